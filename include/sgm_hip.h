@@ -1,0 +1,182 @@
+/*
+ * sgm_hip.h -- C ABI of the MI355X (gfx950) SGM stereo matcher.
+ *
+ * Drop-in replacement for the GPU stereo backend of the Einstar scanner software. The reference
+ * binds its GPU matcher as the C++ class depth_metal::MetalStereo
+ * (libs/depth_metal/include/einstar/depth_metal/metal_stereo.hpp:67-107), whose SGM core runs the
+ * algorithm of depth::sgm_disparity (libs/depth/include/einstar/depth/stereo.hpp:113-115,
+ * libs/depth/src/stereo.cpp:361-463). This header is the C boundary that class's replacement
+ * (include/einstar_hip/hip_stereo.hpp) and foreign callers (ctypes, cgo, JNI) bind: plain
+ * pointers, sizes and status codes; no C++ or HIP types cross it (streams travel as void*).
+ *
+ * Threading: a context is not re-entrant, like MetalStereo (the reference serialises it with a
+ * mutex, libs/pipeline/src/stereo_frontend.cpp:307). Separate contexts are independent. The
+ * server functions are safe to call from any thread.
+ */
+#ifndef SGM_HIP_H
+#define SGM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGM_HIP_ABI_VERSION 1
+
+/* Status codes: the values of einstar::Errc (libs/core/include/einstar/core/error.hpp:8-19). */
+enum {
+    SGM_OK = 0,
+    SGM_ERR_NOT_FOUND = 1,
+    SGM_ERR_IO = 2, /* GPU runtime failure (MetalStereo returns Errc::io, metal_stereo.cpp:716-718) */
+    SGM_ERR_TIMEOUT = 3,
+    SGM_ERR_PROTOCOL = 4,
+    SGM_ERR_INVALID_ARGUMENT = 5, /* size or parameter out of range (metal_stereo.cpp:115-120, 235-236) */
+    SGM_ERR_BLOCKED = 6,
+    SGM_ERR_UNSUPPORTED = 7,
+    SGM_ERR_BUSY = 8, /* serving: not admitted (no free slot or HBM) */
+    SGM_ERR_DISCONNECTED = 9
+};
+
+/* kInvalidDisparity (stereo.hpp:23) and the right-WTA "no match" (stereo.cpp:413). */
+#define SGM_INVALID_DISPARITY (-1.0e9f)
+#define SGM_NO_MATCH INT32_MIN
+
+/* Per-pixel status bits of the left WTA chain (stereo.cpp:419-461); valid == all three. */
+#define SGM_STATUS_WINDOW 1u /* the candidate window [min_k, max_k) is not empty */
+#define SGM_STATUS_UNIQUE 2u /* passed the uniqueness test */
+#define SGM_STATUS_LR 4u     /* passed the left/right check */
+#define SGM_STATUS_VALID 7u
+
+/* depth::SgmParams (stereo.hpp:26-40) plus StereoParams::subpixel (stereo.hpp:87). Booleans are
+ * 0/1. Defaults: sgm_params_default(). Limits (those of MetalStereo::create,
+ * metal_stereo.cpp:115-120): 1 <= num_disparities <= 512, census window <= 9x7,
+ * window cells + max penalty <= 255 (path costs are bytes), 0 <= p1 <= 32767, p2 >= 0. */
+typedef struct {
+    int32_t min_disparity;
+    int32_t num_disparities;
+    int32_t census_radius_x;
+    int32_t census_radius_y;
+    int32_t p1;
+    int32_t p2;
+    int32_t adaptive_p2;
+    int32_t eight_paths;
+    int32_t slant_steps;
+    float uniqueness;
+    int32_t lr_max_diff;
+    int32_t subpixel;
+} sgm_params;
+
+/* Outputs, each W*H (row pitch = width) unless noted; any may be NULL except where noted.
+ * Host pointers for sgm_compute, device pointers for sgm_compute_device and batches. */
+typedef struct {
+    float* disparity;          /* sgm_disparity() result (stereo.hpp:114); invalid = SGM_INVALID_DISPARITY */
+    float* disparity_median;   /* median3(disparity) (stereo.cpp:129-147) */
+    float* confidence;         /* 1 where disparity is valid, else 0 (stereo.cpp:515-519) */
+    int32_t* disparity_int;    /* min_disparity + best_k, SGM_NO_MATCH for an empty window */
+    uint8_t* status;           /* SGM_STATUS_* bits */
+    int32_t* right_disparity;  /* right WTA (stereo.cpp:434-447), SGM_NO_MATCH where none */
+    uint16_t* best_cost;       /* aggregated cost at best_k, 0xFFFF for an empty window */
+    uint16_t* second_cost;     /* best non-adjacent aggregated cost, 0xFFFF if none */
+    uint16_t* sum_volume;      /* debug: aggregated costs S, W*H*D, [y][x][d] */
+    uint16_t* cost_volume;     /* debug: matching costs C, W*H*D, [y][x][d] */
+    uint64_t* census_left;     /* debug: census words */
+    uint64_t* census_right;
+} sgm_outputs;
+
+/* Per-stage GPU times of the last completed compute (hipEvents on the context's stream), as
+ * MetalStereo::last_timings (metal_stereo.hpp:61-65, 97). */
+typedef struct {
+    double census_ms;
+    double paths_ms;
+    double rows_ms;   /* sum of paths, WTA, left/right check */
+    double median_ms;
+    double gpu_ms;    /* first to last kernel */
+    double total_ms;  /* host wall time of the call, including copies */
+} sgm_timings;
+
+typedef struct sgm_context sgm_context;
+
+/* Fills the reference defaults (stereo.hpp:26-40, subpixel = true). */
+void sgm_params_default(sgm_params* p);
+
+/* Replaces MetalStereo::create (metal_stereo.hpp:70-71): validates the parameters and allocates
+ * device buffers for pairs up to max_width x max_height on the current HIP device. */
+int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_context** out);
+
+/* Replaces ~MetalStereo (metal_stereo.hpp:72): frees every buffer. NULL is ignored. */
+void sgm_destroy(sgm_context* ctx);
+
+/* Error text of the last failing call on ctx, or (ctx == NULL) of the last failing sgm_create
+ * or server call on this thread. Never NULL. */
+const char* sgm_last_error(const sgm_context* ctx);
+
+/* Replaces MetalStereo::compute (metal_stereo.hpp:78) for its SGM stage, i.e. depth::sgm_disparity
+ * on rectified images (stereo.hpp:114-115) plus median3: host images with a row stride in bytes,
+ * host outputs; blocks until they are written. */
+int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width, int height,
+                const sgm_outputs* out);
+
+/* The same on device-resident images and outputs, enqueued on `stream` (a hipStream_t; NULL =
+ * the context's own stream) and returning without waiting. out->disparity may be NULL (the
+ * context's buffer is used). Timings become available after sgm_synchronize. */
+int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch, int width,
+                       int height, const sgm_outputs* d_out, void* stream);
+
+/* Waits for the work enqueued by this context and records its timings. */
+int sgm_synchronize(sgm_context* ctx);
+
+/* Replaces MetalStereo::last_timings (metal_stereo.hpp:97). */
+int sgm_last_timings(const sgm_context* ctx, sgm_timings* out);
+
+/* Device memory the context holds, in bytes. */
+size_t sgm_device_bytes(const sgm_context* ctx);
+
+/* Device memory helpers for callers without a HIP binding of their own (ctypes, cgo, JNI):
+ * thin wrappers of hipMalloc / hipFree / hipMemcpy on the current device. No counterpart in the
+ * reference (Metal buffers come from gpu::Context, libs/gpu/include/einstar/gpu/context.hpp:70-74). */
+int sgm_dev_malloc(size_t bytes, void** out);
+int sgm_dev_free(void* ptr);
+int sgm_dev_upload(void* dst, const void* src, size_t bytes);
+int sgm_dev_download(void* dst, const void* src, size_t bytes);
+int sgm_dev_memset(void* dst, int value, size_t bytes);
+/* Free and total device memory in bytes (hipMemGetInfo). */
+int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched SGM: many rectified pairs of uneven sizes and disparity ranges in one graph launch.
+ * No counterpart in the reference (its frontend runs one pair per call,
+ * libs/pipeline/src/stereo_frontend.cpp:302-348); every pair's result equals sgm_compute's.
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    const uint8_t* left;     /* device */
+    const uint8_t* right;    /* device */
+    int64_t pitch;           /* bytes */
+    int32_t width, height;
+    int32_t min_disparity;   /* per-pair disparity range; other parameters come from the batch */
+    int32_t num_disparities;
+    sgm_outputs out;         /* device pointers; disparity is required */
+} sgm_pair;
+
+typedef struct sgm_batch sgm_batch;
+
+/* Plans a batch of n pairs: allocates the scratch of all pairs, picks the launch schedule with the
+ * autotuner (cached per shape class) and captures the launches in a hipGraph. The pair images and
+ * outputs must stay valid until sgm_batch_destroy; their contents may change between runs. */
+int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out);
+/* Replays the graph on `stream` (hipStream_t; NULL = the batch's own stream) without waiting. */
+int sgm_batch_run(sgm_batch* b, void* stream);
+/* The same work launched pair by pair without batching or graphs: the "plain" reference point. */
+int sgm_batch_run_plain(sgm_batch* b, void* stream);
+int sgm_batch_synchronize(sgm_batch* b);
+const char* sgm_batch_last_error(const sgm_batch* b);
+size_t sgm_batch_device_bytes(const sgm_batch* b);
+void sgm_batch_destroy(sgm_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGM_HIP_H */

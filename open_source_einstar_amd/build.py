@@ -1,0 +1,84 @@
+"""Build recipe for the native parts, in-tree (no JIT caches): run `python -m open_source_einstar_amd.build`.
+
+* ``libsgm_hip.so``   -- the gfx950 kernels and the C ABI of include/sgm_hip.h (hipcc)
+* ``libeinstar_synth.so`` -- the synthetic speckle pair generator (gcc)
+* ``hip_stereo_selftest`` -- C++ program driving the HipStereo wrapper (tests only)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIP_LIB = os.path.join(PKG, "libsgm_hip.so")
+SYNTH_LIB = os.path.join(PKG, "libeinstar_synth.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: float stages (subpixel, uniqueness, left/right check) round like the reference.
+# -packed-fp32-ops off: no hand-free v_pk_*_f32 (the kernels are integer; the flag guards the float bits).
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wextra", "-Wno-unused-parameter",
+]
+
+HIP_SOURCES = ["sgm_kernels.hip", "sgm_api.hip"]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_hip(verbose: bool = True) -> str:
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = HIP_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs])
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_synth() -> str:
+    tmp = SYNTH_LIB + ".tmp"
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror",
+          "-o", tmp, os.path.join(CSRC, "speckle_synth.c"), "-lm"])
+    os.replace(tmp, SYNTH_LIB)
+    return SYNTH_LIB
+
+
+def build_oracle() -> str:
+    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle_sgm.so")
+
+
+def build_selftest() -> str | None:
+    src = os.path.join(ROOT, "tests", "cpp", "hip_stereo_selftest.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "hip_stereo_selftest")
+    _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-Werror", f"-I{INCLUDE}", f"-I{os.path.join(ROOT, 'oracle')}",
+          "-o", out, src, os.path.join(ROOT, "oracle", "sgm_oracle.c"), f"-L{PKG}", "-lsgm_hip",
+          f"-Wl,-rpath,{PKG}", "-lpthread", "-lm"])
+    return out
+
+
+def build_all() -> None:
+    build_oracle()
+    build_synth()
+    build_hip()
+    build_selftest()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
+         "selftest": build_selftest}[w]()

@@ -1,0 +1,753 @@
+// C-ABI layer of the gfx950 SGM library (include/sgm_hip.h): contexts, the per-pair pipeline,
+// and batches. Kernels live in sgm_kernels.hip.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sgm_hip.h"
+#include "sgm_internal.h"
+
+using namespace sgm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Stages {
+    hipEvent_t ev[5] = {};
+    bool recorded = false;
+};
+
+std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+// Parameter validation of MetalStereo::create (metal_stereo.cpp:115-120), for byte path costs.
+int validate_params(const sgm_params* p, std::string& err) {
+    if (!p) {
+        err = "params is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (p->num_disparities < 1 || p->num_disparities > 512) {
+        err = "num_disparities must be in [1, 512]";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (p->census_radius_x < 0 || p->census_radius_y < 0 || 2 * p->census_radius_x + 1 > 9 || 2 * p->census_radius_y + 1 > 7) {
+        err = "census window too large for 64 bits (at most 9x7)";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (p->p1 < 0 || p->p1 > 32767 || p->p2 < 0) {
+        err = "p1 must be in [0, 32767] and p2 >= 0";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    const int cells = (2 * p->census_radius_x + 1) * (2 * p->census_radius_y + 1);
+    const int pen = p->adaptive_p2 ? std::max(p->p2, p->p1 + 1) : p->p2;
+    if (cells + pen > 255) {
+        err = "census window + P2 exceed the byte-sized path costs (cells + max(P2, P1+1 if adaptive) <= 255)";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    return SGM_OK;
+}
+
+SgmConsts make_consts(const sgm_params& p) {
+    SgmConsts c{};
+    c.p1 = p.p1;
+    c.p2 = p.p2;
+    c.adaptive_p2 = p.adaptive_p2 ? 1 : 0;
+    c.slant_steps = p.slant_steps ? 1 : 0;
+    c.census_rx = p.census_radius_x;
+    c.census_ry = p.census_radius_y;
+    c.invalid_cost = (2 * p.census_radius_x + 1) * (2 * p.census_radius_y + 1);  // stereo.cpp:366
+    c.lr_max_diff = p.lr_max_diff;
+    c.subpixel = p.subpixel ? 1 : 0;
+    c.uniqueness = p.uniqueness;
+    // stereo.cpp:64: p2 = max(P1+1, P2 / (|dI|/8 + 1)); |dI| <= 255 so |dI|/8 <= 31.
+    for (int q = 0; q < 32; ++q) c.p2_by_q[q] = std::max(p.p1 + 1, p.p2 / (q + 1));
+    return c;
+}
+
+template <typename T>
+hipError_t dalloc(T** ptr, size_t count, size_t& total) {
+    *ptr = nullptr;
+    if (count == 0) return hipSuccess;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
+    if (e == hipSuccess) total += count * sizeof(T);
+    return e;
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
+hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st) {
+    LaunchTable t{};
+    t.one = d;
+    t.n = 1;
+    hipError_t e;
+    if (st && (e = hipEventRecord(st->ev[0], s)) != hipSuccess) return e;
+    t.total_blocks = census_blocks(d.w, d.h);
+    if ((e = launch_census(t, c, s)) != hipSuccess) return e;
+    if (d.cost_volume) {
+        t.total_blocks = (int)(((int64_t)d.w * d.h + kBlock - 1) / kBlock);
+        if ((e = launch_cost_volume(t, c, s)) != hipSuccess) return e;
+    }
+    if (st && (e = hipEventRecord(st->ev[1], s)) != hipSuccess) return e;
+    t.total_blocks = path_blocks(d);
+    if ((e = launch_paths(d.dpl, padded, t, c, s)) != hipSuccess) return e;
+    if (st && (e = hipEventRecord(st->ev[2], s)) != hipSuccess) return e;
+    t.total_blocks = row_blocks(d);
+    if ((e = launch_rows(d.dpl, padded, t, c, d.w, s)) != hipSuccess) return e;
+    if (st && (e = hipEventRecord(st->ev[3], s)) != hipSuccess) return e;
+    if (d.median) {
+        t.total_blocks = median_blocks(d.w, d.h);
+        if ((e = launch_median(t, s)) != hipSuccess) return e;
+    }
+    if (st) {
+        if ((e = hipEventRecord(st->ev[4], s)) != hipSuccess) return e;
+        st->recorded = true;
+    }
+    return hipSuccess;
+}
+
+void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) {
+    d.w = w;
+    d.h = h;
+    d.min_d = min_d;
+    d.nd = nd;
+    d.dpl = dpl_for(nd);
+    d.dp = d.dpl * kGroup;
+    d.ndirs = ndirs;
+    d.lvol_dir_stride = (int64_t)w * h * d.dp;
+    fill_lines(d);
+}
+
+}  // namespace
+
+struct sgm_context {
+    sgm_params p{};
+    SgmConsts consts{};
+    int max_w = 0, max_h = 0, ndirs = 4;
+    hipStream_t stream = nullptr;
+    hipStream_t last_stream = nullptr;
+    Stages st;
+    uint8_t* d_img = nullptr;      // left | right, compact pitch, for host inputs
+    uint64_t* d_census = nullptr;  // left | right
+    uint8_t* d_lvol = nullptr;
+    float* d_disp = nullptr;
+    float* d_median = nullptr;
+    float* d_conf = nullptr;
+    int32_t* d_dint = nullptr;
+    uint8_t* d_status = nullptr;
+    int32_t* d_rdisp = nullptr;
+    uint16_t* d_best = nullptr;
+    uint16_t* d_second = nullptr;
+    uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
+    uint16_t* d_cost = nullptr;
+    size_t bytes = 0;
+    sgm_timings timings{};
+    double pending_wall_ms = 0;
+    std::string err;
+
+    int fail(int code, std::string msg) {
+        err = std::move(msg);
+        return code;
+    }
+    int hip_fail(const char* what, hipError_t e) { return fail(SGM_ERR_IO, hip_msg(what, e)); }
+};
+
+extern "C" {
+
+void sgm_params_default(sgm_params* p) {
+    if (!p) return;
+    *p = sgm_params{};
+    p->min_disparity = 0;
+    p->num_disparities = 64;
+    p->census_radius_x = 2;
+    p->census_radius_y = 2;
+    p->p1 = 8;
+    p->p2 = 96;
+    p->adaptive_p2 = 1;
+    p->eight_paths = 0;
+    p->slant_steps = 1;
+    p->uniqueness = 0.9f;
+    p->lr_max_diff = 1;
+    p->subpixel = 1;
+}
+
+const char* sgm_last_error(const sgm_context* ctx) { return ctx ? ctx->err.c_str() : g_last_error.c_str(); }
+
+void sgm_destroy(sgm_context* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->st.ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
+                    (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
+                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost})
+        dfree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_context** out) {
+    if (!out) {
+        g_last_error = "out is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    std::string err;
+    if (int rc = validate_params(params, err); rc != SGM_OK) {
+        g_last_error = err;
+        return rc;
+    }
+    if (max_width < 1 || max_height < 1 || max_width > kMaxWidth) {
+        g_last_error = "max_width must be in [1, " + std::to_string(kMaxWidth) + "] and max_height >= 1";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    auto* ctx = new sgm_context;
+    ctx->p = *params;
+    ctx->consts = make_consts(*params);
+    ctx->max_w = max_width;
+    ctx->max_h = max_height;
+    ctx->ndirs = params->eight_paths ? 8 : 4;
+    const size_t npx = (size_t)max_width * (size_t)max_height;
+    const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&ctx->st.ev[i]);
+    size_t& b = ctx->bytes;
+    if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_conf, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_dint, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_status, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_rdisp, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_best, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_second, npx, b);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_create: device allocation", e);
+        sgm_destroy(ctx);
+        return SGM_ERR_IO;
+    }
+    *out = ctx;
+    return SGM_OK;
+}
+
+size_t sgm_device_bytes(const sgm_context* ctx) { return ctx ? ctx->bytes : 0; }
+
+int sgm_dev_malloc(size_t bytes, void** out) {
+    if (!out) return SGM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_malloc", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_dev_free(void* ptr) {
+    const hipError_t e = hipFree(ptr);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_free", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_dev_upload(void* dst, const void* src, size_t bytes) {
+    const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_upload", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_dev_download(void* dst, const void* src, size_t bytes) {
+    const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_download", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_dev_memset(void* dst, int value, size_t bytes) {
+    const hipError_t e = hipMemset(dst, value, bytes);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_memset", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_mem_info", e);
+        return SGM_ERR_IO;
+    }
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return SGM_OK;
+}
+
+static int check_size(sgm_context* ctx, int width, int height) {
+    if (width < 1 || height < 1 || width > ctx->max_w || height > ctx->max_h)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "image size " + std::to_string(width) + "x" + std::to_string(height) +
+                                                       " outside the context's 1.." + std::to_string(ctx->max_w) + " x 1.." +
+                                                       std::to_string(ctx->max_h));
+    return SGM_OK;
+}
+
+// Builds the descriptor for one pair over the context's scratch; outputs default to the context's.
+static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t* dr, int64_t pitch, int w, int h,
+                             const sgm_outputs* o) {
+    PairDesc d{};
+    d.left = dl;
+    d.right = dr;
+    d.pitch = pitch;
+    d.census_l = ctx->d_census;
+    d.census_r = ctx->d_census + (size_t)ctx->max_w * ctx->max_h;
+    d.lvol = ctx->d_lvol;
+    set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs);
+    d.disp = (o && o->disparity) ? o->disparity : ctx->d_disp;
+    if (o) {
+        d.median = o->disparity_median;
+        d.conf = o->confidence;
+        d.dint = o->disparity_int;
+        d.status = o->status;
+        d.rdisp = o->right_disparity;
+        d.best = o->best_cost;
+        d.second = o->second_cost;
+        d.sum_volume = o->sum_volume;
+        d.cost_volume = o->cost_volume;
+    }
+    return d;
+}
+
+int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch, int width,
+                       int height, const sgm_outputs* d_out, void* stream) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
+    if (!d_left || !d_right || pitch < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    PairDesc d = context_desc(ctx, d_left, d_right, pitch, width, height, d_out);
+    if (d_out && (d_out->census_left || d_out->census_right))
+        return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
+    const bool padded = d.dp != d.nd;
+    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, &ctx->st);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_device: launch", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = 0;
+    return SGM_OK;
+}
+
+static void read_timings(sgm_context* ctx) {
+    if (!ctx->st.recorded) return;
+    float ms[4] = {0, 0, 0, 0}, all = 0;
+    for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ctx->st.ev[i], ctx->st.ev[i + 1]);
+    (void)hipEventElapsedTime(&all, ctx->st.ev[0], ctx->st.ev[4]);
+    ctx->timings.census_ms = ms[0];
+    ctx->timings.paths_ms = ms[1];
+    ctx->timings.rows_ms = ms[2];
+    ctx->timings.median_ms = ms[3];
+    ctx->timings.gpu_ms = all;
+    ctx->timings.total_ms = ctx->pending_wall_ms > 0 ? ctx->pending_wall_ms : all;
+    ctx->st.recorded = false;
+}
+
+int sgm_synchronize(sgm_context* ctx) {
+    if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
+    hipStream_t s = ctx->last_stream ? ctx->last_stream : ctx->stream;
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_synchronize", e);
+    read_timings(ctx);
+    return SGM_OK;
+}
+
+int sgm_last_timings(const sgm_context* ctx, sgm_timings* out) {
+    if (!ctx || !out) return SGM_ERR_INVALID_ARGUMENT;
+    *out = ctx->timings;
+    return SGM_OK;
+}
+
+int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width, int height,
+                const sgm_outputs* out) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
+    if (!left || !right || stride < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    if (!out || !out->disparity) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "out->disparity is required");
+    const size_t npx = (size_t)width * height;
+    const size_t nvol = npx * (size_t)ctx->p.num_disparities;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (out->sum_volume && !ctx->d_sum) e = dalloc(&ctx->d_sum, (size_t)ctx->max_w * ctx->max_h * ctx->p.num_disparities, ctx->bytes);
+    if (e == hipSuccess && out->cost_volume && !ctx->d_cost)
+        e = dalloc(&ctx->d_cost, (size_t)ctx->max_w * ctx->max_h * ctx->p.num_disparities, ctx->bytes);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: debug volume allocation", e);
+    uint8_t* dl = ctx->d_img;
+    uint8_t* dr = ctx->d_img + (size_t)ctx->max_w * ctx->max_h;
+    e = hipMemcpy2DAsync(dl, width, left, stride, width, height, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpy2DAsync(dr, width, right, stride, width, height, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: upload", e);
+    sgm_outputs o{};
+    o.disparity = ctx->d_disp;
+    o.disparity_median = out->disparity_median ? ctx->d_median : nullptr;
+    o.confidence = out->confidence ? ctx->d_conf : nullptr;
+    o.disparity_int = out->disparity_int ? ctx->d_dint : nullptr;
+    o.status = out->status ? ctx->d_status : nullptr;
+    o.right_disparity = out->right_disparity ? ctx->d_rdisp : nullptr;
+    o.best_cost = out->best_cost ? ctx->d_best : nullptr;
+    o.second_cost = out->second_cost ? ctx->d_second : nullptr;
+    o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
+    o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
+    PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, &ctx->st);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: launch", e);
+    struct Copy {
+        void* host;
+        const void* dev;
+        size_t bytes;
+    };
+    const Copy copies[] = {
+        {out->disparity, ctx->d_disp, npx * 4},
+        {out->disparity_median, ctx->d_median, npx * 4},
+        {out->confidence, ctx->d_conf, npx * 4},
+        {out->disparity_int, ctx->d_dint, npx * 4},
+        {out->status, ctx->d_status, npx},
+        {out->right_disparity, ctx->d_rdisp, npx * 4},
+        {out->best_cost, ctx->d_best, npx * 2},
+        {out->second_cost, ctx->d_second, npx * 2},
+        {out->sum_volume, ctx->d_sum, nvol * 2},
+        {out->cost_volume, ctx->d_cost, nvol * 2},
+        {out->census_left, d.census_l, npx * 8},
+        {out->census_right, d.census_r, npx * 8},
+    };
+    for (const Copy& c : copies)
+        if (c.host && (e = hipMemcpyAsync(c.host, c.dev, c.bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return ctx->hip_fail("sgm_compute: download", e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail("sgm_compute: execution", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    read_timings(ctx);
+    return SGM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------ batches
+
+struct sgm_batch {
+    sgm_params p{};
+    SgmConsts consts{};
+    int n = 0;
+    int ndirs = 4;
+    std::vector<PairDesc> desc;  // host copies
+    PairDesc* d_desc = nullptr;
+    struct Class {
+        int dpl = 0;
+        bool padded = false;
+        std::vector<int> pairs;
+        int32_t* d_first_paths = nullptr;
+        int32_t* d_first_rows = nullptr;
+        PairDesc* d_desc = nullptr;  // this class's descriptors, contiguous
+        int paths_blocks = 0, rows_blocks = 0;
+    };
+    std::vector<Class> classes;
+    int32_t* d_first_census = nullptr;
+    int32_t* d_first_median = nullptr;
+    PairDesc* d_desc_median = nullptr;
+    int census_total = 0, median_total = 0, n_median = 0;
+    uint8_t* d_scratch = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> fork_ev, join_ev;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t last_stream = nullptr;
+    std::string err;
+    int fail(int code, std::string m) {
+        err = std::move(m);
+        return code;
+    }
+};
+
+namespace {
+
+// Uploads a flat-block prefix table and a descriptor array; returns the launch table.
+hipError_t upload_table(const std::vector<PairDesc>& ds, const std::vector<int>& blocks, PairDesc** d_desc,
+                        int32_t** d_first, size_t& bytes, LaunchTable& t) {
+    t = LaunchTable{};
+    t.n = (int)ds.size();
+    std::vector<int32_t> first(ds.size() + 1, 0);
+    for (size_t i = 0; i < ds.size(); ++i) first[i + 1] = first[i] + blocks[i];
+    t.total_blocks = first.back();
+    if (ds.size() == 1) {
+        t.one = ds[0];
+        return hipSuccess;
+    }
+    hipError_t e = dalloc(d_desc, ds.size(), bytes);
+    if (e == hipSuccess) e = dalloc(d_first, first.size(), bytes);
+    if (e == hipSuccess) e = hipMemcpy(*d_desc, ds.data(), ds.size() * sizeof(PairDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(*d_first, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    t.many = *d_desc;
+    t.first_block = *d_first;
+    return e;
+}
+
+struct BatchTables {
+    LaunchTable census, median;
+    std::vector<LaunchTable> paths, rows;
+};
+
+hipError_t build_tables(sgm_batch* b, BatchTables& bt) {
+    std::vector<int> blocks;
+    for (const PairDesc& d : b->desc) blocks.push_back(census_blocks(d.w, d.h));
+    hipError_t e = upload_table(b->desc, blocks, &b->d_desc, &b->d_first_census, b->bytes, bt.census);
+    if (e != hipSuccess) return e;
+    std::vector<PairDesc> med;
+    blocks.clear();
+    for (const PairDesc& d : b->desc)
+        if (d.median) {
+            med.push_back(d);
+            blocks.push_back(median_blocks(d.w, d.h));
+        }
+    b->n_median = (int)med.size();
+    if (!med.empty() && (e = upload_table(med, blocks, &b->d_desc_median, &b->d_first_median, b->bytes, bt.median)) != hipSuccess)
+        return e;
+    for (auto& c : b->classes) {
+        std::vector<PairDesc> ds;
+        std::vector<int> pb, rb;
+        for (int i : c.pairs) {
+            ds.push_back(b->desc[(size_t)i]);
+            pb.push_back(path_blocks(b->desc[(size_t)i]));
+            rb.push_back(row_blocks(b->desc[(size_t)i]));
+        }
+        LaunchTable tp, tr;
+        PairDesc* dd = nullptr;
+        if ((e = upload_table(ds, pb, &dd, &c.d_first_paths, b->bytes, tp)) != hipSuccess) return e;
+        c.d_desc = dd;
+        if ((e = upload_table(ds, rb, &dd, &c.d_first_rows, b->bytes, tr)) != hipSuccess) return e;
+        if (dd && dd != c.d_desc) {
+            // The rows table reuses the paths descriptors: free the duplicate.
+            dfree(dd);
+            b->bytes -= ds.size() * sizeof(PairDesc);
+        }
+        tr.many = tp.many;
+        bt.paths.push_back(tp);
+        bt.rows.push_back(tr);
+    }
+    return hipSuccess;
+}
+
+hipError_t enqueue_batch(sgm_batch* b, const BatchTables& bt, hipStream_t s) {
+    hipError_t e = launch_census(bt.census, b->consts, s);
+    if (e != hipSuccess) return e;
+    const size_t nc = b->classes.size();
+    // Classes run as parallel branches (fork/join through events): their pairs are independent.
+    if (nc > 1) {
+        if ((e = hipEventRecord(b->fork_ev[0], s)) != hipSuccess) return e;
+        for (size_t i = 1; i < nc; ++i)
+            if ((e = hipStreamWaitEvent(b->branch[i], b->fork_ev[0], 0)) != hipSuccess) return e;
+    }
+    for (size_t i = 0; i < nc; ++i) {
+        hipStream_t cs = i == 0 ? s : b->branch[i];
+        const auto& c = b->classes[i];
+        if ((e = launch_paths(c.dpl, c.padded, bt.paths[i], b->consts, cs)) != hipSuccess) return e;
+        int maxw = 1;
+        for (int p : c.pairs) maxw = std::max(maxw, b->desc[(size_t)p].w);
+        if ((e = launch_rows(c.dpl, c.padded, bt.rows[i], b->consts, maxw, cs)) != hipSuccess) return e;
+    }
+    if (nc > 1) {
+        for (size_t i = 1; i < nc; ++i) {
+            if ((e = hipEventRecord(b->join_ev[i], b->branch[i])) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(s, b->join_ev[i], 0)) != hipSuccess) return e;
+        }
+    }
+    if (b->n_median > 0 && (e = launch_median(bt.median, s)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sgm_batch_destroy(sgm_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& c : b->classes) {
+        dfree(c.d_first_paths);
+        dfree(c.d_first_rows);
+        dfree(c.d_desc);
+    }
+    dfree(b->d_first_census);
+    dfree(b->d_first_median);
+    dfree(b->d_desc_median);
+    dfree(b->d_desc);
+    dfree(b->d_scratch);
+    for (hipEvent_t e : b->fork_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->join_ev) (void)hipEventDestroy(e);
+    for (size_t i = 1; i < b->branch.size(); ++i) (void)hipStreamDestroy(b->branch[i]);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* sgm_batch_last_error(const sgm_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+size_t sgm_batch_device_bytes(const sgm_batch* b) { return b ? b->bytes : 0; }
+
+int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out) {
+    if (!out) {
+        g_last_error = "out is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    std::string err;
+    if (int rc = validate_params(params, err); rc != SGM_OK) {
+        g_last_error = err;
+        return rc;
+    }
+    if (n < 1 || n > kMaxBatch || !pairs) {
+        g_last_error = "batch size must be in [1, " + std::to_string(kMaxBatch) + "]";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < n; ++i) {
+        const sgm_pair& q = pairs[i];
+        sgm_params pp = *params;
+        pp.min_disparity = q.min_disparity;
+        pp.num_disparities = q.num_disparities;
+        if (validate_params(&pp, err) != SGM_OK || q.width < 1 || q.height < 1 || q.width > kMaxWidth ||
+            q.pitch < q.width || !q.left || !q.right || !q.out.disparity) {
+            g_last_error = "pair " + std::to_string(i) + ": " + (err.empty() ? "bad size, pointer or pitch" : err);
+            return SGM_ERR_INVALID_ARGUMENT;
+        }
+        if (q.out.census_left || q.out.census_right) {
+            g_last_error = "pair " + std::to_string(i) + ": census outputs are only available from sgm_compute";
+            return SGM_ERR_UNSUPPORTED;
+        }
+    }
+    auto* b = new sgm_batch;
+    b->p = *params;
+    b->consts = make_consts(*params);
+    b->n = n;
+    b->ndirs = params->eight_paths ? 8 : 4;
+    // Scratch: census words (2 per pixel) and path costs (ndirs * dp per pixel) of every pair.
+    std::vector<size_t> off_census(n), off_lvol(n);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t npx = (size_t)pairs[i].width * pairs[i].height;
+        off_census[i] = total;
+        total += (2 * npx * 8 + 255) & ~(size_t)255;
+        off_lvol[i] = total;
+        total += ((size_t)b->ndirs * npx * (size_t)dpl_for(pairs[i].num_disparities) * kGroup + 255) & ~(size_t)255;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = dalloc(&b->d_scratch, total, b->bytes);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_batch_create: allocation", e);
+        sgm_batch_destroy(b);
+        return SGM_ERR_IO;
+    }
+    std::map<std::pair<int, bool>, int> class_of;
+    for (int i = 0; i < n; ++i) {
+        const sgm_pair& q = pairs[i];
+        PairDesc d{};
+        d.left = q.left;
+        d.right = q.right;
+        d.pitch = q.pitch;
+        const size_t npx = (size_t)q.width * q.height;
+        d.census_l = reinterpret_cast<uint64_t*>(b->d_scratch + off_census[i]);
+        d.census_r = d.census_l + npx;
+        d.lvol = b->d_scratch + off_lvol[i];
+        set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);
+        d.disp = q.out.disparity;
+        d.median = q.out.disparity_median;
+        d.conf = q.out.confidence;
+        d.dint = q.out.disparity_int;
+        d.status = q.out.status;
+        d.rdisp = q.out.right_disparity;
+        d.best = q.out.best_cost;
+        d.second = q.out.second_cost;
+        d.sum_volume = q.out.sum_volume;
+        d.cost_volume = nullptr;
+        b->desc.push_back(d);
+        const auto key = std::make_pair(d.dpl, d.dp != d.nd);
+        auto it = class_of.find(key);
+        if (it == class_of.end()) {
+            it = class_of.emplace(key, (int)b->classes.size()).first;
+            sgm_batch::Class c;
+            c.dpl = key.first;
+            c.padded = key.second;
+            b->classes.push_back(c);
+        }
+        b->classes[(size_t)it->second].pairs.push_back(i);
+    }
+    b->branch.assign(b->classes.size(), nullptr);
+    b->branch[0] = b->stream;
+    for (size_t i = 1; e == hipSuccess && i < b->classes.size(); ++i) e = hipStreamCreateWithFlags(&b->branch[i], hipStreamNonBlocking);
+    b->fork_ev.assign(1, nullptr);
+    b->join_ev.assign(b->classes.size(), nullptr);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork_ev[0], hipEventDisableTiming);
+    for (size_t i = 0; e == hipSuccess && i < b->classes.size(); ++i) e = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
+    BatchTables bt;
+    if (e == hipSuccess) e = build_tables(b, bt);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t le = enqueue_batch(b, bt, b->stream);
+        e = hipStreamEndCapture(b->stream, &b->graph);
+        if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_batch_create: tables/graph", e);
+        sgm_batch_destroy(b);
+        return SGM_ERR_IO;
+    }
+    *out = b;
+    return SGM_OK;
+}
+
+int sgm_batch_run(sgm_batch* b, void* stream) {
+    if (!b) return SGM_ERR_INVALID_ARGUMENT;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->stream;
+    const hipError_t e = hipGraphLaunch(b->exec, s);
+    if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_run", e));
+    b->last_stream = s;
+    return SGM_OK;
+}
+
+int sgm_batch_run_plain(sgm_batch* b, void* stream) {
+    if (!b) return SGM_ERR_INVALID_ARGUMENT;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->stream;
+    for (const PairDesc& d : b->desc) {
+        const hipError_t e = enqueue_pair(d, b->consts, d.dp != d.nd, s, nullptr);
+        if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_run_plain", e));
+    }
+    b->last_stream = s;
+    return SGM_OK;
+}
+
+int sgm_batch_synchronize(sgm_batch* b) {
+    if (!b) return SGM_ERR_INVALID_ARGUMENT;
+    const hipError_t e = hipStreamSynchronize(b->last_stream ? b->last_stream : b->stream);
+    if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_synchronize", e));
+    return SGM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// Internal types shared by the gfx950 SGM kernels (sgm_kernels.hip) and the C-ABI layer
+// (sgm_api.hip). Not installed; the public interface is include/sgm_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgm {
+
+// Lanes that cooperate on one scanline: one 16-lane DPP row of a wave64. Each lane owns `dpl`
+// consecutive disparities, so a line covers dp = 16 * dpl >= D disparities.
+constexpr int kGroup = 16;
+constexpr int kBlock = 256;                   // threads per block (4 wave64s)
+constexpr int kLinesPerBlock = kBlock / kGroup;
+constexpr int kMaxBatch = 64;                 // pairs per launch table
+constexpr int kMaxDirs = 8;
+constexpr int kMaxWidth = 4096;               // the row kernel keeps 9 bytes of LDS per column
+
+// Device pointers and geometry of one rectified pair. Every output pointer may be null except
+// disp, which the left/right check and median read back.
+struct PairDesc {
+    const uint8_t* left;
+    const uint8_t* right;
+    uint64_t* census_l;
+    uint64_t* census_r;
+    uint8_t* lvol;           // [dir][y][x][dp] per-direction path costs L (u8)
+    float* disp;             // sgm_disparity output (post left/right check)
+    float* median;           // median3(disp)
+    float* conf;             // 1 where disp is valid, else 0
+    int32_t* dint;           // min_d + best_k, INT32_MIN for an empty candidate window
+    uint8_t* status;         // SGM_STATUS_* bits
+    int32_t* rdisp;          // right WTA, INT32_MIN for no match
+    uint16_t* best;          // S at best_k
+    uint16_t* second;        // best non-adjacent S (0xFFFF: none)
+    uint16_t* sum_volume;    // optional debug S [y][x][nd]
+    uint16_t* cost_volume;   // optional debug C [y][x][nd]
+    int64_t pitch;           // input row pitch in bytes
+    int64_t lvol_dir_stride; // bytes between direction planes of lvol
+    int32_t w, h;
+    int32_t min_d, nd, dpl, dp;
+    int32_t ndirs;
+    int32_t total_lines;             // scanlines over all directions
+    int32_t line_off[kMaxDirs];      // first line index of each direction
+};
+
+// Parameters shared by every pair of a launch (depth::SgmParams minus the disparity range).
+struct SgmConsts {
+    int32_t p1, p2, adaptive_p2, slant_steps;
+    int32_t census_rx, census_ry, invalid_cost;
+    int32_t lr_max_diff, subpixel;
+    float uniqueness;
+    int32_t p2_by_q[32];  // max(P1+1, P2/(q+1)) for q = |dI|/8 in [0, 31]
+};
+
+// Maps a flat block index of one launch to (pair, block within the pair). A single pair travels
+// inline in the kernel arguments; batches point at a device-resident descriptor array.
+struct LaunchTable {
+    PairDesc one;                 // used when n == 1
+    const PairDesc* many;         // device array of n descriptors (n > 1)
+    const int32_t* first_block;   // device array: first flat block of each pair (n > 1)
+    int32_t n;
+    int32_t total_blocks;
+};
+
+// Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
+hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
+hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
+
+// Per-pair block counts for each stage (the launchers' grid geometry).
+int census_blocks(int w, int h);
+int path_blocks(const PairDesc& d);
+int row_blocks(const PairDesc& d);
+int median_blocks(int w, int h);
+
+// Disparities per lane for a disparity count, or 0 when unsupported (> 512).
+int dpl_for(int nd);
+// Fills line_off for the 4 or 8 directions of the reference's schedule.
+void fill_lines(PairDesc& d);
+
+}  // namespace sgm

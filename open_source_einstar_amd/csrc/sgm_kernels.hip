@@ -1,0 +1,641 @@
+// gfx950 (CDNA4) kernels of the SGM stereo hot path.
+//
+// Semantics follow the reference CPU matcher (libs/depth/src/stereo.cpp:44-96, 129-147, 328-346,
+// 361-463); the layout and the parallel decomposition are this library's own:
+//
+//   census   one thread per pixel, LDS tile with halo                  -> census words u64
+//   paths    one 16-lane DPP row per scanline, all directions of all   -> L_dir u8 [dir][y][x][dp]
+//            pairs in one launch; a lane holds `dpl` consecutive
+//            disparities as packed u16 pairs; the costs are recomputed
+//            from the census words on the fly (no cost volume in HBM)
+//   rows     one block per image row: S = sum of the L_dir, left WTA   -> disparity, status, costs
+//            with uniqueness + subpixel, right WTA by LDS atomic-min
+//            of packed (S << 16 | k) keys, then the left/right check
+//   median   3x3 median of valid neighbours                           -> disparity_median
+//
+// Every path value satisfies 0 <= L <= C + p2 <= 255 (the context enforces cells + p2 <= 255), so
+// L fits a byte and S = sum over <= 8 paths <= 2040 never wraps: the direction order and the
+// per-direction split of the sum do not change any result.
+#include <climits>
+
+#include "sgm_internal.h"
+
+namespace sgm {
+namespace {
+
+constexpr float kInvalid = -1.0e9f;
+constexpr uint32_t kSentinel = 0x7FFF7FFFu;  // excluded neighbour: 0x7FFF + P1 never wins a min
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+
+// DPP controls (gfx9 encoding).
+constexpr int kQuadSwap1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kQuadSwap2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kRowHalfMirror = 0x141;
+constexpr int kRowMirror = 0x140;
+constexpr int kRowShl1 = 0x101;       // lane i reads lane i+1 of its row
+constexpr int kRowShr1 = 0x111;       // lane i reads lane i-1 of its row
+
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, Ctrl, 0xF, 0xF, false);
+}
+// Neighbour lane's value, or `edge` for the lane at the end of the 16-lane row.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t v, uint32_t edge) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, Ctrl, 0xF, 0xF, false);
+}
+
+// Minimum over the 16 lanes of a DPP row, returned to all of them.
+__device__ __forceinline__ uint32_t row_min(uint32_t v) {
+    v = min(v, dpp<kQuadSwap1>(v));
+    v = min(v, dpp<kQuadSwap2>(v));
+    v = min(v, dpp<kRowHalfMirror>(v));
+    v = min(v, dpp<kRowMirror>(v));
+    return v;
+}
+
+__device__ __forceinline__ void locate(const LaunchTable& t, int b, int& pi, int& lb) {
+    if (t.n == 1) {
+        pi = 0;
+        lb = b;
+        return;
+    }
+    int lo = 0, hi = t.n - 1;  // last pair whose first block <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.first_block[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    pi = lo;
+    lb = b - t.first_block[lo];
+}
+
+__device__ __forceinline__ PairDesc load_desc(const LaunchTable& t, int pi) { return t.n == 1 ? t.one : t.many[pi]; }
+
+// ------------------------------------------------------------------------------------ census
+// census_transform, stereo.cpp:328-346: rows outer, columns inner, centre skipped, each compare
+// shifted in at bit 0 (the first compare ends at bit cells-2); border pixels are 0.
+
+constexpr int kTile = 16;
+constexpr int kMaxRx = 4, kMaxRy = 3;
+constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
+
+__global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
+    __shared__ uint8_t tile[kTileH * kTileW];
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
+    const int side = lb / (tx_n * ty_n);
+    const int tl = lb - side * tx_n * ty_n;
+    const int x0 = (tl % tx_n) * kTile, y0 = (tl / tx_n) * kTile;
+    const uint8_t* img = side ? P.right : P.left;
+    uint64_t* out = side ? P.census_r : P.census_l;
+    const int rx = c.census_rx, ry = c.census_ry;
+    const int tw = kTile + 2 * rx, th = kTile + 2 * ry;
+    for (int i = threadIdx.x; i < tw * th; i += kBlock) {
+        const int ly = i / tw, lx = i - ly * tw;
+        const int gx = x0 - rx + lx, gy = y0 - ry + ly;
+        tile[ly * kTileW + lx] = (gx >= 0 && gy >= 0 && gx < P.w && gy < P.h) ? img[(int64_t)gy * P.pitch + gx] : 0;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & (kTile - 1), ly = threadIdx.x >> 4;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= P.w || y >= P.h) return;
+    uint64_t bits = 0;
+    if (x >= rx && x < P.w - rx && y >= ry && y < P.h - ry) {
+        const uint8_t center = tile[(ly + ry) * kTileW + lx + rx];
+        for (int v = 0; v <= 2 * ry; ++v) {
+            const uint8_t* row = &tile[(ly + v) * kTileW + lx];
+            for (int u = 0; u <= 2 * rx; ++u) {
+                if (u == rx && v == ry) continue;
+                bits = (bits << 1) | (uint64_t)(row[u] < center);
+            }
+        }
+    }
+    out[(int64_t)y * P.w + x] = bits;
+}
+
+// ------------------------------------------------------------------------------- cost volume
+// Debug output only (the path kernel recomputes costs): the cost fill of stereo.cpp:366-380.
+
+__global__ __launch_bounds__(kBlock) void cost_volume_kernel(LaunchTable t, SgmConsts c) {
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    if (i >= (int64_t)P.w * P.h) return;
+    const int y = (int)(i / P.w), x = (int)(i - (int64_t)y * P.w);
+    const uint64_t a = P.census_l[i];
+    uint16_t* dst = P.cost_volume + i * P.nd;
+    for (int k = 0; k < P.nd; ++k) {
+        const int xr = x - (P.min_d + k);
+        dst[k] = (xr >= 0 && xr < P.w) ? (uint16_t)__popcll(a ^ P.census_r[(int64_t)y * P.w + xr]) : (uint16_t)c.invalid_cost;
+    }
+}
+
+// ------------------------------------------------------------------------------------- paths
+// aggregate_path, stereo.cpp:44-89, for every start pixel of every direction (:385-409).
+// Lane g of a line owns disparities k0..k0+DPL-1 (k0 = g*DPL) as DPL/2 packed u16 pairs
+// (low half = even k). Per step:
+//   L(k) = C(k) + min(L'(k), L'(k+-1) + P1, [L'(k+-2) + P1], pm + p2) - pm,   pm = min_k L'(k)
+// The +-1/+-2 neighbours across lane boundaries come from the adjacent lane by DPP row shifts,
+// and pm by a 4-step DPP butterfly; a line never leaves its DPP row.
+
+template <int DPL>
+__device__ __forceinline__ void load_words(const uint64_t* __restrict__ row, int xr0, int w, uint64_t (&wv)[DPL]) {
+#pragma unroll
+    for (int j = 0; j < DPL; ++j) {
+        int xr = xr0 - j;
+        xr = xr < 0 ? 0 : (xr >= w ? w - 1 : xr);
+        wv[j] = row[xr];
+    }
+}
+
+template <int DPL>
+__device__ __forceinline__ void costs(uint64_t a, const uint64_t (&wv)[DPL], int xr0, int w, uint32_t inv,
+                                      uint32_t (&cc)[DPL / 2]) {
+#pragma unroll
+    for (int r = 0; r < DPL / 2; ++r) {
+        const int xa = xr0 - 2 * r, xb = xa - 1;
+        const uint32_t ca = (unsigned)xa < (unsigned)w ? (uint32_t)__popcll(a ^ wv[2 * r]) : inv;
+        const uint32_t cb = (unsigned)xb < (unsigned)w ? (uint32_t)__popcll(a ^ wv[2 * r + 1]) : inv;
+        cc[r] = ca | (cb << 16);
+    }
+}
+
+// Low bytes of the DPL/2 packed pairs -> DPL bytes at dst (aligned to DPL's power-of-two part).
+template <int DPL>
+__device__ __forceinline__ void store_bytes(uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
+    if constexpr (DPL % 4 == 0) {
+        uint32_t b[DPL / 4];
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) b[i] = __builtin_amdgcn_perm(v[2 * i + 1], v[2 * i], 0x06040200u);
+        if constexpr (DPL % 16 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 16; ++i)
+                reinterpret_cast<uint4*>(dst)[i] = make_uint4(b[4 * i], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]);
+        } else if constexpr (DPL % 8 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<uint2*>(dst)[i] = make_uint2(b[2 * i], b[2 * i + 1]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<uint32_t*>(dst)[i] = b[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < DPL / 2; ++i)
+            reinterpret_cast<uint16_t*>(dst)[i] = (uint16_t)((v[i] & 0xFFu) | ((v[i] >> 8) & 0xFF00u));
+    }
+}
+
+template <int DPL, bool PAD>
+__global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
+    constexpr int R = DPL / 2;
+    __shared__ int32_t p2q[32];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) p2q[q] = c.p2_by_q[q];
+    }
+    __syncthreads();
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    const int line = lb * kLinesPerBlock + (threadIdx.x >> 4);
+    if (line >= P.total_lines) return;
+    int dir = 0, off = 0;
+#pragma unroll
+    for (int d = 1; d < kMaxDirs; ++d)
+        if (d < P.ndirs && line >= P.line_off[d]) {
+            dir = d;
+            off = P.line_off[d];
+        }
+    const int li = line - off;
+    // Direction order of stereo.cpp:387-388.
+    const int dx = (dir == 0 || dir == 4 || dir == 6) ? 1 : ((dir == 1 || dir == 5 || dir == 7) ? -1 : 0);
+    const int dy = (dir == 2 || dir == 4 || dir == 7) ? 1 : ((dir == 3 || dir == 5 || dir == 6) ? -1 : 0);
+    const int W = P.w, H = P.h;
+    int x, y;  // start pixel, in the order of stereo.cpp:391-402
+    {
+        const int ncol = dx != 0 ? H : 0;
+        if (li < ncol) {
+            x = dx > 0 ? 0 : W - 1;
+            y = li;
+        } else {
+            int j = li - ncol;
+            if (dx != 0 && j >= (dx > 0 ? 0 : W - 1)) ++j;
+            x = j;
+            y = dy > 0 ? 0 : H - 1;
+        }
+    }
+    const int g = threadIdx.x & (kGroup - 1);
+    const int k0 = g * DPL;
+    const int nd = P.nd;
+    const int xoff = -P.min_d - k0;  // xr of this lane's first disparity is x + xoff
+    const uint32_t inv = (uint32_t)c.invalid_cost;
+    const uint32_t p1p1 = (uint32_t)c.p1 * 0x10001u;
+    const uint32_t nslant = c.slant_steps ? 0u : 0xFFFFFFFFu;
+    const bool adaptive = c.adaptive_p2 != 0;
+    const int p2_fixed = c.p2;
+    uint32_t padm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        padm[r] = PAD ? ((k0 + 2 * r >= nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= nd ? 0x7FFF0000u : 0u)) : 0u;
+    const uint64_t* __restrict__ cl = P.census_l;
+    const uint64_t* __restrict__ cr = P.census_r;
+    const uint8_t* __restrict__ guide = P.left;
+    uint8_t* __restrict__ lv = P.lvol + (int64_t)dir * P.lvol_dir_stride + k0;
+    const int64_t dp = P.dp;
+
+    // Loads for the current pixel; the next pixel's are issued before the current step is computed.
+    uint64_t a = cl[(int64_t)y * W + x];
+    uint64_t wv[DPL];
+    load_words<DPL>(cr + (int64_t)y * W, x + xoff, W, wv);
+    int I = guide[(int64_t)y * P.pitch + x];
+
+    uint32_t prev[R];
+    uint32_t pm;
+    int nx = x + dx, ny = y + dy;
+    bool more = nx >= 0 && ny >= 0 && nx < W && ny < H;
+    uint64_t na, nwv[DPL];
+    int nI;
+    {
+        const int lx = more ? nx : x, ly = more ? ny : y;
+        na = cl[(int64_t)ly * W + lx];
+        load_words<DPL>(cr + (int64_t)ly * W, lx + xoff, W, nwv);
+        nI = guide[(int64_t)ly * P.pitch + lx];
+    }
+    {
+        uint32_t cc[R];
+        costs<DPL>(a, wv, x + xoff, W, inv, cc);
+        uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            prev[r] = PAD ? pk_max(cc[r], padm[r]) : cc[r];
+            mn = pk_min(mn, prev[r]);
+        }
+        store_bytes<DPL>(lv + ((int64_t)y * W + x) * dp, prev);
+        pm = row_min(min(mn & 0xFFFFu, mn >> 16));
+    }
+    int Iprev = I;
+    while (more) {
+        x = nx;
+        y = ny;
+        a = na;
+#pragma unroll
+        for (int j = 0; j < DPL; ++j) wv[j] = nwv[j];
+        I = nI;
+        nx = x + dx;
+        ny = y + dy;
+        more = nx >= 0 && ny >= 0 && nx < W && ny < H;
+        {
+            const int lx = more ? nx : x, ly = more ? ny : y;
+            na = cl[(int64_t)ly * W + lx];
+            load_words<DPL>(cr + (int64_t)ly * W, lx + xoff, W, nwv);
+            nI = guide[(int64_t)ly * P.pitch + lx];
+        }
+        // stereo.cpp:62-65: adaptive P2 from the left image's intensity step (|dI|/8 <= 31).
+        const int dI = I > Iprev ? I - Iprev : Iprev - I;
+        const int p2 = adaptive ? p2q[dI >> 3] : p2_fixed;
+        Iprev = I;
+        uint32_t cc[R];
+        costs<DPL>(a, wv, x + xoff, W, inv, cc);
+        const uint32_t pmpm = pm * 0x10001u;
+        const uint32_t jj = (pm + (uint32_t)p2) * 0x10001u;
+        const uint32_t nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);  // k0-2, k0-1
+        const uint32_t nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);      // k0+DPL, k0+DPL+1
+        uint32_t cur[R];
+        uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t lo = r == 0 ? nb_lo : prev[r - 1];
+            const uint32_t hi = r == R - 1 ? nb_hi : prev[r + 1];
+            const uint32_t km1 = __builtin_amdgcn_alignbit(prev[r], lo, 16);  // (k-1) pairs
+            const uint32_t kp1 = __builtin_amdgcn_alignbit(hi, prev[r], 16);  // (k+1) pairs
+            uint32_t m = pk_min(pk_min(km1, kp1), pk_min(lo, hi) | nslant);
+            m = pk_add(m, p1p1);
+            const uint32_t best = pk_min(pk_min(prev[r], m), jj);
+            uint32_t v = pk_add(pk_sub(best, pmpm), cc[r]);
+            if (PAD) v = pk_max(v, padm[r]);
+            cur[r] = v;
+            mn = pk_min(mn, v);
+        }
+        store_bytes<DPL>(lv + ((int64_t)y * W + x) * dp, cur);
+        pm = row_min(min(mn & 0xFFFFu, mn >> 16));
+#pragma unroll
+        for (int r = 0; r < R; ++r) prev[r] = cur[r];
+    }
+}
+
+// -------------------------------------------------------------------------------------- rows
+// Sum of the path costs, then stereo.cpp:411-461 for one image row per block.
+
+__device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {  // stereo.cpp:92-96
+    const float denom = cm - 2.0f * c0 + cp;
+    if (denom <= 0.0f) return 0.0f;
+    const float v = 0.5f * (cm - cp) / denom;
+    return v < -0.5f ? -0.5f : (0.5f < v ? 0.5f : v);
+}
+
+template <int DPL>
+__device__ __forceinline__ void load_add(const uint8_t* src, uint32_t (&s)[DPL / 2]) {
+    if constexpr (DPL % 4 == 0) {
+        uint32_t b[DPL / 4];
+        if constexpr (DPL % 16 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 16; ++i) {
+                const uint4 q = reinterpret_cast<const uint4*>(src)[i];
+                b[4 * i] = q.x;
+                b[4 * i + 1] = q.y;
+                b[4 * i + 2] = q.z;
+                b[4 * i + 3] = q.w;
+            }
+        } else if constexpr (DPL % 8 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 8; ++i) {
+                const uint2 q = reinterpret_cast<const uint2*>(src)[i];
+                b[2 * i] = q.x;
+                b[2 * i + 1] = q.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < DPL / 4; ++i) b[i] = reinterpret_cast<const uint32_t*>(src)[i];
+        }
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) {
+            s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, b[i], 0x0C010C00u));
+            s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, b[i], 0x0C030C02u));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < DPL / 2; ++i) {
+            const uint32_t q = reinterpret_cast<const uint16_t*>(src)[i];
+            s[i] = pk_add(s[i], (q & 0xFFu) | ((q & 0xFF00u) << 8));
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
+
+__device__ __forceinline__ uint32_t row_max(uint32_t v) {
+    v = max(v, dpp<kQuadSwap1>(v));
+    v = max(v, dpp<kQuadSwap2>(v));
+    v = max(v, dpp<kRowHalfMirror>(v));
+    v = max(v, dpp<kRowMirror>(v));
+    return v;
+}
+
+template <int DPL, bool PAD>
+__global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
+    extern __shared__ uint32_t smem[];
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    const int W = P.w, y = lb;
+    uint32_t* rkey = smem;
+    float* drow = reinterpret_cast<float*>(smem + W);
+    uint8_t* srow = reinterpret_cast<uint8_t*>(smem + 2 * W);
+    for (int x = threadIdx.x; x < W; x += kBlock) rkey[x] = 0xFFFFFFFFu;
+    __syncthreads();
+
+    const int grp = threadIdx.x >> 4, g = threadIdx.x & (kGroup - 1);
+    const int k0 = g * DPL;
+    const int nd = P.nd, min_d = P.min_d;
+    const int64_t dp = P.dp;
+    const uint8_t* lrow = P.lvol + (int64_t)y * W * dp + k0;
+    const int64_t row0 = (int64_t)y * W;
+    for (int x = grp; x < W; x += kLinesPerBlock) {
+        uint32_t s[DPL / 2];
+#pragma unroll
+        for (int r = 0; r < DPL / 2; ++r) s[r] = 0;
+        for (int d = 0; d < P.ndirs; ++d) load_add<DPL>(lrow + d * P.lvol_dir_stride + (int64_t)x * dp, s);
+
+        // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
+        const int min_k = max(0, x - min_d - (W - 1));
+        const int max_k = min(nd, x - min_d + 1);
+        uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < DPL; ++j) {
+            const int k = k0 + j;
+            if (k >= min_k && k < max_k) {
+                const uint32_t kk = (half16(s[j >> 1], j & 1) << 16) | (uint32_t)k;
+                key = min(key, kk);
+                atomicMin(&rkey[x - min_d - k], kk);  // right WTA, stereo.cpp:434-447
+            }
+        }
+        key = row_min(key);  // lowest S, then lowest k: the first strict argmin of :422-424
+        const int bk = key == 0xFFFFFFFFu ? -1 : (int)(key & 0xFFFFu);
+        const uint32_t bS = key >> 16;
+        uint32_t sec = 0xFFFFu;  // stereo.cpp:425-427
+#pragma unroll
+        for (int j = 0; j < DPL; ++j) {
+            const int k = k0 + j;
+            const int dk = k - bk;
+            if (k >= min_k && k < max_k && (dk > 1 || dk < -1)) sec = min(sec, half16(s[j >> 1], j & 1));
+        }
+        sec = row_min(sec);
+        float d = kInvalid;
+        uint8_t st = 0;
+        if (bk >= 0) {
+            st = 1;
+            if (!(sec != 0xFFFFu && (float)bS >= c.uniqueness * (float)sec)) {  // stereo.cpp:428
+                st |= 2;
+                d = (float)(min_d + bk);
+                if (c.subpixel && bk > min_k && bk + 1 < max_k) {  // stereo.cpp:430-431
+                    // S(bk-1) and S(bk+1): the owning lanes contribute them, the others 0.
+                    uint32_t vm = 0, vp = 0;
+#pragma unroll
+                    for (int j = 0; j < DPL; ++j) {
+                        const uint32_t v = half16(s[j >> 1], j & 1);
+                        vm = (k0 + j == bk - 1) ? v : vm;
+                        vp = (k0 + j == bk + 1) ? v : vp;
+                    }
+                    vm = row_max(vm);
+                    vp = row_max(vp);
+                    d += subpixel_offset((float)vm, (float)bS, (float)vp);
+                }
+            }
+        }
+        if (g == 0) {
+            drow[x] = d;
+            srow[x] = st;
+            if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
+            if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? bS : 0xFFFFu);
+            if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? sec : 0xFFFFu);
+        }
+        if (P.sum_volume) {
+            uint16_t* sv = P.sum_volume + (row0 + x) * nd;
+#pragma unroll
+            for (int j = 0; j < DPL; ++j)
+                if (k0 + j < nd) sv[k0 + j] = (uint16_t)half16(s[j >> 1], j & 1);
+        }
+    }
+    __syncthreads();
+    // Left/right consistency, stereo.cpp:450-461 (lround: halves away from zero).
+    for (int x = threadIdx.x; x < W; x += kBlock) {
+        const uint32_t rk = rkey[x];
+        float d = drow[x];
+        uint8_t st = srow[x];
+        if (d > -1.0e8f) {
+            const long xr = lroundf((float)x - d);
+            bool ok = xr >= 0 && xr < W;
+            if (ok) {
+                const uint32_t k2 = rkey[xr];
+                ok = k2 != 0xFFFFFFFFu && !(fabsf((float)(min_d + (int)(k2 & 0xFFFFu)) - d) > (float)c.lr_max_diff);
+            }
+            if (ok) st |= 4;
+            else d = kInvalid;
+        }
+        P.disp[row0 + x] = d;
+        if (P.status) P.status[row0 + x] = st;
+        if (P.rdisp) P.rdisp[row0 + x] = rk == 0xFFFFFFFFu ? INT_MIN : min_d + (int)(rk & 0xFFFFu);
+        if (P.conf) P.conf[row0 + x] = d > -1.0e8f ? 1.0f : 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------------ median
+// median3, stereo.cpp:129-147: the n/2-th smallest (0-based) of the valid in-image 3x3 values.
+
+__global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    const int tx_n = (P.w + kTile - 1) / kTile;
+    const int x = (lb % tx_n) * kTile + (threadIdx.x & (kTile - 1));
+    const int y = (lb / tx_n) * kTile + (threadIdx.x >> 4);
+    if (x >= P.w || y >= P.h) return;
+    const float* in = P.disp;
+    const float v = in[(int64_t)y * P.w + x];
+    float out = kInvalid;
+    if (v > -1.0e8f) {
+        float buf[9];
+        int n = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx, yy = y + dy;
+                float q = kInvalid;
+                if (xx >= 0 && yy >= 0 && xx < P.w && yy < P.h) q = in[(int64_t)yy * P.w + xx];
+                // Invalid or outside: +inf sorts after every valid value and is never selected.
+                const bool ok = q > -1.0e8f;
+                buf[dy * 3 + dx + 4] = ok ? q : __builtin_inff();
+                n += ok ? 1 : 0;
+            }
+        // Odd-even transposition sort (9 rounds sort 9 values): valid values ascending, +inf last.
+#pragma unroll
+        for (int round = 0; round < 9; ++round)
+#pragma unroll
+            for (int i = round & 1; i + 1 < 9; i += 2) {
+                const float lo = fminf(buf[i], buf[i + 1]);
+                const float hi = fmaxf(buf[i], buf[i + 1]);
+                buf[i] = lo;
+                buf[i + 1] = hi;
+            }
+        float sel = buf[0];
+#pragma unroll
+        for (int i = 1; i < 9; ++i) sel = (n / 2) == i ? buf[i] : sel;
+        out = sel;
+    }
+    P.median[(int64_t)y * P.w + x] = out;
+}
+
+// -------------------------------------------------------------------------- launch dispatch
+
+#define SGM_FOR_DPL(X) X(2) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
+
+}  // namespace
+
+int dpl_for(int nd) {
+    if (nd <= 0) return 0;
+    const int need = (nd + kGroup - 1) / kGroup;
+    for (const int d : {2, 4, 6, 8, 12, 16, 24, 32})
+        if (d >= need) return d;
+    return 0;
+}
+
+void fill_lines(PairDesc& d) {
+    static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}, {1, -1}, {-1, 1}};
+    int total = 0;
+    for (int i = 0; i < kMaxDirs; ++i) {
+        d.line_off[i] = total;
+        if (i >= d.ndirs) continue;
+        int n = 0;
+        if (dirs[i][0] != 0) n += d.h;
+        if (dirs[i][1] != 0) n += dirs[i][0] != 0 ? d.w - 1 : d.w;
+        total += n;
+    }
+    d.total_lines = total;
+}
+
+int census_blocks(int w, int h) { return 2 * ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
+int path_blocks(const PairDesc& d) { return (d.total_lines + kLinesPerBlock - 1) / kLinesPerBlock; }
+int row_blocks(const PairDesc& d) { return d.h; }
+int median_blocks(int w, int h) { return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
+
+hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(census_kernel, dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cost_volume_kernel, dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    switch (dpl) {
+#define SGM_CASE(D)                                                                                             \
+    case D:                                                                                                     \
+        if (padded) hipLaunchKernelGGL((paths_kernel<D, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);  \
+        else hipLaunchKernelGGL((paths_kernel<D, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);        \
+        break;
+        SGM_FOR_DPL(SGM_CASE)
+#undef SGM_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    const size_t lds = (size_t)max_w * 9 + 16;
+    switch (dpl) {
+#define SGM_CASE(D)                                                                                                \
+    case D:                                                                                                        \
+        if (padded) hipLaunchKernelGGL((rows_kernel<D, true>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);    \
+        else hipLaunchKernelGGL((rows_kernel<D, false>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);          \
+        break;
+        SGM_FOR_DPL(SGM_CASE)
+#undef SGM_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_median(const LaunchTable& t, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(median_kernel, dim3(t.total_blocks), dim3(kBlock), 0, stream, t);
+    return hipGetLastError();
+}
+
+}  // namespace sgm

@@ -1,0 +1,344 @@
+"""Python binding of the gfx950 SGM library through its C ABI (include/sgm_hip.h).
+
+`HipStereo` mirrors the reference's GPU matcher, depth_metal::MetalStereo
+(libs/depth_metal/include/einstar/depth_metal/metal_stereo.hpp:67-107): ``create`` validates the
+parameters and sizes the device buffers, ``compute`` runs depth::sgm_disparity
+(libs/depth/src/stereo.cpp:361-463) on a rectified pair, ``last_timings`` reports per-stage GPU
+time. Errors raise `SgmError` carrying the einstar::Errc code (libs/core/include/einstar/core/error.hpp:8-19).
+
+There is no CPU fallback: importing works anywhere, but every call needs ``libsgm_hip.so`` and a
+HIP device, and fails loudly without them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libsgm_hip.so")
+
+INVALID_DISPARITY = np.float32(-1.0e9)
+NO_MATCH = np.int32(-(2**31))
+STATUS_WINDOW, STATUS_UNIQUE, STATUS_LR, STATUS_VALID = 1, 2, 4, 7
+
+ERRC = {0: "ok", 1: "not_found", 2: "io", 3: "timeout", 4: "protocol", 5: "invalid_argument", 6: "blocked",
+        7: "unsupported", 8: "busy", 9: "disconnected"}
+
+
+class SgmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERRC.get(code, code)}: {message}")
+        self.code = code
+        self.errc = ERRC.get(code, str(code))
+
+
+class CSgmParams(C.Structure):
+    _fields_ = [
+        ("min_disparity", C.c_int32), ("num_disparities", C.c_int32), ("census_radius_x", C.c_int32),
+        ("census_radius_y", C.c_int32), ("p1", C.c_int32), ("p2", C.c_int32), ("adaptive_p2", C.c_int32),
+        ("eight_paths", C.c_int32), ("slant_steps", C.c_int32), ("uniqueness", C.c_float),
+        ("lr_max_diff", C.c_int32), ("subpixel", C.c_int32),
+    ]
+
+
+OUTPUT_FIELDS = ["disparity", "disparity_median", "confidence", "disparity_int", "status", "right_disparity",
+                 "best_cost", "second_cost", "sum_volume", "cost_volume", "census_left", "census_right"]
+OUTPUT_DTYPES = {"disparity": np.float32, "disparity_median": np.float32, "confidence": np.float32,
+                 "disparity_int": np.int32, "status": np.uint8, "right_disparity": np.int32,
+                 "best_cost": np.uint16, "second_cost": np.uint16, "sum_volume": np.uint16,
+                 "cost_volume": np.uint16, "census_left": np.uint64, "census_right": np.uint64}
+VOLUME_FIELDS = ("sum_volume", "cost_volume")
+
+
+class CSgmOutputs(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in OUTPUT_FIELDS]
+
+
+class CSgmTimings(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms")]
+
+
+class CSgmPair(C.Structure):
+    _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("pitch", C.c_int64), ("width", C.c_int32),
+                ("height", C.c_int32), ("min_disparity", C.c_int32), ("num_disparities", C.c_int32),
+                ("out", CSgmOutputs)]
+
+
+@dataclass
+class SgmParams:
+    """depth::SgmParams (libs/depth/include/einstar/depth/stereo.hpp:26-40) + StereoParams::subpixel."""
+
+    min_disparity: int = 0
+    num_disparities: int = 64
+    census_radius_x: int = 2
+    census_radius_y: int = 2
+    p1: int = 8
+    p2: int = 96
+    adaptive_p2: bool = True
+    eight_paths: bool = False
+    slant_steps: bool = True
+    uniqueness: float = 0.9
+    lr_max_diff: int = 1
+    subpixel: bool = True
+
+    def to_c(self) -> CSgmParams:
+        return CSgmParams(self.min_disparity, self.num_disparities, self.census_radius_x, self.census_radius_y,
+                          self.p1, self.p2, int(self.adaptive_p2), int(self.eight_paths), int(self.slant_steps),
+                          float(self.uniqueness), self.lr_max_diff, int(self.subpixel))
+
+
+_lib = None
+
+
+def lib():
+    """Loads libsgm_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m open_source_einstar_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, P = C.c_void_p, C.c_int, C.POINTER
+        L.sgm_params_default.argtypes = [P(CSgmParams)]
+        L.sgm_create.argtypes = [P(CSgmParams), i32, i32, P(vp)]
+        L.sgm_destroy.argtypes = [vp]
+        L.sgm_last_error.argtypes = [vp]
+        L.sgm_last_error.restype = C.c_char_p
+        L.sgm_compute.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs)]
+        L.sgm_compute_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp]
+        L.sgm_synchronize.argtypes = [vp]
+        L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
+        L.sgm_device_bytes.argtypes = [vp]
+        L.sgm_device_bytes.restype = C.c_size_t
+        L.sgm_dev_malloc.argtypes = [C.c_size_t, P(vp)]
+        L.sgm_dev_free.argtypes = [vp]
+        L.sgm_dev_upload.argtypes = [vp, vp, C.c_size_t]
+        L.sgm_dev_download.argtypes = [vp, vp, C.c_size_t]
+        L.sgm_dev_memset.argtypes = [vp, i32, C.c_size_t]
+        L.sgm_dev_mem_info.argtypes = [P(C.c_size_t), P(C.c_size_t)]
+        L.sgm_batch_create.argtypes = [P(CSgmParams), i32, P(CSgmPair), P(vp)]
+        L.sgm_batch_run.argtypes = [vp, vp]
+        L.sgm_batch_run_plain.argtypes = [vp, vp]
+        L.sgm_batch_synchronize.argtypes = [vp]
+        L.sgm_batch_last_error.argtypes = [vp]
+        L.sgm_batch_last_error.restype = C.c_char_p
+        L.sgm_batch_device_bytes.argtypes = [vp]
+        L.sgm_batch_device_bytes.restype = C.c_size_t
+        L.sgm_batch_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, ctx=None, batch=None):
+    if rc != 0:
+        L = lib()
+        if batch is not None:
+            msg = L.sgm_batch_last_error(batch)
+        else:
+            msg = L.sgm_last_error(ctx)
+        raise SgmError(rc, (msg or b"").decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (sgm_dev_malloc / sgm_dev_free)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().sgm_dev_malloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        _check(lib().sgm_dev_upload(self.ptr + offset, _ptr(a), a.nbytes))
+
+    def download(self, shape, dtype, offset: int = 0) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert offset + out.nbytes <= self.nbytes
+        _check(lib().sgm_dev_download(_ptr(out), self.ptr + offset, out.nbytes))
+        return out
+
+    def memset(self, value: int = 0):
+        _check(lib().sgm_dev_memset(self.ptr, value, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().sgm_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def mem_info():
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(lib().sgm_dev_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+class HipStereo:
+    """gfx950 replacement of depth_metal::MetalStereo's SGM stage (metal_stereo.hpp:67-107)."""
+
+    def __init__(self, handle, params: SgmParams, width: int, height: int):
+        self._h = handle
+        self.params = params
+        self.width, self.height = width, height
+
+    @classmethod
+    def create(cls, params: SgmParams, width: int, height: int) -> "HipStereo":
+        """MetalStereo::create (metal_stereo.hpp:70-71): validate and allocate for width x height."""
+        h = C.c_void_p()
+        cp = params.to_c()
+        _check(lib().sgm_create(C.byref(cp), width, height, C.byref(h)))
+        return cls(h.value, params, width, height)
+
+    def close(self):
+        if self._h:
+            lib().sgm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute(self, left: np.ndarray, right: np.ndarray, outputs=("disparity",)) -> dict:
+        """sgm_disparity on a rectified u8 pair (+ median3 etc. as requested in `outputs`)."""
+        left = np.ascontiguousarray(left, dtype=np.uint8)
+        right = np.ascontiguousarray(right, dtype=np.uint8)
+        if left.shape != right.shape or left.ndim != 2:
+            raise SgmError(5, "left and right must be 2-D images of the same size")
+        h, w = left.shape
+        res = {}
+        co = CSgmOutputs()
+        for f in outputs:
+            shape = (h, w, self.params.num_disparities) if f in VOLUME_FIELDS else (h, w)
+            res[f] = np.empty(shape, OUTPUT_DTYPES[f])
+            setattr(co, f, res[f].ctypes.data)
+        if "disparity" not in res:
+            res["disparity"] = np.empty((h, w), np.float32)
+            co.disparity = res["disparity"].ctypes.data
+        _check(lib().sgm_compute(self._h, _ptr(left), _ptr(right), w, w, h, C.byref(co)), ctx=self._h)
+        return res
+
+    def compute_device(self, d_left: int, d_right: int, pitch: int, width: int, height: int,
+                       d_outputs: dict | None = None, stream: int | None = None):
+        """Device-resident variant (sgm_compute_device); returns without waiting."""
+        co = CSgmOutputs()
+        for f, p in (d_outputs or {}).items():
+            setattr(co, f, p)
+        _check(lib().sgm_compute_device(self._h, d_left, d_right, pitch, width, height, C.byref(co), stream),
+               ctx=self._h)
+
+    def synchronize(self):
+        _check(lib().sgm_synchronize(self._h), ctx=self._h)
+
+    def last_timings(self) -> dict:
+        """MetalStereo::last_timings (metal_stereo.hpp:97)."""
+        t = CSgmTimings()
+        _check(lib().sgm_last_timings(self._h, C.byref(t)), ctx=self._h)
+        return {f: getattr(t, f) for f, _ in CSgmTimings._fields_}
+
+    def device_bytes(self) -> int:
+        return lib().sgm_device_bytes(self._h)
+
+
+@dataclass
+class PairSpec:
+    """One pair of a batch: host images plus its disparity range."""
+
+    left: np.ndarray
+    right: np.ndarray
+    min_disparity: int
+    num_disparities: int
+    outputs: tuple = ("disparity", "disparity_median", "disparity_int", "status", "right_disparity",
+                      "best_cost", "second_cost")
+
+
+class Batch:
+    """A planned batch (sgm_batch_create): device copies of the pairs, outputs, and the graph."""
+
+    def __init__(self, params: SgmParams, pairs: list[PairSpec]):
+        self.params = params
+        self.specs = pairs
+        self._bufs = []
+        cps = (CSgmPair * len(pairs))()
+        self.out_ptrs = []
+        for i, sp in enumerate(pairs):
+            h, w = sp.left.shape
+            img = DeviceBuffer(2 * w * h)
+            img.upload(np.ascontiguousarray(sp.left, np.uint8), 0)
+            img.upload(np.ascontiguousarray(sp.right, np.uint8), w * h)
+            self._bufs.append(img)
+            cps[i].left = img.ptr
+            cps[i].right = img.ptr + w * h
+            cps[i].pitch = w
+            cps[i].width = w
+            cps[i].height = h
+            cps[i].min_disparity = sp.min_disparity
+            cps[i].num_disparities = sp.num_disparities
+            outs = {}
+            for f in set(sp.outputs) | {"disparity"}:
+                shape = (h, w, sp.num_disparities) if f in VOLUME_FIELDS else (h, w)
+                b = DeviceBuffer(int(np.prod(shape)) * np.dtype(OUTPUT_DTYPES[f]).itemsize)
+                self._bufs.append(b)
+                setattr(cps[i].out, f, b.ptr)
+                outs[f] = (b, shape, OUTPUT_DTYPES[f])
+            self.out_ptrs.append(outs)
+        h = C.c_void_p()
+        cp = params.to_c()
+        rc = lib().sgm_batch_create(C.byref(cp), len(pairs), cps, C.byref(h))
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_batch_last_error(None) or b"").decode())
+        self._h = h.value
+
+    def run(self, stream=None):
+        _check(lib().sgm_batch_run(self._h, stream), batch=self._h)
+
+    def run_plain(self, stream=None):
+        _check(lib().sgm_batch_run_plain(self._h, stream), batch=self._h)
+
+    def synchronize(self):
+        _check(lib().sgm_batch_synchronize(self._h), batch=self._h)
+
+    def results(self, i: int) -> dict:
+        return {f: b.download(shape, dt) for f, (b, shape, dt) in self.out_ptrs[i].items()}
+
+    def clear_outputs(self, value: int = 0):
+        for outs in self.out_ptrs:
+            for b, _, _ in outs.values():
+                b.memset(value)
+
+    def device_bytes(self) -> int:
+        return lib().sgm_batch_device_bytes(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sgm_batch_destroy(self._h)
+            self._h = None
+        for b in self._bufs:
+            b.free()
+        self._bufs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

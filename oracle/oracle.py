@@ -1,0 +1,211 @@
+"""ctypes bindings of the CPU oracle (``liboracle_sgm.so``) -- test infrastructure only.
+
+The oracle restates the reference CPU SGM (libs/depth/src/stereo.cpp:15-96, 129-147, 328-346,
+361-463) in C. Only the tests, the smoke check and the CPU-baseline leg of the benchmark use it,
+as the checker; the library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_sgm.so")
+
+INVALID_DISPARITY = np.float32(-1.0e9)
+NO_MATCH = np.int32(-(2**31))
+ST_WINDOW, ST_UNIQUE, ST_LR, ST_VALID = 1, 2, 4, 7
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("min_disparity", C.c_int32),
+        ("num_disparities", C.c_int32),
+        ("census_radius_x", C.c_int32),
+        ("census_radius_y", C.c_int32),
+        ("p1", C.c_int32),
+        ("p2", C.c_int32),
+        ("adaptive_p2", C.c_int32),
+        ("eight_paths", C.c_int32),
+        ("slant_steps", C.c_int32),
+        ("uniqueness", C.c_float),
+        ("lr_max_diff", C.c_int32),
+        ("subpixel", C.c_int32),
+    ]
+
+
+class OrcOutputs(C.Structure):
+    _fields_ = [
+        ("disparity", C.c_void_p),
+        ("disparity_median", C.c_void_p),
+        ("disparity_int", C.c_void_p),
+        ("status", C.c_void_p),
+        ("right_disparity", C.c_void_p),
+        ("best_cost", C.c_void_p),
+        ("second_cost", C.c_void_p),
+        ("sum_volume", C.c_void_p),
+        ("cost_volume", C.c_void_p),
+        ("census_left", C.c_void_p),
+        ("census_right", C.c_void_p),
+    ]
+
+
+@dataclass
+class SgmParams:
+    """depth::SgmParams (libs/depth/include/einstar/depth/stereo.hpp:26-40) + StereoParams::subpixel."""
+
+    min_disparity: int = 0
+    num_disparities: int = 64
+    census_radius_x: int = 2
+    census_radius_y: int = 2
+    p1: int = 8
+    p2: int = 96
+    adaptive_p2: bool = True
+    eight_paths: bool = False
+    slant_steps: bool = True
+    uniqueness: float = 0.9
+    lr_max_diff: int = 1
+    subpixel: bool = True
+
+    def to_c(self) -> OrcParams:
+        return OrcParams(self.min_disparity, self.num_disparities, self.census_radius_x, self.census_radius_y,
+                         self.p1, self.p2, int(self.adaptive_p2), int(self.eight_paths), int(self.slant_steps),
+                         float(self.uniqueness), self.lr_max_diff, int(self.subpixel))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        vp, i32, f32 = C.c_void_p, C.c_int, C.c_float
+        L.orc_census.argtypes = [vp, i32, i32, C.c_ssize_t, i32, i32, vp, i32]
+        L.orc_cost_volume.argtypes = [vp, vp, i32, i32, i32, i32, C.c_uint16, vp, i32]
+        L.orc_aggregate_direction.argtypes = [vp, vp, C.c_ssize_t, i32, i32, i32, C.POINTER(OrcParams), i32, i32, vp, i32]
+        L.orc_aggregate.argtypes = [vp, vp, C.c_ssize_t, i32, i32, C.POINTER(OrcParams), vp, i32]
+        L.orc_subpixel_offset.argtypes = [f32, f32, f32]
+        L.orc_subpixel_offset.restype = f32
+        L.orc_wta.argtypes = [vp, i32, i32, C.POINTER(OrcParams), vp, vp, vp, vp, vp, vp, i32]
+        L.orc_lr_check.argtypes = [vp, vp, i32, i32, i32, vp, i32]
+        L.orc_median3.argtypes = [vp, i32, i32, vp, i32]
+        L.orc_sgm.argtypes = [vp, vp, C.c_ssize_t, i32, i32, C.POINTER(OrcParams), C.POINTER(OrcOutputs), i32]
+        L.orc_sgm.restype = i32
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def census(img: np.ndarray, rx: int, ry: int, threads: int = 0) -> np.ndarray:
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    out = np.empty((h, w), np.uint64)
+    lib().orc_census(_ptr(img), w, h, w, rx, ry, _ptr(out), threads)
+    return out
+
+
+def cost_volume(cl, cr, min_d: int, nd: int, invalid_cost: int, threads: int = 0) -> np.ndarray:
+    cl = np.ascontiguousarray(cl, dtype=np.uint64)
+    cr = np.ascontiguousarray(cr, dtype=np.uint64)
+    h, w = cl.shape
+    out = np.empty((h, w, nd), np.uint16)
+    lib().orc_cost_volume(_ptr(cl), _ptr(cr), w, h, min_d, nd, invalid_cost, _ptr(out), threads)
+    return out
+
+
+def aggregate_direction(cost, guide, p: SgmParams, dx: int, dy: int, sum_in=None, threads: int = 0) -> np.ndarray:
+    cost = np.ascontiguousarray(cost, dtype=np.uint16)
+    guide = np.ascontiguousarray(guide, dtype=np.uint8)
+    h, w, nd = cost.shape
+    s = np.zeros_like(cost) if sum_in is None else np.array(sum_in, dtype=np.uint16, copy=True)
+    cp = p.to_c()
+    lib().orc_aggregate_direction(_ptr(cost), _ptr(guide), w, w, h, nd, C.byref(cp), dx, dy, _ptr(s), threads)
+    return s
+
+
+def aggregate(cost, guide, p: SgmParams, threads: int = 0) -> np.ndarray:
+    cost = np.ascontiguousarray(cost, dtype=np.uint16)
+    guide = np.ascontiguousarray(guide, dtype=np.uint8)
+    h, w, nd = cost.shape
+    assert nd == p.num_disparities
+    s = np.zeros_like(cost)
+    cp = p.to_c()
+    lib().orc_aggregate(_ptr(cost), _ptr(guide), w, w, h, C.byref(cp), _ptr(s), threads)
+    return s
+
+
+def subpixel_offset(cm: float, c0: float, cp: float) -> float:
+    return lib().orc_subpixel_offset(cm, c0, cp)
+
+
+def wta(sum_volume, p: SgmParams, threads: int = 0) -> dict:
+    s = np.ascontiguousarray(sum_volume, dtype=np.uint16)
+    h, w, nd = s.shape
+    assert nd == p.num_disparities
+    o = {
+        "disparity": np.empty((h, w), np.float32),
+        "right_disparity": np.empty((h, w), np.int32),
+        "disparity_int": np.empty((h, w), np.int32),
+        "status": np.empty((h, w), np.uint8),
+        "best_cost": np.empty((h, w), np.uint16),
+        "second_cost": np.empty((h, w), np.uint16),
+    }
+    cp = p.to_c()
+    lib().orc_wta(_ptr(s), w, h, C.byref(cp), _ptr(o["disparity"]), _ptr(o["right_disparity"]),
+                  _ptr(o["disparity_int"]), _ptr(o["status"]), _ptr(o["best_cost"]), _ptr(o["second_cost"]), threads)
+    return o
+
+
+def lr_check(disp, right_disp, lr_max_diff: int, status=None, threads: int = 0):
+    d = np.array(disp, dtype=np.float32, copy=True)
+    r = np.ascontiguousarray(right_disp, dtype=np.int32)
+    st = None if status is None else np.array(status, dtype=np.uint8, copy=True)
+    h, w = d.shape
+    lib().orc_lr_check(_ptr(d), _ptr(r), w, h, lr_max_diff, _ptr(st), threads)
+    return d if st is None else (d, st)
+
+
+def median3(disp, threads: int = 0) -> np.ndarray:
+    d = np.ascontiguousarray(disp, dtype=np.float32)
+    h, w = d.shape
+    out = np.empty_like(d)
+    lib().orc_median3(_ptr(d), w, h, _ptr(out), threads)
+    return out
+
+
+def sgm(left, right, p: SgmParams, volumes: bool = False, threads: int = 0) -> dict:
+    """sgm_disparity + median3 with every intermediate output; volumes=True also returns C and S."""
+    left = np.ascontiguousarray(left, dtype=np.uint8)
+    right = np.ascontiguousarray(right, dtype=np.uint8)
+    assert left.shape == right.shape
+    h, w = left.shape
+    nd = p.num_disparities
+    o = {
+        "disparity": np.empty((h, w), np.float32),
+        "disparity_median": np.empty((h, w), np.float32),
+        "disparity_int": np.empty((h, w), np.int32),
+        "status": np.empty((h, w), np.uint8),
+        "right_disparity": np.empty((h, w), np.int32),
+        "best_cost": np.empty((h, w), np.uint16),
+        "second_cost": np.empty((h, w), np.uint16),
+        "census_left": np.empty((h, w), np.uint64),
+        "census_right": np.empty((h, w), np.uint64),
+    }
+    if volumes:
+        o["sum_volume"] = np.empty((h, w, nd), np.uint16)
+        o["cost_volume"] = np.empty((h, w, nd), np.uint16)
+    co = OrcOutputs(*[_ptr(o.get(f[0])) for f in OrcOutputs._fields_])
+    cp = p.to_c()
+    rc = lib().orc_sgm(_ptr(left), _ptr(right), w, w, h, C.byref(cp), C.byref(co), threads)
+    if rc != 0:
+        raise MemoryError("oracle sgm: allocation failed")
+    return o

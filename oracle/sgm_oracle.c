@@ -1,0 +1,434 @@
+/*
+ * CPU oracle for the SGM stereo hot path -- TEST INFRASTRUCTURE ONLY (see sgm_oracle.h).
+ * Restates libs/depth/src/stereo.cpp (SGM subset) in C; each function cites the lines it follows.
+ * Threads only split independent rows or scanlines, as tbb::parallel_for does in the reference
+ * (stereo.cpp:18-23, 403-408), so results never depend on the thread count.
+ */
+#include "sgm_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+typedef uint16_t cost_t;
+#define MAX_COST ((cost_t)0xFFFF) /* kMaxCost, stereo.cpp:16 */
+
+/* ---------------------------------------------------------------- parallel_for */
+
+typedef void (*range_fn)(void* ctx, int begin, int end);
+
+typedef struct {
+    range_fn fn;
+    void* ctx;
+    int n;
+    int chunk;
+    int next; /* claimed with __atomic_fetch_add */
+} pf_job;
+
+static void* pf_worker(void* arg) {
+    pf_job* job = (pf_job*)arg;
+    for (;;) {
+        const int b = __atomic_fetch_add(&job->next, job->chunk, __ATOMIC_RELAXED);
+        if (b >= job->n) break;
+        const int e = b + job->chunk < job->n ? b + job->chunk : job->n;
+        job->fn(job->ctx, b, e);
+    }
+    return NULL;
+}
+
+static int resolve_threads(int threads) {
+    if (threads > 0) return threads;
+    const long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (int)n : 1;
+}
+
+static void parallel_for(int n, int threads, range_fn fn, void* ctx) {
+    if (n <= 0) return;
+    threads = resolve_threads(threads);
+    if (threads > n) threads = n;
+    if (threads <= 1) {
+        fn(ctx, 0, n);
+        return;
+    }
+    pf_job job = {fn, ctx, n, 0, 0};
+    job.chunk = n / (threads * 8);
+    if (job.chunk < 1) job.chunk = 1;
+    pthread_t* tid = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)(threads - 1));
+    int started = 0;
+    for (int i = 0; tid && i < threads - 1; ++i)
+        if (pthread_create(&tid[i], NULL, pf_worker, &job) == 0) ++started;
+    pf_worker(&job);
+    for (int i = 0; i < started; ++i) pthread_join(tid[i], NULL);
+    free(tid);
+}
+
+/* ---------------------------------------------------------------- census (stereo.cpp:328-346) */
+
+typedef struct {
+    const uint8_t* img;
+    int w, h, rx, ry;
+    ptrdiff_t stride;
+    uint64_t* out;
+} census_ctx;
+
+static void census_rows(void* vctx, int y0, int y1) {
+    const census_ctx* c = (const census_ctx*)vctx;
+    for (int y = y0; y < y1; ++y) {
+        uint64_t* orow = c->out + (size_t)y * (size_t)c->w;
+        memset(orow, 0, sizeof(uint64_t) * (size_t)c->w); /* border pixels stay 0 */
+        if (y < c->ry || y >= c->h - c->ry) continue;
+        for (int x = c->rx; x < c->w - c->rx; ++x) {
+            const uint8_t center = c->img[(ptrdiff_t)y * c->stride + x];
+            uint64_t bits = 0;
+            for (int v = -c->ry; v <= c->ry; ++v) { /* rows outer, stereo.cpp:335 */
+                const uint8_t* row = c->img + (ptrdiff_t)(y + v) * c->stride;
+                for (int u = -c->rx; u <= c->rx; ++u) { /* columns inner, :337 */
+                    if (u == 0 && v == 0) continue;
+                    bits = (bits << 1) | (uint64_t)(row[x + u] < center);
+                }
+            }
+            orow[x] = bits;
+        }
+    }
+}
+
+void orc_census(const uint8_t* img, int w, int h, ptrdiff_t stride, int rx, int ry, uint64_t* out, int threads) {
+    census_ctx c = {img, w, h, rx, ry, stride, out};
+    parallel_for(h, threads, census_rows, &c);
+}
+
+/* ---------------------------------------------------------------- cost fill (stereo.cpp:366-380) */
+
+typedef struct {
+    const uint64_t *cl, *cr;
+    int w, h, min_d, nd;
+    cost_t invalid;
+    cost_t* cost;
+} cost_ctx;
+
+static void cost_rows(void* vctx, int y0, int y1) {
+    const cost_ctx* c = (const cost_ctx*)vctx;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < c->w; ++x) {
+            cost_t* cv = c->cost + ((size_t)y * (size_t)c->w + (size_t)x) * (size_t)c->nd;
+            for (int k = 0; k < c->nd; ++k) cv[k] = c->invalid; /* CostVolume fill, :368 */
+            const uint64_t a = c->cl[(size_t)y * (size_t)c->w + (size_t)x];
+            for (int k = 0; k < c->nd; ++k) {
+                const int xr = x - (c->min_d + k);
+                if (xr < 0) break; /* the rest stay invalid, :375 */
+                if (xr >= c->w) continue;
+                cv[k] = (cost_t)__builtin_popcountll(a ^ c->cr[(size_t)y * (size_t)c->w + (size_t)xr]);
+            }
+        }
+    }
+}
+
+void orc_cost_volume(const uint64_t* cl, const uint64_t* cr, int w, int h, int min_d, int nd, uint16_t invalid_cost,
+                     uint16_t* cost, int threads) {
+    cost_ctx c = {cl, cr, w, h, min_d, nd, invalid_cost, cost};
+    parallel_for(h, threads, cost_rows, &c);
+}
+
+/* ---------------------------------------------------------------- aggregation (stereo.cpp:44-89, 385-409) */
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+/* aggregate_path, stereo.cpp:44-89: walks from (x0, y0) in steps (dx, dy) until it leaves the image. */
+static void aggregate_path(const cost_t* cost, const uint8_t* guide, ptrdiff_t gstride, int w, int h, int nd,
+                           const orc_params* p, cost_t* sum, int x0, int y0, int dx, int dy, cost_t* prev, cost_t* cur) {
+    int x = x0, y = y0;
+    const cost_t* c = cost + ((size_t)y * (size_t)w + (size_t)x) * (size_t)nd;
+    cost_t prev_min = MAX_COST;
+    for (int k = 0; k < nd; ++k) {
+        prev[k] = c[k];
+        if (c[k] < prev_min) prev_min = c[k];
+    }
+    {
+        cost_t* s = sum + ((size_t)y * (size_t)w + (size_t)x) * (size_t)nd;
+        for (int k = 0; k < nd; ++k) s[k] = (cost_t)(s[k] + prev[k]);
+    }
+    int prev_intensity = guide[(ptrdiff_t)y * gstride + x];
+    x += dx;
+    y += dy;
+    while (x >= 0 && y >= 0 && x < w && y < h) {
+        const int intensity = guide[(ptrdiff_t)y * gstride + x];
+        int p2 = p->p2;
+        if (p->adaptive_p2) p2 = imax(p->p1 + 1, p->p2 / (abs(intensity - prev_intensity) / 8 + 1)); /* :64 */
+        prev_intensity = intensity;
+        c = cost + ((size_t)y * (size_t)w + (size_t)x) * (size_t)nd;
+        const int jump = prev_min + p2;
+        cost_t cur_min = MAX_COST;
+        for (int k = 0; k < nd; ++k) {
+            int best = imin(prev[k], jump);
+            if (k > 0) best = imin(best, prev[k - 1] + p->p1);
+            if (k + 1 < nd) best = imin(best, prev[k + 1] + p->p1);
+            if (p->slant_steps) {
+                if (k > 1) best = imin(best, prev[k - 2] + p->p1);
+                if (k + 2 < nd) best = imin(best, prev[k + 2] + p->p1);
+            }
+            const cost_t v = (cost_t)(c[k] + best - prev_min);
+            cur[k] = v;
+            if (v < cur_min) cur_min = v;
+        }
+        cost_t* s = sum + ((size_t)y * (size_t)w + (size_t)x) * (size_t)nd;
+        for (int k = 0; k < nd; ++k) s[k] = (cost_t)(s[k] + cur[k]);
+        cost_t* t = prev;
+        prev = cur;
+        cur = t;
+        prev_min = cur_min;
+        x += dx;
+        y += dy;
+    }
+}
+
+typedef struct {
+    const cost_t* cost;
+    const uint8_t* guide;
+    ptrdiff_t gstride;
+    int w, h, nd, dx, dy;
+    const orc_params* p;
+    cost_t* sum;
+} dir_ctx;
+
+/* Start pixel i of a direction, in the order of stereo.cpp:391-402: the x-entry column for every y,
+ * then the y-entry row for every x except the corner already taken. */
+static void start_pixel(const dir_ctx* c, int i, int* sx, int* sy) {
+    const int ncol = c->dx != 0 ? c->h : 0;
+    if (i < ncol) {
+        *sx = c->dx > 0 ? 0 : c->w - 1;
+        *sy = i;
+        return;
+    }
+    int j = i - ncol;
+    const int skip = c->dx != 0 ? (c->dx > 0 ? 0 : c->w - 1) : -1;
+    if (skip >= 0 && j >= skip) ++j;
+    *sx = j;
+    *sy = c->dy > 0 ? 0 : c->h - 1;
+}
+
+static int num_starts(int w, int h, int dx, int dy) {
+    int n = 0;
+    if (dx != 0) n += h;
+    if (dy != 0) n += dx != 0 ? w - 1 : w;
+    return n;
+}
+
+static void dir_lines(void* vctx, int i0, int i1) {
+    const dir_ctx* c = (const dir_ctx*)vctx;
+    cost_t* prev = (cost_t*)malloc(sizeof(cost_t) * (size_t)c->nd * 2);
+    if (!prev) abort();
+    cost_t* cur = prev + c->nd;
+    for (int i = i0; i < i1; ++i) {
+        int sx, sy;
+        start_pixel(c, i, &sx, &sy);
+        aggregate_path(c->cost, c->guide, c->gstride, c->w, c->h, c->nd, c->p, c->sum, sx, sy, c->dx, c->dy, prev, cur);
+    }
+    free(prev);
+}
+
+void orc_aggregate_direction(const uint16_t* cost, const uint8_t* guide, ptrdiff_t guide_stride, int w, int h, int nd,
+                             const orc_params* p, int dx, int dy, uint16_t* sum, int threads) {
+    dir_ctx c = {cost, guide, guide_stride, w, h, nd, dx, dy, p, sum};
+    parallel_for(num_starts(w, h, dx, dy), threads, dir_lines, &c);
+}
+
+void orc_aggregate(const uint16_t* cost, const uint8_t* guide, ptrdiff_t guide_stride, int w, int h, const orc_params* p,
+                   uint16_t* sum, int threads) {
+    static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}, {1, -1}, {-1, 1}}; /* :387-388 */
+    const int n = p->eight_paths ? 8 : 4;
+    for (int i = 0; i < n; ++i)
+        orc_aggregate_direction(cost, guide, guide_stride, w, h, p->num_disparities, p, dirs[i][0], dirs[i][1], sum, threads);
+}
+
+/* ---------------------------------------------------------------- WTA (stereo.cpp:92-96, 411-448) */
+
+float orc_subpixel_offset(float cm, float c0, float cp) {
+    const float denom = cm - 2.0f * c0 + cp;
+    if (denom <= 0.0f) return 0.0f;
+    const float v = 0.5f * (cm - cp) / denom;
+    return v < -0.5f ? -0.5f : (0.5f < v ? 0.5f : v); /* std::clamp(v, -0.5f, 0.5f) */
+}
+
+typedef struct {
+    const cost_t* sum;
+    int w, h;
+    const orc_params* p;
+    float* disp;
+    int32_t *right, *dint;
+    uint8_t* status;
+    cost_t *best_cost, *second_cost;
+} wta_ctx;
+
+static void wta_rows(void* vctx, int y0, int y1) {
+    const wta_ctx* c = (const wta_ctx*)vctx;
+    const int w = c->w, nd = c->p->num_disparities, min_d = c->p->min_disparity;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * (size_t)w + (size_t)x;
+            const cost_t* s = c->sum + i * (size_t)nd;
+            c->disp[i] = ORC_INVALID_DISPARITY;
+            if (c->dint) c->dint[i] = ORC_NO_MATCH;
+            if (c->status) c->status[i] = 0;
+            if (c->best_cost) c->best_cost[i] = MAX_COST;
+            if (c->second_cost) c->second_cost[i] = MAX_COST;
+            const int min_k = imax(0, x - min_d - (w - 1)); /* :419 */
+            const int max_k = imin(nd, x - min_d + 1);      /* :420 */
+            if (max_k <= min_k) continue;
+            int best_k = min_k;
+            for (int k = min_k + 1; k < max_k; ++k)
+                if (s[k] < s[best_k]) best_k = k;
+            cost_t second = MAX_COST;
+            for (int k = min_k; k < max_k; ++k)
+                if (abs(k - best_k) > 1 && s[k] < second) second = s[k];
+            if (c->dint) c->dint[i] = min_d + best_k;
+            if (c->best_cost) c->best_cost[i] = s[best_k];
+            if (c->second_cost) c->second_cost[i] = second;
+            uint8_t st = ORC_ST_WINDOW;
+            if (second != MAX_COST && (float)s[best_k] >= c->p->uniqueness * (float)second) { /* :428 */
+                if (c->status) c->status[i] = st;
+                continue;
+            }
+            st |= ORC_ST_UNIQUE;
+            if (c->status) c->status[i] = st;
+            float d = (float)(min_d + best_k);
+            if (c->p->subpixel && best_k > min_k && best_k + 1 < max_k)
+                d += orc_subpixel_offset((float)s[best_k - 1], (float)s[best_k], (float)s[best_k + 1]);
+            c->disp[i] = d;
+        }
+        for (int xr = 0; xr < w; ++xr) { /* right WTA, :434-447 */
+            int best_k = -1;
+            cost_t best = MAX_COST;
+            for (int k = imax(0, -(xr + min_d)); k < nd; ++k) {
+                const int xl = xr + min_d + k;
+                if (xl >= w) break;
+                const cost_t v = c->sum[((size_t)y * (size_t)w + (size_t)xl) * (size_t)nd + (size_t)k];
+                if (v < best) {
+                    best = v;
+                    best_k = k;
+                }
+            }
+            c->right[(size_t)y * (size_t)w + (size_t)xr] = best_k < 0 ? ORC_NO_MATCH : min_d + best_k;
+        }
+    }
+}
+
+void orc_wta(const uint16_t* sum, int w, int h, const orc_params* p, float* disp, int32_t* right_disp,
+             int32_t* disp_int, uint8_t* status, uint16_t* best_cost, uint16_t* second_cost, int threads) {
+    wta_ctx c = {sum, w, h, p, disp, right_disp, disp_int, status, best_cost, second_cost};
+    parallel_for(h, threads, wta_rows, &c);
+}
+
+/* ---------------------------------------------------------------- left/right check (stereo.cpp:450-461) */
+
+typedef struct {
+    float* disp;
+    const int32_t* right;
+    int w, lr;
+    uint8_t* status;
+} lr_ctx;
+
+static void lr_rows(void* vctx, int y0, int y1) {
+    const lr_ctx* c = (const lr_ctx*)vctx;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < c->w; ++x) {
+            const size_t i = (size_t)y * (size_t)c->w + (size_t)x;
+            const float d = c->disp[i];
+            if (!(d > -1.0e8f)) continue; /* valid_disparity, stereo.hpp:24 */
+            const int xr = (int)lroundf((float)x - d);
+            if (xr < 0 || xr >= c->w || c->right[(size_t)y * (size_t)c->w + (size_t)xr] == ORC_NO_MATCH ||
+                fabsf((float)c->right[(size_t)y * (size_t)c->w + (size_t)xr] - d) > (float)c->lr) {
+                c->disp[i] = ORC_INVALID_DISPARITY;
+                continue;
+            }
+            if (c->status) c->status[i] |= ORC_ST_LR;
+        }
+    }
+}
+
+void orc_lr_check(float* disp, const int32_t* right_disp, int w, int h, int lr_max_diff, uint8_t* status, int threads) {
+    lr_ctx c = {disp, right_disp, w, lr_max_diff, status};
+    parallel_for(h, threads, lr_rows, &c);
+}
+
+/* ---------------------------------------------------------------- median3 (stereo.cpp:129-147) */
+
+typedef struct {
+    const float* in;
+    float* out;
+    int w, h;
+} med_ctx;
+
+static void med_rows(void* vctx, int y0, int y1) {
+    const med_ctx* c = (const med_ctx*)vctx;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < c->w; ++x) {
+            const size_t i = (size_t)y * (size_t)c->w + (size_t)x;
+            c->out[i] = ORC_INVALID_DISPARITY;
+            if (!(c->in[i] > -1.0e8f)) continue;
+            float buf[9];
+            int n = 0;
+            for (int v = -1; v <= 1; ++v)
+                for (int u = -1; u <= 1; ++u) {
+                    const int xx = x + u, yy = y + v;
+                    if (xx < 0 || yy < 0 || xx >= c->w || yy >= c->h) continue;
+                    const float q = c->in[(size_t)yy * (size_t)c->w + (size_t)xx];
+                    if (!(q > -1.0e8f)) continue;
+                    buf[n++] = q;
+                }
+            /* std::nth_element(buf, buf + n/2, buf + n) leaves the n/2-th smallest at n/2: sort and take it. */
+            for (int a = 1; a < n; ++a) {
+                const float t = buf[a];
+                int b = a - 1;
+                while (b >= 0 && buf[b] > t) {
+                    buf[b + 1] = buf[b];
+                    --b;
+                }
+                buf[b + 1] = t;
+            }
+            c->out[i] = buf[n / 2];
+        }
+    }
+}
+
+void orc_median3(const float* in, int w, int h, float* out, int threads) {
+    med_ctx c = {in, out, w, h};
+    parallel_for(h, threads, med_rows, &c);
+}
+
+/* ---------------------------------------------------------------- sgm_disparity + median3 (stereo.cpp:361-463) */
+
+int orc_sgm(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const orc_params* p,
+            const orc_outputs* out, int threads) {
+    const size_t npx = (size_t)w * (size_t)h;
+    const size_t nvol = npx * (size_t)p->num_disparities;
+    uint64_t* cl = out->census_left ? out->census_left : (uint64_t*)malloc(sizeof(uint64_t) * (npx ? npx : 1));
+    uint64_t* cr = out->census_right ? out->census_right : (uint64_t*)malloc(sizeof(uint64_t) * (npx ? npx : 1));
+    cost_t* cost = out->cost_volume ? out->cost_volume : (cost_t*)malloc(sizeof(cost_t) * (nvol ? nvol : 1));
+    cost_t* sum = out->sum_volume ? out->sum_volume : (cost_t*)malloc(sizeof(cost_t) * (nvol ? nvol : 1));
+    int32_t* rdisp = out->right_disparity ? out->right_disparity : (int32_t*)malloc(sizeof(int32_t) * (npx ? npx : 1));
+    int rc = 0;
+    if (!cl || !cr || !cost || !sum || !rdisp) {
+        rc = -1;
+        goto done;
+    }
+    orc_census(left, w, h, stride, p->census_radius_x, p->census_radius_y, cl, threads);
+    orc_census(right, w, h, stride, p->census_radius_x, p->census_radius_y, cr, threads);
+    const cost_t invalid_cost = (cost_t)((2 * p->census_radius_x + 1) * (2 * p->census_radius_y + 1)); /* :366 */
+    orc_cost_volume(cl, cr, w, h, p->min_disparity, p->num_disparities, invalid_cost, cost, threads);
+    memset(sum, 0, sizeof(cost_t) * nvol);
+    orc_aggregate(cost, left, stride, w, h, p, sum, threads);
+    orc_wta(sum, w, h, p, out->disparity, rdisp, out->disparity_int, out->status, out->best_cost, out->second_cost, threads);
+    orc_lr_check(out->disparity, rdisp, w, h, p->lr_max_diff, out->status, threads);
+    if (out->disparity_median) orc_median3(out->disparity, w, h, out->disparity_median, threads);
+done:
+    if (cl != out->census_left) free(cl);
+    if (cr != out->census_right) free(cr);
+    if (cost != out->cost_volume) free(cost);
+    if (sum != out->sum_volume) free(sum);
+    if (rdisp != out->right_disparity) free(rdisp);
+    return rc;
+}

@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every function include/*.h declares; host-side argument
+checks answer with einstar::Errc codes without touching a GPU."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+from open_source_einstar_amd import sgm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(sgm_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for n in ("sgm_create", "sgm_compute", "sgm_compute_device", "sgm_destroy", "sgm_last_error",
+              "sgm_last_timings", "sgm_batch_create", "sgm_batch_run"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = sgm.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_defaults_match_reference_sgm_params():
+    p = sgm.CSgmParams()
+    sgm.lib().sgm_params_default(C.byref(p))
+    ref = sgm.SgmParams()  # stereo.hpp:26-40
+    for f, _ in sgm.CSgmParams._fields_:
+        assert getattr(p, f) == pytest.approx(float(getattr(ref, f)))
+
+
+@pytest.mark.parametrize("bad", [
+    dict(num_disparities=0), dict(num_disparities=513), dict(census_radius_x=5), dict(census_radius_y=4),
+    dict(p1=-1), dict(p2=-1), dict(census_radius_x=4, census_radius_y=3, p2=200),
+    dict(census_radius_x=4, census_radius_y=3, p1=200, p2=10),
+])
+def test_invalid_parameters_are_rejected_before_any_gpu_work(bad):
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.HipStereo.create(sgm.SgmParams(**bad), 64, 48)
+    assert e.value.errc == "invalid_argument"
+
+
+def test_invalid_sizes_are_rejected():
+    for (w, h) in [(0, 10), (10, 0), (5000, 10)]:
+        with pytest.raises(sgm.SgmError) as e:
+            sgm.HipStereo.create(sgm.SgmParams(), w, h)
+        assert e.value.errc == "invalid_argument"

@@ -1,0 +1,149 @@
+"""GPU parity: the gfx950 library against the CPU oracle on the same seeded speckle pairs.
+
+Bar: integer outputs (census words, costs, sums, integer disparities, status masks, right
+disparities, best/second costs) bit-exact; float outputs (subpixel disparity, median) compared
+bit for bit as well, which is stricter than the 1e-4 px tolerance the path allows for them.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from open_source_einstar_amd import sgm
+from open_source_einstar_amd.synth import speckle_pair
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["disparity", "disparity_median", "disparity_int", "status", "right_disparity", "best_cost", "second_cost"]
+VOLUMES = ["sum_volume", "cost_volume", "census_left", "census_right"]
+SUBPIXEL_TOL = 1e-4  # px, the tolerance stated for the float outputs
+
+
+def assert_same(ref, got, fields, tag=""):
+    for f in fields:
+        a, b = np.asarray(ref[f]), np.asarray(got[f])
+        assert a.shape == b.shape, (tag, f, a.shape, b.shape)
+        if a.dtype == np.float32:
+            # Bit-exact is expected; the assertion message reports the worst difference.
+            if not np.array_equal(a.view(np.uint32), b.view(np.uint32)):
+                va, vb = a > -1e8, b > -1e8
+                both = va & vb
+                worst = float(np.abs(a[both] - b[both]).max()) if both.any() else 0.0
+                assert (va == vb).all() and worst <= SUBPIXEL_TOL, (tag, f, int((va != vb).sum()), worst)
+        else:
+            if not np.array_equal(a, b):
+                idx = np.argwhere(a != b)
+                raise AssertionError(f"{tag} {f}: {len(idx)} mismatches, first at {idx[0].tolist()}: "
+                                     f"ref {a[tuple(idx[0])]} gpu {b[tuple(idx[0])]}")
+
+
+def run_case(w, h, nd, seed, volumes=False, **kw):
+    pair = speckle_pair(w, h, max(nd, 6), seed) if nd >= 6 else speckle_pair(w, h, 6, seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd)
+    params.update(kw)
+    print(f"case {w}x{h} D={nd} seed={seed} {params}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=volumes)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        got = st.compute(pair.left, pair.right, outputs=FIELDS + (VOLUMES if volumes else []))
+    assert_same(ref, got, FIELDS + (VOLUMES if volumes else []), tag=f"{w}x{h}x{nd}")
+    return ref, got
+
+
+SMALL = [
+    # (w, h, nd, seed, params)
+    (64, 48, 16, 1, dict()),
+    (64, 48, 16, 2, dict(eight_paths=True)),
+    (64, 48, 24, 3, dict(eight_paths=True, census_radius_x=4, census_radius_y=3, p1=10, p2=120)),
+    (96, 64, 32, 4, dict(adaptive_p2=False, slant_steps=False)),
+    (96, 64, 32, 5, dict(adaptive_p2=False, p1=12, p2=7, eight_paths=True)),
+    (80, 40, 7, 6, dict(census_radius_x=1, census_radius_y=1)),
+    (70, 50, 64, 7, dict(eight_paths=True, census_radius_x=3, census_radius_y=1, p1=5, p2=60)),
+    (40, 30, 16, 8, dict(min_disparity=-6)),
+    (40, 30, 16, 9, dict(min_disparity=-20, eight_paths=True)),
+    (20, 12, 64, 10, dict(eight_paths=True)),
+    (33, 17, 40, 11, dict(eight_paths=True, uniqueness=0.7, lr_max_diff=0)),
+    (50, 45, 20, 12, dict(subpixel=False, eight_paths=True)),
+    (48, 36, 16, 13, dict(census_radius_x=0, census_radius_y=0)),
+    (64, 48, 16, 14, dict(p1=0, p2=0, adaptive_p2=False, eight_paths=True)),
+]
+
+
+@pytest.mark.parametrize("w,h,nd,seed,kw", SMALL, ids=[f"{c[0]}x{c[1]}x{c[2]}-s{c[3]}" for c in SMALL])
+def test_small_pairs_bit_exact_with_volumes(w, h, nd, seed, kw):
+    run_case(w, h, nd, seed, volumes=True, **kw)
+
+
+EDGES = [(1, 1, 8), (1, 40, 16), (40, 1, 16), (2, 2, 4), (5, 3, 32)]
+
+
+@pytest.mark.parametrize("w,h,nd", EDGES, ids=[f"{w}x{h}x{nd}" for w, h, nd in EDGES])
+def test_degenerate_sizes(w, h, nd):
+    rng = np.random.default_rng(w * 100 + h)
+    left = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    right = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    params = dict(num_disparities=nd, eight_paths=True, census_radius_x=1, census_radius_y=1)
+    print(f"degenerate {w}x{h} D={nd}", flush=True)
+    ref = O.sgm(left, right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        got = st.compute(left, right, outputs=FIELDS + VOLUMES)
+    assert_same(ref, got, FIELDS + VOLUMES, tag=f"{w}x{h}")
+
+
+LANES = [96, 128, 192, 256, 384, 512]
+
+
+@pytest.mark.parametrize("nd", LANES)
+def test_every_lane_layout(nd):
+    """One case per disparities-per-lane instantiation above 64 (6, 8, 12, 16, 24, 32)."""
+    run_case(nd + 40, 24, nd, 20 + nd, volumes=True, eight_paths=True, census_radius_x=4, census_radius_y=3,
+             p1=10, p2=120)
+
+
+def test_config1_640x400x64_four_paths():
+    run_case(640, 400, 64, 1, census_radius_x=4, census_radius_y=3, p1=10, p2=120)
+
+
+def test_config2_1280x1024x128_eight_paths():
+    run_case(1280, 1024, 128, 2, census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=True)
+
+
+def test_repeated_frames_and_context_reuse_leak_no_state():
+    """test_metal_stereo.cpp:152-158: repeated frames give identical disparities; here also one
+    context reused for pairs of different sizes up to its maximum (the generator's disparity range
+    depends only on D, so min_disparity is shared)."""
+    base = dict(num_disparities=32, eight_paths=True, census_radius_x=4, census_radius_y=3, p1=10, p2=120)
+    min_d = speckle_pair(16, 16, 32, 0).min_disparity
+    params = dict(base, min_disparity=min_d)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), 160, 120) as st:
+        for (w, h, seed) in [(160, 120, 30), (64, 48, 31), (160, 120, 30), (97, 33, 32)]:
+            pair = speckle_pair(w, h, 32, seed)
+            assert pair.min_disparity == min_d
+            a = st.compute(pair.left, pair.right, outputs=FIELDS)
+            b = st.compute(pair.left, pair.right, outputs=FIELDS)
+            ref = O.sgm(pair.left, pair.right, O.SgmParams(**params))
+            assert_same(ref, a, FIELDS, tag=f"reuse {w}x{h}")
+            assert_same(a, b, FIELDS, tag=f"repeat {w}x{h}")
+
+
+def test_device_entry_point_matches_host_entry_point():
+    w, h, nd = 200, 150, 48
+    pair = speckle_pair(w, h, nd, 40)
+    params = sgm.SgmParams(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=True)
+    with sgm.HipStereo.create(params, w, h) as st:
+        host = st.compute(pair.left, pair.right, outputs=FIELDS)
+        img = sgm.DeviceBuffer(2 * w * h)
+        img.upload(pair.left, 0)
+        img.upload(pair.right, w * h)
+        outs = {f: sgm.DeviceBuffer(w * h * np.dtype(sgm.OUTPUT_DTYPES[f]).itemsize) for f in FIELDS}
+        st.compute_device(img.ptr, img.ptr + w * h, w, w, h, {f: b.ptr for f, b in outs.items()})
+        st.synchronize()
+        dev = {f: b.download((h, w), sgm.OUTPUT_DTYPES[f]) for f, b in outs.items()}
+        t = st.last_timings()
+    assert_same(host, dev, FIELDS, tag="device")
+    assert t["gpu_ms"] > 0 and t["paths_ms"] > 0
+
+
+def test_size_errors_on_device():
+    with sgm.HipStereo.create(sgm.SgmParams(num_disparities=16), 32, 32) as st:
+        with pytest.raises(sgm.SgmError) as e:
+            st.compute(np.zeros((40, 32), np.uint8), np.zeros((40, 32), np.uint8))
+        assert e.value.errc == "invalid_argument"
