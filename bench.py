@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 SGM stereo path on one MI355X.
+
+Workload (one step): one rectified 1280x1024 u8 speckle pair, 128 disparities, 8 paths, census 9x7,
+P1 = 10, P2 = 120 (adaptive), slant steps, uniqueness 0.9, L-R 1 px, fp32 subpixel, 3x3 median.
+Inputs and outputs stay resident in HBM; the timed region is K back-to-back sgm_compute_device
+calls on one stream between two synchronisations.
+
+Printed as one JSON line:
+  value            Mpix*disp/s of the timed steps (W*H*D per step / time per step)
+  roofline         the aggregation kernel against HBM: the canonical per-direction dataflow's bytes
+                   for 8 paths (38 B per pixel*disparity + 8 B per pixel) over the kernel's average
+                   duration from HIP events on its stream; traffic = measured HBM bytes per launch
+                   from the committed rocprofv3 counter summary, or null
+  cpu_baseline     the CPU oracle on the host cores, same pair, median of runs
+  latency_ms       p50/p99 of single synchronised steps (launch to host-visible completion)
+  batched          32 uneven pairs (sizes 320-1280 x 240-1024, D in {64,96,128,192}) in one graph
+                   launch vs the same pairs launched pair by pair
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W, H, D = 1280, 1024, 128
+PARAMS = dict(census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=True, adaptive_p2=True,
+              slant_steps=True, uniqueness=0.9, lr_max_diff=1, subpixel=True)
+OUT_FIELDS = ["disparity", "disparity_median", "disparity_int", "status", "right_disparity", "best_cost",
+              "second_cost"]
+HBM_PEAK_GBS = 8000.0
+AGG_BYTES_PER_PXD = 38   # canonical aggregation dataflow, 8 paths: W*H*D*(5P-2) + W*H*P
+AGG_BYTES_PER_PX = 8
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def batch_sizes(n=32, seed=3):
+    rng = np.random.default_rng(seed)
+    ws = rng.integers(320, 1281, n)
+    hs = rng.integers(240, 1025, n)
+    ds = rng.choice([64, 96, 128, 192], n)
+    return [(int(w), int(h), int(d)) for w, h, d in zip(ws, hs, ds)]
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the path kernel at this workload, from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "paths_kernel_1280x1024x128_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
+    ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the path shards by pair with no exchange between GPUs, and this "
+              f"build runs on single-GPU boxes only (asked for --gpus {args.gpus}).", file=sys.stderr)
+        return 2
+
+    from open_source_einstar_amd import sgm
+    from open_source_einstar_amd.synth import speckle_pair
+
+    pair = speckle_pair(W, H, D, seed=2)
+    params = sgm.SgmParams(min_disparity=pair.min_disparity, num_disparities=D, **PARAMS)
+    st = sgm.HipStereo.create(params, W, H)
+    img = sgm.DeviceBuffer(2 * W * H)
+    img.upload(pair.left, 0)
+    img.upload(pair.right, W * H)
+    outs = {f: sgm.DeviceBuffer(W * H * np.dtype(sgm.OUTPUT_DTYPES[f]).itemsize) for f in OUT_FIELDS}
+    d_out = {f: b.ptr for f, b in outs.items()}
+
+    def step():
+        st.compute_device(img.ptr, img.ptr + W * H, W, W, H, d_out)
+
+    for _ in range(args.warmup):
+        step()
+    st.synchronize()
+    st.reset_stage_totals()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    st.synchronize()
+    elapsed = time.perf_counter() - t0
+    totals = st.stage_totals()
+    ms_per_step = elapsed * 1e3 / args.steps
+    pxd = W * H * D
+    value = pxd / (ms_per_step * 1e-3) / 1e6
+    paths_ms = totals["paths_ms"] / max(1, totals["count"])
+    stage_ms = {k: totals[k] / max(1, totals["count"]) for k in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")}
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.0f} Mpix*disp/s, stages {stage_ms}")
+
+    # Single synchronised steps: latency from launch to host-visible completion.
+    lat = []
+    for _ in range(args.latency_steps):
+        t = time.perf_counter()
+        step()
+        st.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+
+    bit_exact = None
+    cpu = None
+    if not args.no_verify or not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        op = O.SgmParams(min_disparity=pair.min_disparity, num_disparities=D, **PARAMS)
+        threads = min(len(os.sched_getaffinity(0)), 64)
+        runs = []
+        ref = None
+        for i in range(1 if args.no_cpu else 4):
+            t = time.perf_counter()
+            ref = O.sgm(pair.left, pair.right, op, threads=threads)
+            runs.append(time.perf_counter() - t)
+        if not args.no_verify:
+            got = {f: b.download((H, W), sgm.OUTPUT_DTYPES[f]) for f, b in outs.items()}
+            bit_exact = all(np.array_equal(np.asarray(ref[f]).view(np.uint8), got[f].view(np.uint8)) for f in OUT_FIELDS)
+            log(f"bit-exact vs oracle: {bit_exact}")
+        if not args.no_cpu:
+            med = float(np.median(runs[1:]))  # first run is the warm-up
+            cpu = {"value": round(pxd / med / 1e6, 1), "unit": "Mpix·disp/s", "cores": threads, "kind": "port",
+                   "sample": f"oracle sgm_disparity+median3 on the same 1280x1024x128 8-path pair, median of "
+                             f"{len(runs) - 1} runs after 1 warm-up ({med * 1e3:.0f} ms each)"}
+            log(f"cpu baseline: {cpu}")
+
+    agg_bytes = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
+    achieved = agg_bytes / (paths_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                "kernel": "paths_kernel<8,false>", "kernel_ms": round(paths_ms, 4),
+                "algorithmic_bytes": agg_bytes}
+
+    batched = None
+    if not args.no_batch:
+        sizes = batch_sizes()
+        specs = []
+        for i, (w, h, d) in enumerate(sizes):
+            p = speckle_pair(w, h, d, seed=1000 + i)
+            specs.append(sgm.PairSpec(p.left, p.right, p.min_disparity, d, outputs=("disparity", "disparity_median",
+                                                                                      "disparity_int", "status")))
+        b = sgm.Batch(sgm.SgmParams(**PARAMS), specs)
+        total_pxd = sum(w * h * d for w, h, d in sizes)
+        res = {}
+        for mode in ("plain", "graph"):
+            run = b.run if mode == "graph" else b.run_plain
+            run()
+            b.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.batch_reps):
+                run()
+            b.synchronize()
+            res[mode] = (time.perf_counter() - t) * 1e3 / args.batch_reps
+        batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1),
+                   "batched_ms": round(res["graph"], 3), "plain_ms": round(res["plain"], 3),
+                   "batched_value": round(total_pxd / (res["graph"] * 1e-3) / 1e6, 0),
+                   "plain_value": round(total_pxd / (res["plain"] * 1e-3) / 1e6, 0),
+                   "ratio": round(res["plain"] / res["graph"], 3)}
+        log(f"batched: {batched}")
+        b.close()
+
+    line = {
+        "metric": "Mpix·disp/s batched vs plain on 1 MI355X (bit-exact); p50/p99 frame latency",
+        "value": round(value, 1),
+        "unit": "Mpix·disp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {"workload": "SGM 1280x1024 D=128, 8 paths, census 9x7, P1=10 P2=120 adaptive, subpixel, median3",
+                   "pairs_per_step": 1, "width": W, "height": H, "disparities": D, "paths": 8},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bit_exact": bit_exact,
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "batched": batched,
+    }
+    print(json.dumps(line, ensure_ascii=False), flush=True)
+    for bbuf in outs.values():
+        bbuf.free()
+    img.free()
+    st.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

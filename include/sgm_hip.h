@@ -96,6 +96,16 @@ typedef struct {
     double total_ms;  /* host wall time of the call, including copies */
 } sgm_timings;
 
+/* Per-stage GPU times summed over every call completed since the last reset (count = calls). */
+typedef struct {
+    double census_ms;
+    double paths_ms;
+    double rows_ms;
+    double median_ms;
+    double gpu_ms;
+    int64_t count;
+} sgm_stage_totals;
+
 typedef struct sgm_context sgm_context;
 
 /* Fills the reference defaults (stereo.hpp:26-40, subpixel = true). */
@@ -129,6 +139,12 @@ int sgm_synchronize(sgm_context* ctx);
 
 /* Replaces MetalStereo::last_timings (metal_stereo.hpp:97). */
 int sgm_last_timings(const sgm_context* ctx, sgm_timings* out);
+
+/* Stage times summed over the calls completed since the last reset (read at sgm_synchronize or
+ * sgm_compute). No counterpart in the reference (its profiler serialises the GPU per stage,
+ * libs/gpu/include/einstar/gpu/profile.hpp:1-6). */
+int sgm_get_stage_totals(const sgm_context* ctx, sgm_stage_totals* out);
+int sgm_reset_stage_totals(sgm_context* ctx);
 
 /* Device memory the context holds, in bytes. */
 size_t sgm_device_bytes(const sgm_context* ctx);

@@ -64,9 +64,10 @@ def build_selftest() -> str | None:
     if not os.path.exists(src):
         return None
     out = os.path.join(ROOT, "tests", "cpp", "hip_stereo_selftest")
+    # Links the built libraries by relative rpath so the binary also runs from a copy of the tree.
     _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-Werror", f"-I{INCLUDE}", f"-I{os.path.join(ROOT, 'oracle')}",
-          "-o", out, src, os.path.join(ROOT, "oracle", "sgm_oracle.c"), f"-L{PKG}", "-lsgm_hip",
-          f"-Wl,-rpath,{PKG}", "-lpthread", "-lm"])
+          "-o", out, src, f"-L{PKG}", "-lsgm_hip", f"-L{os.path.join(ROOT, 'oracle')}", "-loracle_sgm",
+          "-Wl,-rpath,$ORIGIN/../../open_source_einstar_amd:$ORIGIN/../../oracle", "-lpthread", "-lm"])
     return out
 
 

@@ -21,6 +21,10 @@ struct Stages {
     bool recorded = false;
 };
 
+// Stage events of the calls in flight: a ring, so that per-stage times can be summed over many
+// asynchronous calls (bench loops) and not only read for the last one.
+constexpr int kEventRing = 64;
+
 std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
 
 // Parameter validation of MetalStereo::create (metal_stereo.cpp:115-120), for byte path costs.
@@ -131,7 +135,10 @@ struct sgm_context {
     int max_w = 0, max_h = 0, ndirs = 4;
     hipStream_t stream = nullptr;
     hipStream_t last_stream = nullptr;
-    Stages st;
+    Stages ring[kEventRing];
+    int ring_head = 0;     // next set to record
+    int ring_pending = 0;  // recorded, not yet read; the oldest is ring_head - ring_pending
+    sgm_stage_totals totals{};
     uint8_t* d_img = nullptr;      // left | right, compact pitch, for host inputs
     uint64_t* d_census = nullptr;  // left | right
     uint8_t* d_lvol = nullptr;
@@ -181,8 +188,9 @@ const char* sgm_last_error(const sgm_context* ctx) { return ctx ? ctx->err.c_str
 void sgm_destroy(sgm_context* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (hipEvent_t e : ctx->st.ev)
-        if (e) (void)hipEventDestroy(e);
+    for (Stages& st : ctx->ring)
+        for (hipEvent_t e : st.ev)
+            if (e) (void)hipEventDestroy(e);
     for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
                     (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost})
@@ -215,7 +223,8 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
     const size_t npx = (size_t)max_width * (size_t)max_height;
     const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&ctx->st.ev[i]);
+    for (int r = 0; e == hipSuccess && r < kEventRing; ++r)
+        for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
     size_t& b = ctx->bytes;
     if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * npx, b);
@@ -332,6 +341,51 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
     return d;
 }
 
+// Adds one completed event set to the running totals; the newest also becomes last_timings.
+static void account(sgm_context* ctx, Stages& st, bool newest) {
+    float ms[4] = {0, 0, 0, 0}, all = 0;
+    for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], st.ev[i], st.ev[i + 1]);
+    (void)hipEventElapsedTime(&all, st.ev[0], st.ev[4]);
+    ctx->totals.census_ms += ms[0];
+    ctx->totals.paths_ms += ms[1];
+    ctx->totals.rows_ms += ms[2];
+    ctx->totals.median_ms += ms[3];
+    ctx->totals.gpu_ms += all;
+    ctx->totals.count += 1;
+    if (newest) {
+        ctx->timings.census_ms = ms[0];
+        ctx->timings.paths_ms = ms[1];
+        ctx->timings.rows_ms = ms[2];
+        ctx->timings.median_ms = ms[3];
+        ctx->timings.gpu_ms = all;
+        ctx->timings.total_ms = ctx->pending_wall_ms > 0 ? ctx->pending_wall_ms : all;
+    }
+    st.recorded = false;
+}
+
+// Reads every recorded set (the stream has been synchronised).
+static void read_timings(sgm_context* ctx) {
+    while (ctx->ring_pending > 0) {
+        const int i = (ctx->ring_head - ctx->ring_pending + kEventRing) % kEventRing;
+        --ctx->ring_pending;
+        account(ctx, ctx->ring[i], ctx->ring_pending == 0);
+    }
+}
+
+// The event set for the next call; when all are in flight, waits for the oldest and reads it.
+static Stages* next_stages(sgm_context* ctx) {
+    if (ctx->ring_pending == kEventRing) {
+        const int i = (ctx->ring_head - ctx->ring_pending + kEventRing) % kEventRing;
+        (void)hipEventSynchronize(ctx->ring[i].ev[4]);
+        --ctx->ring_pending;
+        account(ctx, ctx->ring[i], false);
+    }
+    Stages* st = &ctx->ring[ctx->ring_head];
+    ctx->ring_head = (ctx->ring_head + 1) % kEventRing;
+    ++ctx->ring_pending;
+    return st;
+}
+
 int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch, int width,
                        int height, const sgm_outputs* d_out, void* stream) {
     if (!ctx) {
@@ -345,25 +399,11 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
     const bool padded = d.dp != d.nd;
-    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, &ctx->st);
+    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx));
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
     return SGM_OK;
-}
-
-static void read_timings(sgm_context* ctx) {
-    if (!ctx->st.recorded) return;
-    float ms[4] = {0, 0, 0, 0}, all = 0;
-    for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ctx->st.ev[i], ctx->st.ev[i + 1]);
-    (void)hipEventElapsedTime(&all, ctx->st.ev[0], ctx->st.ev[4]);
-    ctx->timings.census_ms = ms[0];
-    ctx->timings.paths_ms = ms[1];
-    ctx->timings.rows_ms = ms[2];
-    ctx->timings.median_ms = ms[3];
-    ctx->timings.gpu_ms = all;
-    ctx->timings.total_ms = ctx->pending_wall_ms > 0 ? ctx->pending_wall_ms : all;
-    ctx->st.recorded = false;
 }
 
 int sgm_synchronize(sgm_context* ctx) {
@@ -378,6 +418,18 @@ int sgm_synchronize(sgm_context* ctx) {
 int sgm_last_timings(const sgm_context* ctx, sgm_timings* out) {
     if (!ctx || !out) return SGM_ERR_INVALID_ARGUMENT;
     *out = ctx->timings;
+    return SGM_OK;
+}
+
+int sgm_get_stage_totals(const sgm_context* ctx, sgm_stage_totals* out) {
+    if (!ctx || !out) return SGM_ERR_INVALID_ARGUMENT;
+    *out = ctx->totals;
+    return SGM_OK;
+}
+
+int sgm_reset_stage_totals(sgm_context* ctx) {
+    if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
+    ctx->totals = sgm_stage_totals{};
     return SGM_OK;
 }
 
@@ -416,7 +468,7 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
     o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
     o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
-    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, &ctx->st);
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, next_stages(ctx));
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute: launch", e);
     struct Copy {
         void* host;

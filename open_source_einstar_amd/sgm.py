@@ -61,6 +61,11 @@ class CSgmTimings(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms")]
 
 
+class CSgmStageTotals(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")] + \
+               [("count", C.c_int64)]
+
+
 class CSgmPair(C.Structure):
     _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("pitch", C.c_int64), ("width", C.c_int32),
                 ("height", C.c_int32), ("min_disparity", C.c_int32), ("num_disparities", C.c_int32),
@@ -110,6 +115,8 @@ def lib():
         L.sgm_compute_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp]
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
+        L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
+        L.sgm_reset_stage_totals.argtypes = [vp]
         L.sgm_device_bytes.argtypes = [vp]
         L.sgm_device_bytes.restype = C.c_size_t
         L.sgm_dev_malloc.argtypes = [C.c_size_t, P(vp)]
@@ -255,6 +262,15 @@ class HipStereo:
         t = CSgmTimings()
         _check(lib().sgm_last_timings(self._h, C.byref(t)), ctx=self._h)
         return {f: getattr(t, f) for f, _ in CSgmTimings._fields_}
+
+    def stage_totals(self) -> dict:
+        """Per-stage GPU ms summed over the calls completed since reset_stage_totals()."""
+        t = CSgmStageTotals()
+        _check(lib().sgm_get_stage_totals(self._h, C.byref(t)), ctx=self._h)
+        return {f: getattr(t, f) for f, _ in CSgmStageTotals._fields_}
+
+    def reset_stage_totals(self):
+        _check(lib().sgm_reset_stage_totals(self._h), ctx=self._h)
 
     def device_bytes(self) -> int:
         return lib().sgm_device_bytes(self._h)

@@ -4,9 +4,11 @@
  * Threads only split independent rows or scanlines, as tbb::parallel_for does in the reference
  * (stereo.cpp:18-23, 403-408), so results never depend on the thread count.
  */
+#define _GNU_SOURCE
 #include "sgm_oracle.h"
 
 #include <limits.h>
+#include <sched.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -39,8 +41,14 @@ static void* pf_worker(void* arg) {
     return NULL;
 }
 
+/* threads <= 0: the CPUs this process may run on (its affinity mask), not every online CPU. */
 static int resolve_threads(int threads) {
     if (threads > 0) return threads;
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+        const int n = CPU_COUNT(&set);
+        if (n > 0) return n;
+    }
     const long n = sysconf(_SC_NPROCESSORS_ONLN);
     return n > 0 ? (int)n : 1;
 }
