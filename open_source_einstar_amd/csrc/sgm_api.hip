@@ -124,7 +124,15 @@ void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) {
     d.dp = d.dpl * kGroup;
     d.ndirs = ndirs;
     d.lvol_dir_stride = (int64_t)w * h * d.dp;
+    census_layout(w, min_d, d.dp, d.census_padl, d.census_pitch);
     fill_lines(d);
+}
+
+// Census words of one image (padded rows) for a pair of this geometry.
+size_t census_words(int w, int h, int min_d, int nd) {
+    int32_t padl, pitch;
+    census_layout(w, min_d, dpl_for(nd) * kGroup, padl, pitch);
+    return (size_t)pitch * (size_t)h;
 }
 
 }  // namespace
@@ -140,7 +148,8 @@ struct sgm_context {
     int ring_pending = 0;  // recorded, not yet read; the oldest is ring_head - ring_pending
     sgm_stage_totals totals{};
     uint8_t* d_img = nullptr;      // left | right, compact pitch, for host inputs
-    uint64_t* d_census = nullptr;  // left | right
+    uint64_t* d_census = nullptr;  // left | right, padded rows
+    size_t census_half = 0;        // words per image
     uint8_t* d_lvol = nullptr;
     float* d_disp = nullptr;
     float* d_median = nullptr;
@@ -227,7 +236,8 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
         for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
     size_t& b = ctx->bytes;
     if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * npx, b);
+    ctx->census_half = census_words(max_width, max_height, params->min_disparity, params->num_disparities);
+    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
@@ -323,7 +333,7 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
     d.right = dr;
     d.pitch = pitch;
     d.census_l = ctx->d_census;
-    d.census_r = ctx->d_census + (size_t)ctx->max_w * ctx->max_h;
+    d.census_r = ctx->d_census + ctx->census_half;
     d.lvol = ctx->d_lvol;
     set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs);
     d.disp = (o && o->disparity) ? o->disparity : ctx->d_disp;
@@ -486,12 +496,15 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
         {out->second_cost, ctx->d_second, npx * 2},
         {out->sum_volume, ctx->d_sum, nvol * 2},
         {out->cost_volume, ctx->d_cost, nvol * 2},
-        {out->census_left, d.census_l, npx * 8},
-        {out->census_right, d.census_r, npx * 8},
     };
     for (const Copy& c : copies)
         if (c.host && (e = hipMemcpyAsync(c.host, c.dev, c.bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
             return ctx->hip_fail("sgm_compute: download", e);
+    for (const auto& [host, dev] : {std::pair<uint64_t*, const uint64_t*>{out->census_left, d.census_l},
+                                    std::pair<uint64_t*, const uint64_t*>{out->census_right, d.census_r}})
+        if (host && (e = hipMemcpy2DAsync(host, (size_t)width * 8, dev + d.census_padl, (size_t)d.census_pitch * 8,
+                                          (size_t)width * 8, height, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return ctx->hip_fail("sgm_compute: census download", e);
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail("sgm_compute: execution", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -705,7 +718,8 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
     for (int i = 0; i < n; ++i) {
         const size_t npx = (size_t)pairs[i].width * pairs[i].height;
         off_census[i] = total;
-        total += (2 * npx * 8 + 255) & ~(size_t)255;
+        total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 +
+                  255) & ~(size_t)255;
         off_lvol[i] = total;
         total += ((size_t)b->ndirs * npx * (size_t)dpl_for(pairs[i].num_disparities) * kGroup + 255) & ~(size_t)255;
     }
@@ -723,9 +737,8 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
         d.left = q.left;
         d.right = q.right;
         d.pitch = q.pitch;
-        const size_t npx = (size_t)q.width * q.height;
         d.census_l = reinterpret_cast<uint64_t*>(b->d_scratch + off_census[i]);
-        d.census_r = d.census_l + npx;
+        d.census_r = d.census_l + census_words(q.width, q.height, q.min_disparity, q.num_disparities);
         d.lvol = b->d_scratch + off_lvol[i];
         set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);
         d.disp = q.out.disparity;

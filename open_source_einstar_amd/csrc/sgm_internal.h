@@ -35,6 +35,8 @@ struct PairDesc {
     uint16_t* sum_volume;    // optional debug S [y][x][nd]
     uint16_t* cost_volume;   // optional debug C [y][x][nd]
     int64_t pitch;           // input row pitch in bytes
+    int32_t census_pitch;    // census words per row: pads, then the w words of the row, then pads
+    int32_t census_padl;     // words before x = 0 (every disparity window stays inside the row)
     int64_t lvol_dir_stride; // bytes between direction planes of lvol
     int32_t w, h;
     int32_t min_d, nd, dpl, dp;
@@ -77,6 +79,13 @@ int median_blocks(int w, int h);
 
 // Disparities per lane for a disparity count, or 0 when unsupported (> 512).
 int dpl_for(int nd);
+// Census row layout for width w: x - min_d - k must address a word of the row for every pixel x
+// and lane disparity k < dp, so the path kernel loads disparity windows without clamping.
+inline void census_layout(int w, int min_d, int dp, int32_t& padl, int32_t& pitch) {
+    padl = min_d + dp - 1 > 0 ? min_d + dp - 1 : 0;
+    const int padr = min_d < 0 ? -min_d : 0;
+    pitch = (padl + w + padr + 1) & ~1;
+}
 // Fills line_off for the 4 or 8 directions of the reference's schedule.
 void fill_lines(PairDesc& d);
 

@@ -27,6 +27,8 @@ constexpr float kInvalid = -1.0e9f;
 constexpr uint32_t kSentinel = 0x7FFF7FFFu;  // excluded neighbour: 0x7FFF + P1 never wins a min
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
@@ -59,12 +61,17 @@ __device__ __forceinline__ uint32_t dpp_or(uint32_t v, uint32_t edge) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, Ctrl, 0xF, 0xF, false);
 }
 
-// Minimum over the 16 lanes of a DPP row, returned to all of them.
+// Minimum over the 16 lanes of a DPP row, returned to all of them. The DPP moves use the min's
+// identity as their old value so the compiler can fold them into v_min_u32_dpp.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_id(uint32_t v, uint32_t identity) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, Ctrl, 0xF, 0xF, false);
+}
 __device__ __forceinline__ uint32_t row_min(uint32_t v) {
-    v = min(v, dpp<kQuadSwap1>(v));
-    v = min(v, dpp<kQuadSwap2>(v));
-    v = min(v, dpp<kRowHalfMirror>(v));
-    v = min(v, dpp<kRowMirror>(v));
+    v = min(v, dpp_id<kQuadSwap1>(v, 0xFFFFFFFFu));
+    v = min(v, dpp_id<kQuadSwap2>(v, 0xFFFFFFFFu));
+    v = min(v, dpp_id<kRowHalfMirror>(v, 0xFFFFFFFFu));
+    v = min(v, dpp_id<kRowMirror>(v, 0xFFFFFFFFu));
     return v;
 }
 
@@ -127,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts
             }
         }
     }
-    out[(int64_t)y * P.w + x] = bits;
+    out[(int64_t)y * P.census_pitch + P.census_padl + x] = bits;
 }
 
 // ------------------------------------------------------------------------------- cost volume
@@ -140,11 +147,12 @@ __global__ __launch_bounds__(kBlock) void cost_volume_kernel(LaunchTable t, SgmC
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
     if (i >= (int64_t)P.w * P.h) return;
     const int y = (int)(i / P.w), x = (int)(i - (int64_t)y * P.w);
-    const uint64_t a = P.census_l[i];
+    const int64_t row = (int64_t)y * P.census_pitch + P.census_padl;
+    const uint64_t a = P.census_l[row + x];
     uint16_t* dst = P.cost_volume + i * P.nd;
     for (int k = 0; k < P.nd; ++k) {
         const int xr = x - (P.min_d + k);
-        dst[k] = (xr >= 0 && xr < P.w) ? (uint16_t)__popcll(a ^ P.census_r[(int64_t)y * P.w + xr]) : (uint16_t)c.invalid_cost;
+        dst[k] = (xr >= 0 && xr < P.w) ? (uint16_t)__popcll(a ^ P.census_r[row + xr]) : (uint16_t)c.invalid_cost;
     }
 }
 
@@ -156,31 +164,56 @@ __global__ __launch_bounds__(kBlock) void cost_volume_kernel(LaunchTable t, SgmC
 // The +-1/+-2 neighbours across lane boundaries come from the adjacent lane by DPP row shifts,
 // and pm by a 4-step DPP butterfly; a line never leaves its DPP row.
 
-template <int DPL>
-__device__ __forceinline__ void load_words(const uint64_t* __restrict__ row, int xr0, int w, uint64_t (&wv)[DPL]) {
-#pragma unroll
-    for (int j = 0; j < DPL; ++j) {
-        int xr = xr0 - j;
-        xr = xr < 0 ? 0 : (xr >= w ? w - 1 : xr);
-        wv[j] = row[xr];
-    }
+#define SGM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ SGM_GLOBAL T* gptr(T* p) {
+    return (SGM_GLOBAL T*)p;
 }
 
+// Everything one step of a path reads: the left census word, the lane's DPL right census words
+// (lowest address first, i.e. disparity k0+DPL-1 first) and the guide intensity.
 template <int DPL>
-__device__ __forceinline__ void costs(uint64_t a, const uint64_t (&wv)[DPL], int xr0, int w, uint32_t inv,
-                                      uint32_t (&cc)[DPL / 2]) {
+struct StepIn {
+    uint64_t a;
+    uint64_t w[DPL];
+    int I;
+};
+
+template <int DPL>
+__device__ __forceinline__ void load_step(StepIn<DPL>& s, const SGM_GLOBAL uint64_t* cl, const SGM_GLOBAL uint64_t* cr,
+                                          const SGM_GLOBAL uint8_t* gd) {
+    s.a = *cl;
+#pragma unroll
+    for (int j = 0; j < DPL; ++j) s.w[j] = cr[j];
+    s.I = *gd;
+}
+
+// Packed costs of the lane's disparities k0 + 2r, k0 + 2r + 1 (low, high half). Word w[DPL-1-j]
+// belongs to disparity k0 + j. With `full` every window word lies in the image (the group-uniform
+// common case); otherwise out-of-image words give the invalid cost (stereo.cpp:374-376).
+template <int DPL>
+__device__ __forceinline__ void step_costs(const StepIn<DPL>& s, bool full, int xr_top, int w, uint32_t inv,
+                                           uint32_t (&cc)[DPL / 2]) {
 #pragma unroll
     for (int r = 0; r < DPL / 2; ++r) {
-        const int xa = xr0 - 2 * r, xb = xa - 1;
-        const uint32_t ca = (unsigned)xa < (unsigned)w ? (uint32_t)__popcll(a ^ wv[2 * r]) : inv;
-        const uint32_t cb = (unsigned)xb < (unsigned)w ? (uint32_t)__popcll(a ^ wv[2 * r + 1]) : inv;
+        const uint32_t ca = (uint32_t)__popcll(s.a ^ s.w[DPL - 1 - 2 * r]);
+        const uint32_t cb = (uint32_t)__popcll(s.a ^ s.w[DPL - 2 - 2 * r]);
         cc[r] = ca | (cb << 16);
+    }
+    if (!full) {
+#pragma unroll
+        for (int r = 0; r < DPL / 2; ++r) {
+            const int xa = xr_top - 2 * r, xb = xa - 1;
+            const uint32_t lo = (unsigned)xa < (unsigned)w ? (cc[r] & 0xFFFFu) : inv;
+            const uint32_t hi = (unsigned)xb < (unsigned)w ? (cc[r] >> 16) : inv;
+            cc[r] = lo | (hi << 16);
+        }
     }
 }
 
 // Low bytes of the DPL/2 packed pairs -> DPL bytes at dst (aligned to DPL's power-of-two part).
 template <int DPL>
-__device__ __forceinline__ void store_bytes(uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
+__device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
     if constexpr (DPL % 4 == 0) {
         uint32_t b[DPL / 4];
 #pragma unroll
@@ -188,20 +221,52 @@ __device__ __forceinline__ void store_bytes(uint8_t* dst, const uint32_t (&v)[DP
         if constexpr (DPL % 16 == 0) {
 #pragma unroll
             for (int i = 0; i < DPL / 16; ++i)
-                reinterpret_cast<uint4*>(dst)[i] = make_uint4(b[4 * i], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]);
+                reinterpret_cast<SGM_GLOBAL u32x4*>(dst)[i] = u32x4{b[4 * i], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]};
         } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
-            for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<uint2*>(dst)[i] = make_uint2(b[2 * i], b[2 * i + 1]);
+            for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<SGM_GLOBAL u32x2*>(dst)[i] = u32x2{b[2 * i], b[2 * i + 1]};
         } else {
 #pragma unroll
-            for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<uint32_t*>(dst)[i] = b[i];
+            for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<SGM_GLOBAL uint32_t*>(dst)[i] = b[i];
         }
     } else {
 #pragma unroll
         for (int i = 0; i < DPL / 2; ++i)
-            reinterpret_cast<uint16_t*>(dst)[i] = (uint16_t)((v[i] & 0xFFu) | ((v[i] >> 8) & 0xFF00u));
+            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[i] = (uint16_t)((v[i] & 0xFFu) | ((v[i] >> 8) & 0xFF00u));
     }
 }
+
+// One step of the recurrence (stereo.cpp:62-81) on the lane's packed pairs.
+template <int DPL, bool PAD>
+__device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint32_t (&cc)[DPL / 2], uint32_t pm, int p2,
+                                          uint32_t p1p1, uint32_t nslant, const uint32_t (&padm)[DPL / 2],
+                                          uint32_t& mn) {
+    constexpr int R = DPL / 2;
+    const uint32_t pmpm = pm * 0x10001u;
+    const uint32_t jj = (pm + (uint32_t)p2) * 0x10001u;
+    const uint32_t nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);  // k0-2, k0-1
+    const uint32_t nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);      // k0+DPL, k0+DPL+1
+    uint32_t cur[R];
+    mn = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t lo = r == 0 ? nb_lo : prev[r - 1];
+        const uint32_t hi = r == R - 1 ? nb_hi : prev[r + 1];
+        const uint32_t km1 = __builtin_amdgcn_alignbit(prev[r], lo, 16);  // (k-1) pairs
+        const uint32_t kp1 = __builtin_amdgcn_alignbit(hi, prev[r], 16);  // (k+1) pairs
+        uint32_t m = pk_min(pk_min(km1, kp1), pk_min(lo, hi) | nslant);
+        m = pk_add(m, p1p1);
+        const uint32_t best = pk_min(pk_min(prev[r], m), jj);
+        uint32_t v = pk_add(pk_sub(best, pmpm), cc[r]);
+        if (PAD) v = pk_max(v, padm[r]);
+        cur[r] = v;
+        mn = pk_min(mn, v);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) prev[r] = cur[r];
+}
+
+__device__ __forceinline__ uint32_t pair_min_row(uint32_t mn) { return row_min(min(mn & 0xFFFFu, mn >> 16)); }
 
 template <int DPL, bool PAD>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
@@ -242,102 +307,88 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
             y = dy > 0 ? 0 : H - 1;
         }
     }
+    // Pixels on the line until it leaves the image.
+    int n = 0x7FFFFFFF;
+    if (dx > 0) n = W - x;
+    if (dx < 0) n = x + 1;
+    if (dy > 0) n = min(n, H - y);
+    if (dy < 0) n = min(n, y + 1);
+
     const int g = threadIdx.x & (kGroup - 1);
     const int k0 = g * DPL;
-    const int nd = P.nd;
-    const int xoff = -P.min_d - k0;  // xr of this lane's first disparity is x + xoff
+    const int nd = P.nd, min_d = P.min_d;
     const uint32_t inv = (uint32_t)c.invalid_cost;
     const uint32_t p1p1 = (uint32_t)c.p1 * 0x10001u;
     const uint32_t nslant = c.slant_steps ? 0u : 0xFFFFFFFFu;
     const bool adaptive = c.adaptive_p2 != 0;
     const int p2_fixed = c.p2;
+    // The whole group's window [x - min_d - (dp-1), x - min_d] lies in [0, W) for these x.
+    const int full_lo = min_d + P.dp - 1, full_hi = W - 1 + min_d;
     uint32_t padm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
         padm[r] = PAD ? ((k0 + 2 * r >= nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= nd ? 0x7FFF0000u : 0u)) : 0u;
-    const uint64_t* __restrict__ cl = P.census_l;
-    const uint64_t* __restrict__ cr = P.census_r;
-    const uint8_t* __restrict__ guide = P.left;
-    uint8_t* __restrict__ lv = P.lvol + (int64_t)dir * P.lvol_dir_stride + k0;
-    const int64_t dp = P.dp;
 
-    // Loads for the current pixel; the next pixel's are issued before the current step is computed.
-    uint64_t a = cl[(int64_t)y * W + x];
-    uint64_t wv[DPL];
-    load_words<DPL>(cr + (int64_t)y * W, x + xoff, W, wv);
-    int I = guide[(int64_t)y * P.pitch + x];
+    // Walk pointers (the census rows carry pads, so no window load needs clamping).
+    const int64_t cp = P.census_pitch;
+    const int64_t step_c = (int64_t)dy * cp + dx;
+    const int64_t step_g = (int64_t)dy * P.pitch + dx;
+    const int64_t step_l = ((int64_t)dy * W + dx) * P.dp;
+    const SGM_GLOBAL uint64_t* lcl = gptr(P.census_l) + (int64_t)y * cp + P.census_padl + x;
+    const SGM_GLOBAL uint64_t* lcr = gptr(P.census_r) + (int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1));
+    const SGM_GLOBAL uint8_t* lgd = gptr(P.left) + (int64_t)y * P.pitch + x;
+    SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + (int64_t)dir * P.lvol_dir_stride + ((int64_t)y * W + x) * P.dp + k0;
 
-    uint32_t prev[R];
-    uint32_t pm;
-    int nx = x + dx, ny = y + dy;
-    bool more = nx >= 0 && ny >= 0 && nx < W && ny < H;
-    uint64_t na, nwv[DPL];
-    int nI;
-    {
-        const int lx = more ? nx : x, ly = more ? ny : y;
-        na = cl[(int64_t)ly * W + lx];
-        load_words<DPL>(cr + (int64_t)ly * W, lx + xoff, W, nwv);
-        nI = guide[(int64_t)ly * P.pitch + lx];
-    }
+    StepIn<DPL> A, B;
+    load_step<DPL>(A, lcl, lcr, lgd);
+    // Advance the load pointers while pixels remain (the last pixel is reloaded, harmlessly).
+    auto advance = [&](int i) {
+        if (i + 1 < n) {
+            lcl += step_c;
+            lcr += step_c;
+            lgd += step_g;
+        }
+    };
+    advance(0);
+    load_step<DPL>(B, lcl, lcr, lgd);
+    advance(1);
+
+    uint32_t prev[R], pm;
+    int xc = x;
     {
         uint32_t cc[R];
-        costs<DPL>(a, wv, x + xoff, W, inv, cc);
+        step_costs<DPL>(A, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             prev[r] = PAD ? pk_max(cc[r], padm[r]) : cc[r];
             mn = pk_min(mn, prev[r]);
         }
-        store_bytes<DPL>(lv + ((int64_t)y * W + x) * dp, prev);
-        pm = row_min(min(mn & 0xFFFFu, mn >> 16));
+        store_bytes<DPL>(lv, prev);
+        pm = pair_min_row(mn);
     }
-    int Iprev = I;
-    while (more) {
-        x = nx;
-        y = ny;
-        a = na;
-#pragma unroll
-        for (int j = 0; j < DPL; ++j) wv[j] = nwv[j];
-        I = nI;
-        nx = x + dx;
-        ny = y + dy;
-        more = nx >= 0 && ny >= 0 && nx < W && ny < H;
-        {
-            const int lx = more ? nx : x, ly = more ? ny : y;
-            na = cl[(int64_t)ly * W + lx];
-            load_words<DPL>(cr + (int64_t)ly * W, lx + xoff, W, nwv);
-            nI = guide[(int64_t)ly * P.pitch + lx];
-        }
-        // stereo.cpp:62-65: adaptive P2 from the left image's intensity step (|dI|/8 <= 31).
-        const int dI = I > Iprev ? I - Iprev : Iprev - I;
+    int Iprev = A.I;
+    auto step = [&](const StepIn<DPL>& S) {
+        xc += dx;
+        lv += step_l;
+        const int dI = S.I > Iprev ? S.I - Iprev : Iprev - S.I;  // stereo.cpp:62-65, |dI|/8 <= 31
         const int p2 = adaptive ? p2q[dI >> 3] : p2_fixed;
-        Iprev = I;
+        Iprev = S.I;
         uint32_t cc[R];
-        costs<DPL>(a, wv, x + xoff, W, inv, cc);
-        const uint32_t pmpm = pm * 0x10001u;
-        const uint32_t jj = (pm + (uint32_t)p2) * 0x10001u;
-        const uint32_t nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);  // k0-2, k0-1
-        const uint32_t nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);      // k0+DPL, k0+DPL+1
-        uint32_t cur[R];
-        uint32_t mn = 0xFFFFFFFFu;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t lo = r == 0 ? nb_lo : prev[r - 1];
-            const uint32_t hi = r == R - 1 ? nb_hi : prev[r + 1];
-            const uint32_t km1 = __builtin_amdgcn_alignbit(prev[r], lo, 16);  // (k-1) pairs
-            const uint32_t kp1 = __builtin_amdgcn_alignbit(hi, prev[r], 16);  // (k+1) pairs
-            uint32_t m = pk_min(pk_min(km1, kp1), pk_min(lo, hi) | nslant);
-            m = pk_add(m, p1p1);
-            const uint32_t best = pk_min(pk_min(prev[r], m), jj);
-            uint32_t v = pk_add(pk_sub(best, pmpm), cc[r]);
-            if (PAD) v = pk_max(v, padm[r]);
-            cur[r] = v;
-            mn = pk_min(mn, v);
-        }
-        store_bytes<DPL>(lv + ((int64_t)y * W + x) * dp, cur);
-        pm = row_min(min(mn & 0xFFFFu, mn >> 16));
-#pragma unroll
-        for (int r = 0; r < R; ++r) prev[r] = cur[r];
+        step_costs<DPL>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+        uint32_t mn;
+        path_step<DPL, PAD>(prev, cc, pm, p2, p1p1, nslant, padm, mn);
+        store_bytes<DPL>(lv, prev);
+        pm = pair_min_row(mn);
+    };
+    for (int i = 1; i < n; i += 2) {
+        load_step<DPL>(A, lcl, lcr, lgd);  // pixel i+1 (prefetch)
+        advance(i + 1);
+        step(B);                           // pixel i
+        if (i + 1 >= n) break;
+        load_step<DPL>(B, lcl, lcr, lgd);  // pixel i+2
+        advance(i + 2);
+        step(A);                           // pixel i+1
     }
 }
 
@@ -392,10 +443,10 @@ __device__ __forceinline__ void load_add(const uint8_t* src, uint32_t (&s)[DPL /
 __device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
 
 __device__ __forceinline__ uint32_t row_max(uint32_t v) {
-    v = max(v, dpp<kQuadSwap1>(v));
-    v = max(v, dpp<kQuadSwap2>(v));
-    v = max(v, dpp<kRowHalfMirror>(v));
-    v = max(v, dpp<kRowMirror>(v));
+    v = max(v, dpp_id<kQuadSwap1>(v, 0u));
+    v = max(v, dpp_id<kQuadSwap2>(v, 0u));
+    v = max(v, dpp_id<kRowHalfMirror>(v, 0u));
+    v = max(v, dpp_id<kRowMirror>(v, 0u));
     return v;
 }
 
