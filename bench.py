@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the path shards by pair with no exchange between GPUs, and this "
@@ -80,6 +82,15 @@ def main():
 
     from open_source_einstar_amd import sgm
     from open_source_einstar_amd.synth import speckle_pair
+
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        src, dst = sgm.DeviceBuffer(nb), sgm.DeviceBuffer(nb)
+        src.memset(1)
+        sgm.dev_copy(dst.ptr, src.ptr, nb)
+        src.free()
+        dst.free()
+        log(f"copy probe: {args.copy_probe_mib} MiB copied")
 
     pair = speckle_pair(W, H, D, seed=2)
     params = sgm.SgmParams(min_disparity=pair.min_disparity, num_disparities=D, **PARAMS)

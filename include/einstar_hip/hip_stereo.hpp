@@ -1,0 +1,297 @@
+#pragma once
+
+// einstar::depth_hip::HipStereo -- C++ replacement of einstar::depth_metal::MetalStereo
+// (libs/depth_metal/include/einstar/depth_metal/metal_stereo.hpp:67-107) for its SGM stage, on the
+// gfx950 library behind include/sgm_hip.h. Header-only: it only calls the C ABI.
+//
+// The value types restate the reference's (C++20 here; the reference's need C++23 <expected>):
+//   ImageView / Image            libs/core/include/einstar/core/image.hpp:16-102 (stride in elements)
+//   Errc / Error / Result        libs/core/include/einstar/core/error.hpp:8-31
+//   SgmParams ... StereoParams   libs/depth/include/einstar/depth/stereo.hpp:26-88
+//   StereoResult, StereoTimings  stereo.hpp:90-93, metal_stereo.hpp:61-65
+//
+// Scope: create / compute / last_timings as MetalStereo has them, for a configuration whose whole
+// depth::compute_disparity is the SGM stage (pyramid_levels == 0, occlusion and spike filters off,
+// speckle removal off). Anything else answers Errc::unsupported, naming the missing stage.
+// compute_sgm additionally returns every per-pixel output of the SGM stage (integer disparity,
+// status bits, right disparity, best/second costs, median3).
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "sgm_hip.h"
+
+namespace einstar::depth_hip {
+
+enum class Errc { ok = 0, not_found, io, timeout, protocol, invalid_argument, blocked, unsupported, busy, disconnected };
+
+struct Error {
+    Errc code = Errc::ok;
+    std::string message;
+};
+
+// The subset of std::expected<T, Error> the reference's callers use.
+template <typename T>
+class Result {
+public:
+    Result(T v) : ok_(true), value_(std::move(v)) {}
+    Result(Error e) : ok_(false), error_(std::move(e)) {}
+    [[nodiscard]] bool has_value() const { return ok_; }
+    explicit operator bool() const { return ok_; }
+    T& operator*() { return value_; }
+    const T& operator*() const { return value_; }
+    T* operator->() { return &value_; }
+    const T* operator->() const { return &value_; }
+    T& value() { return value_; }
+    [[nodiscard]] const Error& error() const { return error_; }
+
+private:
+    bool ok_;
+    T value_{};
+    Error error_;
+};
+
+template <>
+class Result<void> {
+public:
+    Result() = default;
+    Result(Error e) : ok_(false), error_(std::move(e)) {}
+    [[nodiscard]] bool has_value() const { return ok_; }
+    explicit operator bool() const { return ok_; }
+    [[nodiscard]] const Error& error() const { return error_; }
+
+private:
+    bool ok_ = true;
+    Error error_;
+};
+
+inline Error make_error(Errc code, std::string message) { return Error{code, std::move(message)}; }
+
+template <typename T>
+struct ImageView {
+    T* data = nullptr;
+    int width = 0;
+    int height = 0;
+    std::ptrdiff_t stride = 0;  // elements
+
+    constexpr ImageView() = default;
+    constexpr ImageView(T* d, int w, int h, std::ptrdiff_t s) : data(d), width(w), height(h), stride(s) {}
+    constexpr ImageView(T* d, int w, int h) : ImageView(d, w, h, w) {}
+    template <typename U>
+        requires std::is_same_v<const U, T>
+    constexpr ImageView(const ImageView<U>& o) : data(o.data), width(o.width), height(o.height), stride(o.stride) {}
+    [[nodiscard]] constexpr T& operator()(int x, int y) const { return data[y * stride + x]; }
+    [[nodiscard]] constexpr T* row(int y) const { return data + y * stride; }
+};
+
+template <typename T>
+class Image {
+public:
+    Image() = default;
+    Image(int w, int h, T fill = T{}) : width_(w), height_(h), pixels_(static_cast<std::size_t>(w) * static_cast<std::size_t>(h), fill) {}
+    [[nodiscard]] int width() const { return width_; }
+    [[nodiscard]] int height() const { return height_; }
+    [[nodiscard]] std::size_t size() const { return pixels_.size(); }
+    [[nodiscard]] T* data() { return pixels_.data(); }
+    [[nodiscard]] const T* data() const { return pixels_.data(); }
+    T& operator()(int x, int y) { return pixels_[static_cast<std::size_t>(y) * static_cast<std::size_t>(width_) + static_cast<std::size_t>(x)]; }
+    const T& operator()(int x, int y) const {
+        return pixels_[static_cast<std::size_t>(y) * static_cast<std::size_t>(width_) + static_cast<std::size_t>(x)];
+    }
+    [[nodiscard]] ImageView<T> view() { return {pixels_.data(), width_, height_}; }
+    [[nodiscard]] ImageView<const T> view() const { return {pixels_.data(), width_, height_}; }
+
+private:
+    int width_ = 0;
+    int height_ = 0;
+    std::vector<T> pixels_;
+};
+
+using ImageU8 = Image<std::uint8_t>;
+using ImageU16 = Image<std::uint16_t>;
+using ImageI32 = Image<std::int32_t>;
+using ImageF32 = Image<float>;
+
+inline constexpr float kInvalidDisparity = SGM_INVALID_DISPARITY;
+[[nodiscard]] constexpr bool valid_disparity(float d) { return d > -1.0e8f; }
+
+struct SgmParams {
+    int min_disparity = 0;
+    int num_disparities = 64;
+    int census_radius_x = 2;
+    int census_radius_y = 2;
+    int p1 = 8;
+    int p2 = 96;
+    bool adaptive_p2 = true;
+    bool eight_paths = false;
+    bool slant_steps = true;
+    float uniqueness = 0.9f;
+    int lr_max_diff = 1;
+};
+struct RefineParams {
+    int search_radius = 2;
+    int zncc_radius = 3;
+    float min_zncc = 0.5f;
+    bool reject_edge_winners = true;
+};
+struct DetailParams {
+    bool enabled = true;
+    int radius = 5;
+    float search = 1.0f;
+    float step = 0.5f;
+    float min_zncc = 0.5f;
+};
+struct FilterParams {
+    bool occlusion = true;
+    float occlusion_tolerance = 1.0f;
+    bool spikes = true;
+    float spike_threshold = 1.0f;
+};
+struct SpeckleParams {
+    int max_region_size = 200;
+    float max_diff = 1.0f;
+};
+struct StereoParams {
+    int pyramid_levels = 2;
+    SgmParams sgm;
+    RefineParams refine;
+    DetailParams detail;
+    FilterParams filter;
+    SpeckleParams speckle;
+    bool subpixel = true;
+};
+
+struct StereoResult {
+    ImageF32 disparity;
+    ImageF32 confidence;
+};
+
+struct StereoTimings {
+    double gpu_ms = 0;
+    double speckle_ms = 0;
+    double total_ms = 0;
+};
+
+// Every per-pixel output of the SGM stage.
+struct SgmResult {
+    ImageF32 disparity;         // sgm_disparity (stereo.cpp:361-463)
+    ImageF32 disparity_median;  // median3 of it (stereo.cpp:129-147)
+    ImageI32 disparity_int;     // min_disparity + best_k, SGM_NO_MATCH for an empty window
+    ImageU8 status;             // SGM_STATUS_* bits
+    ImageI32 right_disparity;   // right WTA, SGM_NO_MATCH where none
+    ImageU16 best_cost;
+    ImageU16 second_cost;
+};
+
+inline sgm_params to_c(const StereoParams& p) {
+    sgm_params c{};
+    c.min_disparity = p.sgm.min_disparity;
+    c.num_disparities = p.sgm.num_disparities;
+    c.census_radius_x = p.sgm.census_radius_x;
+    c.census_radius_y = p.sgm.census_radius_y;
+    c.p1 = p.sgm.p1;
+    c.p2 = p.sgm.p2;
+    c.adaptive_p2 = p.sgm.adaptive_p2 ? 1 : 0;
+    c.eight_paths = p.sgm.eight_paths ? 1 : 0;
+    c.slant_steps = p.sgm.slant_steps ? 1 : 0;
+    c.uniqueness = p.sgm.uniqueness;
+    c.lr_max_diff = p.sgm.lr_max_diff;
+    c.subpixel = p.subpixel ? 1 : 0;
+    return c;
+}
+
+class HipStereo {
+public:
+    // MetalStereo::create (metal_stereo.hpp:70-71) without the Metal context argument: the current
+    // HIP device is used. `width`/`height`: the resolution the disparity is computed at.
+    static Result<std::unique_ptr<HipStereo>> create(const StereoParams& params, int width, int height) {
+        const sgm_params c = to_c(params);
+        sgm_context* ctx = nullptr;
+        const int rc = sgm_create(&c, width, height, &ctx);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(nullptr));
+        return std::unique_ptr<HipStereo>(new HipStereo(ctx, params, width, height));
+    }
+    ~HipStereo() { sgm_destroy(ctx_); }
+    HipStereo(const HipStereo&) = delete;
+    HipStereo& operator=(const HipStereo&) = delete;
+
+    // MetalStereo::compute (metal_stereo.hpp:78): depth::compute_disparity on rectified images at the
+    // configured size. Supported when that is the SGM stage alone (see the header comment).
+    Result<StereoResult> compute(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right) {
+        if (params_.pyramid_levels != 0)
+            return make_error(Errc::unsupported, "coarse-to-fine ZNCC refinement (pyramid_levels > 0) is not implemented");
+        if (params_.filter.occlusion || params_.filter.spikes)
+            return make_error(Errc::unsupported, "occlusion and spike filters are not implemented");
+        if (params_.speckle.max_region_size > 1)
+            return make_error(Errc::unsupported, "speckle removal is not implemented");
+        if (auto r = check(left, right); !r) return r.error();
+        StereoResult res;
+        res.disparity = ImageF32(w_, h_);
+        res.confidence = ImageF32(w_, h_);
+        sgm_outputs o{};
+        o.disparity = res.disparity.data();
+        o.confidence = res.confidence.data();
+        if (auto r = run(left, right, o); !r) return r.error();
+        return res;
+    }
+
+    // The SGM stage's full output set (sgm_disparity plus median3 and the integer intermediates).
+    Result<SgmResult> compute_sgm(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right) {
+        if (auto r = check(left, right); !r) return r.error();
+        SgmResult res;
+        res.disparity = ImageF32(w_, h_);
+        res.disparity_median = ImageF32(w_, h_);
+        res.disparity_int = ImageI32(w_, h_);
+        res.status = ImageU8(w_, h_);
+        res.right_disparity = ImageI32(w_, h_);
+        res.best_cost = ImageU16(w_, h_);
+        res.second_cost = ImageU16(w_, h_);
+        sgm_outputs o{};
+        o.disparity = res.disparity.data();
+        o.disparity_median = res.disparity_median.data();
+        o.disparity_int = res.disparity_int.data();
+        o.status = res.status.data();
+        o.right_disparity = res.right_disparity.data();
+        o.best_cost = res.best_cost.data();
+        o.second_cost = res.second_cost.data();
+        if (auto r = run(left, right, o); !r) return r.error();
+        return res;
+    }
+
+    // MetalStereo::last_timings (metal_stereo.hpp:97).
+    [[nodiscard]] const StereoTimings& last_timings() const { return timings_; }
+
+private:
+    HipStereo(sgm_context* ctx, const StereoParams& p, int w, int h) : ctx_(ctx), params_(p), w_(w), h_(h) {}
+
+    // MetalStereo::compute's size check (metal_stereo.cpp:235-236).
+    Result<void> check(ImageView<const std::uint8_t> l, ImageView<const std::uint8_t> r) const {
+        if (l.width != w_ || l.height != h_ || r.width != w_ || r.height != h_)
+            return make_error(Errc::invalid_argument, "image size does not match the configured stereo size");
+        if (l.stride != r.stride)
+            return make_error(Errc::invalid_argument, "left and right images must share a stride");
+        return {};
+    }
+
+    Result<void> run(ImageView<const std::uint8_t> l, ImageView<const std::uint8_t> r, const sgm_outputs& o) {
+        const int rc = sgm_compute(ctx_, l.data, r.data, l.stride, w_, h_, &o);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        sgm_timings t{};
+        sgm_last_timings(ctx_, &t);
+        timings_.gpu_ms = t.gpu_ms;
+        timings_.speckle_ms = 0;
+        timings_.total_ms = t.total_ms;
+        return {};
+    }
+
+    sgm_context* ctx_;
+    StereoParams params_;
+    int w_, h_;
+    StereoTimings timings_;
+};
+
+}  // namespace einstar::depth_hip

@@ -157,6 +157,9 @@ int sgm_dev_free(void* ptr);
 int sgm_dev_upload(void* dst, const void* src, size_t bytes);
 int sgm_dev_download(void* dst, const void* src, size_t bytes);
 int sgm_dev_memset(void* dst, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned pointers and size); also the
+ * known-size reference for HBM traffic counters. Blocks until done. */
+int sgm_dev_copy(void* dst, const void* src, size_t bytes);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes);
 

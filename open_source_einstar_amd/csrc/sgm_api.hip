@@ -305,6 +305,20 @@ int sgm_dev_memset(void* dst, int value, size_t bytes) {
     return SGM_OK;
 }
 
+int sgm_dev_copy(void* dst, const void* src, size_t bytes) {
+    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 15) {
+        g_last_error = "sgm_dev_copy: pointers and size must be multiples of 16 bytes";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    hipError_t e = launch_copy(dst, src, bytes, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_dev_copy", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
 int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
     size_t f = 0, t = 0;
     const hipError_t e = hipMemGetInfo(&f, &t);

@@ -70,6 +70,7 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);

@@ -16,6 +16,7 @@
 // Every path value satisfies 0 <= L <= C + p2 <= 255 (the context enforces cells + p2 <= 255), so
 // L fits a byte and S = sum over <= 8 paths <= 2040 never wraps: the direction order and the
 // per-direction split of the sum do not change any result.
+#include <algorithm>
 #include <climits>
 
 #include "sgm_internal.h"
@@ -402,42 +403,61 @@ __device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {
     return v < -0.5f ? -0.5f : (0.5f < v ? 0.5f : v);
 }
 
+// Raw path-cost bytes of one pixel for one direction: DPL bytes, as (DPL+3)/4 words.
 template <int DPL>
-__device__ __forceinline__ void load_add(const uint8_t* src, uint32_t (&s)[DPL / 2]) {
-    if constexpr (DPL % 4 == 0) {
-        uint32_t b[DPL / 4];
-        if constexpr (DPL % 16 == 0) {
+__device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t (&w)[(DPL + 3) / 4]) {
+    if constexpr (DPL % 16 == 0) {
 #pragma unroll
-            for (int i = 0; i < DPL / 16; ++i) {
-                const uint4 q = reinterpret_cast<const uint4*>(src)[i];
-                b[4 * i] = q.x;
-                b[4 * i + 1] = q.y;
-                b[4 * i + 2] = q.z;
-                b[4 * i + 3] = q.w;
-            }
-        } else if constexpr (DPL % 8 == 0) {
-#pragma unroll
-            for (int i = 0; i < DPL / 8; ++i) {
-                const uint2 q = reinterpret_cast<const uint2*>(src)[i];
-                b[2 * i] = q.x;
-                b[2 * i + 1] = q.y;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < DPL / 4; ++i) b[i] = reinterpret_cast<const uint32_t*>(src)[i];
+        for (int i = 0; i < DPL / 16; ++i) {
+            const u32x4 q = reinterpret_cast<const SGM_GLOBAL u32x4*>(src)[i];
+            w[4 * i] = q.x;
+            w[4 * i + 1] = q.y;
+            w[4 * i + 2] = q.z;
+            w[4 * i + 3] = q.w;
         }
+    } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) {
-            s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, b[i], 0x0C010C00u));
-            s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, b[i], 0x0C030C02u));
+        for (int i = 0; i < DPL / 8; ++i) {
+            const u32x2 q = reinterpret_cast<const SGM_GLOBAL u32x2*>(src)[i];
+            w[2 * i] = q.x;
+            w[2 * i + 1] = q.y;
         }
+    } else if constexpr (DPL % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) w[i] = reinterpret_cast<const SGM_GLOBAL uint32_t*>(src)[i];
     } else {
 #pragma unroll
         for (int i = 0; i < DPL / 2; ++i) {
-            const uint32_t q = reinterpret_cast<const uint16_t*>(src)[i];
-            s[i] = pk_add(s[i], (q & 0xFFu) | ((q & 0xFF00u) << 8));
+            const uint32_t h = reinterpret_cast<const SGM_GLOBAL uint16_t*>(src)[i];
+            w[i / 2] = (i & 1) ? (w[i / 2] | (h << 16)) : h;
         }
     }
+}
+
+// Adds the bytes of `w` (disparities k0..k0+DPL-1) to the packed u16 sums.
+template <int DPL>
+__device__ __forceinline__ void add_raw(const uint32_t (&w)[(DPL + 3) / 4], uint32_t (&s)[DPL / 2]) {
+#pragma unroll
+    for (int i = 0; i < DPL / 4; ++i) {
+        s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, w[i], 0x0C010C00u));
+        s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, w[i], 0x0C030C02u));
+    }
+    if constexpr (DPL % 4 == 2) {
+        const uint32_t q = w[DPL / 4];
+        s[DPL / 2 - 1] = pk_add(s[DPL / 2 - 1], (q & 0xFFu) | ((q & 0xFF00u) << 8));
+    }
+}
+
+template <int DPL>
+struct RowIn {
+    uint32_t w[kMaxDirs][(DPL + 3) / 4];
+};
+
+template <int DPL>
+__device__ __forceinline__ void load_row_in(RowIn<DPL>& in, const SGM_GLOBAL uint8_t* src, int64_t dir_stride, int ndirs) {
+#pragma unroll
+    for (int d = 0; d < kMaxDirs; ++d)
+        if (d < ndirs) load_raw<DPL>(src + d * dir_stride, in.w[d]);
 }
 
 __device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
@@ -467,13 +487,18 @@ __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c
     const int k0 = g * DPL;
     const int nd = P.nd, min_d = P.min_d;
     const int64_t dp = P.dp;
-    const uint8_t* lrow = P.lvol + (int64_t)y * W * dp + k0;
+    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * dp + k0;
     const int64_t row0 = (int64_t)y * W;
-    for (int x = grp; x < W; x += kLinesPerBlock) {
+    const int ndirs = P.ndirs;
+    const int64_t dstride = P.lvol_dir_stride;
+
+    auto process = [&](const RowIn<DPL>& in, int x) {
         uint32_t s[DPL / 2];
 #pragma unroll
         for (int r = 0; r < DPL / 2; ++r) s[r] = 0;
-        for (int d = 0; d < P.ndirs; ++d) load_add<DPL>(lrow + d * P.lvol_dir_stride + (int64_t)x * dp, s);
+#pragma unroll
+        for (int d = 0; d < kMaxDirs; ++d)
+            if (d < ndirs) add_raw<DPL>(in.w[d], s);
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -534,6 +559,19 @@ __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c
             for (int j = 0; j < DPL; ++j)
                 if (k0 + j < nd) sv[k0 + j] = (uint16_t)half16(s[j >> 1], j & 1);
         }
+    };
+
+    // Pixels grp, grp + 16, ...: the next pixel's path costs load while this one is processed.
+    RowIn<DPL> A, B;
+    int x = grp;
+    if (x < W) load_row_in<DPL>(A, lrow + (int64_t)x * dp, dstride, ndirs);
+    for (; x < W; x += 2 * kLinesPerBlock) {
+        const int x1 = x + kLinesPerBlock;
+        load_row_in<DPL>(B, lrow + (int64_t)min(x1, W - 1) * dp, dstride, ndirs);
+        process(A, x);
+        if (x1 >= W) break;
+        load_row_in<DPL>(A, lrow + (int64_t)min(x1 + kLinesPerBlock, W - 1) * dp, dstride, ndirs);
+        process(B, x1);
     }
     __syncthreads();
     // Left/right consistency, stereo.cpp:450-461 (lround: halves away from zero).
@@ -603,6 +641,13 @@ __global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
         out = sel;
     }
     P.median[(int64_t)y * P.w + x] = out;
+}
+
+// -------------------------------------------------------------------------------------- copy
+// Plain 16-byte-per-lane copy: the known-size reference for HBM traffic counters.
+
+__global__ __launch_bounds__(kBlock) void copy_kernel(u32x4* __restrict__ dst, const u32x4* __restrict__ src, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
 }
 
 // -------------------------------------------------------------------------- launch dispatch
@@ -680,6 +725,15 @@ hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmCons
         default:
             return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, stream, static_cast<u32x4*>(dst),
+                       static_cast<const u32x4*>(src), n16);
     return hipGetLastError();
 }
 

@@ -124,6 +124,7 @@ def lib():
         L.sgm_dev_upload.argtypes = [vp, vp, C.c_size_t]
         L.sgm_dev_download.argtypes = [vp, vp, C.c_size_t]
         L.sgm_dev_memset.argtypes = [vp, i32, C.c_size_t]
+        L.sgm_dev_copy.argtypes = [vp, vp, C.c_size_t]
         L.sgm_dev_mem_info.argtypes = [P(C.c_size_t), P(C.c_size_t)]
         L.sgm_batch_create.argtypes = [P(CSgmParams), i32, P(CSgmPair), P(vp)]
         L.sgm_batch_run.argtypes = [vp, vp]
@@ -185,6 +186,10 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def dev_copy(dst: int, src: int, nbytes: int):
+    _check(lib().sgm_dev_copy(dst, src, nbytes))
 
 
 def mem_info():

@@ -1,0 +1,167 @@
+// C++ self-test of einstar::depth_hip::HipStereo (include/einstar_hip/hip_stereo.hpp) on the GPU,
+// against the CPU oracle: the drop-in class's outputs, error codes and timings, the way
+// tests/depth/test_metal_stereo.cpp exercises MetalStereo. Exit status 0 = pass.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "einstar_hip/hip_stereo.hpp"
+#include "sgm_oracle.h"
+
+using namespace einstar::depth_hip;
+
+namespace {
+
+int failures = 0;
+
+void expect(bool ok, const char* what) {
+    std::printf("%s %s\n", ok ? "ok  " : "FAIL", what);
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+// A textured pair: left random blobs, right = left shifted by `shift` px with fresh columns.
+void make_pair(int w, int h, int shift, ImageU8& l, ImageU8& r) {
+    std::mt19937 rng(1234);
+    std::uniform_int_distribution<int> u(0, 255);
+    l = ImageU8(w, h);
+    r = ImageU8(w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) l(x, y) = static_cast<std::uint8_t>(u(rng));
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) r(x, y) = x + shift < w ? l(x + shift, y) : static_cast<std::uint8_t>(u(rng));
+}
+
+orc_params oracle_params(const StereoParams& p) {
+    orc_params o{};
+    o.min_disparity = p.sgm.min_disparity;
+    o.num_disparities = p.sgm.num_disparities;
+    o.census_radius_x = p.sgm.census_radius_x;
+    o.census_radius_y = p.sgm.census_radius_y;
+    o.p1 = p.sgm.p1;
+    o.p2 = p.sgm.p2;
+    o.adaptive_p2 = p.sgm.adaptive_p2;
+    o.eight_paths = p.sgm.eight_paths;
+    o.slant_steps = p.sgm.slant_steps;
+    o.uniqueness = p.sgm.uniqueness;
+    o.lr_max_diff = p.sgm.lr_max_diff;
+    o.subpixel = p.subpixel;
+    return o;
+}
+
+template <typename T>
+bool same_bytes(const Image<T>& a, const std::vector<T>& b) {
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), b.size() * sizeof(T)) == 0;
+}
+
+}  // namespace
+
+int main() {
+    const int w = 200, h = 120;
+    ImageU8 l, r;
+    make_pair(w, h, 9, l, r);
+
+    StereoParams sp;
+    sp.pyramid_levels = 0;
+    sp.filter.occlusion = false;
+    sp.filter.spikes = false;
+    sp.speckle.max_region_size = 0;
+    sp.sgm.num_disparities = 32;
+    sp.sgm.census_radius_x = 4;
+    sp.sgm.census_radius_y = 3;
+    sp.sgm.p1 = 10;
+    sp.sgm.p2 = 120;
+    sp.sgm.eight_paths = true;
+
+    auto st = HipStereo::create(sp, w, h);
+    expect(st.has_value(), "create");
+    if (!st) {
+        std::printf("%s\n", st.error().message.c_str());
+        return 1;
+    }
+
+    // Oracle on the same pair.
+    const auto n = static_cast<std::size_t>(w * h);
+    std::vector<float> disp(n), med(n);
+    std::vector<std::int32_t> dint(n), rdisp(n);
+    std::vector<std::uint8_t> status(n);
+    std::vector<std::uint16_t> best(n), second(n);
+    orc_outputs oo{};
+    oo.disparity = disp.data();
+    oo.disparity_median = med.data();
+    oo.disparity_int = dint.data();
+    oo.status = status.data();
+    oo.right_disparity = rdisp.data();
+    oo.best_cost = best.data();
+    oo.second_cost = second.data();
+    const orc_params op = oracle_params(sp);
+    expect(orc_sgm(l.data(), r.data(), w, w, h, &op, &oo, 0) == 0, "oracle");
+
+    auto res = (*st)->compute_sgm(l.view(), r.view());
+    expect(res.has_value(), "compute_sgm");
+    if (res) {
+        expect(same_bytes(res->disparity, disp), "compute_sgm disparity bit-exact");
+        expect(same_bytes(res->disparity_median, med), "compute_sgm median3 bit-exact");
+        expect(same_bytes(res->disparity_int, dint), "compute_sgm integer disparity bit-exact");
+        expect(same_bytes(res->status, status), "compute_sgm status bit-exact");
+        expect(same_bytes(res->right_disparity, rdisp), "compute_sgm right disparity bit-exact");
+        expect(same_bytes(res->best_cost, best) && same_bytes(res->second_cost, second), "compute_sgm costs bit-exact");
+        int good = 0, valid = 0;
+        for (int y = 10; y < h - 10; ++y)
+            for (int x = 50; x < w - 10; ++x)
+                if (valid_disparity(res->disparity(x, y))) {
+                    ++valid;
+                    good += std::abs(res->disparity(x, y) - 9.0f) < 0.5f;
+                }
+        expect(valid > 0 && good > 0.97 * valid, "shifted texture recovered (> 97% within 0.5 px)");
+    }
+
+    auto full = (*st)->compute(l.view(), r.view());
+    expect(full.has_value(), "compute (SGM-only configuration)");
+    if (full) {
+        expect(same_bytes(full->disparity, disp), "compute disparity == sgm_disparity");
+        bool conf_ok = true;
+        for (std::size_t i = 0; i < n; ++i) conf_ok &= full->confidence.data()[i] == (valid_disparity(disp[i]) ? 1.0f : 0.0f);
+        expect(conf_ok, "compute confidence = validity (stereo.cpp:515-519)");
+        expect((*st)->last_timings().gpu_ms > 0 && (*st)->last_timings().total_ms >= (*st)->last_timings().gpu_ms,
+               "last_timings");
+    }
+
+    // A strided view of a wider buffer gives the same result.
+    {
+        const int pitch = w + 24;
+        std::vector<std::uint8_t> bl(static_cast<std::size_t>(pitch * h)), br(bl.size());
+        for (int y = 0; y < h; ++y) {
+            std::memcpy(&bl[static_cast<std::size_t>(y * pitch)], l.view().row(y), static_cast<std::size_t>(w));
+            std::memcpy(&br[static_cast<std::size_t>(y * pitch)], r.view().row(y), static_cast<std::size_t>(w));
+        }
+        auto s2 = (*st)->compute_sgm(ImageView<const std::uint8_t>(bl.data(), w, h, pitch),
+                                     ImageView<const std::uint8_t>(br.data(), w, h, pitch));
+        expect(s2.has_value() && same_bytes(s2->disparity, disp), "strided input views");
+    }
+
+    // Error behaviour of MetalStereo (metal_stereo.cpp:115-120, 235-236).
+    ImageU8 small(w - 1, h);
+    auto bad = (*st)->compute(small.view(), r.view());
+    expect(!bad && bad.error().code == Errc::invalid_argument, "size mismatch -> invalid_argument");
+    StereoParams dflt;  // pyramid_levels = 2, filters on
+    auto st2 = HipStereo::create(dflt, w, h);
+    expect(st2.has_value(), "create with reference defaults");
+    if (st2) {
+        auto u = (*st2)->compute(l.view(), r.view());
+        expect(!u && u.error().code == Errc::unsupported, "refinement stages -> unsupported");
+    }
+    StereoParams big = sp;
+    big.sgm.num_disparities = 600;
+    auto st3 = HipStereo::create(big, w, h);
+    expect(!st3 && st3.error().code == Errc::invalid_argument, "num_disparities > 512 -> invalid_argument");
+    StereoParams heavy = sp;
+    heavy.sgm.p2 = 250;
+    auto st4 = HipStereo::create(heavy, w, h);
+    expect(!st4 && st4.error().code == Errc::invalid_argument, "census cells + P2 > 255 -> invalid_argument");
+
+    std::printf("%s (%d failures)\n", failures ? "selftest FAILED" : "selftest ok", failures);
+    return failures ? 1 : 0;
+}
