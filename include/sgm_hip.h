@@ -194,6 +194,69 @@ const char* sgm_batch_last_error(const sgm_batch* b);
 size_t sgm_batch_device_bytes(const sgm_batch* b);
 void sgm_batch_destroy(sgm_batch* b);
 
+/* ------------------------------------------------------------------------------------------
+ * Serving: a request loop with continuous admission. No counterpart in the reference (its live
+ * pipeline queues frames for one worker, libs/pipeline/src/scan_pipeline.cpp:512-609).
+ *
+ * sgm_server_start reserves an HBM arena (hbm_budget_bytes; 0 = half the free HBM, at most 64 GiB)
+ * and `slots` streams with worker threads. A submitted request (host images, copied at submission,
+ * so the caller may reuse them) waits in arrival order; the head is admitted when a slot is free and
+ * its device footprint fits the arena. Frames arriving while others run start as soon as a slot and
+ * memory free up and overlap on the device. Results are written to the request's host output
+ * buffers, which must stay valid until its completion is polled. All functions are thread-safe.
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t max_width, max_height; /* largest request */
+    int32_t max_disparities;       /* largest num_disparities of a request */
+    int32_t slots;                 /* requests on the device at once (1..64) */
+    int64_t hbm_budget_bytes;      /* arena size; 0 = default */
+} sgm_server_config;
+
+typedef struct {
+    const uint8_t* left;  /* host */
+    const uint8_t* right; /* host */
+    int64_t stride;       /* bytes */
+    int32_t width, height;
+    int32_t min_disparity, num_disparities;
+    sgm_outputs out; /* host buffers; disparity required; no volumes or census */
+    uint64_t user_tag;
+} sgm_request;
+
+typedef struct {
+    uint64_t ticket;
+    uint64_t user_tag;
+    int32_t status;      /* SGM_OK or the request's error */
+    int64_t t_submit_ns; /* sgm_server_now_ns() clock */
+    int64_t t_admit_ns;
+    int64_t t_done_ns;   /* results in the caller's buffers */
+    double gpu_ms;       /* first to last kernel of this request */
+} sgm_completion;
+
+typedef struct {
+    int32_t queued;
+    int32_t in_flight;
+    int64_t completed;
+    int64_t failed;
+    int64_t arena_bytes;
+    int64_t arena_free_bytes;
+} sgm_server_stats;
+
+typedef struct sgm_server sgm_server;
+
+int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm_server** out);
+/* SGM_ERR_BUSY when the request could never fit the arena; SGM_ERR_INVALID_ARGUMENT for sizes,
+ * ranges or outputs the server does not take. */
+int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket);
+/* Up to max completions; waits up to timeout_us (0: no wait, < 0: until one is available). */
+int sgm_server_poll(sgm_server* sv, sgm_completion* out, int max, int timeout_us, int* n);
+int sgm_server_get_stats(sgm_server* sv, sgm_server_stats* st);
+const char* sgm_server_last_error(sgm_server* sv);
+/* Finishes the requests on the device, drops the queued ones, frees everything. */
+int sgm_server_stop(sgm_server* sv);
+/* The steady clock (ns) the completion timestamps use. */
+int64_t sgm_server_now_ns(void);
+
 #ifdef __cplusplus
 }
 #endif

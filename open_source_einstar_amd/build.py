@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wextra", "-Wno-unused-parameter",
 ]
 
-HIP_SOURCES = ["sgm_kernels.hip", "sgm_api.hip"]
+HIP_SOURCES = ["sgm_kernels.hip", "sgm_api.hip", "sgm_server.hip"]
 
 
 def _run(cmd, cwd=None):
@@ -71,15 +71,23 @@ def build_selftest() -> str | None:
     return out
 
 
+def build_serve_core_test() -> str:
+    out = os.path.join(ROOT, "tests", "cpp", "serve_core_test")
+    _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-Werror", f"-I{CSRC}", "-o", out,
+          os.path.join(ROOT, "tests", "cpp", "serve_core_test.cpp")])
+    return out
+
+
 def build_all() -> None:
     build_oracle()
     build_synth()
     build_hip()
     build_selftest()
+    build_serve_core_test()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
-         "selftest": build_selftest}[w]()
+         "selftest": build_selftest, "serve_core": build_serve_core_test}[w]()

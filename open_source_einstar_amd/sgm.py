@@ -66,6 +66,28 @@ class CSgmStageTotals(C.Structure):
                [("count", C.c_int64)]
 
 
+class CSgmServerConfig(C.Structure):
+    _fields_ = [("max_width", C.c_int32), ("max_height", C.c_int32), ("max_disparities", C.c_int32),
+                ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
+
+
+class CSgmRequest(C.Structure):
+    _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("stride", C.c_int64), ("width", C.c_int32),
+                ("height", C.c_int32), ("min_disparity", C.c_int32), ("num_disparities", C.c_int32),
+                ("out", CSgmOutputs), ("user_tag", C.c_uint64)]
+
+
+class CSgmCompletion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("user_tag", C.c_uint64), ("status", C.c_int32),
+                ("t_submit_ns", C.c_int64), ("t_admit_ns", C.c_int64), ("t_done_ns", C.c_int64),
+                ("gpu_ms", C.c_double)]
+
+
+class CSgmServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int32), ("in_flight", C.c_int32), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("arena_bytes", C.c_int64), ("arena_free_bytes", C.c_int64)]
+
+
 class CSgmPair(C.Structure):
     _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("pitch", C.c_int64), ("width", C.c_int32),
                 ("height", C.c_int32), ("min_disparity", C.c_int32), ("num_disparities", C.c_int32),
@@ -135,6 +157,15 @@ def lib():
         L.sgm_batch_device_bytes.argtypes = [vp]
         L.sgm_batch_device_bytes.restype = C.c_size_t
         L.sgm_batch_destroy.argtypes = [vp]
+        L.sgm_server_start.argtypes = [P(CSgmParams), P(CSgmServerConfig), P(vp)]
+        L.sgm_server_submit.argtypes = [vp, P(CSgmRequest), P(C.c_uint64)]
+        L.sgm_server_poll.argtypes = [vp, P(CSgmCompletion), i32, i32, P(C.c_int)]
+        L.sgm_server_get_stats.argtypes = [vp, P(CSgmServerStats)]
+        L.sgm_server_last_error.argtypes = [vp]
+        L.sgm_server_last_error.restype = C.c_char_p
+        L.sgm_server_stop.argtypes = [vp]
+        L.sgm_server_now_ns.argtypes = []
+        L.sgm_server_now_ns.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -361,5 +392,80 @@ class Batch:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (sgm_server_*): submit host pairs as they arrive, poll completions.
+
+    Output buffers of a request are numpy arrays owned here until its completion is polled."""
+
+    def __init__(self, params: SgmParams, max_width: int, max_height: int, max_disparities: int, slots: int = 4,
+                 hbm_budget_bytes: int = 0):
+        cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes)
+        h = C.c_void_p()
+        cp = params.to_c()
+        rc = lib().sgm_server_start(C.byref(cp), C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
+        self._h = h.value
+        self._pending = {}  # ticket -> (outputs dict, keep-alive)
+
+    @staticmethod
+    def now_ns() -> int:
+        return lib().sgm_server_now_ns()
+
+    def submit(self, left: np.ndarray, right: np.ndarray, min_disparity: int, num_disparities: int,
+               outputs=("disparity",), tag: int = 0) -> int:
+        left = np.ascontiguousarray(left, dtype=np.uint8)
+        right = np.ascontiguousarray(right, dtype=np.uint8)
+        h, w = left.shape
+        res = {f: np.empty((h, w), OUTPUT_DTYPES[f]) for f in set(outputs) | {"disparity"}}
+        rq = CSgmRequest()
+        rq.left, rq.right, rq.stride = left.ctypes.data, right.ctypes.data, w
+        rq.width, rq.height, rq.min_disparity, rq.num_disparities = w, h, min_disparity, num_disparities
+        for f, a in res.items():
+            setattr(rq.out, f, a.ctypes.data)
+        rq.user_tag = tag
+        t = C.c_uint64()
+        rc = lib().sgm_server_submit(self._h, C.byref(rq), C.byref(t))
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_server_last_error(self._h) or b"").decode())
+        self._pending[t.value] = res
+        return t.value
+
+    def poll(self, max_n: int = 64, timeout_us: int = 0) -> list:
+        """Completions as dicts (ticket, tag, status, t_submit_ns, t_admit_ns, t_done_ns, gpu_ms, outputs)."""
+        arr = (CSgmCompletion * max_n)()
+        n = C.c_int()
+        _check(lib().sgm_server_poll(self._h, arr, max_n, timeout_us, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            c = arr[i]
+            out.append({"ticket": c.ticket, "tag": c.user_tag, "status": c.status, "t_submit_ns": c.t_submit_ns,
+                        "t_admit_ns": c.t_admit_ns, "t_done_ns": c.t_done_ns, "gpu_ms": c.gpu_ms,
+                        "outputs": self._pending.pop(c.ticket, None)})
+        return out
+
+    def stats(self) -> dict:
+        st = CSgmServerStats()
+        _check(lib().sgm_server_get_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in CSgmServerStats._fields_}
+
+    def stop(self):
+        if getattr(self, "_h", None):
+            lib().sgm_server_stop(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass
