@@ -140,6 +140,7 @@ def main():
     pair = speckle_pair(W, H, D, seed=2)
     params = sgm.SgmParams(min_disparity=pair.min_disparity, num_disparities=D, **PARAMS)
     st = sgm.HipStereo.create(params, W, H)
+    st.set_option(sgm.OPT_COST_VOLUME, 1)  # allocate up front: the timed region never allocates
     img = sgm.DeviceBuffer(2 * W * H)
     img.upload(pair.left, 0)
     img.upload(pair.right, W * H)
@@ -149,22 +150,33 @@ def main():
     def step():
         st.compute_device(img.ptr, img.ptr + W * H, W, W, H, d_out)
 
-    for _ in range(args.warmup):
-        step()
-    st.synchronize()
-    st.reset_stage_totals()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    st.synchronize()
-    elapsed = time.perf_counter() - t0
-    totals = st.stage_totals()
-    ms_per_step = elapsed * 1e3 / args.steps
+    def timed(cost_volume):
+        st.set_option(sgm.OPT_COST_VOLUME, cost_volume)
+        for _ in range(args.warmup):
+            step()
+        st.synchronize()
+        st.reset_stage_totals()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        st.synchronize()
+        elapsed = time.perf_counter() - t0
+        totals = st.stage_totals()
+        return elapsed * 1e3 / args.steps, totals
+
+    # Both cost modes (they give the same bytes); the faster one is the configuration reported.
+    modes = {}
+    for cv in (0, 1):
+        ms, totals = timed(cv)
+        modes[cv] = (ms, totals)
+        log(f"cost_volume={cv}: {ms:.3f} ms/step, paths {totals['paths_ms'] / max(1, totals['count']):.3f} ms")
+    best_cv = min(modes, key=lambda k: modes[k][0])
+    ms_per_step, totals = timed(best_cv)
     pxd = W * H * D
     value = pxd / (ms_per_step * 1e-3) / 1e6
     paths_ms = totals["paths_ms"] / max(1, totals["count"])
     stage_ms = {k: totals[k] / max(1, totals["count"]) for k in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")}
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.0f} Mpix*disp/s, stages {stage_ms}")
+    log(f"timed {args.steps} steps (cost_volume={best_cv}): {ms_per_step:.3f} ms/step, {value:.0f} Mpix*disp/s, stages {stage_ms}")
 
     # Single synchronised steps: latency from launch to host-visible completion.
     lat = []
@@ -268,6 +280,8 @@ def main():
         "bit_exact": bit_exact,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "cost_mode": "volume" if best_cv else "recompute",
+        "cost_modes_ms_per_step": {"recompute": round(modes[0][0], 4), "volume": round(modes[1][0], 4)},
         "batched": batched,
         "served": served,
     }

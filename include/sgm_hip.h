@@ -146,6 +146,13 @@ int sgm_last_timings(const sgm_context* ctx, sgm_timings* out);
 int sgm_get_stage_totals(const sgm_context* ctx, sgm_stage_totals* out);
 int sgm_reset_stage_totals(sgm_context* ctx);
 
+/* Tuning knobs: they change speed and memory, never results. */
+enum {
+    SGM_OPT_COST_VOLUME = 1 /* 1: matching costs written once as bytes and read by every path
+                               (W*H*dp more bytes of HBM); 0 (default): recomputed from census words */
+};
+int sgm_set_option(sgm_context* ctx, int option, int64_t value);
+
 /* Device memory the context holds, in bytes. */
 size_t sgm_device_bytes(const sgm_context* ctx);
 

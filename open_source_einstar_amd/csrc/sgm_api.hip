@@ -97,6 +97,10 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
         t.total_blocks = (int)(((int64_t)d.w * d.h + kBlock - 1) / kBlock);
         if ((e = launch_cost_volume(t, c, s)) != hipSuccess) return e;
     }
+    if (d.cvol) {
+        t.total_blocks = cost8_blocks(d.w, d.h);
+        if ((e = launch_cost8(d.dpl, t, c, s)) != hipSuccess) return e;
+    }
     if (st && (e = hipEventRecord(st->ev[1], s)) != hipSuccess) return e;
     t.total_blocks = path_blocks(d);
     if ((e = launch_paths(d.dpl, padded, t, c, s)) != hipSuccess) return e;
@@ -174,6 +178,8 @@ struct sgm_context {
     int32_t* d_rdisp = nullptr;
     uint16_t* d_best = nullptr;
     uint16_t* d_second = nullptr;
+    uint8_t* d_cvol = nullptr;   // u8 matching costs when the cost-volume mode is on
+    bool use_cvol = false;
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     size_t bytes = 0;
@@ -217,7 +223,7 @@ void sgm_destroy(sgm_context* ctx) {
             if (e) (void)hipEventDestroy(e);
     for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
-                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost})
+                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol})
         dfree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -272,6 +278,23 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
 }
 
 size_t sgm_device_bytes(const sgm_context* ctx) { return ctx ? ctx->bytes : 0; }
+
+int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
+    if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
+    switch (option) {
+        case SGM_OPT_COST_VOLUME: {
+            if (value && !ctx->d_cvol) {
+                const size_t n = (size_t)ctx->max_w * ctx->max_h * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
+                const hipError_t e = dalloc(&ctx->d_cvol, n, ctx->bytes);
+                if (e != hipSuccess) return ctx->hip_fail("sgm_set_option: cost volume allocation", e);
+            }
+            ctx->use_cvol = value != 0;
+            return SGM_OK;
+        }
+        default:
+            return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));
+    }
+}
 
 int sgm_dev_malloc(size_t bytes, void** out) {
     if (!out) return SGM_ERR_INVALID_ARGUMENT;
@@ -364,6 +387,7 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
     d.census_l = ctx->d_census;
     d.census_r = ctx->d_census + ctx->census_half;
     d.lvol = ctx->d_lvol;
+    d.cvol = ctx->use_cvol ? ctx->d_cvol : nullptr;
     set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs);
     d.disp = (o && o->disparity) ? o->disparity : ctx->d_disp;
     if (o) {

@@ -24,6 +24,7 @@ struct PairDesc {
     uint64_t* census_l;
     uint64_t* census_r;
     uint8_t* lvol;           // [dir][y][x][dp] per-direction path costs L (u8)
+    uint8_t* cvol;           // [y][x][dp] matching costs C (u8), or null: the path kernel recomputes them
     float* disp;             // sgm_disparity output (post left/right check)
     float* median;           // median3(disp)
     float* conf;             // 1 where disp is valid, else 0
@@ -62,6 +63,7 @@ struct LaunchTable {
     const int32_t* first_block;   // device array: first flat block of each pair (n > 1)
     int32_t n;
     int32_t total_blocks;
+    int32_t cost_volume;          // batches (n > 1): every pair has a cost volume
 };
 
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
@@ -70,12 +72,14 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
+hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);
 int path_blocks(const PairDesc& d);
 int row_blocks(const PairDesc& d);
+int cost8_blocks(int w, int h);
 int median_blocks(int w, int h);
 
 // Disparities per lane for a disparity count, or 0 when unsupported (> 512).

@@ -102,11 +102,9 @@ constexpr int kTile = 16;
 constexpr int kMaxRx = 4, kMaxRy = 3;
 constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
 
-__global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
-    __shared__ uint8_t tile[kTileH * kTileW];
-    int pi, lb;
-    locate(t, blockIdx.x, pi, lb);
-    const PairDesc P = load_desc(t, pi);
+// One 16x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
+// the block calls it (it synchronises once).
+__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, uint8_t* tile) {
     const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
     const int side = lb / (tx_n * ty_n);
     const int tl = lb - side * tx_n * ty_n;
@@ -136,6 +134,14 @@ __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts
         }
     }
     out[(int64_t)y * P.census_pitch + P.census_padl + x] = bits;
+}
+
+__global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
+    __shared__ uint8_t tile[kTileH * kTileW];
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    census_body(P, c, lb, tile);
 }
 
 // ------------------------------------------------------------------------------- cost volume
@@ -171,44 +177,127 @@ __device__ __forceinline__ SGM_GLOBAL T* gptr(T* p) {
     return (SGM_GLOBAL T*)p;
 }
 
-// Everything one step of a path reads: the left census word, the lane's DPL right census words
-// (lowest address first, i.e. disparity k0+DPL-1 first) and the guide intensity.
+// DPL bytes (path costs or matching costs of one pixel), as (DPL+3)/4 words.
 template <int DPL>
-struct StepIn {
+__device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t (&w)[(DPL + 3) / 4]) {
+    if constexpr (DPL % 16 == 0) {
+#pragma unroll
+        for (int i = 0; i < DPL / 16; ++i) {
+            const u32x4 q = reinterpret_cast<const SGM_GLOBAL u32x4*>(src)[i];
+            w[4 * i] = q.x;
+            w[4 * i + 1] = q.y;
+            w[4 * i + 2] = q.z;
+            w[4 * i + 3] = q.w;
+        }
+    } else if constexpr (DPL % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < DPL / 8; ++i) {
+            const u32x2 q = reinterpret_cast<const SGM_GLOBAL u32x2*>(src)[i];
+            w[2 * i] = q.x;
+            w[2 * i + 1] = q.y;
+        }
+    } else if constexpr (DPL % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) w[i] = reinterpret_cast<const SGM_GLOBAL uint32_t*>(src)[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < DPL / 2; ++i) {
+            const uint32_t h = reinterpret_cast<const SGM_GLOBAL uint16_t*>(src)[i];
+            w[i / 2] = (i & 1) ? (w[i / 2] | (h << 16)) : h;
+        }
+    }
+}
+
+// Adds the bytes of `w` (disparities k0..k0+DPL-1) to the packed u16 sums.
+template <int DPL>
+__device__ __forceinline__ void add_raw(const uint32_t (&w)[(DPL + 3) / 4], uint32_t (&s)[DPL / 2]) {
+#pragma unroll
+    for (int i = 0; i < DPL / 4; ++i) {
+        s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, w[i], 0x0C010C00u));
+        s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, w[i], 0x0C030C02u));
+    }
+    if constexpr (DPL % 4 == 2) {
+        const uint32_t q = w[DPL / 4];
+        s[DPL / 2 - 1] = pk_add(s[DPL / 2 - 1], (q & 0xFFu) | ((q & 0xFF00u) << 8));
+    }
+}
+
+// Everything one step of a path reads. Recomputed costs (CV = false): the left census word, the
+// lane's DPL right census words (lowest address first, i.e. disparity k0+DPL-1 first) and the guide
+// intensity. Cost volume (CV = true): the lane's DPL cost bytes and the guide intensity.
+template <int DPL, bool CV>
+struct StepIn;
+template <int DPL>
+struct StepIn<DPL, false> {
     uint64_t a;
     uint64_t w[DPL];
     int I;
 };
-
 template <int DPL>
-__device__ __forceinline__ void load_step(StepIn<DPL>& s, const SGM_GLOBAL uint64_t* cl, const SGM_GLOBAL uint64_t* cr,
-                                          const SGM_GLOBAL uint8_t* gd) {
-    s.a = *cl;
+struct StepIn<DPL, true> {
+    uint32_t c[(DPL + 3) / 4];
+    int I;
+};
+
+// Load pointers of a walk; census pointers for recomputed costs, the cost-volume pointer otherwise.
+struct WalkPtrs {
+    const SGM_GLOBAL uint64_t* cl;
+    const SGM_GLOBAL uint64_t* cr;
+    const SGM_GLOBAL uint8_t* cv;
+    const SGM_GLOBAL uint8_t* gd;
+};
+
+template <int DPL, bool CV>
+__device__ __forceinline__ void load_step(StepIn<DPL, CV>& s, const WalkPtrs& p) {
+    if constexpr (CV) {
+        load_raw<DPL>(p.cv, s.c);
+    } else {
+        s.a = *p.cl;
 #pragma unroll
-    for (int j = 0; j < DPL; ++j) s.w[j] = cr[j];
-    s.I = *gd;
+        for (int j = 0; j < DPL; ++j) s.w[j] = p.cr[j];
+    }
+    s.I = *p.gd;
 }
 
 // Packed costs of the lane's disparities k0 + 2r, k0 + 2r + 1 (low, high half). Word w[DPL-1-j]
 // belongs to disparity k0 + j. With `full` every window word lies in the image (the group-uniform
 // common case); otherwise out-of-image words give the invalid cost (stereo.cpp:374-376).
 template <int DPL>
-__device__ __forceinline__ void step_costs(const StepIn<DPL>& s, bool full, int xr_top, int w, uint32_t inv,
-                                           uint32_t (&cc)[DPL / 2]) {
+__device__ __forceinline__ void census_costs(uint64_t a, const uint64_t (&w)[DPL], bool full, int xr_top, int width,
+                                             uint32_t inv, uint32_t (&cc)[DPL / 2]) {
 #pragma unroll
     for (int r = 0; r < DPL / 2; ++r) {
-        const uint32_t ca = (uint32_t)__popcll(s.a ^ s.w[DPL - 1 - 2 * r]);
-        const uint32_t cb = (uint32_t)__popcll(s.a ^ s.w[DPL - 2 - 2 * r]);
+        const uint32_t ca = (uint32_t)__popcll(a ^ w[DPL - 1 - 2 * r]);
+        const uint32_t cb = (uint32_t)__popcll(a ^ w[DPL - 2 - 2 * r]);
         cc[r] = ca | (cb << 16);
     }
     if (!full) {
 #pragma unroll
         for (int r = 0; r < DPL / 2; ++r) {
             const int xa = xr_top - 2 * r, xb = xa - 1;
-            const uint32_t lo = (unsigned)xa < (unsigned)w ? (cc[r] & 0xFFFFu) : inv;
-            const uint32_t hi = (unsigned)xb < (unsigned)w ? (cc[r] >> 16) : inv;
+            const uint32_t lo = (unsigned)xa < (unsigned)width ? (cc[r] & 0xFFFFu) : inv;
+            const uint32_t hi = (unsigned)xb < (unsigned)width ? (cc[r] >> 16) : inv;
             cc[r] = lo | (hi << 16);
         }
+    }
+}
+
+template <int DPL, bool CV>
+__device__ __forceinline__ void step_costs(const StepIn<DPL, CV>& s, bool full, int xr_top, int width, uint32_t inv,
+                                           uint32_t (&cc)[DPL / 2]) {
+    if constexpr (CV) {
+        // The cost kernel already wrote the invalid cost outside the image.
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) {
+            cc[2 * i] = __builtin_amdgcn_perm(0u, s.c[i], 0x0C010C00u);
+            cc[2 * i + 1] = __builtin_amdgcn_perm(0u, s.c[i], 0x0C030C02u);
+        }
+        if constexpr (DPL % 4 == 2) {
+            const uint32_t q = s.c[DPL / 4];
+            cc[DPL / 2 - 1] = (q & 0xFFu) | ((q & 0xFF00u) << 8);
+        }
+    } else {
+        census_costs<DPL>(s.a, s.w, full, xr_top, width, inv, cc);
     }
 }
 
@@ -269,18 +358,10 @@ __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint3
 
 __device__ __forceinline__ uint32_t pair_min_row(uint32_t mn) { return row_min(min(mn & 0xFFFFu, mn >> 16)); }
 
-template <int DPL, bool PAD>
-__global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
+// Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
+template <int DPL, bool PAD, bool CV>
+__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const int32_t* p2q) {
     constexpr int R = DPL / 2;
-    __shared__ int32_t p2q[32];
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < 32; ++q) p2q[q] = c.p2_by_q[q];
-    }
-    __syncthreads();
-    int pi, lb;
-    locate(t, blockIdx.x, pi, lb);
-    const PairDesc P = load_desc(t, pi);
     const int line = lb * kLinesPerBlock + (threadIdx.x >> 4);
     if (line >= P.total_lines) return;
     int dir = 0, off = 0;
@@ -335,30 +416,36 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
     const int64_t step_c = (int64_t)dy * cp + dx;
     const int64_t step_g = (int64_t)dy * P.pitch + dx;
     const int64_t step_l = ((int64_t)dy * W + dx) * P.dp;
-    const SGM_GLOBAL uint64_t* lcl = gptr(P.census_l) + (int64_t)y * cp + P.census_padl + x;
-    const SGM_GLOBAL uint64_t* lcr = gptr(P.census_r) + (int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1));
-    const SGM_GLOBAL uint8_t* lgd = gptr(P.left) + (int64_t)y * P.pitch + x;
+    WalkPtrs wp;
+    wp.cl = gptr(P.census_l) + (int64_t)y * cp + P.census_padl + x;
+    wp.cr = gptr(P.census_r) + (int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1));
+    wp.cv = gptr(P.cvol) + ((int64_t)y * W + x) * P.dp + k0;
+    wp.gd = gptr(P.left) + (int64_t)y * P.pitch + x;
     SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + (int64_t)dir * P.lvol_dir_stride + ((int64_t)y * W + x) * P.dp + k0;
 
-    StepIn<DPL> A, B;
-    load_step<DPL>(A, lcl, lcr, lgd);
+    StepIn<DPL, CV> A, B;
+    load_step<DPL, CV>(A, wp);
     // Advance the load pointers while pixels remain (the last pixel is reloaded, harmlessly).
     auto advance = [&](int i) {
         if (i + 1 < n) {
-            lcl += step_c;
-            lcr += step_c;
-            lgd += step_g;
+            if constexpr (CV) {
+                wp.cv += step_l;
+            } else {
+                wp.cl += step_c;
+                wp.cr += step_c;
+            }
+            wp.gd += step_g;
         }
     };
     advance(0);
-    load_step<DPL>(B, lcl, lcr, lgd);
+    load_step<DPL, CV>(B, wp);
     advance(1);
 
     uint32_t prev[R], pm;
     int xc = x;
     {
         uint32_t cc[R];
-        step_costs<DPL>(A, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+        step_costs<DPL, CV>(A, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -369,28 +456,76 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
         pm = pair_min_row(mn);
     }
     int Iprev = A.I;
-    auto step = [&](const StepIn<DPL>& S) {
+    auto step = [&](const StepIn<DPL, CV>& S) {
         xc += dx;
         lv += step_l;
         const int dI = S.I > Iprev ? S.I - Iprev : Iprev - S.I;  // stereo.cpp:62-65, |dI|/8 <= 31
         const int p2 = adaptive ? p2q[dI >> 3] : p2_fixed;
         Iprev = S.I;
         uint32_t cc[R];
-        step_costs<DPL>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+        step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
         uint32_t mn;
         path_step<DPL, PAD>(prev, cc, pm, p2, p1p1, nslant, padm, mn);
         store_bytes<DPL>(lv, prev);
         pm = pair_min_row(mn);
     };
     for (int i = 1; i < n; i += 2) {
-        load_step<DPL>(A, lcl, lcr, lgd);  // pixel i+1 (prefetch)
+        load_step<DPL, CV>(A, wp);  // pixel i+1 (prefetch)
         advance(i + 1);
-        step(B);                           // pixel i
+        step(B);                    // pixel i
         if (i + 1 >= n) break;
-        load_step<DPL>(B, lcl, lcr, lgd);  // pixel i+2
+        load_step<DPL, CV>(B, wp);  // pixel i+2
         advance(i + 2);
-        step(A);                           // pixel i+1
+        step(A);                    // pixel i+1
     }
+}
+
+__device__ __forceinline__ void load_p2q(const SgmConsts& c, int32_t* p2q) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) p2q[q] = c.p2_by_q[q];
+    }
+}
+
+template <int DPL, bool PAD, bool CV>
+__global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
+    __shared__ int32_t p2q[32];
+    load_p2q(c, p2q);
+    __syncthreads();
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    paths_body<DPL, PAD, CV>(P, c, lb, p2q);
+}
+
+// -------------------------------------------------------------------------------- cost volume
+// Matching costs as bytes, [y][x][dp], for the path kernel's cost-volume mode: one 16-lane group per
+// pixel with the path kernel's lane layout; the cost fill of stereo.cpp:366-380 (invalid outside).
+
+template <int DPL>
+__device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c, int lb) {
+    const int64_t pix = (int64_t)lb * kLinesPerBlock + (threadIdx.x >> 4);
+    if (pix >= (int64_t)P.w * P.h) return;
+    const int y = (int)(pix / P.w), x = (int)(pix - (int64_t)y * P.w);
+    const int k0 = (threadIdx.x & (kGroup - 1)) * DPL;
+    const int64_t row = (int64_t)y * P.census_pitch + P.census_padl;
+    const uint64_t a = gptr(P.census_l)[row + x];
+    const SGM_GLOBAL uint64_t* cr = gptr(P.census_r) + row + (x - P.min_d - k0 - (DPL - 1));
+    uint64_t w[DPL];
+#pragma unroll
+    for (int j = 0; j < DPL; ++j) w[j] = cr[j];
+    const bool full = x >= P.min_d + P.dp - 1 && x <= P.w - 1 + P.min_d;
+    uint32_t cc[DPL / 2];
+    census_costs<DPL>(a, w, full, x - P.min_d - k0, P.w, (uint32_t)c.invalid_cost, cc);
+    store_bytes<DPL>(gptr(P.cvol) + pix * P.dp + k0, cc);
+}
+
+template <int DPL>
+__global__ __launch_bounds__(kBlock) void cost8_kernel(LaunchTable t, SgmConsts c) {
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    cost8_body<DPL>(P, c, lb);
 }
 
 // -------------------------------------------------------------------------------------- rows
@@ -401,51 +536,6 @@ __device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {
     if (denom <= 0.0f) return 0.0f;
     const float v = 0.5f * (cm - cp) / denom;
     return v < -0.5f ? -0.5f : (0.5f < v ? 0.5f : v);
-}
-
-// Raw path-cost bytes of one pixel for one direction: DPL bytes, as (DPL+3)/4 words.
-template <int DPL>
-__device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t (&w)[(DPL + 3) / 4]) {
-    if constexpr (DPL % 16 == 0) {
-#pragma unroll
-        for (int i = 0; i < DPL / 16; ++i) {
-            const u32x4 q = reinterpret_cast<const SGM_GLOBAL u32x4*>(src)[i];
-            w[4 * i] = q.x;
-            w[4 * i + 1] = q.y;
-            w[4 * i + 2] = q.z;
-            w[4 * i + 3] = q.w;
-        }
-    } else if constexpr (DPL % 8 == 0) {
-#pragma unroll
-        for (int i = 0; i < DPL / 8; ++i) {
-            const u32x2 q = reinterpret_cast<const SGM_GLOBAL u32x2*>(src)[i];
-            w[2 * i] = q.x;
-            w[2 * i + 1] = q.y;
-        }
-    } else if constexpr (DPL % 4 == 0) {
-#pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) w[i] = reinterpret_cast<const SGM_GLOBAL uint32_t*>(src)[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < DPL / 2; ++i) {
-            const uint32_t h = reinterpret_cast<const SGM_GLOBAL uint16_t*>(src)[i];
-            w[i / 2] = (i & 1) ? (w[i / 2] | (h << 16)) : h;
-        }
-    }
-}
-
-// Adds the bytes of `w` (disparities k0..k0+DPL-1) to the packed u16 sums.
-template <int DPL>
-__device__ __forceinline__ void add_raw(const uint32_t (&w)[(DPL + 3) / 4], uint32_t (&s)[DPL / 2]) {
-#pragma unroll
-    for (int i = 0; i < DPL / 4; ++i) {
-        s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, w[i], 0x0C010C00u));
-        s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, w[i], 0x0C030C02u));
-    }
-    if constexpr (DPL % 4 == 2) {
-        const uint32_t q = w[DPL / 4];
-        s[DPL / 2 - 1] = pk_add(s[DPL / 2 - 1], (q & 0xFFu) | ((q & 0xFF00u) << 8));
-    }
 }
 
 template <int DPL>
@@ -470,12 +560,10 @@ __device__ __forceinline__ uint32_t row_max(uint32_t v) {
     return v;
 }
 
+// Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
+// 9 bytes per column.
 template <int DPL, bool PAD>
-__global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
-    extern __shared__ uint32_t smem[];
-    int pi, lb;
-    locate(t, blockIdx.x, pi, lb);
-    const PairDesc P = load_desc(t, pi);
+__device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, uint32_t* smem) {
     const int W = P.w, y = lb;
     uint32_t* rkey = smem;
     float* drow = reinterpret_cast<float*>(smem + W);
@@ -596,13 +684,19 @@ __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c
     }
 }
 
-// ------------------------------------------------------------------------------------ median
-// median3, stereo.cpp:129-147: the n/2-th smallest (0-based) of the valid in-image 3x3 values.
-
-__global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
+template <int DPL, bool PAD>
+__global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
+    extern __shared__ uint32_t smem[];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
+    rows_body<DPL, PAD>(P, c, lb, smem);
+}
+
+// ------------------------------------------------------------------------------------ median
+// median3, stereo.cpp:129-147: the n/2-th smallest (0-based) of the valid in-image 3x3 values.
+
+__device__ __forceinline__ void median_body(const PairDesc& P, int lb) {
     const int tx_n = (P.w + kTile - 1) / kTile;
     const int x = (lb % tx_n) * kTile + (threadIdx.x & (kTile - 1));
     const int y = (lb / tx_n) * kTile + (threadIdx.x >> 4);
@@ -643,6 +737,13 @@ __global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
     P.median[(int64_t)y * P.w + x] = out;
 }
 
+__global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
+    int pi, lb;
+    locate(t, blockIdx.x, pi, lb);
+    const PairDesc P = load_desc(t, pi);
+    median_body(P, lb);
+}
+
 // -------------------------------------------------------------------------------------- copy
 // Plain 16-byte-per-lane copy: the known-size reference for HBM traffic counters.
 
@@ -681,6 +782,7 @@ void fill_lines(PairDesc& d) {
 int census_blocks(int w, int h) { return 2 * ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
 int path_blocks(const PairDesc& d) { return (d.total_lines + kLinesPerBlock - 1) / kLinesPerBlock; }
 int row_blocks(const PairDesc& d) { return d.h; }
+int cost8_blocks(int w, int h) { return (int)(((int64_t)w * h + kLinesPerBlock - 1) / kLinesPerBlock); }
 int median_blocks(int w, int h) { return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
 
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
@@ -697,11 +799,32 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
 
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
+    const bool cv = (t.n == 1 ? t.one.cvol : nullptr) != nullptr || t.cost_volume;
     switch (dpl) {
-#define SGM_CASE(D)                                                                                             \
-    case D:                                                                                                     \
-        if (padded) hipLaunchKernelGGL((paths_kernel<D, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);  \
-        else hipLaunchKernelGGL((paths_kernel<D, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);        \
+#define SGM_CASE(D)                                                                                                  \
+    case D:                                                                                                          \
+        if (cv) {                                                                                                    \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);  \
+            else hipLaunchKernelGGL((paths_kernel<D, false, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);        \
+        } else {                                                                                                     \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c); \
+            else hipLaunchKernelGGL((paths_kernel<D, false, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);       \
+        }                                                                                                            \
+        break;
+        SGM_FOR_DPL(SGM_CASE)
+#undef SGM_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    switch (dpl) {
+#define SGM_CASE(D)                                                                                     \
+    case D:                                                                                             \
+        hipLaunchKernelGGL((cost8_kernel<D>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);     \
         break;
         SGM_FOR_DPL(SGM_CASE)
 #undef SGM_CASE

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_PKG, "libsgm_hip.so")
 INVALID_DISPARITY = np.float32(-1.0e9)
 NO_MATCH = np.int32(-(2**31))
 STATUS_WINDOW, STATUS_UNIQUE, STATUS_LR, STATUS_VALID = 1, 2, 4, 7
+OPT_COST_VOLUME = 1
 
 ERRC = {0: "ok", 1: "not_found", 2: "io", 3: "timeout", 4: "protocol", 5: "invalid_argument", 6: "blocked",
         7: "unsupported", 8: "busy", 9: "disconnected"}
@@ -138,6 +139,7 @@ def lib():
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
         L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
+        L.sgm_set_option.argtypes = [vp, i32, C.c_int64]
         L.sgm_reset_stage_totals.argtypes = [vp]
         L.sgm_device_bytes.argtypes = [vp]
         L.sgm_device_bytes.restype = C.c_size_t
@@ -304,6 +306,10 @@ class HipStereo:
         t = CSgmStageTotals()
         _check(lib().sgm_get_stage_totals(self._h, C.byref(t)), ctx=self._h)
         return {f: getattr(t, f) for f, _ in CSgmStageTotals._fields_}
+
+    def set_option(self, option: int, value: int):
+        """Tuning knob (OPT_COST_VOLUME): changes speed and memory, never results."""
+        _check(lib().sgm_set_option(self._h, option, value), ctx=self._h)
 
     def reset_stage_totals(self):
         _check(lib().sgm_reset_stage_totals(self._h), ctx=self._h)

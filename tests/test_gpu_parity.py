@@ -147,3 +147,26 @@ def test_size_errors_on_device():
         with pytest.raises(sgm.SgmError) as e:
             st.compute(np.zeros((40, 32), np.uint8), np.zeros((40, 32), np.uint8))
         assert e.value.errc == "invalid_argument"
+
+
+CV_CASES = [(64, 48, 16, 51, dict()), (64, 48, 24, 52, dict(eight_paths=True, census_radius_x=4, census_radius_y=3,
+            p1=10, p2=120)), (40, 30, 16, 53, dict(min_disparity=-20, eight_paths=True)), (20, 12, 64, 54, dict()),
+            (230, 40, 192, 55, dict(eight_paths=True)), (136, 24, 96, 56, dict(eight_paths=True)),
+            (552, 24, 512, 57, dict(eight_paths=True)), (1, 40, 16, 58, dict(eight_paths=True))]
+
+
+@pytest.mark.parametrize("w,h,nd,seed,kw", CV_CASES, ids=[f"cv-{c[0]}x{c[1]}x{c[2]}" for c in CV_CASES])
+def test_cost_volume_mode_bit_exact(w, h, nd, seed, kw):
+    """The cost-volume mode (costs written once as bytes) gives the same bytes as recomputation."""
+    pair = speckle_pair(w, h, max(nd, 6), seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd)
+    params.update(kw)
+    print(f"cost-volume case {w}x{h} D={nd} {params}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params))
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, 1)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS)
+        st.set_option(sgm.OPT_COST_VOLUME, 0)
+        again = st.compute(pair.left, pair.right, outputs=FIELDS)
+    assert_same(ref, got, FIELDS, tag=f"cv {w}x{h}x{nd}")
+    assert_same(ref, again, FIELDS, tag=f"recompute after cv {w}x{h}x{nd}")
