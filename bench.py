@@ -75,6 +75,7 @@ def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=No
                 for c in sv.poll(64, 0):
                     lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
                     done_t.append(c["t_done_ns"])
+                    sv.release(c)
                     got += 1
                 time.sleep(0.0001)
             sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=("disparity", "disparity_median"), tag=i)
@@ -83,6 +84,7 @@ def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=No
             for c in sv.poll(64, 200000):
                 lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
                 done_t.append(c["t_done_ns"])
+                sv.release(c)
                 got += 1
         span_s = (max(done_t) - arrival[0]) / 1e9
         lat = np.array(lat)

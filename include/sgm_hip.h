@@ -167,6 +167,9 @@ int sgm_dev_memset(void* dst, int value, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel (16-byte aligned pointers and size); also the
  * known-size reference for HBM traffic counters. Blocks until done. */
 int sgm_dev_copy(void* dst, const void* src, size_t bytes);
+/* Pinned (page-locked) host memory: device copies to and from it are direct DMA transfers. */
+int sgm_host_alloc(size_t bytes, void** out);
+int sgm_host_free(void* ptr);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes);
 
@@ -209,8 +212,9 @@ void sgm_batch_destroy(sgm_batch* b);
  * and `slots` streams with worker threads. A submitted request (host images, copied at submission,
  * so the caller may reuse them) waits in arrival order; the head is admitted when a slot is free and
  * its device footprint fits the arena. Frames arriving while others run start as soon as a slot and
- * memory free up and overlap on the device. Results are written to the request's host output
- * buffers, which must stay valid until its completion is polled. All functions are thread-safe.
+ * memory free up and overlap on the device. Results are copied straight into the request's host
+ * output buffers (pinned ones, e.g. from sgm_host_alloc, take a direct DMA), which must stay valid
+ * until its completion is polled. All functions are thread-safe.
  * ------------------------------------------------------------------------------------------ */
 
 typedef struct {

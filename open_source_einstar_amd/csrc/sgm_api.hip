@@ -357,6 +357,26 @@ int sgm_dev_copy(void* dst, const void* src, size_t bytes) {
     return SGM_OK;
 }
 
+int sgm_host_alloc(size_t bytes, void** out) {
+    if (!out) return SGM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_host_alloc", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_host_free(void* ptr) {
+    const hipError_t e = hipHostFree(ptr);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_host_free", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
 int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
     size_t f = 0, t = 0;
     const hipError_t e = hipMemGetInfo(&f, &t);

@@ -499,33 +499,57 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
 }
 
 // -------------------------------------------------------------------------------- cost volume
-// Matching costs as bytes, [y][x][dp], for the path kernel's cost-volume mode: one 16-lane group per
-// pixel with the path kernel's lane layout; the cost fill of stereo.cpp:366-380 (invalid outside).
+// Matching costs as bytes, [y][x][dp], for the path kernel's cost-volume mode (the cost fill of
+// stereo.cpp:366-380, invalid outside the image). A block covers kCostSeg consecutive pixels of a
+// row: the right-image census words all of them need (dp + kCostSeg - 1) are staged in LDS once;
+// each 16-lane group then takes pixels grp, grp + 16 with the path kernel's lane layout.
+
+constexpr int kCostSeg = 32;
+// LDS position of window word i: one spare word after every 8, so that the 16 lanes of a group,
+// whose windows start 8 words apart, read 16 different bank pairs (a 72-byte lane stride).
+__device__ __forceinline__ int win_pos(int i) { return i + (i >> 3); }
+constexpr int kWinWords = (512 + kCostSeg) + (512 + kCostSeg) / 8 + 1;
+
+__host__ __device__ inline int cost8_segments(int w) { return (w + kCostSeg - 1) / kCostSeg; }
 
 template <int DPL>
-__device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c, int lb) {
-    const int64_t pix = (int64_t)lb * kLinesPerBlock + (threadIdx.x >> 4);
-    if (pix >= (int64_t)P.w * P.h) return;
-    const int y = (int)(pix / P.w), x = (int)(pix - (int64_t)y * P.w);
-    const int k0 = (threadIdx.x & (kGroup - 1)) * DPL;
+__device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c, int lb, uint64_t* win) {
+    const int nseg = cost8_segments(P.w);
+    const int y = lb / nseg, x0 = (lb - y * nseg) * kCostSeg;
     const int64_t row = (int64_t)y * P.census_pitch + P.census_padl;
-    const uint64_t a = gptr(P.census_l)[row + x];
-    const SGM_GLOBAL uint64_t* cr = gptr(P.census_r) + row + (x - P.min_d - k0 - (DPL - 1));
-    uint64_t w[DPL];
+    const int base = x0 - P.min_d - (P.dp - 1);  // xr of win[0]
+    const int nwin = P.dp + kCostSeg - 1;
+    const int lo = -P.census_padl, hi = P.census_pitch - P.census_padl - 1;  // addressable xr of the row
+    for (int i = threadIdx.x; i < nwin; i += kBlock) {
+        const int xr = base + i;
+        win[win_pos(i)] = (xr >= lo && xr <= hi) ? gptr(P.census_r)[row + xr] : 0ull;
+    }
+    __syncthreads();
+    const int grp = threadIdx.x >> 4;
+    const int k0 = (threadIdx.x & (kGroup - 1)) * DPL;
 #pragma unroll
-    for (int j = 0; j < DPL; ++j) w[j] = cr[j];
-    const bool full = x >= P.min_d + P.dp - 1 && x <= P.w - 1 + P.min_d;
-    uint32_t cc[DPL / 2];
-    census_costs<DPL>(a, w, full, x - P.min_d - k0, P.w, (uint32_t)c.invalid_cost, cc);
-    store_bytes<DPL>(gptr(P.cvol) + pix * P.dp + k0, cc);
+    for (int rep = 0; rep < kCostSeg / kLinesPerBlock; ++rep) {
+        const int x = x0 + grp + rep * kLinesPerBlock;
+        if (x >= P.w) break;
+        const uint64_t a = gptr(P.census_l)[row + x];
+        const int i0 = x - P.min_d - k0 - (DPL - 1) - base;  // win index of the lane's lowest word
+        uint64_t w[DPL];
+#pragma unroll
+        for (int j = 0; j < DPL; ++j) w[j] = win[win_pos(i0 + j)];
+        const bool full = x >= P.min_d + P.dp - 1 && x <= P.w - 1 + P.min_d;
+        uint32_t cc[DPL / 2];
+        census_costs<DPL>(a, w, full, x - P.min_d - k0, P.w, (uint32_t)c.invalid_cost, cc);
+        store_bytes<DPL>(gptr(P.cvol) + ((int64_t)y * P.w + x) * P.dp + k0, cc);
+    }
 }
 
 template <int DPL>
 __global__ __launch_bounds__(kBlock) void cost8_kernel(LaunchTable t, SgmConsts c) {
+    __shared__ uint64_t win[kWinWords];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    cost8_body<DPL>(P, c, lb);
+    cost8_body<DPL>(P, c, lb, win);
 }
 
 // -------------------------------------------------------------------------------------- rows
@@ -782,7 +806,7 @@ void fill_lines(PairDesc& d) {
 int census_blocks(int w, int h) { return 2 * ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
 int path_blocks(const PairDesc& d) { return (d.total_lines + kLinesPerBlock - 1) / kLinesPerBlock; }
 int row_blocks(const PairDesc& d) { return d.h; }
-int cost8_blocks(int w, int h) { return (int)(((int64_t)w * h + kLinesPerBlock - 1) / kLinesPerBlock); }
+int cost8_blocks(int w, int h) { return h * cost8_segments(w); }
 int median_blocks(int w, int h) { return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
 
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {

@@ -44,7 +44,7 @@ size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 // Byte offsets of one request's buffers inside its arena block.
 struct Footprint {
-    size_t img, census, lvol, disp, median, conf, dint, status, rdisp, best, second, total;
+    size_t img, census, cvol, lvol, disp, median, conf, dint, status, rdisp, best, second, total;
 };
 
 Footprint footprint(int w, int h, int min_d, int nd, int ndirs) {
@@ -56,6 +56,8 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs) {
     o = align_up(o + 2 * npx);
     f.census = o;
     o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8);
+    f.cvol = o;  // matching costs as bytes: the path kernel's cost-volume mode
+    o = align_up(o + npx * dp);
     f.lvol = o;
     o = align_up(o + (size_t)ndirs * npx * dp);
     f.disp = o;
@@ -81,7 +83,8 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs) {
 struct Request {
     uint64_t ticket = 0, tag = 0;
     int w = 0, h = 0, min_d = 0, nd = 0;
-    std::vector<uint8_t> img;  // left rows then right rows, pitch w
+    uint8_t* pinned = nullptr;  // left rows then right rows (pitch w) in a pooled pinned buffer, or
+    std::vector<uint8_t> img;   // the same in heap memory when the pool was empty
     sgm_outputs out{};         // caller's host buffers
     int64_t t_submit = 0, t_admit = 0;
     size_t bytes = 0, offset = 0;
@@ -104,11 +107,12 @@ struct sgm_server {
     bool stopping = false;
     std::string err;
 
+    std::vector<uint8_t*> pinned_free;  // pinned input buffers (2 * max_w * max_h bytes each)
+    std::vector<uint8_t*> pinned_all;
+
     struct Slot {
         hipStream_t stream = nullptr;
         hipEvent_t ev[5] = {};
-        uint8_t* pin_in = nullptr;
-        uint8_t* pin_out = nullptr;
         std::thread th;
         std::condition_variable cv;
         bool has_job = false;
@@ -152,6 +156,7 @@ void sgm_server::worker(int si) {
         d.census_l = reinterpret_cast<uint64_t*>(base + f.census);
         d.census_r = d.census_l + census_words_ext(r.w, r.h, r.min_d, r.nd);
         d.lvol = base + f.lvol;
+        d.cvol = base + f.cvol;
         set_geometry_ext(d, r.w, r.h, r.min_d, r.nd, ndirs);
         d.disp = reinterpret_cast<float*>(base + f.disp);
         d.median = r.out.disparity_median ? reinterpret_cast<float*>(base + f.median) : nullptr;
@@ -173,26 +178,18 @@ void sgm_server::worker(int si) {
             {r.out.status, f.status, npx},            {r.out.right_disparity, f.rdisp, npx * 4},
             {r.out.best_cost, f.best, npx * 2},       {r.out.second_cost, f.second, npx * 2},
         };
-        std::memcpy(s.pin_in, r.img.data(), 2 * npx);
-        hipError_t e = hipMemcpyAsync(base + f.img, s.pin_in, 2 * npx, hipMemcpyHostToDevice, s.stream);
+        // Inputs from the pooled pinned buffer (or heap); results straight into the caller's buffers
+        // (a direct DMA when they are pinned, e.g. from sgm_host_alloc).
+        const uint8_t* src = r.pinned ? r.pinned : r.img.data();
+        hipError_t e = hipMemcpyAsync(base + f.img, src, 2 * npx, hipMemcpyHostToDevice, s.stream);
         if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, s.stream, s.ev);
-        size_t pin_off = 0;
         for (const Copy& c : copies) {
             if (!c.host || e != hipSuccess) continue;
-            e = hipMemcpyAsync(s.pin_out + pin_off, base + c.dev_off, c.bytes, hipMemcpyDeviceToHost, s.stream);
-            pin_off += align_up(c.bytes);
+            e = hipMemcpyAsync(c.host, base + c.dev_off, c.bytes, hipMemcpyDeviceToHost, s.stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
         float gpu_ms = 0;
-        if (e == hipSuccess) {
-            pin_off = 0;
-            for (const Copy& c : copies) {
-                if (!c.host) continue;
-                std::memcpy(c.host, s.pin_out + pin_off, c.bytes);
-                pin_off += align_up(c.bytes);
-            }
-            (void)hipEventElapsedTime(&gpu_ms, s.ev[0], s.ev[4]);
-        }
+        if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, s.ev[0], s.ev[4]);
         sgm_completion cpl{};
         cpl.ticket = r.ticket;
         cpl.user_tag = r.tag;
@@ -208,6 +205,7 @@ void sgm_server::worker(int si) {
                 ++failed;
             }
             ++completed;
+            if (r.pinned) pinned_free.push_back(r.pinned);
             reqs.erase(r.ticket);
             s.has_job = false;
             aq.finish(si, r.offset);
@@ -235,10 +233,9 @@ int sgm_server_stop(sgm_server* sv) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         for (hipEvent_t e : s.ev)
             if (e) (void)hipEventDestroy(e);
-        if (s.pin_in) (void)hipHostFree(s.pin_in);
-        if (s.pin_out) (void)hipHostFree(s.pin_out);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
+    for (uint8_t* b : sv->pinned_all) (void)hipHostFree(b);
     if (sv->d_arena) (void)hipFree(sv->d_arena);
     delete sv;
     return SGM_OK;
@@ -286,8 +283,14 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     for (auto& s : sv->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
         for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&s.ev[i]);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s.pin_in), 2 * npx, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s.pin_out), npx * 32, hipHostMallocDefault);
+    }
+    for (int i = 0; e == hipSuccess && i < 2 * cfg->slots + 2; ++i) {
+        uint8_t* b = nullptr;
+        e = hipHostMalloc(reinterpret_cast<void**>(&b), 2 * npx, hipHostMallocDefault);
+        if (e == hipSuccess) {
+            sv->pinned_all.push_back(b);
+            sv->pinned_free.push_back(b);
+        }
     }
     if (e != hipSuccess) {
         set_thread_error(std::string("sgm_server_start: slots: ") + hipGetErrorString(e));
@@ -320,18 +323,30 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
     r.nd = rq->num_disparities;
     r.out = rq->out;
     r.bytes = footprint(r.w, r.h, r.min_d, r.nd, sv->ndirs).total;
-    r.img.resize(2 * (size_t)r.w * r.h);
-    for (int y = 0; y < r.h; ++y) {
-        std::memcpy(&r.img[(size_t)y * r.w], rq->left + (ptrdiff_t)y * rq->stride, (size_t)r.w);
-        std::memcpy(&r.img[((size_t)r.h + y) * r.w], rq->right + (ptrdiff_t)y * rq->stride, (size_t)r.w);
-    }
     r.t_submit = now_ns();
-    std::lock_guard lk(sv->mu);
-    if (sv->stopping) {
-        sv->err = "sgm_server_submit: server is stopping";
-        return SGM_ERR_DISCONNECTED;
+    {
+        std::lock_guard lk(sv->mu);
+        if (!sv->pinned_free.empty()) {
+            r.pinned = sv->pinned_free.back();
+            sv->pinned_free.pop_back();
+        }
     }
-    if (!sv->aq.can_ever_fit(r.bytes)) {
+    uint8_t* dst = r.pinned;
+    if (!dst) {
+        r.img.resize(2 * (size_t)r.w * r.h);
+        dst = r.img.data();
+    }
+    for (int y = 0; y < r.h; ++y) {
+        std::memcpy(dst + (size_t)y * r.w, rq->left + (ptrdiff_t)y * rq->stride, (size_t)r.w);
+        std::memcpy(dst + ((size_t)r.h + y) * r.w, rq->right + (ptrdiff_t)y * rq->stride, (size_t)r.w);
+    }
+    std::lock_guard lk(sv->mu);
+    if (sv->stopping || !sv->aq.can_ever_fit(r.bytes)) {
+        if (r.pinned) sv->pinned_free.push_back(r.pinned);
+        if (sv->stopping) {
+            sv->err = "sgm_server_submit: server is stopping";
+            return SGM_ERR_DISCONNECTED;
+        }
         sv->err = "sgm_server_submit: request needs " + std::to_string(r.bytes) + " bytes, more than the HBM arena";
         return SGM_ERR_BUSY;
     }

@@ -150,6 +150,8 @@ def lib():
         L.sgm_dev_memset.argtypes = [vp, i32, C.c_size_t]
         L.sgm_dev_copy.argtypes = [vp, vp, C.c_size_t]
         L.sgm_dev_mem_info.argtypes = [P(C.c_size_t), P(C.c_size_t)]
+        L.sgm_host_alloc.argtypes = [C.c_size_t, P(vp)]
+        L.sgm_host_free.argtypes = [vp]
         L.sgm_batch_create.argtypes = [P(CSgmParams), i32, P(CSgmPair), P(vp)]
         L.sgm_batch_run.argtypes = [vp, vp]
         L.sgm_batch_run_plain.argtypes = [vp, vp]
@@ -402,10 +404,41 @@ class Batch:
             pass
 
 
+class PinnedPool:
+    """Reusable pinned host arrays (sgm_host_alloc), so device copies into them are direct DMA."""
+
+    def __init__(self):
+        self._free = {}   # (nbytes) -> [ptr]
+        self._all = []
+
+    def take(self, shape, dtype) -> np.ndarray:
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        lst = self._free.get(nbytes)
+        if lst:
+            ptr = lst.pop()
+        else:
+            p = C.c_void_p()
+            _check(lib().sgm_host_alloc(nbytes, C.byref(p)))
+            ptr = p.value
+            self._all.append(ptr)
+        buf = (C.c_uint8 * nbytes).from_address(ptr)
+        a = np.frombuffer(buf, dtype=dtype).reshape(shape)
+        return a
+
+    def give(self, a: np.ndarray):
+        self._free.setdefault(a.nbytes, []).append(a.ctypes.data)
+
+    def free_all(self):
+        for p in self._all:
+            lib().sgm_host_free(p)
+        self._all, self._free = [], {}
+
+
 class Server:
     """The serving loop (sgm_server_*): submit host pairs as they arrive, poll completions.
 
-    Output buffers of a request are numpy arrays owned here until its completion is polled."""
+    Output arrays live in pinned host memory owned by this object; a completion's arrays stay valid
+    until `release(completion)` returns them to the pool (or the server stops)."""
 
     def __init__(self, params: SgmParams, max_width: int, max_height: int, max_disparities: int, slots: int = 4,
                  hbm_budget_bytes: int = 0):
@@ -416,7 +449,8 @@ class Server:
         if rc != 0:
             raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
         self._h = h.value
-        self._pending = {}  # ticket -> (outputs dict, keep-alive)
+        self._pending = {}  # ticket -> outputs dict
+        self._pool = PinnedPool()
 
     @staticmethod
     def now_ns() -> int:
@@ -427,7 +461,7 @@ class Server:
         left = np.ascontiguousarray(left, dtype=np.uint8)
         right = np.ascontiguousarray(right, dtype=np.uint8)
         h, w = left.shape
-        res = {f: np.empty((h, w), OUTPUT_DTYPES[f]) for f in set(outputs) | {"disparity"}}
+        res = {f: self._pool.take((h, w), OUTPUT_DTYPES[f]) for f in set(outputs) | {"disparity"}}
         rq = CSgmRequest()
         rq.left, rq.right, rq.stride = left.ctypes.data, right.ctypes.data, w
         rq.width, rq.height, rq.min_disparity, rq.num_disparities = w, h, min_disparity, num_disparities
@@ -437,6 +471,8 @@ class Server:
         t = C.c_uint64()
         rc = lib().sgm_server_submit(self._h, C.byref(rq), C.byref(t))
         if rc != 0:
+            for a in res.values():
+                self._pool.give(a)
             raise SgmError(rc, (lib().sgm_server_last_error(self._h) or b"").decode())
         self._pending[t.value] = res
         return t.value
@@ -454,6 +490,12 @@ class Server:
                         "outputs": self._pending.pop(c.ticket, None)})
         return out
 
+    def release(self, completion: dict):
+        """Returns a polled completion's output arrays to the pinned pool (they must not be used after)."""
+        for a in (completion.get("outputs") or {}).values():
+            self._pool.give(a)
+        completion["outputs"] = None
+
     def stats(self) -> dict:
         st = CSgmServerStats()
         _check(lib().sgm_server_get_stats(self._h, C.byref(st)))
@@ -463,6 +505,7 @@ class Server:
         if getattr(self, "_h", None):
             lib().sgm_server_stop(self._h)
             self._h = None
+            self._pool.free_all()
 
     def __enter__(self):
         return self
