@@ -240,7 +240,7 @@ def main():
                 run()
             b.synchronize()
             res[mode] = (time.perf_counter() - t) * 1e3 / args.batch_reps
-        batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1),
+        batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1), "plan": b.plan(),
                    "batched_ms": round(res["graph"], 3), "plain_ms": round(res["plain"], 3),
                    "batched_value": round(total_pxd / (res["graph"] * 1e-3) / 1e6, 0),
                    "plain_value": round(total_pxd / (res["plain"] * 1e-3) / 1e6, 0),

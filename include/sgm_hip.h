@@ -148,8 +148,9 @@ int sgm_reset_stage_totals(sgm_context* ctx);
 
 /* Tuning knobs: they change speed and memory, never results. */
 enum {
-    SGM_OPT_COST_VOLUME = 1 /* 1: matching costs written once as bytes and read by every path
-                               (W*H*dp more bytes of HBM); 0 (default): recomputed from census words */
+    SGM_OPT_COST_VOLUME = 1, /* 1 (default): matching costs written once as bytes and read by every
+                                path (W*H*dp bytes of HBM); 0: recomputed from census words in each path */
+    SGM_OPT_PATH_LINES = 2   /* scanlines per path-kernel block: 4, 8 or 16 (default) */
 };
 int sgm_set_option(sgm_context* ctx, int option, int64_t value);
 
@@ -191,10 +192,29 @@ typedef struct {
 
 typedef struct sgm_batch sgm_batch;
 
-/* Plans a batch of n pairs: allocates the scratch of all pairs, picks the launch schedule with the
- * autotuner (cached per shape class) and captures the launches in a hipGraph. The pair images and
- * outputs must stay valid until sgm_batch_destroy; their contents may change between runs. */
+/* Launch plan knobs: -1 = let the autotuner choose. None changes any result. */
+typedef struct {
+    int32_t cost_volume; /* 0: costs recomputed in the path kernel; 1: written once as bytes */
+    int32_t path_lines;  /* scanlines per path-kernel block: 4, 8 or 16 */
+    int32_t branches;    /* 0: one stream; 1: a graph branch per lane class; 2: two per class */
+} sgm_batch_tuning;
+
+/* The plan a batch runs and how it was chosen. */
+typedef struct {
+    int32_t cost_volume, path_lines, branches;
+    int32_t candidates_timed; /* plans timed at creation (0 when taken from the cache) */
+    double best_ms, worst_ms; /* replay time of the fastest and slowest candidate */
+    int32_t from_cache;
+} sgm_batch_plan;
+
+/* Plans a batch of n pairs: allocates the scratch of all pairs, times every candidate plan on the
+ * batch itself (a graph replay each) and keeps the fastest, cached per batch signature for the
+ * process; the chosen launches are captured in a hipGraph. The pair images and outputs must stay
+ * valid until sgm_batch_destroy; their contents may change between runs. */
 int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out);
+int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pairs, const sgm_batch_tuning* tuning,
+                           sgm_batch** out);
+int sgm_batch_get_plan(const sgm_batch* b, sgm_batch_plan* out);
 /* Replays the graph on `stream` (hipStream_t; NULL = the batch's own stream) without waiting. */
 int sgm_batch_run(sgm_batch* b, void* stream);
 /* The same work launched pair by pair without batching or graphs: the "plain" reference point. */

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,7 +86,7 @@ void dfree(void* p) {
 }
 
 // Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
-hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st) {
+hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 16) {
     LaunchTable t{};
     t.one = d;
     t.n = 1;
@@ -102,8 +103,8 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
         if ((e = launch_cost8(d.dpl, t, c, s)) != hipSuccess) return e;
     }
     if (st && (e = hipEventRecord(st->ev[1], s)) != hipSuccess) return e;
-    t.total_blocks = path_blocks(d);
-    if ((e = launch_paths(d.dpl, padded, t, c, s)) != hipSuccess) return e;
+    t.total_blocks = path_blocks(d, path_lines);
+    if ((e = launch_paths(d.dpl, padded, t, c, path_lines, s)) != hipSuccess) return e;
     if (st && (e = hipEventRecord(st->ev[2], s)) != hipSuccess) return e;
     t.total_blocks = row_blocks(d);
     if ((e = launch_rows(d.dpl, padded, t, c, d.w, s)) != hipSuccess) return e;
@@ -180,6 +181,7 @@ struct sgm_context {
     uint16_t* d_second = nullptr;
     uint8_t* d_cvol = nullptr;   // u8 matching costs when the cost-volume mode is on
     bool use_cvol = false;
+    int path_lines = 16;         // scanlines per path-kernel block
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     size_t bytes = 0;
@@ -260,6 +262,8 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
     ctx->census_half = census_words(max_width, max_height, params->min_disparity, params->num_disparities);
     if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_cvol, npx * dp, b);  // cost-volume mode is the default
+    ctx->use_cvol = true;
     if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_conf, npx, b);
@@ -291,6 +295,11 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
             ctx->use_cvol = value != 0;
             return SGM_OK;
         }
+        case SGM_OPT_PATH_LINES:
+            if (value != 4 && value != 8 && value != 16)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "path lines per block must be 4, 8 or 16");
+            ctx->path_lines = (int)value;
+            return SGM_OK;
         default:
             return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));
     }
@@ -482,7 +491,7 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
     const bool padded = d.dp != d.nd;
-    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx));
+    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx), ctx->path_lines);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
@@ -551,7 +560,7 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
     o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
     o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
-    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, next_stages(ctx));
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, next_stages(ctx), ctx->path_lines);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute: launch", e);
     struct Copy {
         void* host;
@@ -588,35 +597,44 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
 }  // extern "C"
 
 // ------------------------------------------------------------------------------------ batches
+//
+// A batch runs as one hipGraph: a census launch over every pair; per lane group (pairs sharing a
+// lane layout, optionally split in two by work) a cost launch, a path launch and a row launch, each
+// covering all the group's pairs through a flat block -> (pair, block) table; groups run as
+// parallel graph branches; one median launch closes the graph. How it is laid out (cost mode,
+// scanlines per path block, branches) is the plan; sgm_batch_create times candidate plans on the
+// batch itself and keeps the fastest (cached per batch signature for the process).
+
+namespace {
+
+struct BatchPlan {
+    int cost_volume = 0;  // 0: recompute costs in the path kernel, 1: cost volume
+    int path_lines = 16;  // scanlines per path-kernel block
+    int branches = 1;     // 0: one stream, 1: a branch per lane class, 2: two per class
+};
+
+std::mutex g_plan_mu;
+std::map<std::string, BatchPlan> g_plan_cache;
+
+}  // namespace
 
 struct sgm_batch {
     sgm_params p{};
     SgmConsts consts{};
     int n = 0;
     int ndirs = 4;
-    std::vector<PairDesc> desc;  // host copies
-    PairDesc* d_desc = nullptr;
-    struct Class {
-        int dpl = 0;
-        bool padded = false;
-        std::vector<int> pairs;
-        int32_t* d_first_paths = nullptr;
-        int32_t* d_first_rows = nullptr;
-        PairDesc* d_desc = nullptr;  // this class's descriptors, contiguous
-        int paths_blocks = 0, rows_blocks = 0;
-    };
-    std::vector<Class> classes;
-    int32_t* d_first_census = nullptr;
-    int32_t* d_first_median = nullptr;
-    PairDesc* d_desc_median = nullptr;
-    int census_total = 0, median_total = 0, n_median = 0;
+    std::vector<PairDesc> desc;  // host copies; cvol points at each pair's cost volume
     uint8_t* d_scratch = nullptr;
     size_t bytes = 0;
     hipStream_t stream = nullptr;
-    std::vector<hipStream_t> branch;
-    std::vector<hipEvent_t> fork_ev, join_ev;
+    std::vector<hipStream_t> branch;  // extra streams for parallel branches
+    std::vector<hipEvent_t> events;
+    BatchPlan plan;
+    std::vector<void*> plan_allocs;  // device tables of the current plan's graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    sgm_batch_plan info{};
+    std::string signature;
     hipStream_t last_stream = nullptr;
     std::string err;
     int fail(int code, std::string m) {
@@ -627,9 +645,8 @@ struct sgm_batch {
 
 namespace {
 
-// Uploads a flat-block prefix table and a descriptor array; returns the launch table.
-hipError_t upload_table(const std::vector<PairDesc>& ds, const std::vector<int>& blocks, PairDesc** d_desc,
-                        int32_t** d_first, size_t& bytes, LaunchTable& t) {
+// Uploads a descriptor array (or inlines a single pair) and its flat-block prefix table.
+hipError_t make_table(sgm_batch* b, const std::vector<PairDesc>& ds, const std::vector<int>& blocks, LaunchTable& t) {
     t = LaunchTable{};
     t.n = (int)ds.size();
     std::vector<int32_t> first(ds.size() + 1, 0);
@@ -639,86 +656,156 @@ hipError_t upload_table(const std::vector<PairDesc>& ds, const std::vector<int>&
         t.one = ds[0];
         return hipSuccess;
     }
-    hipError_t e = dalloc(d_desc, ds.size(), bytes);
-    if (e == hipSuccess) e = dalloc(d_first, first.size(), bytes);
-    if (e == hipSuccess) e = hipMemcpy(*d_desc, ds.data(), ds.size() * sizeof(PairDesc), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(*d_first, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-    t.many = *d_desc;
-    t.first_block = *d_first;
+    PairDesc* dd = nullptr;
+    int32_t* df = nullptr;
+    size_t unused = 0;
+    hipError_t e = dalloc(&dd, ds.size(), unused);
+    if (e == hipSuccess) b->plan_allocs.push_back(dd);
+    if (e == hipSuccess) e = dalloc(&df, first.size(), unused);
+    if (e == hipSuccess) b->plan_allocs.push_back(df);
+    if (e == hipSuccess) e = hipMemcpy(dd, ds.data(), ds.size() * sizeof(PairDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(df, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    t.many = dd;
+    t.first_block = df;
     return e;
 }
 
-struct BatchTables {
-    LaunchTable census, median;
-    std::vector<LaunchTable> paths, rows;
+struct Group {
+    int dpl = 0;
+    bool padded = false;
+    int max_w = 1;
+    std::vector<PairDesc> ds;
+    LaunchTable cost, paths, rows;
 };
 
-hipError_t build_tables(sgm_batch* b, BatchTables& bt) {
+void drop_plan(sgm_batch* b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    for (void* q : b->plan_allocs) dfree(q);
+    b->plan_allocs.clear();
+}
+
+// Builds and instantiates the graph of `plan`.
+hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
+    drop_plan(b);
+    b->plan = plan;
+    std::vector<PairDesc> all = b->desc;
+    for (PairDesc& d : all)
+        if (!plan.cost_volume) d.cvol = nullptr;
+    // Lane classes, each split in `parts` chunks of about equal work.
+    std::map<std::pair<int, bool>, std::vector<int>> cls;
+    for (int i = 0; i < b->n; ++i) cls[{all[(size_t)i].dpl, all[(size_t)i].dp != all[(size_t)i].nd}].push_back(i);
+    std::vector<Group> groups;
+    const int parts = plan.branches == 2 ? 2 : 1;
+    for (auto& [key, idx] : cls) {
+        // Longest work first inside a launch: its scanlines start earliest.
+        std::sort(idx.begin(), idx.end(), [&](int x, int y) {
+            return (int64_t)all[(size_t)x].w * all[(size_t)x].h * all[(size_t)x].nd > (int64_t)all[(size_t)y].w * all[(size_t)y].h * all[(size_t)y].nd;
+        });
+        const int k = std::min<int>(parts, (int)idx.size());
+        std::vector<Group> gs((size_t)k);
+        std::vector<int64_t> work((size_t)k, 0);
+        for (int i : idx) {  // greedy: next pair to the lightest chunk
+            const size_t g = (size_t)(std::min_element(work.begin(), work.end()) - work.begin());
+            work[g] += (int64_t)all[(size_t)i].w * all[(size_t)i].h * all[(size_t)i].nd;
+            gs[g].ds.push_back(all[(size_t)i]);
+        }
+        for (Group& g : gs) {
+            g.dpl = key.first;
+            g.padded = key.second;
+            for (const PairDesc& d : g.ds) g.max_w = std::max(g.max_w, d.w);
+            groups.push_back(std::move(g));
+        }
+    }
+    hipError_t e = hipSuccess;
+    for (Group& g : groups) {
+        std::vector<int> cb, pb, rb;
+        for (const PairDesc& d : g.ds) {
+            cb.push_back(cost8_blocks(d.w, d.h));
+            pb.push_back(path_blocks(d, plan.path_lines));
+            rb.push_back(row_blocks(d));
+        }
+        if (e == hipSuccess && plan.cost_volume) e = make_table(b, g.ds, cb, g.cost);
+        if (e == hipSuccess) e = make_table(b, g.ds, pb, g.paths);
+        if (e == hipSuccess) e = make_table(b, g.ds, rb, g.rows);
+    }
+    LaunchTable census{}, median{};
     std::vector<int> blocks;
-    for (const PairDesc& d : b->desc) blocks.push_back(census_blocks(d.w, d.h));
-    hipError_t e = upload_table(b->desc, blocks, &b->d_desc, &b->d_first_census, b->bytes, bt.census);
-    if (e != hipSuccess) return e;
+    for (const PairDesc& d : all) blocks.push_back(census_blocks(d.w, d.h));
+    if (e == hipSuccess) e = make_table(b, all, blocks, census);
     std::vector<PairDesc> med;
     blocks.clear();
-    for (const PairDesc& d : b->desc)
+    for (const PairDesc& d : all)
         if (d.median) {
             med.push_back(d);
             blocks.push_back(median_blocks(d.w, d.h));
         }
-    b->n_median = (int)med.size();
-    if (!med.empty() && (e = upload_table(med, blocks, &b->d_desc_median, &b->d_first_median, b->bytes, bt.median)) != hipSuccess)
-        return e;
-    for (auto& c : b->classes) {
-        std::vector<PairDesc> ds;
-        std::vector<int> pb, rb;
-        for (int i : c.pairs) {
-            ds.push_back(b->desc[(size_t)i]);
-            pb.push_back(path_blocks(b->desc[(size_t)i]));
-            rb.push_back(row_blocks(b->desc[(size_t)i]));
-        }
-        LaunchTable tp, tr;
-        PairDesc* dd = nullptr;
-        if ((e = upload_table(ds, pb, &dd, &c.d_first_paths, b->bytes, tp)) != hipSuccess) return e;
-        c.d_desc = dd;
-        if ((e = upload_table(ds, rb, &dd, &c.d_first_rows, b->bytes, tr)) != hipSuccess) return e;
-        if (dd && dd != c.d_desc) {
-            // The rows table reuses the paths descriptors: free the duplicate.
-            dfree(dd);
-            b->bytes -= ds.size() * sizeof(PairDesc);
-        }
-        tr.many = tp.many;
-        bt.paths.push_back(tp);
-        bt.rows.push_back(tr);
+    if (e == hipSuccess && !med.empty()) e = make_table(b, med, blocks, median);
+    // Streams and events for the branches.
+    const size_t nbr = plan.branches == 0 ? 1 : groups.size();
+    while (e == hipSuccess && b->branch.size() + 1 < nbr) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) b->branch.push_back(st);
     }
-    return hipSuccess;
+    while (e == hipSuccess && b->events.size() < nbr + 1) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->events.push_back(ev);
+    }
+    if (e != hipSuccess) return e;
+    hipStream_t s = b->stream;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = launch_census(census, b->consts, s);
+    if (le == hipSuccess && nbr > 1) {
+        le = hipEventRecord(b->events[0], s);
+        for (size_t i = 1; le == hipSuccess && i < nbr; ++i) le = hipStreamWaitEvent(b->branch[i - 1], b->events[0], 0);
+    }
+    for (size_t gi = 0; le == hipSuccess && gi < groups.size(); ++gi) {
+        const Group& g = groups[gi];
+        hipStream_t gs = (nbr == 1 || gi == 0) ? s : b->branch[gi - 1];
+        if (plan.cost_volume) le = launch_cost8(g.dpl, g.cost, b->consts, gs);
+        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, gs);
+        if (le == hipSuccess) le = launch_rows(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
+    }
+    for (size_t i = 1; le == hipSuccess && nbr > 1 && i < nbr; ++i) {
+        le = hipEventRecord(b->events[i], b->branch[i - 1]);
+        if (le == hipSuccess) le = hipStreamWaitEvent(s, b->events[i], 0);
+    }
+    if (le == hipSuccess && !med.empty()) le = launch_median(median, s);
+    e = hipStreamEndCapture(s, &b->graph);
+    if (le != hipSuccess) return le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    return e;
 }
 
-hipError_t enqueue_batch(sgm_batch* b, const BatchTables& bt, hipStream_t s) {
-    hipError_t e = launch_census(bt.census, b->consts, s);
-    if (e != hipSuccess) return e;
-    const size_t nc = b->classes.size();
-    // Classes run as parallel branches (fork/join through events): their pairs are independent.
-    if (nc > 1) {
-        if ((e = hipEventRecord(b->fork_ev[0], s)) != hipSuccess) return e;
-        for (size_t i = 1; i < nc; ++i)
-            if ((e = hipStreamWaitEvent(b->branch[i], b->fork_ev[0], 0)) != hipSuccess) return e;
-    }
-    for (size_t i = 0; i < nc; ++i) {
-        hipStream_t cs = i == 0 ? s : b->branch[i];
-        const auto& c = b->classes[i];
-        if ((e = launch_paths(c.dpl, c.padded, bt.paths[i], b->consts, cs)) != hipSuccess) return e;
-        int maxw = 1;
-        for (int p : c.pairs) maxw = std::max(maxw, b->desc[(size_t)p].w);
-        if ((e = launch_rows(c.dpl, c.padded, bt.rows[i], b->consts, maxw, cs)) != hipSuccess) return e;
-    }
-    if (nc > 1) {
-        for (size_t i = 1; i < nc; ++i) {
-            if ((e = hipEventRecord(b->join_ev[i], b->branch[i])) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(s, b->join_ev[i], 0)) != hipSuccess) return e;
-        }
-    }
-    if (b->n_median > 0 && (e = launch_median(bt.median, s)) != hipSuccess) return e;
-    return hipSuccess;
+// Mean time of one replay of the current plan's graph (after one warm-up replay).
+hipError_t time_plan(sgm_batch* b, int reps, float* ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipGraphLaunch(b->exec, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+    for (int i = 0; e == hipSuccess && i < reps; ++i) e = hipGraphLaunch(b->exec, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (e == hipSuccess) *ms /= (float)reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return e;
+}
+
+std::string batch_signature(const sgm_params& p, int n, const sgm_pair* pairs) {
+    std::string k = std::to_string(p.eight_paths) + ":" + std::to_string(p.census_radius_x) + "x" + std::to_string(p.census_radius_y);
+    for (int i = 0; i < n; ++i)
+        k += ";" + std::to_string(pairs[i].width) + "x" + std::to_string(pairs[i].height) + "x" + std::to_string(pairs[i].num_disparities) +
+             "@" + std::to_string(pairs[i].min_disparity) + (pairs[i].out.disparity_median ? "m" : "");
+    return k;
 }
 
 }  // namespace
@@ -727,22 +814,10 @@ extern "C" {
 
 void sgm_batch_destroy(sgm_batch* b) {
     if (!b) return;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (auto& c : b->classes) {
-        dfree(c.d_first_paths);
-        dfree(c.d_first_rows);
-        dfree(c.d_desc);
-    }
-    dfree(b->d_first_census);
-    dfree(b->d_first_median);
-    dfree(b->d_desc_median);
-    dfree(b->d_desc);
+    drop_plan(b);
     dfree(b->d_scratch);
-    for (hipEvent_t e : b->fork_ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : b->join_ev) (void)hipEventDestroy(e);
-    for (size_t i = 1; i < b->branch.size(); ++i) (void)hipStreamDestroy(b->branch[i]);
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -750,7 +825,14 @@ void sgm_batch_destroy(sgm_batch* b) {
 const char* sgm_batch_last_error(const sgm_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
 size_t sgm_batch_device_bytes(const sgm_batch* b) { return b ? b->bytes : 0; }
 
-int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out) {
+int sgm_batch_get_plan(const sgm_batch* b, sgm_batch_plan* out) {
+    if (!b || !out) return SGM_ERR_INVALID_ARGUMENT;
+    *out = b->info;
+    return SGM_OK;
+}
+
+int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pairs, const sgm_batch_tuning* tuning,
+                           sgm_batch** out) {
     if (!out) {
         g_last_error = "out is NULL";
         return SGM_ERR_INVALID_ARGUMENT;
@@ -765,6 +847,13 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
         g_last_error = "batch size must be in [1, " + std::to_string(kMaxBatch) + "]";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    sgm_batch_tuning tn{-1, -1, -1};
+    if (tuning) tn = *tuning;
+    if ((tn.cost_volume < -1 || tn.cost_volume > 1) || (tn.path_lines != -1 && tn.path_lines != 4 && tn.path_lines != 8 && tn.path_lines != 16) ||
+        (tn.branches < -1 || tn.branches > 2)) {
+        g_last_error = "tuning: cost_volume in {-1,0,1}, path_lines in {-1,4,8,16}, branches in {-1,0,1,2}";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     for (int i = 0; i < n; ++i) {
         const sgm_pair& q = pairs[i];
         sgm_params pp = *params;
@@ -775,8 +864,8 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
             g_last_error = "pair " + std::to_string(i) + ": " + (err.empty() ? "bad size, pointer or pitch" : err);
             return SGM_ERR_INVALID_ARGUMENT;
         }
-        if (q.out.census_left || q.out.census_right) {
-            g_last_error = "pair " + std::to_string(i) + ": census outputs are only available from sgm_compute";
+        if (q.out.census_left || q.out.census_right || q.out.cost_volume) {
+            g_last_error = "pair " + std::to_string(i) + ": census and cost-volume outputs are only available from sgm_compute";
             return SGM_ERR_UNSUPPORTED;
         }
     }
@@ -785,16 +874,19 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
     b->consts = make_consts(*params);
     b->n = n;
     b->ndirs = params->eight_paths ? 8 : 4;
-    // Scratch: census words (2 per pixel) and path costs (ndirs * dp per pixel) of every pair.
-    std::vector<size_t> off_census(n), off_lvol(n);
+    // Scratch per pair: census words (2 images, padded rows), u8 cost volume, u8 path costs.
+    std::vector<size_t> off_census(n), off_cvol(n), off_lvol(n);
     size_t total = 0;
     for (int i = 0; i < n; ++i) {
         const size_t npx = (size_t)pairs[i].width * pairs[i].height;
+        const size_t dp = (size_t)dpl_for(pairs[i].num_disparities) * kGroup;
         off_census[i] = total;
-        total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 +
-                  255) & ~(size_t)255;
+        total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 + 255) &
+                 ~(size_t)255;
+        off_cvol[i] = total;
+        total += (npx * dp + 255) & ~(size_t)255;
         off_lvol[i] = total;
-        total += ((size_t)b->ndirs * npx * (size_t)dpl_for(pairs[i].num_disparities) * kGroup + 255) & ~(size_t)255;
+        total += ((size_t)b->ndirs * npx * dp + 255) & ~(size_t)255;
     }
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = dalloc(&b->d_scratch, total, b->bytes);
@@ -803,7 +895,6 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
         sgm_batch_destroy(b);
         return SGM_ERR_IO;
     }
-    std::map<std::pair<int, bool>, int> class_of;
     for (int i = 0; i < n; ++i) {
         const sgm_pair& q = pairs[i];
         PairDesc d{};
@@ -812,6 +903,7 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
         d.pitch = q.pitch;
         d.census_l = reinterpret_cast<uint64_t*>(b->d_scratch + off_census[i]);
         d.census_r = d.census_l + census_words(q.width, q.height, q.min_disparity, q.num_disparities);
+        d.cvol = b->d_scratch + off_cvol[i];
         d.lvol = b->d_scratch + off_lvol[i];
         set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);
         d.disp = q.out.disparity;
@@ -825,40 +917,68 @@ int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm
         d.sum_volume = q.out.sum_volume;
         d.cost_volume = nullptr;
         b->desc.push_back(d);
-        const auto key = std::make_pair(d.dpl, d.dp != d.nd);
-        auto it = class_of.find(key);
-        if (it == class_of.end()) {
-            it = class_of.emplace(key, (int)b->classes.size()).first;
-            sgm_batch::Class c;
-            c.dpl = key.first;
-            c.padded = key.second;
-            b->classes.push_back(c);
+    }
+    // Candidate plans: every combination of the knobs left open.
+    std::vector<BatchPlan> cands;
+    for (int cv : {0, 1})
+        for (int pl : {4, 8, 16})
+            for (int br : {0, 1, 2}) {
+                if ((tn.cost_volume >= 0 && cv != tn.cost_volume) || (tn.path_lines > 0 && pl != tn.path_lines) ||
+                    (tn.branches >= 0 && br != tn.branches))
+                    continue;
+                if (tn.branches < 0 && br == 0) continue;  // one stream only when asked for
+                cands.push_back({cv, pl, br});
+            }
+    b->signature = batch_signature(*params, n, pairs);
+    BatchPlan best = cands.front();
+    float best_ms = 0, worst_ms = 0;
+    bool cached = false;
+    if (cands.size() > 1) {
+        std::lock_guard lk(g_plan_mu);
+        auto it = g_plan_cache.find(b->signature);
+        if (it != g_plan_cache.end()) {
+            best = it->second;
+            cached = true;
         }
-        b->classes[(size_t)it->second].pairs.push_back(i);
     }
-    b->branch.assign(b->classes.size(), nullptr);
-    b->branch[0] = b->stream;
-    for (size_t i = 1; e == hipSuccess && i < b->classes.size(); ++i) e = hipStreamCreateWithFlags(&b->branch[i], hipStreamNonBlocking);
-    b->fork_ev.assign(1, nullptr);
-    b->join_ev.assign(b->classes.size(), nullptr);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork_ev[0], hipEventDisableTiming);
-    for (size_t i = 0; e == hipSuccess && i < b->classes.size(); ++i) e = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
-    BatchTables bt;
-    if (e == hipSuccess) e = build_tables(b, bt);
-    if (e == hipSuccess) e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        const hipError_t le = enqueue_batch(b, bt, b->stream);
-        e = hipStreamEndCapture(b->stream, &b->graph);
-        if (le != hipSuccess) e = le;
+    int timed = 0;
+    if (!cached && cands.size() > 1) {
+        for (const BatchPlan& pl : cands) {
+            float ms = 0;
+            e = build_plan(b, pl);
+            if (e == hipSuccess) e = time_plan(b, 2, &ms);
+            if (e != hipSuccess) break;
+            ++timed;
+            if (timed == 1 || ms < best_ms) {
+                best_ms = ms;
+                best = pl;
+            }
+            worst_ms = std::max(worst_ms, ms);
+        }
+        if (e == hipSuccess) {
+            std::lock_guard lk(g_plan_mu);
+            g_plan_cache[b->signature] = best;
+        }
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = build_plan(b, best);
     if (e != hipSuccess) {
-        g_last_error = hip_msg("sgm_batch_create: tables/graph", e);
+        g_last_error = hip_msg("sgm_batch_create: plan/graph", e);
         sgm_batch_destroy(b);
         return SGM_ERR_IO;
     }
+    b->info.cost_volume = best.cost_volume;
+    b->info.path_lines = best.path_lines;
+    b->info.branches = best.branches;
+    b->info.candidates_timed = timed;
+    b->info.best_ms = best_ms;
+    b->info.worst_ms = worst_ms;
+    b->info.from_cache = cached ? 1 : 0;
     *out = b;
     return SGM_OK;
+}
+
+int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out) {
+    return sgm_batch_create_tuned(params, n, pairs, nullptr, out);
 }
 
 int sgm_batch_run(sgm_batch* b, void* stream) {
@@ -873,8 +993,10 @@ int sgm_batch_run(sgm_batch* b, void* stream) {
 int sgm_batch_run_plain(sgm_batch* b, void* stream) {
     if (!b) return SGM_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->stream;
-    for (const PairDesc& d : b->desc) {
-        const hipError_t e = enqueue_pair(d, b->consts, d.dp != d.nd, s, nullptr);
+    for (PairDesc d : b->desc) {
+        // The same kernels and knobs as the graph, one pair after another: only batching differs.
+        if (!b->plan.cost_volume) d.cvol = nullptr;
+        const hipError_t e = enqueue_pair(d, b->consts, d.dp != d.nd, s, nullptr, b->plan.path_lines);
         if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_run_plain", e));
     }
     b->last_stream = s;

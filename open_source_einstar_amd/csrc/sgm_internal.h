@@ -69,7 +69,7 @@ struct LaunchTable {
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
-hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, hipStream_t stream);
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
@@ -77,7 +77,7 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);
-int path_blocks(const PairDesc& d);
+int path_blocks(const PairDesc& d, int lines_per_block);
 int row_blocks(const PairDesc& d);
 int cost8_blocks(int w, int h);
 int median_blocks(int w, int h);

@@ -94,6 +94,14 @@ __device__ __forceinline__ void locate(const LaunchTable& t, int b, int& pi, int
 
 __device__ __forceinline__ PairDesc load_desc(const LaunchTable& t, int pi) { return t.n == 1 ? t.one : t.many[pi]; }
 
+// LDS pointers carry their address space so that stage bodies compiled out of line still use
+// ds_* instructions.
+#define SGM_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ SGM_LDS T* lds(T* p) {
+    return (SGM_LDS T*)p;
+}
+
 // ------------------------------------------------------------------------------------ census
 // census_transform, stereo.cpp:328-346: rows outer, columns inner, centre skipped, each compare
 // shifted in at bit 0 (the first compare ends at bit cells-2); border pixels are 0.
@@ -104,7 +112,7 @@ constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
 
 // One 16x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
 // the block calls it (it synchronises once).
-__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, uint8_t* tile) {
+__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint8_t* tile) {
     const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
     const int side = lb / (tx_n * ty_n);
     const int tl = lb - side * tx_n * ty_n;
@@ -126,7 +134,7 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
     if (x >= rx && x < P.w - rx && y >= ry && y < P.h - ry) {
         const uint8_t center = tile[(ly + ry) * kTileW + lx + rx];
         for (int v = 0; v <= 2 * ry; ++v) {
-            const uint8_t* row = &tile[(ly + v) * kTileW + lx];
+            const SGM_LDS uint8_t* row = &tile[(ly + v) * kTileW + lx];
             for (int u = 0; u <= 2 * rx; ++u) {
                 if (u == rx && v == ry) continue;
                 bits = (bits << 1) | (uint64_t)(row[u] < center);
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    census_body(P, c, lb, tile);
+    census_body(P, c, lb, lds(tile));
 }
 
 // ------------------------------------------------------------------------------- cost volume
@@ -360,9 +368,9 @@ __device__ __forceinline__ uint32_t pair_min_row(uint32_t mn) { return row_min(m
 
 // Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
 template <int DPL, bool PAD, bool CV>
-__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const int32_t* p2q) {
+__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const SGM_LDS int32_t* p2q) {
     constexpr int R = DPL / 2;
-    const int line = lb * kLinesPerBlock + (threadIdx.x >> 4);
+    const int line = lb * (int)(blockDim.x >> 4) + (threadIdx.x >> 4);  // 4, 8 or 16 lines per block
     if (line >= P.total_lines) return;
     int dir = 0, off = 0;
 #pragma unroll
@@ -480,7 +488,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     }
 }
 
-__device__ __forceinline__ void load_p2q(const SgmConsts& c, int32_t* p2q) {
+__device__ __forceinline__ void load_p2q(const SgmConsts& c, SGM_LDS int32_t* p2q) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 32; ++q) p2q[q] = c.p2_by_q[q];
@@ -490,12 +498,12 @@ __device__ __forceinline__ void load_p2q(const SgmConsts& c, int32_t* p2q) {
 template <int DPL, bool PAD, bool CV>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
     __shared__ int32_t p2q[32];
-    load_p2q(c, p2q);
+    load_p2q(c, lds(p2q));
     __syncthreads();
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    paths_body<DPL, PAD, CV>(P, c, lb, p2q);
+    paths_body<DPL, PAD, CV>(P, c, lb, lds(p2q));
 }
 
 // -------------------------------------------------------------------------------- cost volume
@@ -513,7 +521,7 @@ constexpr int kWinWords = (512 + kCostSeg) + (512 + kCostSeg) / 8 + 1;
 __host__ __device__ inline int cost8_segments(int w) { return (w + kCostSeg - 1) / kCostSeg; }
 
 template <int DPL>
-__device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c, int lb, uint64_t* win) {
+__device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint64_t* win) {
     const int nseg = cost8_segments(P.w);
     const int y = lb / nseg, x0 = (lb - y * nseg) * kCostSeg;
     const int64_t row = (int64_t)y * P.census_pitch + P.census_padl;
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void cost8_kernel(LaunchTable t, SgmConsts 
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    cost8_body<DPL>(P, c, lb, win);
+    cost8_body<DPL>(P, c, lb, lds(win));
 }
 
 // -------------------------------------------------------------------------------------- rows
@@ -587,11 +595,11 @@ __device__ __forceinline__ uint32_t row_max(uint32_t v) {
 // Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
 // 9 bytes per column.
 template <int DPL, bool PAD>
-__device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, uint32_t* smem) {
+__device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
     const int W = P.w, y = lb;
-    uint32_t* rkey = smem;
-    float* drow = reinterpret_cast<float*>(smem + W);
-    uint8_t* srow = reinterpret_cast<uint8_t*>(smem + 2 * W);
+    SGM_LDS uint32_t* rkey = smem;
+    SGM_LDS float* drow = (SGM_LDS float*)(smem + W);
+    SGM_LDS uint8_t* srow = (SGM_LDS uint8_t*)(smem + 2 * W);
     for (int x = threadIdx.x; x < W; x += kBlock) rkey[x] = 0xFFFFFFFFu;
     __syncthreads();
 
@@ -622,7 +630,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             if (k >= min_k && k < max_k) {
                 const uint32_t kk = (half16(s[j >> 1], j & 1) << 16) | (uint32_t)k;
                 key = min(key, kk);
-                atomicMin(&rkey[x - min_d - k], kk);  // right WTA, stereo.cpp:434-447
+                __hip_atomic_fetch_min(&rkey[x - min_d - k], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA, stereo.cpp:434-447
             }
         }
         key = row_min(key);  // lowest S, then lowest k: the first strict argmin of :422-424
@@ -714,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    rows_body<DPL, PAD>(P, c, lb, smem);
+    rows_body<DPL, PAD>(P, c, lb, lds(smem));
 }
 
 // ------------------------------------------------------------------------------------ median
@@ -804,7 +812,7 @@ void fill_lines(PairDesc& d) {
 }
 
 int census_blocks(int w, int h) { return 2 * ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
-int path_blocks(const PairDesc& d) { return (d.total_lines + kLinesPerBlock - 1) / kLinesPerBlock; }
+int path_blocks(const PairDesc& d, int lines_per_block) { return (d.total_lines + lines_per_block - 1) / lines_per_block; }
 int row_blocks(const PairDesc& d) { return d.h; }
 int cost8_blocks(int w, int h) { return h * cost8_segments(w); }
 int median_blocks(int w, int h) { return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
@@ -821,18 +829,20 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
+    if (lines_per_block != 4 && lines_per_block != 8 && lines_per_block != 16) return hipErrorInvalidValue;
+    const unsigned threads = (unsigned)(lines_per_block * kGroup);
     const bool cv = (t.n == 1 ? t.one.cvol : nullptr) != nullptr || t.cost_volume;
     switch (dpl) {
 #define SGM_CASE(D)                                                                                                  \
     case D:                                                                                                          \
         if (cv) {                                                                                                    \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);  \
-            else hipLaunchKernelGGL((paths_kernel<D, false, true>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);        \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);  \
+            else hipLaunchKernelGGL((paths_kernel<D, false, true>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);        \
         } else {                                                                                                     \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c); \
-            else hipLaunchKernelGGL((paths_kernel<D, false, false>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);       \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c); \
+            else hipLaunchKernelGGL((paths_kernel<D, false, false>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);       \
         }                                                                                                            \
         break;
         SGM_FOR_DPL(SGM_CASE)

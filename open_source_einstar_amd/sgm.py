@@ -24,6 +24,7 @@ INVALID_DISPARITY = np.float32(-1.0e9)
 NO_MATCH = np.int32(-(2**31))
 STATUS_WINDOW, STATUS_UNIQUE, STATUS_LR, STATUS_VALID = 1, 2, 4, 7
 OPT_COST_VOLUME = 1
+OPT_PATH_LINES = 2
 
 ERRC = {0: "ok", 1: "not_found", 2: "io", 3: "timeout", 4: "protocol", 5: "invalid_argument", 6: "blocked",
         7: "unsupported", 8: "busy", 9: "disconnected"}
@@ -87,6 +88,16 @@ class CSgmCompletion(C.Structure):
 class CSgmServerStats(C.Structure):
     _fields_ = [("queued", C.c_int32), ("in_flight", C.c_int32), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("arena_bytes", C.c_int64), ("arena_free_bytes", C.c_int64)]
+
+
+class CSgmBatchTuning(C.Structure):
+    _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32)]
+
+
+class CSgmBatchPlan(C.Structure):
+    _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32),
+                ("candidates_timed", C.c_int32), ("best_ms", C.c_double), ("worst_ms", C.c_double),
+                ("from_cache", C.c_int32)]
 
 
 class CSgmPair(C.Structure):
@@ -153,6 +164,8 @@ def lib():
         L.sgm_host_alloc.argtypes = [C.c_size_t, P(vp)]
         L.sgm_host_free.argtypes = [vp]
         L.sgm_batch_create.argtypes = [P(CSgmParams), i32, P(CSgmPair), P(vp)]
+        L.sgm_batch_create_tuned.argtypes = [P(CSgmParams), i32, P(CSgmPair), P(CSgmBatchTuning), P(vp)]
+        L.sgm_batch_get_plan.argtypes = [vp, P(CSgmBatchPlan)]
         L.sgm_batch_run.argtypes = [vp, vp]
         L.sgm_batch_run_plain.argtypes = [vp, vp]
         L.sgm_batch_synchronize.argtypes = [vp]
@@ -335,7 +348,8 @@ class PairSpec:
 class Batch:
     """A planned batch (sgm_batch_create): device copies of the pairs, outputs, and the graph."""
 
-    def __init__(self, params: SgmParams, pairs: list[PairSpec]):
+    def __init__(self, params: SgmParams, pairs: list[PairSpec], tuning: dict | None = None):
+        """tuning: cost_volume / path_lines / branches fixed to a value (omitted = autotuned)."""
         self.params = params
         self.specs = pairs
         self._bufs = []
@@ -364,10 +378,18 @@ class Batch:
             self.out_ptrs.append(outs)
         h = C.c_void_p()
         cp = params.to_c()
-        rc = lib().sgm_batch_create(C.byref(cp), len(pairs), cps, C.byref(h))
+        t = tuning or {}
+        tn = CSgmBatchTuning(t.get("cost_volume", -1), t.get("path_lines", -1), t.get("branches", -1))
+        rc = lib().sgm_batch_create_tuned(C.byref(cp), len(pairs), cps, C.byref(tn), C.byref(h))
         if rc != 0:
             raise SgmError(rc, (lib().sgm_batch_last_error(None) or b"").decode())
         self._h = h.value
+
+    def plan(self) -> dict:
+        """The launch plan the autotuner chose (sgm_batch_get_plan)."""
+        pl = CSgmBatchPlan()
+        _check(lib().sgm_batch_get_plan(self._h, C.byref(pl)), batch=self._h)
+        return {f: getattr(pl, f) for f, _ in CSgmBatchPlan._fields_}
 
     def run(self, stream=None):
         _check(lib().sgm_batch_run(self._h, stream), batch=self._h)

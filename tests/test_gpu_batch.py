@@ -53,3 +53,42 @@ def test_single_pair_batch():
         assert_same(refs[0], b.results(0), FIELDS, tag="single")
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("tuning", [dict(cost_volume=0, path_lines=4, branches=0), dict(cost_volume=1, path_lines=8, branches=2),
+                                    dict(cost_volume=1, path_lines=16, branches=1)],
+                         ids=["recompute-4-onestream", "volume-8-split", "volume-16-perclass"])
+def test_every_plan_gives_the_same_bytes(tuning):
+    sizes = [(96, 64, 64), (130, 70, 96), (64, 48, 128), (77, 91, 64), (40, 30, 24)]
+    specs, refs = make_specs(sizes, 3000)
+    b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=tuning)
+    try:
+        plan = b.plan()
+        assert all(plan[k] == v for k, v in tuning.items()) and plan["candidates_timed"] == 0
+        for mode in ("graph", "plain"):
+            b.clear_outputs(0xEE)
+            b.run() if mode == "graph" else b.run_plain()
+            b.synchronize()
+            for i in range(len(sizes)):
+                assert_same(refs[i], b.results(i), FIELDS, tag=f"{tuning} {mode} pair {i}")
+    finally:
+        b.close()
+
+
+def test_autotuner_times_candidates_and_caches_the_choice():
+    sizes = [(96, 64, 64), (64, 48, 128), (40, 30, 24)]
+    specs, refs = make_specs(sizes, 4000)
+    b1 = sgm.Batch(sgm.SgmParams(**BASE), specs)
+    p1 = b1.plan()
+    b1.close()
+    assert p1["candidates_timed"] == 12 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
+    b2 = sgm.Batch(sgm.SgmParams(**BASE), specs)
+    try:
+        p2 = b2.plan()
+        assert p2["from_cache"] == 1 and all(p2[k] == p1[k] for k in ("cost_volume", "path_lines", "branches"))
+        b2.run()
+        b2.synchronize()
+        for i in range(len(sizes)):
+            assert_same(refs[i], b2.results(i), FIELDS, tag=f"tuned pair {i}")
+    finally:
+        b2.close()

@@ -165,8 +165,10 @@ def test_cost_volume_mode_bit_exact(w, h, nd, seed, kw):
     ref = O.sgm(pair.left, pair.right, O.SgmParams(**params))
     with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
         st.set_option(sgm.OPT_COST_VOLUME, 1)
+        st.set_option(sgm.OPT_PATH_LINES, 4)
         got = st.compute(pair.left, pair.right, outputs=FIELDS)
         st.set_option(sgm.OPT_COST_VOLUME, 0)
+        st.set_option(sgm.OPT_PATH_LINES, 8)
         again = st.compute(pair.left, pair.right, outputs=FIELDS)
     assert_same(ref, got, FIELDS, tag=f"cv {w}x{h}x{nd}")
     assert_same(ref, again, FIELDS, tag=f"recompute after cv {w}x{h}x{nd}")
