@@ -96,6 +96,66 @@ def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=No
         sv.stop()
 
 
+def large_batched(sgm, speckle_pair, resident=16, total=64, distinct=8, verify=True):
+    """64 pairs of 2048x1536 with 256 disparities, 8 paths, streamed through a resident batch of
+    `resident` pairs (about 116 GB of path costs and scratch): per chunk, the next pairs' images are
+    uploaded into the batch's input buffers and the graph is replayed. `distinct` synthetic pairs
+    are cycled as inputs."""
+    W5, H5, D5 = 2048, 1536, 256
+    src = [speckle_pair(W5, H5, D5, seed=5000 + i) for i in range(distinct)]
+    specs = [sgm.PairSpec(src[j % distinct].left, src[j % distinct].right, src[j % distinct].min_disparity, D5,
+                          outputs=("disparity", "disparity_int", "status")) for j in range(resident)]
+    t = time.perf_counter()
+    b = sgm.Batch(sgm.SgmParams(**PARAMS), specs)
+    plan_s = time.perf_counter() - t
+    try:
+        def upload(chunk):
+            for j in range(resident):
+                p = src[(chunk * resident + j) % distinct]
+                b.set_inputs(j, p.left, p.right)
+
+        chunks = total // resident
+        b.run()
+        b.synchronize()
+        t = time.perf_counter()
+        for ch in range(chunks):
+            upload(ch)
+            b.run()
+            b.synchronize()
+        streamed_s = time.perf_counter() - t
+        t = time.perf_counter()
+        for ch in range(chunks):
+            b.run()
+        b.synchronize()
+        graph_s = time.perf_counter() - t
+        t = time.perf_counter()
+        for ch in range(chunks):
+            b.run_plain()
+        b.synchronize()
+        plain_s = time.perf_counter() - t
+        pxd = total * W5 * H5 * D5
+        bit_exact = None
+        if verify:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            p0 = src[0]
+            upload(0)
+            b.run()
+            b.synchronize()
+            got = b.results(0)
+            ref = O.sgm(p0.left, p0.right, O.SgmParams(min_disparity=p0.min_disparity, num_disparities=D5, **PARAMS),
+                        threads=min(len(os.sched_getaffinity(0)), 64))
+            bit_exact = all(np.array_equal(np.asarray(ref[f]).view(np.uint8), got[f].view(np.uint8))
+                            for f in ("disparity", "disparity_int", "status"))
+        return {"pairs": total, "resident": resident, "width": W5, "height": H5, "disparities": D5,
+                "device_gb": round(b.device_bytes() / 1e9, 1), "plan": b.plan(), "plan_s": round(plan_s, 2),
+                "streamed_value": round(pxd / streamed_s / 1e6, 0), "graph_value": round(pxd / graph_s / 1e6, 0),
+                "plain_value": round(pxd / plain_s / 1e6, 0), "ratio": round(plain_s / graph_s, 3),
+                "ms_per_pair": round(graph_s * 1e3 / total, 3), "bit_exact_pair0": bit_exact}
+    finally:
+        b.close()
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the path kernel at this workload, from the committed counter summary."""
     path = os.path.join(ROOT, "profiles", "paths_kernel_1280x1024x128_traffic.json")
@@ -116,6 +176,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
+    ap.add_argument("--no-large", action="store_true", help="skip the 64 x 2048x1536x256 streamed batch")
     ap.add_argument("--serve-frames", type=int, default=600, help="frames of the 60 Hz served stream (0: skip)")
     ap.add_argument("--serve-sweep", type=str, default="240,480,720,960",
                     help="arrival rates (Hz) of the saturation sweep, 300 frames each ('' to skip)")
@@ -248,6 +309,11 @@ def main():
         log(f"batched: {batched}")
         b.close()
 
+    large = None
+    if not args.no_large:
+        large = large_batched(sgm, speckle_pair, verify=not args.no_verify)
+        log(f"large batched: {large}")
+
     served = None
     if args.serve_frames > 0:
         full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
@@ -286,6 +352,7 @@ def main():
         "cost_modes_ms_per_step": {"recompute": round(modes[0][0], 4), "volume": round(modes[1][0], 4)},
         "batched": batched,
         "served": served,
+        "large_batched": large,
     }
     print(json.dumps(line, ensure_ascii=False), flush=True)
     for bbuf in outs.values():

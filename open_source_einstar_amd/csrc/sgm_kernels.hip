@@ -624,25 +624,48 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         const int min_k = max(0, x - min_d - (W - 1));
         const int max_k = min(nd, x - min_d + 1);
         uint32_t key = 0xFFFFFFFFu;
+        uint32_t sec = 0xFFFFu;
+        if (!PAD && min_k == 0 && max_k == nd) {
+            // Every disparity is a candidate (most pixels): no per-element window tests.
+            SGM_LDS uint32_t* rk = rkey + (x - min_d - k0 - (DPL - 1));  // rk[DPL-1-j]: right pixel of k0+j
 #pragma unroll
-        for (int j = 0; j < DPL; ++j) {
-            const int k = k0 + j;
-            if (k >= min_k && k < max_k) {
-                const uint32_t kk = (half16(s[j >> 1], j & 1) << 16) | (uint32_t)k;
+            for (int j = 0; j < DPL; ++j) {
+                const uint32_t kk = ((j & 1) ? (s[j >> 1] & 0xFFFF0000u) : (s[j >> 1] << 16)) | (uint32_t)(k0 + j);
                 key = min(key, kk);
-                __hip_atomic_fetch_min(&rkey[x - min_d - k], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA, stereo.cpp:434-447
+                __hip_atomic_fetch_min(&rk[DPL - 1 - j], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA
+            }
+            key = row_min(key);
+            const int bk = (int)(key & 0xFFFFu);
+            // Second best over |k - bk| > 1: the excluded halves are forced to 0xFFFF.
+            uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+            for (int r = 0; r < DPL / 2; ++r) {
+                const int ka = k0 + 2 * r;
+                const uint32_t ex = ((unsigned)(ka - bk + 1) <= 2u ? 0xFFFFu : 0u) | ((unsigned)(ka - bk + 2) <= 2u ? 0xFFFF0000u : 0u);
+                m = pk_min(m, s[r] | ex);
+            }
+            sec = min(m & 0xFFFFu, m >> 16);
+        } else {
+#pragma unroll
+            for (int j = 0; j < DPL; ++j) {
+                const int k = k0 + j;
+                if (k >= min_k && k < max_k) {
+                    const uint32_t kk = (half16(s[j >> 1], j & 1) << 16) | (uint32_t)k;
+                    key = min(key, kk);
+                    __hip_atomic_fetch_min(&rkey[x - min_d - k], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA, stereo.cpp:434-447
+                }
+            }
+            key = row_min(key);  // lowest S, then lowest k: the first strict argmin of :422-424
+            const int bk0 = key == 0xFFFFFFFFu ? -1 : (int)(key & 0xFFFFu);
+#pragma unroll
+            for (int j = 0; j < DPL; ++j) {  // stereo.cpp:425-427
+                const int k = k0 + j;
+                const int dk = k - bk0;
+                if (k >= min_k && k < max_k && (dk > 1 || dk < -1)) sec = min(sec, half16(s[j >> 1], j & 1));
             }
         }
-        key = row_min(key);  // lowest S, then lowest k: the first strict argmin of :422-424
         const int bk = key == 0xFFFFFFFFu ? -1 : (int)(key & 0xFFFFu);
         const uint32_t bS = key >> 16;
-        uint32_t sec = 0xFFFFu;  // stereo.cpp:425-427
-#pragma unroll
-        for (int j = 0; j < DPL; ++j) {
-            const int k = k0 + j;
-            const int dk = k - bk;
-            if (k >= min_k && k < max_k && (dk > 1 || dk < -1)) sec = min(sec, half16(s[j >> 1], j & 1));
-        }
         sec = row_min(sec);
         float d = kInvalid;
         uint8_t st = 0;

@@ -353,6 +353,7 @@ class Batch:
         self.params = params
         self.specs = pairs
         self._bufs = []
+        self._img_index = []  # position in _bufs of each pair's input buffer (left rows, then right)
         cps = (CSgmPair * len(pairs))()
         self.out_ptrs = []
         for i, sp in enumerate(pairs):
@@ -360,6 +361,7 @@ class Batch:
             img = DeviceBuffer(2 * w * h)
             img.upload(np.ascontiguousarray(sp.left, np.uint8), 0)
             img.upload(np.ascontiguousarray(sp.right, np.uint8), w * h)
+            self._img_index.append(len(self._bufs))
             self._bufs.append(img)
             cps[i].left = img.ptr
             cps[i].right = img.ptr + w * h
@@ -399,6 +401,14 @@ class Batch:
 
     def synchronize(self):
         _check(lib().sgm_batch_synchronize(self._h), batch=self._h)
+
+    def set_inputs(self, i: int, left: np.ndarray, right: np.ndarray):
+        """Replaces pair i's images (same size) in device memory; the next run uses them."""
+        h, w = self.specs[i].left.shape
+        assert left.shape == (h, w) and right.shape == (h, w)
+        buf = self._bufs[self._img_index[i]]
+        buf.upload(np.ascontiguousarray(left, np.uint8), 0)
+        buf.upload(np.ascontiguousarray(right, np.uint8), w * h)
 
     def results(self, i: int) -> dict:
         return {f: b.download(shape, dt) for f, (b, shape, dt) in self.out_ptrs[i].items()}
