@@ -170,6 +170,13 @@ struct StereoResult {
     ImageF32 confidence;
 };
 
+// calib::RemapTable (libs/calib/include/einstar/calib/rectify.hpp:40-43): for each rectified pixel,
+// the source pixel in the raw (distorted) frame.
+struct RemapTable {
+    int width = 0, height = 0;
+    ImageF32 map_x, map_y;
+};
+
 struct StereoTimings {
     double gpu_ms = 0;
     double speckle_ms = 0;
@@ -262,6 +269,46 @@ public:
         return res;
     }
 
+    // MetalStereo::set_rectification (metal_stereo.hpp:74-75): raw full-resolution frames + remap
+    // tables at twice the stereo resolution; the tables stay resident on the device.
+    Result<void> set_rectification(const RemapTable& left, const RemapTable& right, int raw_width, int raw_height) {
+        if (left.width / 2 != w_ || left.height / 2 != h_ || right.width != left.width || right.height != left.height)
+            return make_error(Errc::invalid_argument, "remap resolution must be twice the stereo resolution");
+        const int rc = sgm_set_rectification(ctx_, left.map_x.data(), left.map_y.data(), right.map_x.data(),
+                                             right.map_y.data(), left.width, left.height, raw_width, raw_height);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        raw_w_ = raw_width;
+        raw_h_ = raw_height;
+        return {};
+    }
+
+    // MetalStereo::compute_raw (metal_stereo.hpp:79-81): rectify (to half of the remap resolution)
+    // and run stereo; optionally returns the rectified pair.
+    Result<StereoResult> compute_raw(ImageView<const std::uint8_t> raw_left, ImageView<const std::uint8_t> raw_right,
+                                     ImageU8* rect_left = nullptr, ImageU8* rect_right = nullptr) {
+        if (params_.pyramid_levels != 0 || params_.filter.occlusion || params_.filter.spikes ||
+            params_.speckle.max_region_size > 1)
+            return make_error(Errc::unsupported, "only the SGM stage is implemented (pyramid_levels 0, filters off)");
+        if (raw_w_ == 0) return make_error(Errc::invalid_argument, "set_rectification() not called");
+        if (raw_left.width != raw_w_ || raw_left.height != raw_h_ || raw_right.width != raw_w_ || raw_right.height != raw_h_)
+            return make_error(Errc::invalid_argument, "raw size mismatch");
+        if (raw_left.stride != raw_right.stride)
+            return make_error(Errc::invalid_argument, "left and right images must share a stride");
+        StereoResult res;
+        res.disparity = ImageF32(w_, h_);
+        res.confidence = ImageF32(w_, h_);
+        if (rect_left) *rect_left = ImageU8(w_, h_);
+        if (rect_right) *rect_right = ImageU8(w_, h_);
+        sgm_outputs o{};
+        o.disparity = res.disparity.data();
+        o.confidence = res.confidence.data();
+        const int rc = sgm_compute_raw(ctx_, raw_left.data, raw_right.data, raw_left.stride, raw_w_, raw_h_, &o,
+                                       rect_left ? rect_left->data() : nullptr, rect_right ? rect_right->data() : nullptr);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        read_timings();
+        return res;
+    }
+
     // MetalStereo::last_timings (metal_stereo.hpp:97).
     [[nodiscard]] const StereoTimings& last_timings() const { return timings_; }
 
@@ -280,17 +327,22 @@ private:
     Result<void> run(ImageView<const std::uint8_t> l, ImageView<const std::uint8_t> r, const sgm_outputs& o) {
         const int rc = sgm_compute(ctx_, l.data, r.data, l.stride, w_, h_, &o);
         if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        read_timings();
+        return {};
+    }
+
+    void read_timings() {
         sgm_timings t{};
         sgm_last_timings(ctx_, &t);
         timings_.gpu_ms = t.gpu_ms;
         timings_.speckle_ms = 0;
         timings_.total_ms = t.total_ms;
-        return {};
     }
 
     sgm_context* ctx_;
     StereoParams params_;
     int w_, h_;
+    int raw_w_ = 0, raw_h_ = 0;
     StereoTimings timings_;
 };
 

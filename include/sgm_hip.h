@@ -94,6 +94,7 @@ typedef struct {
     double median_ms;
     double gpu_ms;    /* first to last kernel */
     double total_ms;  /* host wall time of the call, including copies */
+    double rectify_ms; /* the fused rectification of sgm_compute_raw*, else 0 */
 } sgm_timings;
 
 /* Per-stage GPU times summed over every call completed since the last reset (count = calls). */
@@ -104,6 +105,7 @@ typedef struct {
     double median_ms;
     double gpu_ms;
     int64_t count;
+    double rectify_ms;
 } sgm_stage_totals;
 
 typedef struct sgm_context sgm_context;
@@ -133,6 +135,31 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
  * context's buffer is used). Timings become available after sgm_synchronize. */
 int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch, int width,
                        int height, const sgm_outputs* d_out, void* stream);
+
+/* Replaces MetalStereo::set_rectification (metal_stereo.hpp:74-75, metal_stereo.cpp:203-231): uploads
+ * the left and right calib::RemapTable maps (libs/calib/include/einstar/calib/rectify.hpp:40-43;
+ * map_x and map_y as row-major map_w x map_h host arrays of source pixel coordinates) once; they stay
+ * resident on the device until the next call or sgm_destroy. The stereo size becomes
+ * (map_w / 2) x (map_h / 2), which must fit the context ("remap resolution must be twice the stereo
+ * resolution"); raw frames are raw_w x raw_h. */
+int sgm_set_rectification(sgm_context* ctx, const float* left_map_x, const float* left_map_y, const float* right_map_x,
+                          const float* right_map_y, int map_w, int map_h, int raw_w, int raw_h);
+
+/* Replaces MetalStereo::compute_raw (metal_stereo.hpp:79-81, metal_stereo.cpp:340-349): rectifies raw
+ * host frames with the resident maps -- calib::remap (libs/calib/src/rectify.cpp:133-148) at the
+ * maps' resolution and downsample2 (libs/depth/src/stereo.cpp:348-359), fused in one kernel and
+ * bit-exact with the two -- then runs sgm_compute's pipeline on the result. rect_left / rect_right
+ * (may be NULL) receive the rectified pair, (map_w / 2) x (map_h / 2) with pitch map_w / 2.
+ * Errors as the reference: "set_rectification() not called", "raw size mismatch". */
+int sgm_compute_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride, int raw_w,
+                    int raw_h, const sgm_outputs* out, uint8_t* rect_left, uint8_t* rect_right);
+
+/* The same on device-resident raw frames and outputs, enqueued on `stream` (NULL = the context's)
+ * without waiting. d_rect_left / d_rect_right (pitch map_w / 2) receive the rectified pair; NULL
+ * uses the context's input buffers. */
+int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right, ptrdiff_t raw_pitch,
+                           int raw_w, int raw_h, const sgm_outputs* d_out, uint8_t* d_rect_left, uint8_t* d_rect_right,
+                           void* stream);
 
 /* Waits for the work enqueued by this context and records its timings. */
 int sgm_synchronize(sgm_context* ctx);

@@ -17,9 +17,12 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// ev[0..4] bracket census, paths, rows and median; ev[5] marks the start of a rectification
+// launch when the call had one (rectified).
 struct Stages {
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[6] = {};
     bool recorded = false;
+    bool rectified = false;
 };
 
 // Stage events of the calls in flight: a ring, so that per-stage times can be summed over many
@@ -184,6 +187,12 @@ struct sgm_context {
     int path_lines = 16;         // scanlines per path-kernel block
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
+    // Resident rectification (sgm_set_rectification): both maps as interleaved (x, y) floats, and
+    // the raw frames of host calls.
+    float* d_maps = nullptr;     // left | right, map_w * map_h * 2 floats each
+    uint8_t* d_raw = nullptr;    // left | right, raw_w * raw_h each
+    size_t maps_bytes = 0, raw_bytes = 0;
+    int map_w = 0, map_h = 0, raw_w = 0, raw_h = 0;
     size_t bytes = 0;
     sgm_timings timings{};
     double pending_wall_ms = 0;
@@ -225,7 +234,8 @@ void sgm_destroy(sgm_context* ctx) {
             if (e) (void)hipEventDestroy(e);
     for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
-                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol})
+                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol, (void*)ctx->d_maps,
+                    (void*)ctx->d_raw})
         dfree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -256,7 +266,7 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
     const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int r = 0; e == hipSuccess && r < kEventRing; ++r)
-        for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
+        for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
     size_t& b = ctx->bytes;
     if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * npx, b);
     ctx->census_half = census_words(max_width, max_height, params->min_disparity, params->num_disparities);
@@ -435,9 +445,11 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
 
 // Adds one completed event set to the running totals; the newest also becomes last_timings.
 static void account(sgm_context* ctx, Stages& st, bool newest) {
-    float ms[4] = {0, 0, 0, 0}, all = 0;
+    float ms[4] = {0, 0, 0, 0}, all = 0, rect = 0;
     for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], st.ev[i], st.ev[i + 1]);
-    (void)hipEventElapsedTime(&all, st.ev[0], st.ev[4]);
+    if (st.rectified) (void)hipEventElapsedTime(&rect, st.ev[5], st.ev[0]);
+    (void)hipEventElapsedTime(&all, st.rectified ? st.ev[5] : st.ev[0], st.ev[4]);
+    ctx->totals.rectify_ms += rect;
     ctx->totals.census_ms += ms[0];
     ctx->totals.paths_ms += ms[1];
     ctx->totals.rows_ms += ms[2];
@@ -451,8 +463,10 @@ static void account(sgm_context* ctx, Stages& st, bool newest) {
         ctx->timings.median_ms = ms[3];
         ctx->timings.gpu_ms = all;
         ctx->timings.total_ms = ctx->pending_wall_ms > 0 ? ctx->pending_wall_ms : all;
+        ctx->timings.rectify_ms = rect;
     }
     st.recorded = false;
+    st.rectified = false;
 }
 
 // Reads every recorded set (the stream has been synchronised).
@@ -525,15 +539,55 @@ int sgm_reset_stage_totals(sgm_context* ctx) {
     return SGM_OK;
 }
 
-int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width, int height,
-                const sgm_outputs* out) {
-    if (!ctx) {
-        g_last_error = "ctx is NULL";
-        return SGM_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+// Rectifies raw device frames with the resident maps into rect_l / rect_r (pitch map_w / 2).
+static hipError_t enqueue_rectify(sgm_context* ctx, const uint8_t* raw_l, const uint8_t* raw_r, int64_t raw_pitch,
+                                  uint8_t* rect_l, uint8_t* rect_r, hipStream_t s, Stages* st) {
+    RectifyDesc r{};
+    const size_t n = (size_t)ctx->map_w * ctx->map_h;
+    r.raw[0] = raw_l;
+    r.raw[1] = raw_r;
+    r.map[0] = ctx->d_maps;
+    r.map[1] = ctx->d_maps + 2 * n;
+    r.out[0] = rect_l;
+    r.out[1] = rect_r;
+    r.raw_pitch = raw_pitch;
+    r.raw_w = ctx->raw_w;
+    r.raw_h = ctx->raw_h;
+    r.map_w = ctx->map_w;
+    r.out_w = ctx->map_w / 2;
+    r.out_h = ctx->map_h / 2;
+    r.out_pitch = r.out_w;
+    hipError_t e;
+    if (st) {
+        if ((e = hipEventRecord(st->ev[5], s)) != hipSuccess) return e;
+        st->rectified = true;
     }
+    return launch_rectify(r, s);
+}
+
+static int check_raw(sgm_context* ctx, int raw_w, int raw_h) {
+    if (!ctx->d_maps) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "set_rectification() not called");
+    if (raw_w != ctx->raw_w || raw_h != ctx->raw_h) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "raw size mismatch");
+    return SGM_OK;
+}
+
+// Host frames in, host outputs back, blocking: the shared body of sgm_compute (rectified input,
+// raw == false) and sgm_compute_raw (raw frames rectified on the device first).
+struct HostFrame {
+    const uint8_t* left;
+    const uint8_t* right;
+    ptrdiff_t stride;
+    bool raw;
+    uint8_t* rect_left;   // raw only: rectified pair back to the host, may be null
+    uint8_t* rect_right;
+};
+
+static int compute_host(sgm_context* ctx, const HostFrame& f, int width, int height, const sgm_outputs* out,
+                        const char* who) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
-    if (!left || !right || stride < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    const std::string w(who);
     if (!out || !out->disparity) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "out->disparity is required");
     const size_t npx = (size_t)width * height;
     const size_t nvol = npx * (size_t)ctx->p.num_disparities;
@@ -542,12 +596,23 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
     if (out->sum_volume && !ctx->d_sum) e = dalloc(&ctx->d_sum, (size_t)ctx->max_w * ctx->max_h * ctx->p.num_disparities, ctx->bytes);
     if (e == hipSuccess && out->cost_volume && !ctx->d_cost)
         e = dalloc(&ctx->d_cost, (size_t)ctx->max_w * ctx->max_h * ctx->p.num_disparities, ctx->bytes);
-    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: debug volume allocation", e);
+    if (e != hipSuccess) return ctx->hip_fail((w + ": debug volume allocation").c_str(), e);
     uint8_t* dl = ctx->d_img;
     uint8_t* dr = ctx->d_img + (size_t)ctx->max_w * ctx->max_h;
-    e = hipMemcpy2DAsync(dl, width, left, stride, width, height, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpy2DAsync(dr, width, right, stride, width, height, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: upload", e);
+    Stages* st = next_stages(ctx);
+    if (f.raw) {
+        const size_t rpx = (size_t)ctx->raw_w * ctx->raw_h;
+        e = hipMemcpy2DAsync(ctx->d_raw, ctx->raw_w, f.left, f.stride, ctx->raw_w, ctx->raw_h, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(ctx->d_raw + rpx, ctx->raw_w, f.right, f.stride, ctx->raw_w, ctx->raw_h, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return ctx->hip_fail((w + ": upload").c_str(), e);
+        if ((e = enqueue_rectify(ctx, ctx->d_raw, ctx->d_raw + rpx, ctx->raw_w, dl, dr, s, st)) != hipSuccess)
+            return ctx->hip_fail((w + ": rectify").c_str(), e);
+    } else {
+        e = hipMemcpy2DAsync(dl, width, f.left, f.stride, width, height, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpy2DAsync(dr, width, f.right, f.stride, width, height, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return ctx->hip_fail((w + ": upload").c_str(), e);
+    }
     sgm_outputs o{};
     o.disparity = ctx->d_disp;
     o.disparity_median = out->disparity_median ? ctx->d_median : nullptr;
@@ -560,8 +625,8 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
     o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
     o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
-    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, next_stages(ctx), ctx->path_lines);
-    if (e != hipSuccess) return ctx->hip_fail("sgm_compute: launch", e);
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines);
+    if (e != hipSuccess) return ctx->hip_fail((w + ": launch").c_str(), e);
     struct Copy {
         void* host;
         const void* dev;
@@ -578,19 +643,137 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
         {out->second_cost, ctx->d_second, npx * 2},
         {out->sum_volume, ctx->d_sum, nvol * 2},
         {out->cost_volume, ctx->d_cost, nvol * 2},
+        {f.raw ? f.rect_left : nullptr, dl, npx},
+        {f.raw ? f.rect_right : nullptr, dr, npx},
     };
     for (const Copy& c : copies)
         if (c.host && (e = hipMemcpyAsync(c.host, c.dev, c.bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
-            return ctx->hip_fail("sgm_compute: download", e);
+            return ctx->hip_fail((w + ": download").c_str(), e);
     for (const auto& [host, dev] : {std::pair<uint64_t*, const uint64_t*>{out->census_left, d.census_l},
                                     std::pair<uint64_t*, const uint64_t*>{out->census_right, d.census_r}})
         if (host && (e = hipMemcpy2DAsync(host, (size_t)width * 8, dev + d.census_padl, (size_t)d.census_pitch * 8,
                                           (size_t)width * 8, height, hipMemcpyDeviceToHost, s)) != hipSuccess)
-            return ctx->hip_fail("sgm_compute: census download", e);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail("sgm_compute: execution", e);
+            return ctx->hip_fail((w + ": census download").c_str(), e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail((w + ": execution").c_str(), e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     read_timings(ctx);
+    return SGM_OK;
+}
+
+extern "C" {
+
+int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width, int height,
+                const sgm_outputs* out) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
+    if (!left || !right || stride < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    return compute_host(ctx, HostFrame{left, right, stride, false, nullptr, nullptr}, width, height, out, "sgm_compute");
+}
+
+int sgm_set_rectification(sgm_context* ctx, const float* left_map_x, const float* left_map_y, const float* right_map_x,
+                          const float* right_map_y, int map_w, int map_h, int raw_w, int raw_h) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (!left_map_x || !left_map_y || !right_map_x || !right_map_y) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null map");
+    if (map_w < 2 || map_h < 2 || map_w / 2 > ctx->max_w || map_h / 2 > ctx->max_h)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "remap resolution must be twice the stereo resolution (stereo size " +
+                                                       std::to_string(map_w / 2) + "x" + std::to_string(map_h / 2) +
+                                                       " outside the context's " + std::to_string(ctx->max_w) + "x" +
+                                                       std::to_string(ctx->max_h) + ")");
+    // Raw sizes stay below 2^24 so that the kernel's float bounds test is exact.
+    if (raw_w < 1 || raw_h < 1 || raw_w >= (1 << 24) || raw_h >= (1 << 24))
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "raw size must be in [1, 2^24)");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_set_rectification: synchronize", e);
+    dfree(ctx->d_maps);
+    dfree(ctx->d_raw);
+    ctx->bytes -= ctx->maps_bytes + ctx->raw_bytes;
+    ctx->d_maps = nullptr;
+    ctx->d_raw = nullptr;
+    ctx->maps_bytes = ctx->raw_bytes = 0;
+    ctx->map_w = ctx->map_h = ctx->raw_w = ctx->raw_h = 0;
+    const size_t n = (size_t)map_w * map_h;
+    const size_t rpx = (size_t)raw_w * raw_h;
+    size_t got = 0;
+    if ((e = dalloc(&ctx->d_maps, 4 * n, got)) != hipSuccess) return ctx->hip_fail("sgm_set_rectification: map allocation", e);
+    ctx->maps_bytes = got;
+    got = 0;
+    if ((e = dalloc(&ctx->d_raw, 2 * rpx, got)) != hipSuccess) {
+        dfree(ctx->d_maps);
+        ctx->d_maps = nullptr;
+        ctx->maps_bytes = 0;
+        return ctx->hip_fail("sgm_set_rectification: raw frame allocation", e);
+    }
+    ctx->raw_bytes = got;
+    ctx->bytes += ctx->maps_bytes + ctx->raw_bytes;
+    // The kernel reads the (x, y) pairs of two neighbouring map pixels as one 16-byte load.
+    std::vector<float> packed(2 * n);
+    const float* mx[2] = {left_map_x, right_map_x};
+    const float* my[2] = {left_map_y, right_map_y};
+    for (int side = 0; side < 2 && e == hipSuccess; ++side) {
+        for (size_t i = 0; i < n; ++i) {
+            packed[2 * i] = mx[side][i];
+            packed[2 * i + 1] = my[side][i];
+        }
+        e = hipMemcpy(ctx->d_maps + 2 * n * side, packed.data(), 8 * n, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        dfree(ctx->d_maps);
+        dfree(ctx->d_raw);
+        ctx->bytes -= ctx->maps_bytes + ctx->raw_bytes;
+        ctx->d_maps = nullptr;
+        ctx->d_raw = nullptr;
+        ctx->maps_bytes = ctx->raw_bytes = 0;
+        return ctx->hip_fail("sgm_set_rectification: map upload", e);
+    }
+    ctx->map_w = map_w;
+    ctx->map_h = map_h;
+    ctx->raw_w = raw_w;
+    ctx->raw_h = raw_h;
+    return SGM_OK;
+}
+
+int sgm_compute_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride, int raw_w,
+                    int raw_h, const sgm_outputs* out, uint8_t* rect_left, uint8_t* rect_right) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!raw_left || !raw_right || raw_stride < raw_w) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    return compute_host(ctx, HostFrame{raw_left, raw_right, raw_stride, true, rect_left, rect_right}, ctx->map_w / 2,
+                        ctx->map_h / 2, out, "sgm_compute_raw");
+}
+
+int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right, ptrdiff_t raw_pitch,
+                           int raw_w, int raw_h, const sgm_outputs* d_out, uint8_t* d_rect_left, uint8_t* d_rect_right,
+                           void* stream) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!d_raw_left || !d_raw_right || raw_pitch < raw_w) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
+    if (d_out && (d_out->census_left || d_out->census_right))
+        return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
+    const int width = ctx->map_w / 2, height = ctx->map_h / 2;
+    uint8_t* dl = d_rect_left ? d_rect_left : ctx->d_img;
+    uint8_t* dr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)ctx->max_w * ctx->max_h;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    Stages* st = next_stages(ctx);
+    hipError_t e = enqueue_rectify(ctx, d_raw_left, d_raw_right, raw_pitch, dl, dr, s, st);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_raw_device: rectify", e);
+    PairDesc d = context_desc(ctx, dl, dr, width, width, height, d_out);
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines)) != hipSuccess)
+        return ctx->hip_fail("sgm_compute_raw_device: launch", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = 0;
     return SGM_OK;
 }
 

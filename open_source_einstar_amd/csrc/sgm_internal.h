@@ -66,6 +66,19 @@ struct LaunchTable {
     int32_t cost_volume;          // batches (n > 1): every pair has a cost volume
 };
 
+// Raw full-resolution frames -> rectified half-resolution pair (calib::remap at the maps'
+// resolution, then downsample2). side 0 is the left camera, side 1 the right.
+struct RectifyDesc {
+    const uint8_t* raw[2];
+    const float* map[2];   // [map_h][map_w][2] (x, y) source coordinates, 16-byte aligned
+    uint8_t* out[2];
+    int64_t raw_pitch;     // bytes
+    int64_t out_pitch;     // bytes
+    int32_t raw_w, raw_h;
+    int32_t map_w;         // map row length in entries (map_h >= 2 * out_h)
+    int32_t out_w, out_h;  // map_w / 2, map_h / 2
+};
+
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
@@ -74,6 +87,7 @@ hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmCons
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+hipError_t launch_rectify(const RectifyDesc& r, hipStream_t stream);
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);

@@ -799,6 +799,44 @@ __global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
     median_body(P, lb);
 }
 
+// ------------------------------------------------------------------------------ rectification
+// calib::remap (libs/calib/src/rectify.cpp:133-148) at the maps' resolution followed by
+// downsample2 (stereo.cpp:348-359), fused: a thread makes one output pixel from its 2x2 block of
+// full-resolution samples, reading their four (x, y) map entries as two 16-byte loads, and the
+// full-resolution image never exists. The bilinear expression keeps the reference's operation
+// order and the library is built with -ffp-contract=off, so every sample rounds as on the CPU.
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRectW = 64, kRectH = kBlock / kRectW;
+
+__device__ __forceinline__ uint32_t remap_sample(const SGM_GLOBAL uint8_t* src, int64_t pitch, int sw, int sh, float sx,
+                                                 float sy) {
+    // floor(s) in [0, size - 2]  <=>  0 <= s < size - 1 for every finite s; NaN fails both tests.
+    if (!(sx >= 0.0f && sy >= 0.0f && sx < (float)(sw - 1) && sy < (float)(sh - 1))) return 0;
+    const float flx = __builtin_floorf(sx), fly = __builtin_floorf(sy);
+    const float fx = sx - flx, fy = sy - fly;
+    const SGM_GLOBAL uint8_t* p = src + (int64_t)(int)fly * pitch + (int)flx;
+    const float a = p[0], b = p[1], c = p[pitch], d = p[pitch + 1];
+    const float v = (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy;
+    return (uint32_t)__builtin_fminf(v + 0.5f, 255.0f);  // v >= 0: the lower clamp never binds
+}
+
+__global__ __launch_bounds__(kBlock) void rectify_kernel(RectifyDesc r) {
+    const int side = (int)blockIdx.z;
+    const int x = (int)blockIdx.x * kRectW + (int)threadIdx.x;
+    const int y = (int)blockIdx.y * kRectH + (int)threadIdx.y;
+    if (x >= r.out_w || y >= r.out_h) return;
+    const SGM_GLOBAL uint8_t* src = (const SGM_GLOBAL uint8_t*)r.raw[side];
+    const SGM_GLOBAL float* map = (const SGM_GLOBAL float*)r.map[side] + ((int64_t)(2 * y) * r.map_w + 2 * x) * 2;
+    const f32x4 m0 = *(const SGM_GLOBAL f32x4*)map;
+    const f32x4 m1 = *(const SGM_GLOBAL f32x4*)(map + 2 * (int64_t)r.map_w);
+    const uint32_t acc = remap_sample(src, r.raw_pitch, r.raw_w, r.raw_h, m0.x, m0.y) +
+                         remap_sample(src, r.raw_pitch, r.raw_w, r.raw_h, m0.z, m0.w) +
+                         remap_sample(src, r.raw_pitch, r.raw_w, r.raw_h, m1.x, m1.y) +
+                         remap_sample(src, r.raw_pitch, r.raw_w, r.raw_h, m1.z, m1.w);
+    ((SGM_GLOBAL uint8_t*)r.out[side])[(int64_t)y * r.out_pitch + x] = (uint8_t)((acc + 2) >> 2);
+}
+
 // -------------------------------------------------------------------------------------- copy
 // Plain 16-byte-per-lane copy: the known-size reference for HBM traffic counters.
 
@@ -914,6 +952,13 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
     const int64_t blocks = std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 64);
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, stream, static_cast<u32x4*>(dst),
                        static_cast<const u32x4*>(src), n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_rectify(const RectifyDesc& r, hipStream_t stream) {
+    if (r.out_w <= 0 || r.out_h <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((r.out_w + kRectW - 1) / kRectW), (unsigned)((r.out_h + kRectH - 1) / kRectH), 2);
+    hipLaunchKernelGGL(rectify_kernel, grid, dim3(kRectW, kRectH), 0, stream, r);
     return hipGetLastError();
 }
 

@@ -60,12 +60,13 @@ class CSgmOutputs(C.Structure):
 
 
 class CSgmTimings(C.Structure):
-    _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms")]
+    _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms",
+                                          "rectify_ms")]
 
 
 class CSgmStageTotals(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")] + \
-               [("count", C.c_int64)]
+               [("count", C.c_int64), ("rectify_ms", C.c_double)]
 
 
 class CSgmServerConfig(C.Structure):
@@ -147,6 +148,9 @@ def lib():
         L.sgm_last_error.restype = C.c_char_p
         L.sgm_compute.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs)]
         L.sgm_compute_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp]
+        L.sgm_set_rectification.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32]
+        L.sgm_compute_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp, vp]
+        L.sgm_compute_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp, vp, vp]
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
         L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
@@ -279,13 +283,7 @@ class HipStereo:
         except Exception:
             pass
 
-    def compute(self, left: np.ndarray, right: np.ndarray, outputs=("disparity",)) -> dict:
-        """sgm_disparity on a rectified u8 pair (+ median3 etc. as requested in `outputs`)."""
-        left = np.ascontiguousarray(left, dtype=np.uint8)
-        right = np.ascontiguousarray(right, dtype=np.uint8)
-        if left.shape != right.shape or left.ndim != 2:
-            raise SgmError(5, "left and right must be 2-D images of the same size")
-        h, w = left.shape
+    def _host_outputs(self, w: int, h: int, outputs) -> tuple[dict, CSgmOutputs]:
         res = {}
         co = CSgmOutputs()
         for f in outputs:
@@ -295,8 +293,58 @@ class HipStereo:
         if "disparity" not in res:
             res["disparity"] = np.empty((h, w), np.float32)
             co.disparity = res["disparity"].ctypes.data
+        return res, co
+
+    def compute(self, left: np.ndarray, right: np.ndarray, outputs=("disparity",)) -> dict:
+        """sgm_disparity on a rectified u8 pair (+ median3 etc. as requested in `outputs`)."""
+        left = np.ascontiguousarray(left, dtype=np.uint8)
+        right = np.ascontiguousarray(right, dtype=np.uint8)
+        if left.shape != right.shape or left.ndim != 2:
+            raise SgmError(5, "left and right must be 2-D images of the same size")
+        h, w = left.shape
+        res, co = self._host_outputs(w, h, outputs)
         _check(lib().sgm_compute(self._h, _ptr(left), _ptr(right), w, w, h, C.byref(co)), ctx=self._h)
         return res
+
+    def set_rectification(self, left_map_x, left_map_y, right_map_x, right_map_y, raw_width: int, raw_height: int):
+        """MetalStereo::set_rectification (metal_stereo.hpp:74-75): resident left/right remap tables
+        (float32 map_x / map_y, each map_h x map_w); the stereo size becomes half the maps'."""
+        maps = [np.ascontiguousarray(m, dtype=np.float32) for m in (left_map_x, left_map_y, right_map_x, right_map_y)]
+        if any(m.ndim != 2 or m.shape != maps[0].shape for m in maps):
+            raise SgmError(5, "the four maps must be 2-D arrays of one size")
+        mh, mw = maps[0].shape
+        _check(lib().sgm_set_rectification(self._h, *[_ptr(m) for m in maps], mw, mh, raw_width, raw_height), ctx=self._h)
+        self.rect_size = (mw // 2, mh // 2)
+        self.raw_size = (raw_width, raw_height)
+
+    def compute_raw(self, raw_left: np.ndarray, raw_right: np.ndarray, outputs=("disparity",),
+                    rectified: bool = False) -> dict:
+        """MetalStereo::compute_raw (metal_stereo.hpp:79-81): rectify with the resident maps, then SGM.
+        rectified=True also returns the rectified pair as rect_left / rect_right."""
+        raw_left = np.ascontiguousarray(raw_left, dtype=np.uint8)
+        raw_right = np.ascontiguousarray(raw_right, dtype=np.uint8)
+        if raw_left.shape != raw_right.shape or raw_left.ndim != 2:
+            raise SgmError(5, "raw frames must be 2-D images of the same size")
+        rh, rw = raw_left.shape
+        w, h = getattr(self, "rect_size", (0, 0))
+        res, co = self._host_outputs(max(w, 1), max(h, 1), outputs)
+        rl = rr = None
+        if rectified:
+            res["rect_left"] = rl = np.empty((max(h, 1), max(w, 1)), np.uint8)
+            res["rect_right"] = rr = np.empty((max(h, 1), max(w, 1)), np.uint8)
+        _check(lib().sgm_compute_raw(self._h, _ptr(raw_left), _ptr(raw_right), rw, rw, rh, C.byref(co), _ptr(rl),
+                                     _ptr(rr)), ctx=self._h)
+        return res
+
+    def compute_raw_device(self, d_raw_left: int, d_raw_right: int, pitch: int, raw_width: int, raw_height: int,
+                           d_outputs: dict | None = None, d_rect_left: int | None = None,
+                           d_rect_right: int | None = None, stream: int | None = None):
+        """Device-resident variant (sgm_compute_raw_device); returns without waiting."""
+        co = CSgmOutputs()
+        for f, p in (d_outputs or {}).items():
+            setattr(co, f, p)
+        _check(lib().sgm_compute_raw_device(self._h, d_raw_left, d_raw_right, pitch, raw_width, raw_height,
+                                            C.byref(co), d_rect_left, d_rect_right, stream), ctx=self._h)
 
     def compute_device(self, d_left: int, d_right: int, pitch: int, width: int, height: int,
                        d_outputs: dict | None = None, stream: int | None = None):

@@ -58,3 +58,48 @@ def speckle_pair(width: int, height: int, num_disparities: int, seed: int, *, mi
     if md == -(2**31):
         raise ValueError("synth_speckle_pair: invalid parameters")
     return SpecklePair(left, right, gt, int(md), num_disparities)
+
+
+@dataclass
+class RemapPair:
+    """Left and right remap tables (calib::RemapTable, libs/calib/include/einstar/calib/rectify.hpp:40-43):
+    for each rectified pixel, the float32 source coordinates in the raw (distorted) frame."""
+
+    left_x: np.ndarray
+    left_y: np.ndarray
+    right_x: np.ndarray
+    right_y: np.ndarray
+
+
+def _rotation(rng, max_angle_deg: float) -> np.ndarray:
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    a = np.deg2rad(rng.uniform(-max_angle_deg, max_angle_deg))
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(a) * k + (1 - np.cos(a)) * (k @ k)
+
+
+def rectification_maps(map_w: int, map_h: int, raw_w: int, raw_h: int, seed: int, *, max_angle_deg: float = 0.6,
+                       k1: float = -0.08, k2: float = 0.02, tangential: float = 5e-4) -> RemapPair:
+    """Rectification tables of a synthetic rig, built like calib::build_remap (rectify.cpp:117-131):
+    a rectified pinhole pixel -> ray, rotated back into the raw camera by a small random rotation,
+    projected through a Brown radial/tangential model with a jittered principal point, and stored as
+    float32. Pixels near the border may map outside the raw frame (rectification then gives 0)."""
+    rng = np.random.default_rng(seed)
+    ys, xs = np.mgrid[0:map_h, 0:map_w].astype(np.float64)
+    out = []
+    for _ in range(2):
+        f = 0.9 * raw_w * rng.uniform(0.98, 1.02)
+        cx = (raw_w - 1) / 2 + rng.uniform(-0.01, 0.01) * raw_w
+        cy = (raw_h - 1) / 2 + rng.uniform(-0.01, 0.01) * raw_h
+        fr = 0.9 * raw_w * map_w / raw_w
+        rays = np.stack([(xs - (map_w - 1) / 2) / fr, (ys - (map_h - 1) / 2) / fr, np.ones_like(xs)], axis=-1)
+        r = rays @ _rotation(rng, max_angle_deg)  # row vectors: (R^T ray)^T = ray^T R
+        nx, ny = r[..., 0] / r[..., 2], r[..., 1] / r[..., 2]
+        r2 = nx * nx + ny * ny
+        radial = 1 + k1 * r2 + k2 * r2 * r2
+        t1, t2 = tangential * rng.uniform(-1, 1), tangential * rng.uniform(-1, 1)
+        dx = nx * radial + 2 * t1 * nx * ny + t2 * (r2 + 2 * nx * nx)
+        dy = ny * radial + t1 * (r2 + 2 * ny * ny) + 2 * t2 * nx * ny
+        out += [(f * dx + cx).astype(np.float32), (f * dy + cy).astype(np.float32)]
+    return RemapPair(*out)

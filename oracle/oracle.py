@@ -96,6 +96,10 @@ def lib():
         L.orc_lr_check.argtypes = [vp, vp, i32, i32, i32, vp, i32]
         L.orc_median3.argtypes = [vp, i32, i32, vp, i32]
         L.orc_sgm.argtypes = [vp, vp, C.c_ssize_t, i32, i32, C.POINTER(OrcParams), C.POINTER(OrcOutputs), i32]
+        L.orc_remap.argtypes = [vp, i32, i32, C.c_ssize_t, vp, vp, i32, i32, vp, i32]
+        L.orc_downsample2.argtypes = [vp, i32, i32, C.c_ssize_t, vp, i32]
+        L.orc_rectify_half.argtypes = [vp, i32, i32, C.c_ssize_t, vp, vp, i32, i32, vp, i32]
+        L.orc_rectify_half.restype = i32
         L.orc_sgm.restype = i32
         _lib = L
     return _lib
@@ -209,3 +213,37 @@ def sgm(left, right, p: SgmParams, volumes: bool = False, threads: int = 0) -> d
     if rc != 0:
         raise MemoryError("oracle sgm: allocation failed")
     return o
+
+
+def remap(src, map_x, map_y, threads: int = 0) -> np.ndarray:
+    """calib::remap (libs/calib/src/rectify.cpp:133-148)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    mx = np.ascontiguousarray(map_x, dtype=np.float32)
+    my = np.ascontiguousarray(map_y, dtype=np.float32)
+    sh, sw = src.shape
+    h, w = mx.shape
+    out = np.empty((h, w), np.uint8)
+    lib().orc_remap(_ptr(src), sw, sh, sw, _ptr(mx), _ptr(my), w, h, _ptr(out), threads)
+    return out
+
+
+def downsample2(src, threads: int = 0) -> np.ndarray:
+    """downsample2 (libs/depth/src/stereo.cpp:348-359)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    h, w = src.shape
+    out = np.empty((h // 2, w // 2), np.uint8)
+    lib().orc_downsample2(_ptr(src), w, h, w, _ptr(out), threads)
+    return out
+
+
+def rectify_half(raw, map_x, map_y, threads: int = 0) -> np.ndarray:
+    """remap to the maps' resolution, then downsample2 (the SGM-level rectified image)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    mx = np.ascontiguousarray(map_x, dtype=np.float32)
+    my = np.ascontiguousarray(map_y, dtype=np.float32)
+    rh, rw = raw.shape
+    h, w = mx.shape
+    out = np.empty((h // 2, w // 2), np.uint8)
+    if lib().orc_rectify_half(_ptr(raw), rw, rh, rw, _ptr(mx), _ptr(my), w, h, _ptr(out), threads) != 0:
+        raise MemoryError("oracle rectify_half: allocation failed")
+    return out

@@ -407,6 +407,76 @@ void orc_median3(const float* in, int w, int h, float* out, int threads) {
     parallel_for(h, threads, med_rows, &c);
 }
 
+/* ---------------------------------------------------------------- rectification (rectify.cpp:133-148, stereo.cpp:348-359) */
+
+typedef struct {
+    const uint8_t* src;
+    int sw, sh;
+    ptrdiff_t sstride;
+    const float *mx, *my;
+    int w;
+    uint8_t* dst;
+} remap_ctx;
+
+static void remap_rows(void* vctx, int y0, int y1) {
+    const remap_ctx* c = (const remap_ctx*)vctx;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < c->w; ++x) {
+            const size_t i = (size_t)y * (size_t)c->w + (size_t)x;
+            const float sx = c->mx[i], sy = c->my[i];
+            const int x0 = (int)floorf(sx), yy0 = (int)floorf(sy);
+            if (x0 < 0 || yy0 < 0 || x0 + 1 >= c->sw || yy0 + 1 >= c->sh) {
+                c->dst[i] = 0;
+                continue;
+            }
+            const float fx = sx - (float)x0, fy = sy - (float)yy0;
+            const uint8_t* r0 = c->src + (ptrdiff_t)yy0 * c->sstride;
+            const uint8_t* r1 = r0 + c->sstride;
+            const float a = r0[x0], b = r0[x0 + 1], cc = r1[x0], d = r1[x0 + 1];
+            const float v = (a * (1 - fx) + b * fx) * (1 - fy) + (cc * (1 - fx) + d * fx) * fy;
+            const float r = v + 0.5f;
+            c->dst[i] = (uint8_t)(r < 0.0f ? 0.0f : (r > 255.0f ? 255.0f : r)); /* std::clamp then truncation */
+        }
+}
+
+void orc_remap(const uint8_t* src, int src_w, int src_h, ptrdiff_t src_stride, const float* map_x, const float* map_y,
+               int w, int h, uint8_t* dst, int threads) {
+    remap_ctx c = {src, src_w, src_h, src_stride, map_x, map_y, w, dst};
+    parallel_for(h, threads, remap_rows, &c);
+}
+
+typedef struct {
+    const uint8_t* src;
+    ptrdiff_t stride;
+    int ow;
+    uint8_t* dst;
+} down_ctx;
+
+static void down_rows(void* vctx, int y0, int y1) {
+    const down_ctx* c = (const down_ctx*)vctx;
+    for (int y = y0; y < y1; ++y) {
+        const uint8_t* r0 = c->src + (ptrdiff_t)(2 * y) * c->stride;
+        const uint8_t* r1 = r0 + c->stride;
+        for (int x = 0; x < c->ow; ++x)
+            c->dst[(size_t)y * (size_t)c->ow + (size_t)x] = (uint8_t)((r0[2 * x] + r0[2 * x + 1] + r1[2 * x] + r1[2 * x + 1] + 2) / 4);
+    }
+}
+
+void orc_downsample2(const uint8_t* src, int w, int h, ptrdiff_t stride, uint8_t* dst, int threads) {
+    down_ctx c = {src, stride, w / 2, dst};
+    parallel_for(h / 2, threads, down_rows, &c);
+}
+
+int orc_rectify_half(const uint8_t* raw, int raw_w, int raw_h, ptrdiff_t raw_stride, const float* map_x, const float* map_y,
+                     int map_w, int map_h, uint8_t* dst, int threads) {
+    uint8_t* full = (uint8_t*)malloc((size_t)map_w * (size_t)map_h + 1);
+    if (!full) return -1;
+    orc_remap(raw, raw_w, raw_h, raw_stride, map_x, map_y, map_w, map_h, full, threads);
+    orc_downsample2(full, map_w, map_h, map_w, dst, threads);
+    free(full);
+    return 0;
+}
+
 /* ---------------------------------------------------------------- sgm_disparity + median3 (stereo.cpp:361-463) */
 
 int orc_sgm(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const orc_params* p,

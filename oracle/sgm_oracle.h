@@ -103,6 +103,19 @@ void orc_lr_check(float* disp, const int32_t* right_disp, int w, int h, int lr_m
 /* median3, stereo.cpp:129-147. */
 void orc_median3(const float* in, int w, int h, float* out, int threads);
 
+/* calib::remap, libs/calib/src/rectify.cpp:133-148: bilinear lookup of src at (map_x, map_y) per
+ * output pixel, 0 where the 2x2 neighbourhood leaves src, rounded by +0.5 and truncation. */
+void orc_remap(const uint8_t* src, int src_w, int src_h, ptrdiff_t src_stride, const float* map_x, const float* map_y,
+               int w, int h, uint8_t* dst, int threads);
+
+/* downsample2, stereo.cpp:348-359: (a + b + c + d + 2) / 4 over 2x2 blocks; output (w/2) x (h/2). */
+void orc_downsample2(const uint8_t* src, int w, int h, ptrdiff_t stride, uint8_t* dst, int threads);
+
+/* The stereo front end's rectification to the SGM level (libs/pipeline/src/stereo_frontend.cpp:331-334):
+ * remap at the maps' resolution (map_w x map_h), then downsample2. dst is (map_w/2) x (map_h/2). */
+int orc_rectify_half(const uint8_t* raw, int raw_w, int raw_h, ptrdiff_t raw_stride, const float* map_x, const float* map_y,
+                     int map_w, int map_h, uint8_t* dst, int threads);
+
 /* sgm_disparity (stereo.cpp:361-463) followed by median3; fills whatever outputs are non-NULL.
  * Returns 0, or -1 when an allocation fails. */
 int orc_sgm(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const orc_params* p,

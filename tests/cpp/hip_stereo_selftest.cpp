@@ -142,6 +142,45 @@ int main() {
         expect(s2.has_value() && same_bytes(s2->disparity, disp), "strided input views");
     }
 
+    // Fused rectification, as tests/depth/test_metal_stereo.cpp:70-103 checks it: raw frames at twice
+    // the stereo resolution through slightly rotated, scaled remap tables; the rectified pair is
+    // bit-exact with remap + downsample2 and the disparity with sgm_disparity on that pair.
+    {
+        const int mw = 2 * w, mh = 2 * h, rw = 2 * w + 30, rh = 2 * h + 20;
+        ImageU8 raw_l, raw_r;
+        make_pair(rw, rh, 18, raw_l, raw_r);
+        RemapTable ml{mw, mh, ImageF32(mw, mh), ImageF32(mw, mh)}, mr = ml;
+        for (int y = 0; y < mh; ++y)
+            for (int x = 0; x < mw; ++x) {
+                const double u = x - mw / 2.0, v = y - mh / 2.0;
+                ml.map_x(x, y) = static_cast<float>(1.02 * (u * 0.9998 - v * 0.02) + rw / 2.0);
+                ml.map_y(x, y) = static_cast<float>(1.02 * (u * 0.02 + v * 0.9998) + rh / 2.0);
+                mr.map_x(x, y) = static_cast<float>(0.99 * u + rw / 2.0 + 0.37);
+                mr.map_y(x, y) = static_cast<float>(0.99 * v + rh / 2.0 - 0.61);
+            }
+        auto none = (*st)->compute_raw(raw_l.view(), raw_r.view());
+        expect(!none && none.error().message == "set_rectification() not called", "compute_raw before set_rectification");
+        RemapTable wrong{mw + 2, mh, ImageF32(mw + 2, mh), ImageF32(mw + 2, mh)};
+        auto bad_map = (*st)->set_rectification(wrong, wrong, rw, rh);
+        expect(!bad_map && bad_map.error().code == Errc::invalid_argument, "remap resolution != 2x stereo -> invalid_argument");
+        expect((*st)->set_rectification(ml, mr, rw, rh).has_value(), "set_rectification");
+        ImageU8 gl, gr;
+        auto rr = (*st)->compute_raw(raw_l.view(), raw_r.view(), &gl, &gr);
+        expect(rr.has_value(), "compute_raw");
+        std::vector<std::uint8_t> hl(n), hr(n);
+        orc_rectify_half(raw_l.data(), rw, rh, rw, ml.map_x.data(), ml.map_y.data(), mw, mh, hl.data(), 0);
+        orc_rectify_half(raw_r.data(), rw, rh, rw, mr.map_x.data(), mr.map_y.data(), mw, mh, hr.data(), 0);
+        expect(same_bytes(gl, hl) && same_bytes(gr, hr), "rectified pair bit-exact");
+        std::vector<float> rd(n);
+        orc_outputs ro{};
+        ro.disparity = rd.data();
+        expect(orc_sgm(hl.data(), hr.data(), w, w, h, &op, &ro, 0) == 0, "oracle on the rectified pair");
+        if (rr) expect(same_bytes(rr->disparity, rd), "compute_raw disparity bit-exact");
+        ImageU8 short_raw(rw, rh - 1);
+        auto mism = (*st)->compute_raw(short_raw.view(), short_raw.view());
+        expect(!mism && mism.error().message == "raw size mismatch", "raw size mismatch");
+    }
+
     // Error behaviour of MetalStereo (metal_stereo.cpp:115-120, 235-236).
     ImageU8 small(w - 1, h);
     auto bad = (*st)->compute(small.view(), r.view());
