@@ -52,25 +52,28 @@ def batch_sizes(n=32, seed=3):
     return [(int(w), int(h), int(d)) for w, h, d in zip(ws, hs, ds)]
 
 
-def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=None):
+def serve_stream(sgm, rate_hz, n_frames, pairs, seed=7, slots=4, maps=None):
     """Poisson arrivals at rate_hz of a 3:1 mix of 1280x1024x128 frames and 640x512x64 ROI crops,
-    submitted to the serving loop as they arrive. Returns latency percentiles (arrival -> results in
-    host memory) and the sustained Mpix*disp/s."""
+    submitted to the serving loop as they arrive. pairs = (full, roi) lists of input pairs. With
+    maps = (full_maps, roi_maps), the inputs are raw sensor frames at twice the stereo resolution,
+    read in place from pinned memory and rectified on the device with the resident maps. Returns
+    latency percentiles (arrival -> results in host memory) and the sustained Mpix*disp/s."""
     rng = np.random.default_rng(seed)
-    if pairs is None:
-        full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
-        roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
-        pairs = (full, roi)
     full, roi = pairs
     kinds = rng.random(n_frames) < 0.75
     gaps = rng.exponential(1.0 / rate_hz, n_frames)
     sv = sgm.Server(sgm.SgmParams(**PARAMS), 1280, 1024, 128, slots=slots)
     try:
+        ids = (0, 0)
+        if maps is not None:
+            ids = tuple(sv.add_rectification(m.left_x, m.left_y, m.right_x, m.right_y, p[0].left.shape[1],
+                                             p[0].left.shape[0]) for m, p in zip(maps, pairs))
         t0 = sgm.Server.now_ns() + 2_000_000
         arrival = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
         lat, done_t, pxd, got = [], [], 0, 0
         for i in range(n_frames):
-            p = full[i % len(full)] if kinds[i] else roi[i % len(roi)]
+            k = 0 if kinds[i] else 1
+            p = full[i % len(full)] if k == 0 else roi[i % len(roi)]
             while sgm.Server.now_ns() < arrival[i]:
                 for c in sv.poll(64, 0):
                     lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
@@ -78,8 +81,10 @@ def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=No
                     sv.release(c)
                     got += 1
                 time.sleep(0.0001)
-            sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=("disparity", "disparity_median"), tag=i)
-            pxd += p.left.size * p.num_disparities
+            sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=("disparity", "disparity_median"),
+                      tag=i, rectification=ids[k], borrow=maps is not None)
+            h, w = p.left.shape
+            pxd += (w * h // 4 if maps is not None else w * h) * p.num_disparities
         while got < n_frames:
             for c in sv.poll(64, 200000):
                 lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
@@ -94,6 +99,22 @@ def serve_stream(sgm, speckle_pair, rate_hz, n_frames, seed=7, slots=4, pairs=No
                 "failed": sv.stats()["failed"]}
     finally:
         sv.stop()
+
+
+def raw_serving_inputs(sgm, speckle_pair, pool):
+    """Raw frames for the served stream: 2560x2048 sensor frames rectified to 1280x1024 (D=128) and
+    1280x1024 ROI readouts rectified to 640x512 (D=64), in pinned host memory, with their maps."""
+    from open_source_einstar_amd.synth import rectification_maps
+
+    def pinned(p):
+        a, b = pool.take(p.left.shape, np.uint8), pool.take(p.right.shape, np.uint8)
+        a[:], b[:] = p.left, p.right
+        return sgm.PairSpec(a, b, p.min_disparity // 2, p.num_disparities // 2)
+
+    full = [pinned(speckle_pair(2560, 2048, 256, seed=7200 + i)) for i in range(2)]
+    roi = [pinned(speckle_pair(1280, 1024, 128, seed=7300 + i)) for i in range(2)]
+    maps = (rectification_maps(2560, 2048, 2560, 2048, 31), rectification_maps(1280, 1024, 1280, 1024, 32))
+    return (full, roi), maps
 
 
 def large_batched(sgm, speckle_pair, resident=16, total=64, distinct=8, verify=True):
@@ -274,6 +295,43 @@ def main():
                              f"{len(runs) - 1} runs after 1 warm-up ({med * 1e3:.0f} ms each)"}
             log(f"cpu baseline: {cpu}")
 
+    # The raw form of the same step: 2560x2048 raw frames rectified on the device with resident maps
+    # (fused remap + downsample2), then SGM. The rectification kernel's algorithmic bytes per launch:
+    # both maps (8 B per full-resolution pixel), both raw frames once, both rectified images written.
+    from open_source_einstar_amd.synth import rectification_maps
+    maps = rectification_maps(2 * W, 2 * H, 2 * W, 2 * H, 31)
+    raw = speckle_pair(2 * W, 2 * H, 2 * D, seed=3)
+    st.set_rectification(maps.left_x, maps.left_y, maps.right_x, maps.right_y, 2 * W, 2 * H)
+    d_raw = sgm.DeviceBuffer(2 * 4 * W * H)
+    d_raw.upload(raw.left, 0)
+    d_raw.upload(raw.right, 4 * W * H)
+    rect_buf = sgm.DeviceBuffer(2 * W * H)
+
+    def raw_step():
+        st.compute_raw_device(d_raw.ptr, d_raw.ptr + 4 * W * H, 2 * W, 2 * W, 2 * H, d_out, rect_buf.ptr,
+                              rect_buf.ptr + W * H)
+
+    for _ in range(args.warmup):
+        raw_step()
+    st.synchronize()
+    st.reset_stage_totals()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        raw_step()
+    st.synchronize()
+    raw_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    rt = st.stage_totals()
+    rect_ms = rt["rectify_ms"] / max(1, rt["count"])
+    rect_bytes = 2 * (4 * W * H * 8) + 2 * (4 * W * H) + 2 * (W * H)
+    rectify = {"kernel_ms": round(rect_ms, 4), "bytes_per_launch": rect_bytes,
+               "achieved_gbs": round(rect_bytes / (rect_ms * 1e-3) / 1e9, 1),
+               "frac": round(rect_bytes / (rect_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+               "raw_ms_per_step": round(raw_ms, 4), "raw_value": round(pxd / (raw_ms * 1e-3) / 1e6, 1),
+               "workload": "2 x 2560x2048 raw frames -> 1280x1024 rectified pair, then the SGM step"}
+    log(f"rectify: {rectify}")
+    d_raw.free()
+    rect_buf.free()
+
     agg_bytes = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
     achieved = agg_bytes / (paths_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -316,17 +374,26 @@ def main():
 
     served = None
     if args.serve_frames > 0:
-        full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
-        roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
-        served = serve_stream(sgm, speckle_pair, 60.0, args.serve_frames, pairs=(full, roi))
-        log(f"served 60 Hz: {served}")
+        # Raw sensor frames rectified on the device with resident maps (the served form), then the
+        # same stream of already rectified pairs.
+        pool = sgm.PinnedPool()
+        raw_pairs, maps = raw_serving_inputs(sgm, speckle_pair, pool)
+        served = serve_stream(sgm, 60.0, args.serve_frames, raw_pairs, maps=maps)
+        served["inputs"] = "raw 2560x2048 / 1280x1024 frames, resident rectification maps, read in place"
+        log(f"served 60 Hz (raw): {served}")
         sweep = []
         for r in [float(x) for x in args.serve_sweep.split(",") if x.strip()]:
-            sweep.append(serve_stream(sgm, speckle_pair, r, 300, seed=int(r), pairs=(full, roi)))
-            log(f"served {r:.0f} Hz: {sweep[-1]}")
+            sweep.append(serve_stream(sgm, r, 300, raw_pairs, seed=int(r), maps=maps))
+            log(f"served {r:.0f} Hz (raw): {sweep[-1]}")
         ok = [x for x in sweep if x["p99_ms"] < 5.0]
         served["sweep"] = sweep
         served["max_rate_p99_under_5ms_hz"] = max((x["rate_hz"] for x in ok), default=None)
+        full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
+        roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
+        rect = serve_stream(sgm, 60.0, args.serve_frames, (full, roi))
+        log(f"served 60 Hz (rectified inputs, copied at submission): {rect}")
+        served["rectified_inputs"] = rect
+        pool.free_all()
 
     line = {
         "metric": "Mpix·disp/s batched vs plain on 1 MI355X (bit-exact); p50/p99 frame latency",
@@ -348,6 +415,7 @@ def main():
         "bit_exact": bit_exact,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "rectify": rectify,
         "cost_mode": "volume" if best_cv else "recompute",
         "cost_modes_ms_per_step": {"recompute": round(modes[0][0], 4), "volume": round(modes[1][0], 4)},
         "batched": batched,

@@ -279,6 +279,14 @@ typedef struct {
     int32_t min_disparity, num_disparities;
     sgm_outputs out; /* host buffers; disparity required; no volumes or census */
     uint64_t user_tag;
+    int32_t rectification; /* 0: left/right are rectified width x height images. k > 0: they are raw
+                              frames of map set k (sgm_server_add_rectification), rectified on the
+                              device; width x height must then be half the maps' size */
+    int32_t borrow_inputs; /* 0: left/right are copied at submission and may be reused at once.
+                              1: they are read in place (a direct DMA when pinned, e.g. from
+                              sgm_host_alloc) and must stay valid and unchanged until the
+                              completion is polled, as MetalStereo::compute_frame uses page-aligned
+                              raw images in place (metal_stereo.hpp:90-93) */
 } sgm_request;
 
 typedef struct {
@@ -306,6 +314,13 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
 /* SGM_ERR_BUSY when the request could never fit the arena; SGM_ERR_INVALID_ARGUMENT for sizes,
  * ranges or outputs the server does not take. */
 int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket);
+/* Registers a rectification map set (both cameras' calib::RemapTable, as sgm_set_rectification)
+ * for raw frames of one size and returns its id (> 0) in *id. The maps stay resident on the device
+ * until sgm_server_stop, as MetalStereo keeps those of set_rectification (metal_stereo.cpp:203-231)
+ * across frames; requests name the set in sgm_request.rectification. */
+int sgm_server_add_rectification(sgm_server* sv, const float* left_map_x, const float* left_map_y,
+                                 const float* right_map_x, const float* right_map_y, int map_w, int map_h, int raw_w,
+                                 int raw_h, int32_t* id);
 /* Up to max completions; waits up to timeout_us (0: no wait, < 0: until one is available). */
 int sgm_server_poll(sgm_server* sv, sgm_completion* out, int max, int timeout_us, int* n);
 int sgm_server_get_stats(sgm_server* sv, sgm_server_stats* st);

@@ -11,6 +11,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -42,16 +43,19 @@ int64_t now_ns() {
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
-// Byte offsets of one request's buffers inside its arena block.
+// Byte offsets of one request's buffers inside its arena block. raw holds the two raw frames of a
+// request that is rectified on the device (raw_px pixels each).
 struct Footprint {
-    size_t img, census, cvol, lvol, disp, median, conf, dint, status, rdisp, best, second, total;
+    size_t raw, img, census, cvol, lvol, disp, median, conf, dint, status, rdisp, best, second, total;
 };
 
-Footprint footprint(int w, int h, int min_d, int nd, int ndirs) {
+Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px) {
     const size_t npx = (size_t)w * h;
     const size_t dp = (size_t)dpl_for(nd) * kGroup;
     Footprint f{};
     size_t o = 0;
+    f.raw = o;
+    o = align_up(o + 2 * raw_px);
     f.img = o;
     o = align_up(o + 2 * npx);
     f.census = o;
@@ -80,11 +84,21 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs) {
     return f;
 }
 
+// A resident rectification map set: both cameras' (x, y) maps interleaved, left then right.
+struct MapSet {
+    float* d_maps = nullptr;
+    int map_w = 0, map_h = 0, raw_w = 0, raw_h = 0;
+};
+
 struct Request {
     uint64_t ticket = 0, tag = 0;
     int w = 0, h = 0, min_d = 0, nd = 0;
-    uint8_t* pinned = nullptr;  // left rows then right rows (pitch w) in a pooled pinned buffer, or
-    std::vector<uint8_t> img;   // the same in heap memory when the pool was empty
+    MapSet rect;                 // d_maps == null: the images are already rectified
+    uint8_t* pinned = nullptr;   // left rows then right rows (raw frames when rectified) in a pooled
+    size_t pinned_class = 0;     // pinned buffer of this size class, or
+    std::vector<uint8_t> img;    // the same in heap memory when the pool was empty, or
+    const uint8_t* borrowed[2] = {nullptr, nullptr};  // the caller's own buffers, read in place
+    int64_t borrowed_stride = 0;
     sgm_outputs out{};         // caller's host buffers
     int64_t t_submit = 0, t_admit = 0;
     size_t bytes = 0, offset = 0;
@@ -107,12 +121,38 @@ struct sgm_server {
     bool stopping = false;
     std::string err;
 
-    std::vector<uint8_t*> pinned_free;  // pinned input buffers (2 * max_w * max_h bytes each)
+    // Pinned input buffers by size class (bytes): 2 * max_w * max_h at start, plus one class per
+    // raw frame size that a map set brings in.
+    std::map<size_t, std::vector<uint8_t*>> pinned_free;
     std::vector<uint8_t*> pinned_all;
+    std::vector<MapSet> map_sets;  // id k is map_sets[k - 1]
+
+    // Takes a pooled pinned buffer of at least `bytes` (null when every such buffer is in use).
+    uint8_t* take_pinned(size_t bytes, size_t& cls) {  // with mu held
+        for (auto it = pinned_free.lower_bound(bytes); it != pinned_free.end(); ++it)
+            if (!it->second.empty()) {
+                uint8_t* b = it->second.back();
+                it->second.pop_back();
+                cls = it->first;
+                return b;
+            }
+        return nullptr;
+    }
+    hipError_t add_pinned_class(size_t bytes, int count) {  // with mu held
+        auto& v = pinned_free[bytes];
+        for (int i = 0; i < count; ++i) {
+            uint8_t* b = nullptr;
+            const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&b), bytes, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            pinned_all.push_back(b);
+            v.push_back(b);
+        }
+        return hipSuccess;
+    }
 
     struct Slot {
         hipStream_t stream = nullptr;
-        hipEvent_t ev[5] = {};
+        hipEvent_t ev[6] = {};  // census .. median as sgm_compute, [5] = rectification start
         std::thread th;
         std::condition_variable cv;
         bool has_job = false;
@@ -146,7 +186,8 @@ void sgm_server::worker(int si) {
             if (!s.has_job) return;
             r = std::move(reqs.at(s.ticket));  // the worker owns the request until it completes
         }
-        const Footprint f = footprint(r.w, r.h, r.min_d, r.nd, ndirs);
+        const size_t raw_px = r.rect.d_maps ? (size_t)r.rect.raw_w * r.rect.raw_h : 0;
+        const Footprint f = footprint(r.w, r.h, r.min_d, r.nd, ndirs, raw_px);
         uint8_t* base = d_arena + r.offset;
         const size_t npx = (size_t)r.w * r.h;
         PairDesc d{};
@@ -181,7 +222,38 @@ void sgm_server::worker(int si) {
         // Inputs from the pooled pinned buffer (or heap); results straight into the caller's buffers
         // (a direct DMA when they are pinned, e.g. from sgm_host_alloc).
         const uint8_t* src = r.pinned ? r.pinned : r.img.data();
-        hipError_t e = hipMemcpyAsync(base + f.img, src, 2 * npx, hipMemcpyHostToDevice, s.stream);
+        // Both inputs (raw frames when rectified on the device) to `dst`, rows of in_w bytes.
+        auto upload = [&](uint8_t* dst, int in_w, int in_h) {
+            const size_t n = (size_t)in_w * in_h;
+            if (!r.borrowed[0]) return hipMemcpyAsync(dst, src, 2 * n, hipMemcpyHostToDevice, s.stream);
+            hipError_t ue = hipMemcpy2DAsync(dst, in_w, r.borrowed[0], r.borrowed_stride, in_w, in_h, hipMemcpyHostToDevice, s.stream);
+            if (ue == hipSuccess)
+                ue = hipMemcpy2DAsync(dst + n, in_w, r.borrowed[1], r.borrowed_stride, in_w, in_h, hipMemcpyHostToDevice, s.stream);
+            return ue;
+        };
+        hipError_t e;
+        if (r.rect.d_maps) {
+            e = upload(base + f.raw, r.rect.raw_w, r.rect.raw_h);
+            RectifyDesc rd{};
+            const size_t nmap = (size_t)r.rect.map_w * r.rect.map_h;
+            rd.raw[0] = base + f.raw;
+            rd.raw[1] = base + f.raw + raw_px;
+            rd.map[0] = r.rect.d_maps;
+            rd.map[1] = r.rect.d_maps + 2 * nmap;
+            rd.out[0] = base + f.img;
+            rd.out[1] = base + f.img + npx;
+            rd.raw_pitch = r.rect.raw_w;
+            rd.out_pitch = r.w;
+            rd.raw_w = r.rect.raw_w;
+            rd.raw_h = r.rect.raw_h;
+            rd.map_w = r.rect.map_w;
+            rd.out_w = r.w;
+            rd.out_h = r.h;
+            if (e == hipSuccess) e = hipEventRecord(s.ev[5], s.stream);
+            if (e == hipSuccess) e = launch_rectify(rd, s.stream);
+        } else {
+            e = upload(base + f.img, r.w, r.h);
+        }
         if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, s.stream, s.ev);
         for (const Copy& c : copies) {
             if (!c.host || e != hipSuccess) continue;
@@ -189,7 +261,7 @@ void sgm_server::worker(int si) {
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
         float gpu_ms = 0;
-        if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, s.ev[0], s.ev[4]);
+        if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, r.rect.d_maps ? s.ev[5] : s.ev[0], s.ev[4]);
         sgm_completion cpl{};
         cpl.ticket = r.ticket;
         cpl.user_tag = r.tag;
@@ -205,7 +277,7 @@ void sgm_server::worker(int si) {
                 ++failed;
             }
             ++completed;
-            if (r.pinned) pinned_free.push_back(r.pinned);
+            if (r.pinned) pinned_free[r.pinned_class].push_back(r.pinned);
             reqs.erase(r.ticket);
             s.has_job = false;
             aq.finish(si, r.offset);
@@ -236,6 +308,7 @@ int sgm_server_stop(sgm_server* sv) {
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     for (uint8_t* b : sv->pinned_all) (void)hipHostFree(b);
+    for (const MapSet& m : sv->map_sets) (void)hipFree(m.d_maps);
     if (sv->d_arena) (void)hipFree(sv->d_arena);
     delete sv;
     return SGM_OK;
@@ -282,16 +355,9 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     const size_t npx = (size_t)cfg->max_width * cfg->max_height;
     for (auto& s : sv->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
-        for (int i = 0; e == hipSuccess && i < 5; ++i) e = hipEventCreate(&s.ev[i]);
+        for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&s.ev[i]);
     }
-    for (int i = 0; e == hipSuccess && i < 2 * cfg->slots + 2; ++i) {
-        uint8_t* b = nullptr;
-        e = hipHostMalloc(reinterpret_cast<void**>(&b), 2 * npx, hipHostMallocDefault);
-        if (e == hipSuccess) {
-            sv->pinned_all.push_back(b);
-            sv->pinned_free.push_back(b);
-        }
-    }
+    if (e == hipSuccess) e = sv->add_pinned_class(2 * npx, 2 * cfg->slots + 2);
     if (e != hipSuccess) {
         set_thread_error(std::string("sgm_server_start: slots: ") + hipGetErrorString(e));
         sgm_server_stop(sv);
@@ -308,41 +374,56 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
     sgm_params pp = sv->p;
     pp.min_disparity = rq->min_disparity;
     pp.num_disparities = rq->num_disparities;
-    if (validate_params_ext(&pp, err) != SGM_OK || rq->width < 1 || rq->height < 1 || rq->width > sv->max_w ||
-        rq->height > sv->max_h || rq->num_disparities > sv->max_nd || !rq->left || !rq->right || rq->stride < rq->width ||
+    MapSet ms;
+    bool bad_rect = rq->rectification < 0;
+    if (rq->rectification > 0) {
+        std::lock_guard lk(sv->mu);
+        if ((size_t)rq->rectification > sv->map_sets.size()) bad_rect = true;
+        else ms = sv->map_sets[(size_t)rq->rectification - 1];
+        bad_rect = bad_rect || rq->width != ms.map_w / 2 || rq->height != ms.map_h / 2;
+    }
+    const int in_w = ms.d_maps ? ms.raw_w : rq->width, in_h = ms.d_maps ? ms.raw_h : rq->height;
+    if (validate_params_ext(&pp, err) != SGM_OK || bad_rect || rq->width < 1 || rq->height < 1 || rq->width > sv->max_w ||
+        rq->height > sv->max_h || rq->num_disparities > sv->max_nd || !rq->left || !rq->right || rq->stride < in_w ||
         !rq->out.disparity || rq->out.sum_volume || rq->out.cost_volume || rq->out.census_left || rq->out.census_right) {
         std::lock_guard lk(sv->mu);
-        sv->err = "sgm_server_submit: size, disparity range, pointers or outputs not servable" + (err.empty() ? "" : ": " + err);
+        sv->err = "sgm_server_submit: size, disparity range, rectification, pointers or outputs not servable" +
+                  (err.empty() ? "" : ": " + err);
         return SGM_ERR_INVALID_ARGUMENT;
     }
     Request r;
+    r.rect = ms;
     r.tag = rq->user_tag;
     r.w = rq->width;
     r.h = rq->height;
     r.min_d = rq->min_disparity;
     r.nd = rq->num_disparities;
     r.out = rq->out;
-    r.bytes = footprint(r.w, r.h, r.min_d, r.nd, sv->ndirs).total;
+    const size_t in_px = (size_t)in_w * in_h;
+    r.bytes = footprint(r.w, r.h, r.min_d, r.nd, sv->ndirs, ms.d_maps ? in_px : 0).total;
     r.t_submit = now_ns();
-    {
-        std::lock_guard lk(sv->mu);
-        if (!sv->pinned_free.empty()) {
-            r.pinned = sv->pinned_free.back();
-            sv->pinned_free.pop_back();
+    if (rq->borrow_inputs) {
+        r.borrowed[0] = rq->left;
+        r.borrowed[1] = rq->right;
+        r.borrowed_stride = rq->stride;
+    } else {
+        {
+            std::lock_guard lk(sv->mu);
+            r.pinned = sv->take_pinned(2 * in_px, r.pinned_class);
         }
-    }
-    uint8_t* dst = r.pinned;
-    if (!dst) {
-        r.img.resize(2 * (size_t)r.w * r.h);
-        dst = r.img.data();
-    }
-    for (int y = 0; y < r.h; ++y) {
-        std::memcpy(dst + (size_t)y * r.w, rq->left + (ptrdiff_t)y * rq->stride, (size_t)r.w);
-        std::memcpy(dst + ((size_t)r.h + y) * r.w, rq->right + (ptrdiff_t)y * rq->stride, (size_t)r.w);
+        uint8_t* dst = r.pinned;
+        if (!dst) {
+            r.img.resize(2 * in_px);
+            dst = r.img.data();
+        }
+        for (int y = 0; y < in_h; ++y) {
+            std::memcpy(dst + (size_t)y * in_w, rq->left + (ptrdiff_t)y * rq->stride, (size_t)in_w);
+            std::memcpy(dst + ((size_t)in_h + y) * in_w, rq->right + (ptrdiff_t)y * rq->stride, (size_t)in_w);
+        }
     }
     std::lock_guard lk(sv->mu);
     if (sv->stopping || !sv->aq.can_ever_fit(r.bytes)) {
-        if (r.pinned) sv->pinned_free.push_back(r.pinned);
+        if (r.pinned) sv->pinned_free[r.pinned_class].push_back(r.pinned);
         if (sv->stopping) {
             sv->err = "sgm_server_submit: server is stopping";
             return SGM_ERR_DISCONNECTED;
@@ -355,6 +436,53 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
     sv->aq.push(r.ticket, r.bytes);
     sv->reqs.emplace(r.ticket, std::move(r));
     sv->dispatch(sv->aq.admit());
+    return SGM_OK;
+}
+
+int sgm_server_add_rectification(sgm_server* sv, const float* left_map_x, const float* left_map_y,
+                                 const float* right_map_x, const float* right_map_y, int map_w, int map_h, int raw_w,
+                                 int raw_h, int32_t* id) {
+    if (!sv || !id) return SGM_ERR_INVALID_ARGUMENT;
+    auto fail = [&](int code, const std::string& m) {
+        std::lock_guard lk(sv->mu);
+        sv->err = "sgm_server_add_rectification: " + m;
+        return code;
+    };
+    if (!left_map_x || !left_map_y || !right_map_x || !right_map_y) return fail(SGM_ERR_INVALID_ARGUMENT, "null map");
+    if (map_w < 2 || map_h < 2 || map_w / 2 > sv->max_w || map_h / 2 > sv->max_h)
+        return fail(SGM_ERR_INVALID_ARGUMENT, "remap resolution must be twice the stereo resolution (at most the server's maximum)");
+    if (raw_w < 1 || raw_h < 1 || raw_w >= (1 << 24) || raw_h >= (1 << 24))
+        return fail(SGM_ERR_INVALID_ARGUMENT, "raw size must be in [1, 2^24)");
+    MapSet m;
+    m.map_w = map_w;
+    m.map_h = map_h;
+    m.raw_w = raw_w;
+    m.raw_h = raw_h;
+    const size_t n = (size_t)map_w * map_h;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&m.d_maps), 16 * n);
+    std::vector<float> packed(e == hipSuccess ? 2 * n : 0);
+    const float* mx[2] = {left_map_x, right_map_x};
+    const float* my[2] = {left_map_y, right_map_y};
+    for (int side = 0; side < 2 && e == hipSuccess; ++side) {
+        for (size_t i = 0; i < n; ++i) {
+            packed[2 * i] = mx[side][i];
+            packed[2 * i + 1] = my[side][i];
+        }
+        e = hipMemcpy(m.d_maps + 2 * n * side, packed.data(), 8 * n, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (m.d_maps) (void)hipFree(m.d_maps);
+        return fail(SGM_ERR_IO, std::string("maps: ") + hipGetErrorString(e));
+    }
+    std::lock_guard lk(sv->mu);
+    const size_t raw_bytes = 2 * (size_t)raw_w * raw_h;
+    if (sv->pinned_free.lower_bound(raw_bytes) == sv->pinned_free.end()) {
+        // A size class for this set's raw frames; if pinned memory runs out, submissions of such
+        // frames fall back to heap staging.
+        (void)sv->add_pinned_class(raw_bytes, 2 * (int)sv->slots.size() + 2);
+    }
+    sv->map_sets.push_back(m);
+    *id = (int32_t)sv->map_sets.size();
     return SGM_OK;
 }
 

@@ -77,7 +77,8 @@ class CSgmServerConfig(C.Structure):
 class CSgmRequest(C.Structure):
     _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("stride", C.c_int64), ("width", C.c_int32),
                 ("height", C.c_int32), ("min_disparity", C.c_int32), ("num_disparities", C.c_int32),
-                ("out", CSgmOutputs), ("user_tag", C.c_uint64)]
+                ("out", CSgmOutputs), ("user_tag", C.c_uint64), ("rectification", C.c_int32),
+                ("borrow_inputs", C.c_int32)]
 
 
 class CSgmCompletion(C.Structure):
@@ -180,6 +181,7 @@ def lib():
         L.sgm_batch_destroy.argtypes = [vp]
         L.sgm_server_start.argtypes = [P(CSgmParams), P(CSgmServerConfig), P(vp)]
         L.sgm_server_submit.argtypes = [vp, P(CSgmRequest), P(C.c_uint64)]
+        L.sgm_server_add_rectification.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, P(C.c_int32)]
         L.sgm_server_poll.argtypes = [vp, P(CSgmCompletion), i32, i32, P(C.c_int)]
         L.sgm_server_get_stats.argtypes = [vp, P(CSgmServerStats)]
         L.sgm_server_last_error.argtypes = [vp]
@@ -531,23 +533,48 @@ class Server:
         self._h = h.value
         self._pending = {}  # ticket -> outputs dict
         self._pool = PinnedPool()
+        self._rect_sizes = {}  # map set id -> rectified (width, height)
+        self._borrowed = {}    # ticket -> input arrays read in place, kept alive until polled
 
     @staticmethod
     def now_ns() -> int:
         return lib().sgm_server_now_ns()
 
+    def add_rectification(self, left_map_x, left_map_y, right_map_x, right_map_y, raw_width: int,
+                          raw_height: int) -> int:
+        """Registers a resident map set (sgm_server_add_rectification); returns its id for submit()."""
+        maps = [np.ascontiguousarray(m, dtype=np.float32) for m in (left_map_x, left_map_y, right_map_x, right_map_y)]
+        if any(m.ndim != 2 or m.shape != maps[0].shape for m in maps):
+            raise SgmError(5, "the four maps must be 2-D arrays of one size")
+        mh, mw = maps[0].shape
+        i = C.c_int32()
+        rc = lib().sgm_server_add_rectification(self._h, *[m.ctypes.data for m in maps], mw, mh, raw_width, raw_height,
+                                                C.byref(i))
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_server_last_error(self._h) or b"").decode())
+        self._rect_sizes[i.value] = (mw // 2, mh // 2)
+        return i.value
+
     def submit(self, left: np.ndarray, right: np.ndarray, min_disparity: int, num_disparities: int,
-               outputs=("disparity",), tag: int = 0) -> int:
+               outputs=("disparity",), tag: int = 0, rectification: int = 0, borrow: bool = False) -> int:
+        """A rectified pair, or (rectification = a map set id) a raw pair rectified on the device.
+        borrow=True: the arrays are read in place and must stay unchanged until the completion is
+        polled (contiguous uint8 arrays, ideally in pinned memory)."""
+        if borrow and not all(a.dtype == np.uint8 and a.flags.c_contiguous for a in (left, right)):
+            raise SgmError(5, "borrowed inputs must be C-contiguous uint8 arrays")
         left = np.ascontiguousarray(left, dtype=np.uint8)
         right = np.ascontiguousarray(right, dtype=np.uint8)
-        h, w = left.shape
+        ih, iw = left.shape
+        w, h = self._rect_sizes.get(rectification, (iw, ih)) if rectification else (iw, ih)
         res = {f: self._pool.take((h, w), OUTPUT_DTYPES[f]) for f in set(outputs) | {"disparity"}}
         rq = CSgmRequest()
-        rq.left, rq.right, rq.stride = left.ctypes.data, right.ctypes.data, w
+        rq.left, rq.right, rq.stride = left.ctypes.data, right.ctypes.data, iw
         rq.width, rq.height, rq.min_disparity, rq.num_disparities = w, h, min_disparity, num_disparities
         for f, a in res.items():
             setattr(rq.out, f, a.ctypes.data)
         rq.user_tag = tag
+        rq.rectification = rectification
+        rq.borrow_inputs = int(borrow)
         t = C.c_uint64()
         rc = lib().sgm_server_submit(self._h, C.byref(rq), C.byref(t))
         if rc != 0:
@@ -555,6 +582,8 @@ class Server:
                 self._pool.give(a)
             raise SgmError(rc, (lib().sgm_server_last_error(self._h) or b"").decode())
         self._pending[t.value] = res
+        if borrow:
+            self._borrowed[t.value] = (left, right)
         return t.value
 
     def poll(self, max_n: int = 64, timeout_us: int = 0) -> list:
@@ -568,6 +597,7 @@ class Server:
             out.append({"ticket": c.ticket, "tag": c.user_tag, "status": c.status, "t_submit_ns": c.t_submit_ns,
                         "t_admit_ns": c.t_admit_ns, "t_done_ns": c.t_done_ns, "gpu_ms": c.gpu_ms,
                         "outputs": self._pending.pop(c.ticket, None)})
+            self._borrowed.pop(c.ticket, None)
         return out
 
     def release(self, completion: dict):

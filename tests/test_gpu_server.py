@@ -92,3 +92,50 @@ def test_oversize_and_invalid_requests_are_refused():
             sv.submit(np.zeros((300, 400), np.uint8), np.zeros((300, 400), np.uint8), 0, 64)
         assert e.value.errc == "invalid_argument"
         assert sv.stats()["completed"] == 0
+
+
+def test_raw_frames_rectified_with_resident_maps():
+    """Raw frames of two map sets (full frame and crop size) mixed with rectified frames: each result
+    equals compute_raw alone and the oracle on the oracle's rectified pair."""
+    from open_source_einstar_amd.synth import rectification_maps
+
+    sets = [(640, 512, 660, 530, 64), (320, 256, 300, 240, 48)]  # map w, h, raw w, h, D
+    maps = [rectification_maps(mw, mh, rw, rh, 70 + i) for i, (mw, mh, rw, rh, _) in enumerate(sets)]
+    raws = [speckle_pair(rw, rh, 16, 80 + i) for i, (_, _, rw, rh, _) in enumerate(sets * 2)]
+    rect = frames(2, 900)
+    pinned = sgm.PinnedPool()
+    with sgm.Server(sgm.SgmParams(**BASE), 320, 256, 96, slots=3) as sv:
+        ids = [sv.add_rectification(m.left_x, m.left_y, m.right_x, m.right_y, s[2], s[3]) for m, s in zip(maps, sets)]
+        assert ids == [1, 2]
+        jobs = []
+        for i, p in enumerate(raws):
+            k = i % 2
+            left, right, borrow = p.left, p.right, i >= 2
+            if borrow:  # the second half is read in place from pinned host memory
+                left, right = pinned.take(p.left.shape, np.uint8), pinned.take(p.right.shape, np.uint8)
+                left[:], right[:] = p.left, p.right
+            jobs.append(("raw", k, p, sv.submit(left, right, -4, sets[k][4], outputs=SERVE_FIELDS, tag=len(jobs),
+                                                rectification=ids[k], borrow=borrow)))
+            q = rect[i % 2]
+            jobs.append(("rect", None, q, sv.submit(q.left, q.right, q.min_disparity, q.num_disparities,
+                                                    outputs=SERVE_FIELDS, tag=len(jobs))))
+        got = {c["tag"]: c for c in drain(sv, len(jobs))}
+        with pytest.raises(sgm.SgmError):  # raw frames of the wrong size for the set
+            sv.submit(raws[1].left, raws[1].right, -4, 64, rectification=ids[0])
+    pinned.free_all()
+    assert len(got) == len(jobs) and all(c["status"] == 0 for c in got.values())
+    for tag, (kind, k, p, _) in enumerate(jobs):
+        c = got[tag]
+        if kind == "rect":
+            assert_same(alone(p), c["outputs"], SERVE_FIELDS, tag=f"rectified frame {tag}")
+            continue
+        mw, mh, rw, rh, nd = sets[k]
+        m = maps[k]
+        kw = dict(BASE, min_disparity=-4, num_disparities=nd)
+        with sgm.HipStereo.create(sgm.SgmParams(**kw), mw // 2, mh // 2) as st:
+            st.set_rectification(m.left_x, m.left_y, m.right_x, m.right_y, rw, rh)
+            a = st.compute_raw(p.left, p.right, outputs=SERVE_FIELDS)
+        assert_same(a, c["outputs"], SERVE_FIELDS, tag=f"raw frame {tag} vs compute_raw alone")
+        rl = O.rectify_half(p.left, m.left_x, m.left_y)
+        rr = O.rectify_half(p.right, m.right_x, m.right_y)
+        assert_same(O.sgm(rl, rr, O.SgmParams(**kw)), c["outputs"], SERVE_FIELDS, tag=f"raw frame {tag} vs oracle")
