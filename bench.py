@@ -177,13 +177,13 @@ def large_batched(sgm, speckle_pair, resident=16, total=64, distinct=8, verify=T
         b.close()
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the path kernel at this workload, from the committed counter summary."""
-    path = os.path.join(ROOT, "profiles", "paths_kernel_1280x1024x128_traffic.json")
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` at this workload, from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "sgm_1280x1024x128_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
+            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -324,6 +324,7 @@ def main():
     rect_ms = rt["rectify_ms"] / max(1, rt["count"])
     rect_bytes = 2 * (4 * W * H * 8) + 2 * (4 * W * H) + 2 * (W * H)
     rectify = {"kernel_ms": round(rect_ms, 4), "bytes_per_launch": rect_bytes,
+               "traffic": traffic_from_profiles("rectify_kernel"),
                "achieved_gbs": round(rect_bytes / (rect_ms * 1e-3) / 1e9, 1),
                "frac": round(rect_bytes / (rect_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
                "raw_ms_per_step": round(raw_ms, 4), "raw_value": round(pxd / (raw_ms * 1e-3) / 1e6, 1),
@@ -334,10 +335,10 @@ def main():
 
     agg_bytes = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
     achieved = agg_bytes / (paths_ms * 1e-3) / 1e9
+    kernel = f"paths_kernel<8, false, {'true' if best_cv else 'false'}>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                "kernel": "paths_kernel<8,false>", "kernel_ms": round(paths_ms, 4),
-                "algorithmic_bytes": agg_bytes}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel),
+                "kernel": kernel, "kernel_ms": round(paths_ms, 4), "algorithmic_bytes": agg_bytes}
 
     batched = None
     if not args.no_batch:
