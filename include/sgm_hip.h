@@ -256,8 +256,8 @@ void sgm_batch_destroy(sgm_batch* b);
  * pipeline queues frames for one worker, libs/pipeline/src/scan_pipeline.cpp:512-609).
  *
  * sgm_server_start reserves an HBM arena (hbm_budget_bytes; 0 = half the free HBM, at most 64 GiB)
- * and `slots` streams with worker threads. A submitted request (host images, copied at submission,
- * so the caller may reuse them) waits in arrival order; the head is admitted when a slot is free and
+ * and `slots` streams with worker threads. A submitted request (host images, copied at submission
+ * unless it borrows them) waits in arrival order; the head is admitted when a slot is free and
  * its device footprint fits the arena. Frames arriving while others run start as soon as a slot and
  * memory free up and overlap on the device. Results are copied straight into the request's host
  * output buffers (pinned ones, e.g. from sgm_host_alloc, take a direct DMA), which must stay valid
