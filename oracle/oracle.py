@@ -76,6 +76,47 @@ class SgmParams:
                          float(self.uniqueness), self.lr_max_diff, int(self.subpixel))
 
 
+class OrcStereoParams(C.Structure):
+    _fields_ = [
+        ("pyramid_levels", C.c_int32),
+        ("refine_search_radius", C.c_int32), ("refine_zncc_radius", C.c_int32), ("refine_min_zncc", C.c_float),
+        ("refine_reject_edge_winners", C.c_int32),
+        ("detail_enabled", C.c_int32), ("detail_radius", C.c_int32), ("detail_search", C.c_float),
+        ("detail_step", C.c_float), ("detail_min_zncc", C.c_float),
+        ("filter_occlusion", C.c_int32), ("filter_occlusion_tolerance", C.c_float), ("filter_spikes", C.c_int32),
+        ("filter_spike_threshold", C.c_float),
+        ("speckle_max_region_size", C.c_int32), ("speckle_max_diff", C.c_float),
+    ]
+
+
+@dataclass
+class StereoParams:
+    """depth::StereoParams (libs/depth/include/einstar/depth/stereo.hpp:42-89) minus sgm/subpixel,
+    with the reference defaults."""
+
+    pyramid_levels: int = 2
+    refine_search_radius: int = 2
+    refine_zncc_radius: int = 3
+    refine_min_zncc: float = 0.5
+    refine_reject_edge_winners: bool = True
+    detail_enabled: bool = True
+    detail_radius: int = 5
+    detail_search: float = 1.0
+    detail_step: float = 0.5
+    detail_min_zncc: float = 0.5
+    filter_occlusion: bool = True
+    filter_occlusion_tolerance: float = 1.0
+    filter_spikes: bool = True
+    filter_spike_threshold: float = 1.0
+    speckle_max_region_size: int = 200
+    speckle_max_diff: float = 1.0
+
+    def to_c(self) -> OrcStereoParams:
+        return OrcStereoParams(*[type_(getattr(self, f)) if type_ is float else int(getattr(self, f))
+                                 for f, type_ in ((f, float if t is C.c_float else int)
+                                                  for f, t in OrcStereoParams._fields_)])
+
+
 _lib = None
 
 
@@ -101,6 +142,19 @@ def lib():
         L.orc_rectify_half.argtypes = [vp, i32, i32, C.c_ssize_t, vp, vp, i32, i32, vp, i32]
         L.orc_rectify_half.restype = i32
         L.orc_sgm.restype = i32
+        SP = C.POINTER(OrcStereoParams)
+        L.orc_zncc_slanted.argtypes = [vp, vp, C.c_ssize_t, i32, i32, i32, f32, f32, f32, i32]
+        L.orc_zncc_slanted.restype = f32
+        L.orc_zncc_window.argtypes = [vp, vp, C.c_ssize_t, i32, i32, i32, i32, f32, f32, f32, f32, f32]
+        L.orc_zncc_window.restype = f32
+        L.orc_refine_level.argtypes = [vp, vp, C.c_ssize_t, i32, i32, vp, i32, i32, SP, i32, vp, vp, i32]
+        L.orc_refine_detail.argtypes = [vp, vp, C.c_ssize_t, i32, i32, vp, i32, i32, SP, i32, i32]
+        L.orc_filter_disparity.argtypes = [vp, i32, i32, SP, i32]
+        L.orc_remove_speckles.argtypes = [vp, i32, i32, SP]
+        L.orc_remove_speckles.restype = i32
+        L.orc_compute_disparity.argtypes = [vp, vp, C.c_ssize_t, i32, i32, vp, vp, C.c_ssize_t, i32, i32,
+                                            C.POINTER(OrcParams), SP, vp, vp, i32]
+        L.orc_compute_disparity.restype = i32
         _lib = L
     return _lib
 
@@ -247,3 +301,83 @@ def rectify_half(raw, map_x, map_y, threads: int = 0) -> np.ndarray:
     if lib().orc_rectify_half(_ptr(raw), rw, rh, rw, _ptr(mx), _ptr(my), w, h, _ptr(out), threads) != 0:
         raise MemoryError("oracle rectify_half: allocation failed")
     return out
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def zncc_slanted(a, b, x, y, d, gx, gy, r) -> float:
+    """zncc_slanted (libs/depth/src/stereo.cpp:101-125)."""
+    a, b = _u8(a), _u8(b)
+    assert a.shape == b.shape
+    return lib().orc_zncc_slanted(_ptr(a), _ptr(b), a.shape[1], a.shape[1], x, y, d, gx, gy, r)
+
+
+def zncc_window(a, b, x0, y0, n, cx, cy, D, gx, gy) -> float:
+    """zncc_window (libs/depth/src/stereo.cpp:211-239)."""
+    a, b = _u8(a), _u8(b)
+    assert a.shape == b.shape
+    return lib().orc_zncc_window(_ptr(a), _ptr(b), a.shape[1], a.shape[1], x0, y0, n, cx, cy, D, gx, gy)
+
+
+def refine_level(left, right, coarse_raw, sp: StereoParams, subpixel=True, threads: int = 0):
+    """refine_level (libs/depth/src/stereo.cpp:151-207): (disparity, confidence) at left's size."""
+    left, right = _u8(left), _u8(right)
+    coarse = np.ascontiguousarray(coarse_raw, dtype=np.float32)
+    h, w = left.shape
+    ch, cw = coarse.shape
+    d = np.empty((h, w), np.float32)
+    c = np.empty((h, w), np.float32)
+    csp = sp.to_c()
+    lib().orc_refine_level(_ptr(left), _ptr(right), w, w, h, _ptr(coarse), cw, ch, C.byref(csp), int(subpixel),
+                           _ptr(d), _ptr(c), threads)
+    return d, c
+
+
+def refine_detail(full_left, full_right, disparity, sp: StereoParams, subpixel=True, threads: int = 0):
+    """refine_detail (libs/depth/src/stereo.cpp:243-284) on a copy of disparity."""
+    fl, fr = _u8(full_left), _u8(full_right)
+    d = np.array(disparity, dtype=np.float32, copy=True)
+    H, W = fl.shape
+    h, w = d.shape
+    csp = sp.to_c()
+    lib().orc_refine_detail(_ptr(fl), _ptr(fr), W, W, H, _ptr(d), w, h, C.byref(csp), int(subpixel), threads)
+    return d
+
+
+def filter_disparity(disparity, sp: StereoParams, threads: int = 0):
+    """filter_disparity (libs/depth/src/stereo.cpp:286-326) on a copy."""
+    d = np.array(disparity, dtype=np.float32, copy=True)
+    h, w = d.shape
+    csp = sp.to_c()
+    lib().orc_filter_disparity(_ptr(d), w, h, C.byref(csp), threads)
+    return d
+
+
+def remove_speckles(disparity, sp: StereoParams):
+    """remove_speckles (libs/depth/src/stereo.cpp:465-494) on a copy."""
+    d = np.array(disparity, dtype=np.float32, copy=True)
+    h, w = d.shape
+    csp = sp.to_c()
+    if lib().orc_remove_speckles(_ptr(d), w, h, C.byref(csp)) != 0:
+        raise MemoryError("oracle remove_speckles: allocation failed")
+    return d
+
+
+def compute_disparity(left, right, sgm_params: SgmParams, sp: StereoParams, full_left=None, full_right=None,
+                      threads: int = 0) -> dict:
+    """compute_disparity (libs/depth/src/stereo.cpp:496-536), both forms."""
+    left, right = _u8(left), _u8(right)
+    h, w = left.shape
+    fl = None if full_left is None else _u8(full_left)
+    fr = None if full_right is None else _u8(full_right)
+    fh, fw = (0, 0) if fl is None else fl.shape
+    disp = np.empty((h, w), np.float32)
+    conf = np.empty((h, w), np.float32)
+    cp, csp = sgm_params.to_c(), sp.to_c()
+    rc = lib().orc_compute_disparity(_ptr(left), _ptr(right), w, w, h, _ptr(fl), _ptr(fr), fw, fw, fh, C.byref(cp),
+                                     C.byref(csp), _ptr(disp), _ptr(conf), threads)
+    if rc != 0:
+        raise MemoryError("oracle compute_disparity failed")
+    return {"disparity": disp, "confidence": conf}

@@ -510,3 +510,410 @@ done:
     if (rdisp != out->right_disparity) free(rdisp);
     return rc;
 }
+
+/* ---------------------------------------------------------------- refinement (stereo.cpp:98-284) */
+
+static inline int valid_d(float d) { return d > -1.0e8f; } /* valid_disparity, stereo.hpp:24 */
+
+static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); } /* std::clamp */
+
+float orc_zncc_slanted(const uint8_t* a, const uint8_t* b, ptrdiff_t stride, int bw, int x, int y, float d, float gx,
+                       float gy, int r) {
+    double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+    const int n = (2 * r + 1) * (2 * r + 1);
+    for (int v = -r; v <= r; ++v) {
+        const uint8_t* ra = a + (ptrdiff_t)(y + v) * stride;
+        const uint8_t* rb = b + (ptrdiff_t)(y + v) * stride;
+        for (int u = -r; u <= r; ++u) {
+            const float xb = (float)(x + u) - (d + gx * (float)u + gy * (float)v);
+            const int x0 = (int)floorf(xb);
+            if (x0 < 0 || x0 + 1 >= bw) return -2.0f;
+            const float f = xb - (float)x0;
+            const double ib = rb[x0] * (1.0f - f) + rb[x0 + 1] * f;
+            const double ia = ra[x + u];
+            sa += ia;
+            sb += ib;
+            saa += ia * ia;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    const double va = saa - sa * sa / n;
+    const double vb = sbb - sb * sb / n;
+    if (va < 1e-6 || vb < 1e-6) return -1.0f;
+    return (float)((sab - sa * sb / n) / sqrt(va * vb));
+}
+
+typedef struct {
+    const uint8_t *left, *right;
+    ptrdiff_t stride;
+    int w, h;
+    const float* coarse; /* median3 of the coarse disparity */
+    int cw, ch;
+    const orc_stereo_params* sp;
+    int subpixel;
+    float *out_disp, *out_conf;
+} refine_ctx;
+
+/* The slope lambda of refine_level (stereo.cpp:159-170). */
+static void coarse_slope(const refine_ctx* c, int cx, int cy, float* gx, float* gy) {
+    const float* C = c->coarse;
+    const int cw = c->cw, ch = c->ch;
+    *gx = *gy = 0.0f;
+    const float m = C[(size_t)cy * cw + cx];
+    if (cx > 0 && cx + 1 < cw && valid_d(C[(size_t)cy * cw + cx - 1]) && valid_d(C[(size_t)cy * cw + cx + 1])) {
+        const float l = C[(size_t)cy * cw + cx - 1], r = C[(size_t)cy * cw + cx + 1];
+        const float g = 0.5f * (r - l);
+        if (fabsf(r - m) < 2 && fabsf(m - l) < 2) *gx = g;
+    }
+    if (cy > 0 && cy + 1 < ch && valid_d(C[(size_t)(cy - 1) * cw + cx]) && valid_d(C[(size_t)(cy + 1) * cw + cx])) {
+        const float u = C[(size_t)(cy - 1) * cw + cx], dn = C[(size_t)(cy + 1) * cw + cx];
+        const float g = 0.5f * (dn - u);
+        if (fabsf(dn - m) < 2 && fabsf(m - u) < 2) *gy = g;
+    }
+    *gx = clampf(*gx, -0.9f, 0.9f);
+    *gy = clampf(*gy, -0.9f, 0.9f);
+}
+
+static void refine_rows(void* vctx, int y0, int y1) {
+    const refine_ctx* c = (const refine_ctx*)vctx;
+    const int r = c->sp->refine_zncc_radius;
+    const float kStep = 0.5f;
+    for (int y = y0; y < y1; ++y) {
+        if (y < r || y >= c->h - r) continue;
+        const int cy = imin(y / 2, c->ch - 1);
+        float scores[128];
+        for (int x = r; x < c->w - r; ++x) {
+            const int cx = imin(x / 2, c->cw - 1);
+            const float d0 = c->coarse[(size_t)cy * c->cw + cx];
+            if (!valid_d(d0)) continue;
+            float gx, gy;
+            coarse_slope(c, cx, cy, &gx, &gy);
+            const float center = 2.0f * d0;
+            const int nd = imin(127, (int)((float)(2 * c->sp->refine_search_radius) / kStep) + 1);
+            const float lo = center - (float)c->sp->refine_search_radius;
+            float best = -2.0f;
+            int best_k = -1;
+            for (int k = 0; k < nd; ++k) {
+                const float d = lo + kStep * (float)k;
+                scores[k] = orc_zncc_slanted(c->left, c->right, c->stride, c->w, x, y, d, gx, gy, r);
+                if (scores[k] > best) {
+                    best = scores[k];
+                    best_k = k;
+                }
+            }
+            if (best_k < 0 || best < c->sp->refine_min_zncc) continue;
+            if (c->sp->refine_reject_edge_winners && (best_k == 0 || best_k == nd - 1)) continue;
+            float d = lo + kStep * (float)best_k;
+            if (c->subpixel && best_k > 0 && best_k + 1 < nd && scores[best_k - 1] > -2.0f && scores[best_k + 1] > -2.0f)
+                d += kStep * orc_subpixel_offset(1.0f - scores[best_k - 1], 1.0f - best, 1.0f - scores[best_k + 1]);
+            c->out_disp[(size_t)y * c->w + x] = d;
+            c->out_conf[(size_t)y * c->w + x] = best;
+        }
+    }
+}
+
+void orc_refine_level(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const float* coarse_raw,
+                      int cw, int ch, const orc_stereo_params* sp, int subpixel, float* out_disp, float* out_conf,
+                      int threads) {
+    for (size_t i = 0; i < (size_t)w * (size_t)h; ++i) {
+        out_disp[i] = ORC_INVALID_DISPARITY;
+        out_conf[i] = 0.0f;
+    }
+    float* coarse = (float*)malloc(sizeof(float) * ((size_t)cw * (size_t)ch + 1));
+    if (!coarse) return;
+    orc_median3(coarse_raw, cw, ch, coarse, threads);
+    refine_ctx c = {left, right, stride, w, h, coarse, cw, ch, sp, subpixel, out_disp, out_conf};
+    parallel_for(h, threads, refine_rows, &c);
+    free(coarse);
+}
+
+float orc_zncc_window(const uint8_t* a, const uint8_t* b, ptrdiff_t stride, int bw, int x0, int y0, int n, float cx,
+                      float cy, float D, float gx, float gy) {
+    double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+    for (int j = 0; j < n; ++j) {
+        const int Y = y0 + j;
+        const uint8_t* ra = a + (ptrdiff_t)Y * stride;
+        const uint8_t* rb = b + (ptrdiff_t)Y * stride;
+        const float dy = gy * ((float)Y - cy);
+        for (int i = 0; i < n; ++i) {
+            const int X = x0 + i;
+            const float xb = (float)X - (D + gx * ((float)X - cx) + dy);
+            const int xi = (int)floorf(xb);
+            if (xi < 0 || xi + 1 >= bw) return -2.0f;
+            const float f = xb - (float)xi;
+            const double ib = rb[xi] * (1.0f - f) + rb[xi + 1] * f;
+            const double ia = ra[X];
+            sa += ia;
+            sb += ib;
+            saa += ia * ia;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    const double m = (double)n * n;
+    const double va = saa - sa * sa / m;
+    const double vb = sbb - sb * sb / m;
+    if (va < 1e-6 || vb < 1e-6) return -1.0f;
+    return (float)((sab - sa * sb / m) / sqrt(va * vb));
+}
+
+typedef struct {
+    const uint8_t *fl, *fr;
+    ptrdiff_t fstride;
+    int W, H;
+    const float* first;
+    float* disparity;
+    int w, h;
+    const orc_stereo_params* sp;
+    int subpixel;
+} detail_ctx;
+
+/* The grad lambda of refine_detail (stereo.cpp:252-255). */
+static float detail_grad(float l, float c, float r) {
+    if (!valid_d(l) || !valid_d(r) || fabsf(r - c) >= 1.0f || fabsf(c - l) >= 1.0f) return 0.0f;
+    return clampf(0.5f * (r - l), -0.9f, 0.9f);
+}
+
+static void detail_rows(void* vctx, int ya, int yb) {
+    const detail_ctx* c = (const detail_ctx*)vctx;
+    const orc_stereo_params* p = c->sp;
+    const int K = imin(16, (int)lroundf(p->detail_search / p->detail_step));
+    const int nc = 2 * K + 1, n = 2 * p->detail_radius;
+    const float* F = c->first;
+    const int w = c->w;
+    for (int y = ya; y < yb; ++y) {
+        float scores[33];
+        const int y0 = 2 * y - p->detail_radius + 1;
+        if (y < 1 || y + 1 >= c->h || y0 < 0 || y0 + n > c->H) continue;
+        for (int x = 1; x + 1 < w; ++x) {
+            const float d = F[(size_t)y * w + x];
+            if (!valid_d(d)) continue;
+            const int x0 = 2 * x - p->detail_radius + 1;
+            if (x0 < 0 || x0 + n > c->W) continue;
+            const float gx = detail_grad(F[(size_t)y * w + x - 1], d, F[(size_t)y * w + x + 1]);
+            const float gy = detail_grad(F[(size_t)(y - 1) * w + x], d, F[(size_t)(y + 1) * w + x]);
+            const float cx = 2.0f * (float)x + 0.5f, cy = 2.0f * (float)y + 0.5f;
+            const float D0 = 2.0f * d;
+            float best = -2.0f;
+            int best_k = -1;
+            for (int k = 0; k < nc; ++k) {
+                scores[k] = orc_zncc_window(c->fl, c->fr, c->fstride, c->W, x0, y0, n, cx, cy,
+                                            D0 + p->detail_step * (float)(k - K), gx, gy);
+                if (scores[k] > best) {
+                    best = scores[k];
+                    best_k = k;
+                }
+            }
+            if (best_k <= 0 || best_k + 1 >= nc || best < p->detail_min_zncc || scores[best_k - 1] <= -2.0f ||
+                scores[best_k + 1] <= -2.0f)
+                continue;
+            float D = D0 + p->detail_step * (float)(best_k - K);
+            if (c->subpixel)
+                D += p->detail_step * orc_subpixel_offset(1.0f - scores[best_k - 1], 1.0f - best, 1.0f - scores[best_k + 1]);
+            c->disparity[(size_t)y * w + x] = 0.5f * D;
+        }
+    }
+}
+
+void orc_refine_detail(const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride, int W, int H,
+                       float* disparity, int w, int h, const orc_stereo_params* sp, int subpixel, int threads) {
+    if (!sp->detail_enabled || sp->detail_radius < 1 || sp->detail_step <= 0) return;
+    float* first = (float*)malloc(sizeof(float) * ((size_t)w * (size_t)h + 1));
+    if (!first) return;
+    memcpy(first, disparity, sizeof(float) * (size_t)w * (size_t)h);
+    detail_ctx c = {full_left, full_right, full_stride, W, H, first, disparity, w, h, sp, subpixel};
+    parallel_for(h, threads, detail_rows, &c);
+    free(first);
+}
+
+/* ---------------------------------------------------------------- filters (stereo.cpp:286-326, 465-494) */
+
+typedef struct {
+    float* disp;
+    const float* in; /* spikes: the disparity after the occlusion filter */
+    int w, h;
+    const orc_stereo_params* sp;
+} filt_ctx;
+
+static void occlusion_rows(void* vctx, int y0, int y1) {
+    const filt_ctx* c = (const filt_ctx*)vctx;
+    const int w = c->w;
+    float* nearest = (float*)malloc(sizeof(float) * ((size_t)w + 1));
+    if (!nearest) return;
+    for (int y = y0; y < y1; ++y) {
+        float* row = c->disp + (size_t)y * w;
+        for (int x = 0; x < w; ++x) nearest[x] = ORC_INVALID_DISPARITY;
+        for (int x = 0; x < w; ++x) {
+            const float d = row[x];
+            if (!valid_d(d)) continue;
+            const long xr = lroundf((float)x - d);
+            if (xr >= 0 && xr < w) nearest[xr] = nearest[xr] < d ? d : nearest[xr]; /* std::max(nearest, d) */
+        }
+        for (int x = 0; x < w; ++x) {
+            const float d = row[x];
+            if (!valid_d(d)) continue;
+            const long xr = lroundf((float)x - d);
+            if (xr >= 0 && xr < w && d < nearest[xr] - c->sp->filter_occlusion_tolerance) row[x] = ORC_INVALID_DISPARITY;
+        }
+    }
+    free(nearest);
+}
+
+static void spike_rows(void* vctx, int y0, int y1) {
+    const filt_ctx* c = (const filt_ctx*)vctx;
+    const int w = c->w, h = c->h;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const float d = c->in[(size_t)y * w + x];
+            if (!valid_d(d)) continue;
+            float buf[9];
+            int n = 0;
+            for (int v = -1; v <= 1; ++v)
+                for (int u = -1; u <= 1; ++u) {
+                    const int xx = x + u, yy = y + v;
+                    if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                    const float q = c->in[(size_t)yy * w + xx];
+                    if (!valid_d(q)) continue;
+                    buf[n++] = q;
+                }
+            if (n < 3) continue;
+            for (int a = 1; a < n; ++a) { /* n/2-th smallest, as std::nth_element leaves it */
+                const float t = buf[a];
+                int b = a - 1;
+                while (b >= 0 && buf[b] > t) {
+                    buf[b + 1] = buf[b];
+                    --b;
+                }
+                buf[b + 1] = t;
+            }
+            if (fabsf(d - buf[n / 2]) > c->sp->filter_spike_threshold) c->disp[(size_t)y * w + x] = ORC_INVALID_DISPARITY;
+        }
+    }
+}
+
+void orc_filter_disparity(float* disparity, int w, int h, const orc_stereo_params* sp, int threads) {
+    filt_ctx c = {disparity, NULL, w, h, sp};
+    if (sp->filter_occlusion) parallel_for(h, threads, occlusion_rows, &c);
+    if (sp->filter_spikes) {
+        float* in = (float*)malloc(sizeof(float) * ((size_t)w * (size_t)h + 1));
+        if (!in) return;
+        memcpy(in, disparity, sizeof(float) * (size_t)w * (size_t)h);
+        c.in = in;
+        parallel_for(h, threads, spike_rows, &c);
+        free(in);
+    }
+}
+
+int orc_remove_speckles(float* disparity, int w, int h, const orc_stereo_params* sp) {
+    const size_t npx = (size_t)w * (size_t)h;
+    uint8_t* seen = (uint8_t*)calloc(npx + 1, 1);
+    int* stack = (int*)malloc(sizeof(int) * (npx + 1));
+    int* region = (int*)malloc(sizeof(int) * (npx + 1));
+    if (!seen || !stack || !region) {
+        free(seen);
+        free(stack);
+        free(region);
+        return -1;
+    }
+    for (int start = 0; start < (int)npx; ++start) {
+        if (seen[start] || !valid_d(disparity[start])) continue;
+        int ns = 0, nr = 0;
+        stack[ns++] = start;
+        seen[start] = 1;
+        while (ns > 0) {
+            const int i = stack[--ns];
+            region[nr++] = i;
+            const int x = i % w, y = i / w;
+            const float d = disparity[i];
+            const int nb[4][2] = {{x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
+            for (int k = 0; k < 4; ++k) {
+                if (nb[k][0] < 0 || nb[k][1] < 0 || nb[k][0] >= w || nb[k][1] >= h) continue;
+                const int j = nb[k][1] * w + nb[k][0];
+                const float dj = disparity[j];
+                if (seen[j] || !valid_d(dj) || fabsf(dj - d) > sp->speckle_max_diff) continue;
+                seen[j] = 1;
+                stack[ns++] = j;
+            }
+        }
+        if (nr < sp->speckle_max_region_size)
+            for (int k = 0; k < nr; ++k) disparity[region[k]] = ORC_INVALID_DISPARITY;
+    }
+    free(seen);
+    free(stack);
+    free(region);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- compute_disparity (stereo.cpp:496-536) */
+
+int orc_compute_disparity(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h,
+                          const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride, int full_w,
+                          int full_h, const orc_params* sgm, const orc_stereo_params* sp, float* disparity,
+                          float* confidence, int threads) {
+    const int L = sp->pyramid_levels;
+    int rc = -1;
+    /* Level i (1..L) images, compact; level 0 is the caller's. */
+    uint8_t* pl[17] = {0};
+    uint8_t* pr[17] = {0};
+    int lw[17], lh[17];
+    float *disp = NULL, *finer = NULL;
+    if (L < 0 || L > 16) return -1;
+    lw[0] = w;
+    lh[0] = h;
+    for (int i = 1; i <= L; ++i) {
+        lw[i] = lw[i - 1] / 2;
+        lh[i] = lh[i - 1] / 2;
+        pl[i] = (uint8_t*)malloc((size_t)lw[i] * (size_t)lh[i] + 1);
+        pr[i] = (uint8_t*)malloc((size_t)lw[i] * (size_t)lh[i] + 1);
+        if (!pl[i] || !pr[i]) goto done;
+        const uint8_t* sl = i == 1 ? left : pl[i - 1];
+        const uint8_t* sr = i == 1 ? right : pr[i - 1];
+        const ptrdiff_t ss = i == 1 ? stride : lw[i - 1];
+        orc_downsample2(sl, lw[i - 1], lh[i - 1], ss, pl[i], threads);
+        orc_downsample2(sr, lw[i - 1], lh[i - 1], ss, pr[i], threads);
+    }
+    {
+        const int tw = lw[L], th = lh[L];
+        disp = (float*)malloc(sizeof(float) * ((size_t)tw * (size_t)th + 1));
+        if (!disp) goto done;
+        orc_outputs o;
+        memset(&o, 0, sizeof(o));
+        o.disparity = disp;
+        if (orc_sgm(L ? pl[L] : left, L ? pr[L] : right, L ? tw : stride, tw, th, sgm, &o, threads) != 0) goto done;
+        const size_t npx = (size_t)w * (size_t)h;
+        if (L == 0)
+            for (size_t i = 0; i < npx; ++i) confidence[i] = valid_d(disp[i]) ? 1.0f : 0.0f;
+        for (int level = L - 1; level >= 0; --level) {
+            const size_t n = (size_t)lw[level] * (size_t)lh[level];
+            finer = (float*)malloc(sizeof(float) * (n + 1));
+            if (!finer) goto done;
+            float* cf = level == 0 ? confidence : (float*)malloc(sizeof(float) * (n + 1));
+            if (!cf) goto done;
+            orc_refine_level(level == 0 ? left : pl[level], level == 0 ? right : pr[level], level == 0 ? stride : lw[level],
+                             lw[level], lh[level], disp, lw[level + 1], lh[level + 1], sp, sgm->subpixel, finer, cf,
+                             threads);
+            if (cf != confidence) free(cf);
+            free(disp);
+            disp = finer;
+            finer = NULL;
+        }
+        if (sp->detail_enabled && full_left && full_right && full_w == 2 * w && full_h == 2 * h)
+            orc_refine_detail(full_left, full_right, full_stride, full_w, full_h, disp, w, h, sp, sgm->subpixel, threads);
+        orc_filter_disparity(disp, w, h, sp, threads);
+        if (orc_remove_speckles(disp, w, h, sp) != 0) goto done;
+        for (size_t i = 0; i < npx; ++i) {
+            disparity[i] = disp[i];
+            if (!valid_d(disp[i])) confidence[i] = 0.0f;
+        }
+    }
+    rc = 0;
+done:
+    for (int i = 1; i <= L; ++i) {
+        free(pl[i]);
+        free(pr[i]);
+    }
+    free(disp);
+    free(finer);
+    return rc;
+}

@@ -116,6 +116,53 @@ void orc_downsample2(const uint8_t* src, int w, int h, ptrdiff_t stride, uint8_t
 int orc_rectify_half(const uint8_t* raw, int raw_w, int raw_h, ptrdiff_t raw_stride, const float* map_x, const float* map_y,
                      int map_w, int map_h, uint8_t* dst, int threads);
 
+/* depth::StereoParams (stereo.hpp:42-89) minus its SgmParams, which travel as orc_params (whose
+ * subpixel field is StereoParams::subpixel and applies to every stage). */
+typedef struct {
+    int32_t pyramid_levels;
+    int32_t refine_search_radius, refine_zncc_radius; /* RefineParams (stereo.hpp:42-48) */
+    float refine_min_zncc;
+    int32_t refine_reject_edge_winners;
+    int32_t detail_enabled, detail_radius;             /* DetailParams (stereo.hpp:55-64) */
+    float detail_search, detail_step, detail_min_zncc;
+    int32_t filter_occlusion;                          /* FilterParams (stereo.hpp:69-74) */
+    float filter_occlusion_tolerance;
+    int32_t filter_spikes;
+    float filter_spike_threshold;
+    int32_t speckle_max_region_size;                   /* SpeckleParams (stereo.hpp:76-79) */
+    float speckle_max_diff;
+} orc_stereo_params;
+
+/* zncc_slanted, stereo.cpp:101-125 (-2: window leaves b, -1: a flat window). */
+float orc_zncc_slanted(const uint8_t* a, const uint8_t* b, ptrdiff_t stride, int bw, int x, int y, float d, float gx,
+                       float gy, int r);
+
+/* refine_level, stereo.cpp:151-207: out (w x h) from coarse_raw (cw x ch, median3 applied inside). */
+void orc_refine_level(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const float* coarse_raw,
+                      int cw, int ch, const orc_stereo_params* sp, int subpixel, float* out_disp, float* out_conf,
+                      int threads);
+
+/* zncc_window, stereo.cpp:211-239. */
+float orc_zncc_window(const uint8_t* a, const uint8_t* b, ptrdiff_t stride, int bw, int x0, int y0, int n, float cx,
+                      float cy, float D, float gx, float gy);
+
+/* refine_detail, stereo.cpp:243-284, in place on disparity (w x h) with full images (W x H). */
+void orc_refine_detail(const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride, int W, int H,
+                       float* disparity, int w, int h, const orc_stereo_params* sp, int subpixel, int threads);
+
+/* filter_disparity, stereo.cpp:286-326 (occlusion then spikes), in place. */
+void orc_filter_disparity(float* disparity, int w, int h, const orc_stereo_params* sp, int threads);
+
+/* remove_speckles, stereo.cpp:465-494, in place. Returns 0, or -1 when an allocation fails. */
+int orc_remove_speckles(float* disparity, int w, int h, const orc_stereo_params* sp);
+
+/* compute_disparity, stereo.cpp:496-536 (both forms: full_left/full_right may be NULL). disparity and
+ * confidence are w x h. Returns 0, or -1 when an allocation fails. */
+int orc_compute_disparity(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h,
+                          const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride, int full_w,
+                          int full_h, const orc_params* sgm, const orc_stereo_params* sp, float* disparity,
+                          float* confidence, int threads);
+
 /* sgm_disparity (stereo.cpp:361-463) followed by median3; fills whatever outputs are non-NULL.
  * Returns 0, or -1 when an allocation fails. */
 int orc_sgm(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const orc_params* p,
