@@ -94,7 +94,9 @@ typedef struct {
     double median_ms;
     double gpu_ms;    /* first to last kernel */
     double total_ms;  /* host wall time of the call, including copies */
-    double rectify_ms; /* the fused rectification of sgm_compute_raw*, else 0 */
+    double rectify_ms; /* rectification (raw calls) and pyramid (compute_disparity), else 0 */
+    double refine_ms;  /* compute_disparity: ZNCC refinement of every level and the detail refinement */
+    double filter_ms;  /* compute_disparity: occlusion and spike filters, speckle removal, confidence */
 } sgm_timings;
 
 /* Per-stage GPU times summed over every call completed since the last reset (count = calls). */
@@ -106,6 +108,8 @@ typedef struct {
     double gpu_ms;
     int64_t count;
     double rectify_ms;
+    double refine_ms;
+    double filter_ms;
 } sgm_stage_totals;
 
 typedef struct sgm_context sgm_context;
@@ -160,6 +164,66 @@ int sgm_compute_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* ra
 int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right, ptrdiff_t raw_pitch,
                            int raw_w, int raw_h, const sgm_outputs* d_out, uint8_t* d_rect_left, uint8_t* d_rect_right,
                            void* stream);
+
+/* depth::StereoParams (libs/depth/include/einstar/depth/stereo.hpp:42-89): SGM at 1/2^pyramid_levels
+ * of the input, ZNCC refinement to each finer level, detail refinement on the full-resolution pair,
+ * occlusion and spike filters, speckle removal. sgm.subpixel is StereoParams::subpixel. */
+typedef struct {
+    sgm_params sgm;
+    int32_t pyramid_levels;              /* 0..4 */
+    int32_t refine_search_radius;        /* RefineParams (stereo.hpp:42-48): 1..4 */
+    int32_t refine_zncc_radius;          /* 1..7 */
+    float refine_min_zncc;
+    int32_t refine_reject_edge_winners;
+    int32_t detail_enabled;              /* DetailParams (stereo.hpp:55-64) */
+    int32_t detail_radius;               /* 1..16 */
+    float detail_search, detail_step;    /* lround(search / step) <= 8 */
+    float detail_min_zncc;
+    int32_t filter_occlusion;            /* FilterParams (stereo.hpp:69-74) */
+    float filter_occlusion_tolerance;
+    int32_t filter_spikes;
+    float filter_spike_threshold;
+    int32_t speckle_max_region_size;     /* SpeckleParams (stereo.hpp:76-79) */
+    float speckle_max_diff;
+} sgm_stereo_params;
+
+/* Fills the reference defaults (stereo.hpp:26-89: two pyramid levels, every stage on). */
+void sgm_stereo_params_default(sgm_stereo_params* p);
+
+/* Replaces MetalStereo::create with its full StereoParams (metal_stereo.hpp:70-71, checks
+ * metal_stereo.cpp:115-149): a context for depth::compute_disparity on width x height images (the
+ * finest level); SGM runs at (width >> levels) x (height >> levels). Everything is allocated here. */
+int sgm_create_stereo(const sgm_stereo_params* params, int width, int height, sgm_context** out);
+
+/* Replaces MetalStereo::compute (metal_stereo.hpp:77-78) = depth::compute_disparity
+ * (libs/depth/src/stereo.cpp:496-536), both forms: full_left / full_right (may be NULL) are the
+ * full_width x full_height images left and right were box-downsampled from, used by the detail
+ * refinement when full_width == 2 * width and full_height == 2 * height (otherwise skipped, as in
+ * the reference). Host images in, host disparity and confidence (may be NULL) out; blocking. On a
+ * context from sgm_create it is sgm_disparity with confidence 1 where valid. */
+int sgm_compute_disparity(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width,
+                          int height, const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride,
+                          int full_width, int full_height, float* disparity, float* confidence);
+
+/* The same on device-resident images and outputs, enqueued on `stream` without waiting. */
+int sgm_compute_disparity_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch,
+                                 int width, int height, const uint8_t* d_full_left, const uint8_t* d_full_right,
+                                 ptrdiff_t full_pitch, int full_width, int full_height, float* d_disparity,
+                                 float* d_confidence, void* stream);
+
+/* Replaces MetalStereo::compute_raw with its full StereoParams (metal_stereo.cpp:340-349, 448-479):
+ * raw frames rectified with the resident maps (sgm_set_rectification; maps at twice the context's
+ * width x height) to the full-resolution pair when the detail refinement is on (then
+ * downsample2 of it) or straight to the stereo resolution, then compute_disparity. rect_left /
+ * rect_right (may be NULL) receive the stereo-resolution rectified pair. */
+int sgm_compute_disparity_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride,
+                              int raw_w, int raw_h, float* disparity, float* confidence, uint8_t* rect_left,
+                              uint8_t* rect_right);
+
+/* The same on device-resident raw frames and outputs, enqueued on `stream` without waiting. */
+int sgm_compute_disparity_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right,
+                                     ptrdiff_t raw_pitch, int raw_w, int raw_h, float* d_disparity, float* d_confidence,
+                                     uint8_t* d_rect_left, uint8_t* d_rect_right, void* stream);
 
 /* Waits for the work enqueued by this context and records its timings. */
 int sgm_synchronize(sgm_context* ctx);

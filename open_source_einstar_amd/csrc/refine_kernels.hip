@@ -1,0 +1,498 @@
+// gfx950 kernels of the stages compute_disparity runs around SGM (libs/depth/src/stereo.cpp:98-326,
+// 348-359, 465-536): the image pyramid, slanted-window ZNCC refinement to each finer level, detail
+// refinement on the full-resolution pair, the occlusion and spike filters, and speckle removal by
+// connected components.
+//
+// The ZNCC scores accumulate in double in the reference's order (window rows outer, columns inner)
+// and the float sample positions keep its operation order; with -ffp-contract=off and IEEE double
+// add/mul/div/sqrt every score, and so every winner and subpixel offset, equals the CPU's.
+#include <climits>
+
+#include "sgm_internal.h"
+
+namespace sgm {
+namespace {
+
+constexpr float kInvalid = -1.0e9f;
+constexpr int kTw = 16, kTh = 16;  // 2-D blocks of 256 threads
+
+__device__ __forceinline__ bool valid_d(float d) { return d > -1.0e8f; }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// subpixel_offset, stereo.cpp:92-96.
+__device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {
+    const float denom = cm - 2.0f * c0 + cp;
+    if (denom <= 0.0f) return 0.0f;
+    const float v = 0.5f * (cm - cp) / denom;
+    return clampf(v, -0.5f, 0.5f);
+}
+
+// The n/2-th smallest valid value of the in-image 3x3 neighbourhood and the count n (median3,
+// stereo.cpp:129-147, and the spike filter's median, :306-322).
+__device__ __forceinline__ float median9(const float* in, int w, int h, int x, int y, int& n_out) {
+    float buf[9];
+    int n = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx, yy = y + dy;
+            float q = kInvalid;
+            if (xx >= 0 && yy >= 0 && xx < w && yy < h) q = in[(int64_t)yy * w + xx];
+            const bool ok = q > -1.0e8f;
+            buf[dy * 3 + dx + 4] = ok ? q : __builtin_inff();  // +inf sorts after every valid value
+            n += ok ? 1 : 0;
+        }
+#pragma unroll
+    for (int round = 0; round < 9; ++round)
+#pragma unroll
+        for (int i = round & 1; i + 1 < 9; i += 2) {
+            const float lo = fminf(buf[i], buf[i + 1]);
+            const float hi = fmaxf(buf[i], buf[i + 1]);
+            buf[i] = lo;
+            buf[i + 1] = hi;
+        }
+    float sel = buf[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) sel = (n / 2) == i ? buf[i] : sel;
+    n_out = n;
+    return sel;
+}
+
+// ------------------------------------------------------------------------------- pyramid
+// downsample2, stereo.cpp:348-359; both images in one launch (grid z).
+
+__global__ __launch_bounds__(kTw* kTh) void downsample2_kernel(const uint8_t* s0, const uint8_t* s1, int64_t spitch,
+                                                               uint8_t* d0, uint8_t* d1, int64_t dpitch, int ow, int oh) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    const uint8_t* s = blockIdx.z ? s1 : s0;
+    const uint8_t* r0 = s + (int64_t)(2 * y) * spitch + 2 * x;
+    const uint8_t* r1 = r0 + spitch;
+    (blockIdx.z ? d1 : d0)[(int64_t)y * dpitch + x] = (uint8_t)(((uint32_t)r0[0] + r0[1] + r1[0] + r1[1] + 2) >> 2);
+}
+
+__global__ __launch_bounds__(kTw* kTh) void median3_kernel(const float* in, float* out, int w, int h) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    float m = kInvalid;
+    if (valid_d(in[(int64_t)y * w + x])) {
+        int n;
+        m = median9(in, w, h, x, y, n);
+    }
+    out[(int64_t)y * w + x] = m;
+}
+
+// ------------------------------------------------------------------------------- refine_level
+// stereo.cpp:151-207. One thread per fine pixel. The left window's sums do not depend on the
+// candidate and are formed once, in the order the reference forms them for every candidate.
+
+// zncc_slanted (stereo.cpp:101-125) with the left sums given.
+__device__ float zncc_slanted(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x, int y, float d, float gx,
+                              float gy, int r, double sa, double saa) {
+    double sb = 0, sbb = 0, sab = 0;
+    const int n = (2 * r + 1) * (2 * r + 1);
+    for (int v = -r; v <= r; ++v) {
+        const uint8_t* ra = a + (int64_t)(y + v) * pitch;
+        const uint8_t* rb = b + (int64_t)(y + v) * pitch;
+        for (int u = -r; u <= r; ++u) {
+            const float xb = (float)(x + u) - (d + gx * (float)u + gy * (float)v);
+            const float fl = __builtin_floorf(xb);
+            // floor(xb) in [0, bw - 2]  <=>  0 <= xb < bw - 1 (bw < 2^24; NaN fails)
+            if (!(xb >= 0.0f && xb < (float)(bw - 1))) return -2.0f;
+            const int x0 = (int)fl;
+            const float f = xb - fl;
+            const double ib = (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
+            const double ia = ra[x + u];
+            sb += ib;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    const double va = saa - sa * sa / n;
+    const double vb = sbb - sb * sb / n;
+    if (va < 1e-6 || vb < 1e-6) return -1.0f;
+    return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
+}
+
+__global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= R.w || y >= R.h) return;
+    const int64_t o = (int64_t)y * R.w + x;
+    float out_d = kInvalid, out_c = 0.0f;
+    const int r = R.radius;
+    if (y >= r && y < R.h - r && x >= r && x < R.w - r) {
+        const int cy = min(y / 2, R.ch - 1), cx = min(x / 2, R.cw - 1);
+        const float* C = R.coarse;
+        const float d0 = C[(int64_t)cy * R.cw + cx];
+        if (valid_d(d0)) {
+            // The slope lambda, stereo.cpp:159-170.
+            float gx = 0.0f, gy = 0.0f;
+            if (cx > 0 && cx + 1 < R.cw) {
+                const float l = C[(int64_t)cy * R.cw + cx - 1], rr = C[(int64_t)cy * R.cw + cx + 1];
+                if (valid_d(l) && valid_d(rr)) {
+                    const float g = 0.5f * (rr - l);
+                    if (fabsf(rr - d0) < 2 && fabsf(d0 - l) < 2) gx = g;
+                }
+            }
+            if (cy > 0 && cy + 1 < R.ch) {
+                const float u = C[(int64_t)(cy - 1) * R.cw + cx], dn = C[(int64_t)(cy + 1) * R.cw + cx];
+                if (valid_d(u) && valid_d(dn)) {
+                    const float g = 0.5f * (dn - u);
+                    if (fabsf(dn - d0) < 2 && fabsf(d0 - u) < 2) gy = g;
+                }
+            }
+            gx = clampf(gx, -0.9f, 0.9f);
+            gy = clampf(gy, -0.9f, 0.9f);
+            double sa = 0, saa = 0;
+            for (int v = -r; v <= r; ++v) {
+                const uint8_t* ra = R.left + (int64_t)(y + v) * R.pitch;
+                for (int u = -r; u <= r; ++u) {
+                    const double ia = ra[x + u];
+                    sa += ia;
+                    saa += ia * ia;
+                }
+            }
+            const float kStep = 0.5f;
+            const float center = 2.0f * d0;
+            const int nd = min(127, (int)((float)(2 * R.search_radius) / kStep) + 1);
+            const float lo = center - (float)R.search_radius;
+            // Only the winner and its two neighbours' scores matter afterwards.
+            float best = -2.0f, prev = -3.0f, best_prev = -3.0f, best_next = -3.0f;
+            int best_k = -1;
+            for (int k = 0; k < nd; ++k) {
+                const float d = lo + kStep * (float)k;
+                const float s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
+                if (s > best) {
+                    best = s;
+                    best_k = k;
+                    best_prev = prev;
+                } else if (k == best_k + 1) {
+                    best_next = s;
+                }
+                prev = s;
+            }
+            bool keep = !(best_k < 0 || best < R.min_zncc);
+            if (keep && R.reject_edge && (best_k == 0 || best_k == nd - 1)) keep = false;
+            if (keep) {
+                float d = lo + kStep * (float)best_k;
+                if (R.subpixel && best_k > 0 && best_k + 1 < nd && best_prev > -2.0f && best_next > -2.0f)
+                    d += kStep * subpixel_offset(1.0f - best_prev, 1.0f - best, 1.0f - best_next);
+                out_d = d;
+                out_c = best;
+            }
+        }
+    }
+    R.disp[o] = out_d;
+    R.conf[o] = out_c;
+}
+
+// ------------------------------------------------------------------------------- refine_detail
+// stereo.cpp:211-284. One thread per half-resolution pixel; pixels it does not change keep the
+// first estimate (the output starts as a copy).
+
+__device__ float zncc_window(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x0, int y0, int n, float cx,
+                             float cy, float D, float gx, float gy, double sa, double saa) {
+    double sb = 0, sbb = 0, sab = 0;
+    for (int j = 0; j < n; ++j) {
+        const int Y = y0 + j;
+        const uint8_t* ra = a + (int64_t)Y * pitch;
+        const uint8_t* rb = b + (int64_t)Y * pitch;
+        const float dy = gy * ((float)Y - cy);
+        for (int i = 0; i < n; ++i) {
+            const int X = x0 + i;
+            const float xb = (float)X - (D + gx * ((float)X - cx) + dy);
+            const float fl = __builtin_floorf(xb);
+            if (!(xb >= 0.0f && xb < (float)(bw - 1))) return -2.0f;
+            const int xi = (int)fl;
+            const float f = xb - fl;
+            const double ib = (float)rb[xi] * (1.0f - f) + (float)rb[xi + 1] * f;
+            const double ia = ra[X];
+            sb += ib;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    const double m = (double)n * n;
+    const double va = saa - sa * sa / m;
+    const double vb = sbb - sb * sb / m;
+    if (va < 1e-6 || vb < 1e-6) return -1.0f;
+    return (float)((sab - sa * sb / m) / __builtin_sqrt(va * vb));
+}
+
+__device__ __forceinline__ float detail_grad(float l, float c, float r) {
+    if (!valid_d(l) || !valid_d(r) || fabsf(r - c) >= 1.0f || fabsf(c - l) >= 1.0f) return 0.0f;
+    return clampf(0.5f * (r - l), -0.9f, 0.9f);
+}
+
+__global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= D.w || y >= D.h) return;
+    const float* F = D.first;
+    const int w = D.w;
+    const float d = F[(int64_t)y * w + x];
+    float out = d;
+    const int n = 2 * D.radius, K = D.K, nc = 2 * K + 1;
+    const int y0 = 2 * y - D.radius + 1, x0 = 2 * x - D.radius + 1;
+    if (y >= 1 && y + 1 < D.h && y0 >= 0 && y0 + n <= D.H && x >= 1 && x + 1 < w && valid_d(d) && x0 >= 0 &&
+        x0 + n <= D.W) {
+        const float gx = detail_grad(F[(int64_t)y * w + x - 1], d, F[(int64_t)y * w + x + 1]);
+        const float gy = detail_grad(F[(int64_t)(y - 1) * w + x], d, F[(int64_t)(y + 1) * w + x]);
+        const float cx = 2.0f * (float)x + 0.5f, cy = 2.0f * (float)y + 0.5f;
+        const float D0 = 2.0f * d;
+        double sa = 0, saa = 0;
+        for (int j = 0; j < n; ++j) {
+            const uint8_t* ra = D.fl + (int64_t)(y0 + j) * D.fpitch;
+            for (int i = 0; i < n; ++i) {
+                const double ia = ra[x0 + i];
+                sa += ia;
+                saa += ia * ia;
+            }
+        }
+        float best = -2.0f, prev = -3.0f, best_prev = -3.0f, best_next = -3.0f;
+        int best_k = -1;
+        for (int k = 0; k < nc; ++k) {
+            const float s = zncc_window(D.fl, D.fr, D.fpitch, D.W, x0, y0, n, cx, cy, D0 + D.step * (float)(k - K), gx, gy,
+                                        sa, saa);
+            if (s > best) {
+                best = s;
+                best_k = k;
+                best_prev = prev;
+            } else if (k == best_k + 1) {
+                best_next = s;
+            }
+            prev = s;
+        }
+        if (!(best_k <= 0 || best_k + 1 >= nc || best < D.min_zncc || best_prev <= -2.0f || best_next <= -2.0f)) {
+            float Dv = D0 + D.step * (float)(best_k - K);
+            if (D.subpixel) Dv += D.step * subpixel_offset(1.0f - best_prev, 1.0f - best, 1.0f - best_next);
+            out = 0.5f * Dv;
+        }
+    }
+    D.out[(int64_t)y * w + x] = out;
+}
+
+// ------------------------------------------------------------------------------- filters
+// Occlusion (stereo.cpp:288-304): one block per row; the nearest claimant of each right pixel is an
+// LDS max over order-preserving integer keys of the claimants' disparities.
+
+__device__ __forceinline__ uint32_t order_key(float f) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) {
+    return __builtin_bit_cast(float, (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__global__ __launch_bounds__(kBlock) void occlusion_kernel(float* disp, int w, float tol) {
+    extern __shared__ uint32_t nearest[];
+    float* row = disp + (int64_t)blockIdx.x * w;
+    const uint32_t none = order_key(kInvalid);
+    for (int x = (int)threadIdx.x; x < w; x += kBlock) nearest[x] = none;
+    __syncthreads();
+    for (int x = (int)threadIdx.x; x < w; x += kBlock) {
+        const float d = row[x];
+        if (!valid_d(d)) continue;
+        const long xr = lroundf((float)x - d);
+        if (xr >= 0 && xr < w) atomicMax(&nearest[xr], order_key(d));
+    }
+    __syncthreads();
+    for (int x = (int)threadIdx.x; x < w; x += kBlock) {
+        const float d = row[x];
+        if (!valid_d(d)) continue;
+        const long xr = lroundf((float)x - d);
+        if (xr >= 0 && xr < w && d < key_value(nearest[xr]) - tol) row[x] = kInvalid;
+    }
+}
+
+// Spikes (stereo.cpp:305-323), out of place: in is the disparity after the occlusion filter.
+__global__ __launch_bounds__(kTw* kTh) void spike_kernel(const float* in, float* out, int w, int h, float thr) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    const float d = in[(int64_t)y * w + x];
+    float o = d;
+    if (valid_d(d)) {
+        int n;
+        const float m = median9(in, w, h, x, y, n);
+        if (n >= 3 && fabsf(d - m) > thr) o = kInvalid;
+    }
+    out[(int64_t)y * w + x] = o;
+}
+
+// ------------------------------------------------------------------------------- speckles
+// remove_speckles (stereo.cpp:465-494) labels the connected components of the graph whose edges
+// join 4-neighbours that are both valid with |d_a - d_b| <= max_diff; a flood fill and a union-find
+// find the same components. Labels are pixel indices; every label points at a smaller or equal
+// index, so every find ends, and unions link the larger root under the smaller by atomic min.
+
+__device__ __forceinline__ int32_t load_label(const int32_t* L, int32_t i) {
+    return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t find_root(const int32_t* L, int32_t a) {
+    int32_t p = load_label(L, a);
+    while (p != a) {
+        a = p;
+        p = load_label(L, a);
+    }
+    return a;
+}
+
+__device__ void unite(int32_t* L, int32_t a, int32_t b) {
+    for (;;) {
+        a = find_root(L, a);
+        b = find_root(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const int32_t old = atomicMin(L + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const int32_t old = atomicMin(L + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTw* kTh) void ccl_init_kernel(const float* disp, int32_t* L, uint32_t* sizes, int w, int h) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int32_t i = y * w + x;
+    L[i] = valid_d(disp[i]) ? i : -1;
+    sizes[i] = 0;
+}
+
+__global__ __launch_bounds__(kTw* kTh) void ccl_merge_kernel(const float* disp, int32_t* L, int w, int h, float max_diff) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int32_t i = y * w + x;
+    const float d = disp[i];
+    if (!valid_d(d)) return;
+    if (x + 1 < w) {
+        const float e = disp[i + 1];
+        if (valid_d(e) && !(fabsf(e - d) > max_diff)) unite(L, i, i + 1);
+    }
+    if (y + 1 < h) {
+        const float e = disp[i + w];
+        if (valid_d(e) && !(fabsf(e - d) > max_diff)) unite(L, i, i + w);
+    }
+}
+
+__global__ __launch_bounds__(kTw* kTh) void ccl_count_kernel(int32_t* L, uint32_t* sizes, int w, int h) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int32_t i = y * w + x;
+    if (L[i] < 0) return;
+    const int32_t root = find_root(L, i);
+    L[i] = root;
+    atomicAdd(sizes + root, 1u);
+}
+
+// The end of compute_disparity (stereo.cpp:527-534): speckle regions removed, confidence 0 where
+// the disparity is invalid (conf_src null: 1 elsewhere, the level-0 SGM case).
+__global__ __launch_bounds__(kTw* kTh) void finalize_kernel(const float* disp, const int32_t* L, const uint32_t* sizes,
+                                                            int max_region, const float* conf_src, float* out_disp,
+                                                            float* out_conf, int w, int h) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int32_t i = y * w + x;
+    float d = disp[i];
+    if (L && valid_d(d) && sizes[L[i]] < (uint32_t)max_region) d = kInvalid;
+    out_disp[i] = d;
+    if (out_conf) out_conf[i] = valid_d(d) ? (conf_src ? conf_src[i] : 1.0f) : 0.0f;
+}
+
+// ------------------------------------------------------------------------------- rectify_full
+// calib::remap at the maps' resolution (rectify.cpp:133-148), kept for the detail refinement; the
+// stereo-resolution pair is then downsample2 of it, which equals the fused rectify_kernel's output.
+
+__global__ __launch_bounds__(kTw* kTh) void rectify_full_kernel(RectifyDesc r, uint8_t* f0, uint8_t* f1, int64_t fpitch,
+                                                                int fw, int fh) {
+    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    if (x >= fw || y >= fh) return;
+    const int side = (int)blockIdx.z;
+    const uint8_t* src = r.raw[side];
+    const float* m = r.map[side] + ((int64_t)y * r.map_w + x) * 2;
+    const float sx = m[0], sy = m[1];
+    uint32_t v = 0;
+    if (sx >= 0.0f && sy >= 0.0f && sx < (float)(r.raw_w - 1) && sy < (float)(r.raw_h - 1)) {
+        const float flx = __builtin_floorf(sx), fly = __builtin_floorf(sy);
+        const float fx = sx - flx, fy = sy - fly;
+        const uint8_t* p = src + (int64_t)(int)fly * r.raw_pitch + (int)flx;
+        const float a = p[0], b = p[1], c = p[r.raw_pitch], d = p[r.raw_pitch + 1];
+        const float s = (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy;
+        v = (uint32_t)__builtin_fminf(s + 0.5f, 255.0f);
+    }
+    (side ? f1 : f0)[(int64_t)y * fpitch + x] = (uint8_t)v;
+}
+
+dim3 grid2d(int w, int h, int z = 1) { return dim3((unsigned)((w + kTw - 1) / kTw), (unsigned)((h + kTh - 1) / kTh), (unsigned)z); }
+
+}  // namespace
+
+hipError_t launch_downsample2(const uint8_t* s0, const uint8_t* s1, int64_t spitch, int sw, int sh, uint8_t* d0, uint8_t* d1,
+                              int64_t dpitch, hipStream_t stream) {
+    const int ow = sw / 2, oh = sh / 2;
+    if (ow <= 0 || oh <= 0) return hipSuccess;
+    hipLaunchKernelGGL(downsample2_kernel, grid2d(ow, oh, 2), dim3(kTw, kTh), 0, stream, s0, s1, spitch, d0, d1, dpitch, ow, oh);
+    return hipGetLastError();
+}
+
+hipError_t launch_median3(const float* in, float* out, int w, int h, hipStream_t stream) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(median3_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, in, out, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine(const RefineDesc& r, hipStream_t stream) {
+    if (r.w <= 0 || r.h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(refine_kernel, grid2d(r.w, r.h), dim3(kTw, kTh), 0, stream, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_detail(const DetailDesc& d, hipStream_t stream) {
+    if (d.w <= 0 || d.h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(detail_kernel, grid2d(d.w, d.h), dim3(kTw, kTh), 0, stream, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_occlusion(float* disp, int w, int h, float tol, hipStream_t stream) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(occlusion_kernel, dim3((unsigned)h), dim3(kBlock), (size_t)w * 4, stream, disp, w, tol);
+    return hipGetLastError();
+}
+
+hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, hipStream_t stream) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(spike_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, in, out, w, h, thr);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* sizes, int w, int h, float max_diff, hipStream_t stream) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ccl_init_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, sizes, w, h);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ccl_merge_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, w, h, max_diff);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(ccl_count_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, labels, sizes, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const float* disp, const int32_t* labels, const uint32_t* sizes, int max_region,
+                           const float* conf_src, float* out_disp, float* out_conf, int w, int h, hipStream_t stream) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(finalize_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, sizes, max_region, conf_src,
+                       out_disp, out_conf, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_rectify_full(const RectifyDesc& r, uint8_t* f0, uint8_t* f1, int64_t fpitch, int fw, int fh,
+                               hipStream_t stream) {
+    if (fw <= 0 || fh <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rectify_full_kernel, grid2d(fw, fh, 2), dim3(kTw, kTh), 0, stream, r, f0, f1, fpitch, fw, fh);
+    return hipGetLastError();
+}
+
+}  // namespace sgm

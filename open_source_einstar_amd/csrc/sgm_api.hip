@@ -2,6 +2,7 @@
 // and batches. Kernels live in sgm_kernels.hip.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -17,12 +18,15 @@ namespace {
 
 thread_local std::string g_last_error;
 
-// ev[0..4] bracket census, paths, rows and median; ev[5] marks the start of a rectification
-// launch when the call had one (rectified).
+// ev[0..4] bracket census, paths, rows and median; ev[5] marks the start of the front end
+// (rectification and pyramid) when the call had one (front); ev[6..8] bracket the refinement and the
+// filters of compute_disparity (post).
+constexpr int kStageEvents = 9;
 struct Stages {
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[kStageEvents] = {};
     bool recorded = false;
-    bool rectified = false;
+    bool front = false;
+    bool post = false;
 };
 
 // Stage events of the calls in flight: a ring, so that per-stage times can be summed over many
@@ -193,6 +197,19 @@ struct sgm_context {
     uint8_t* d_raw = nullptr;    // left | right, raw_w * raw_h each
     size_t maps_bytes = 0, raw_bytes = 0;
     int map_w = 0, map_h = 0, raw_w = 0, raw_h = 0;
+    // compute_disparity: SGM runs at level `levels` of a pyramid over stereo_w x stereo_h images
+    // (sgm_create: level 0 and no refinement or filters).
+    sgm_stereo_params sp{};
+    int levels = 0;
+    int stereo_w = 0, stereo_h = 0;
+    uint8_t* d_pyr = nullptr;    // levels 1..L, left then right of each level, compact rows
+    float* d_wa = nullptr;       // level-0 sized work maps: disparity ping-pong, median, confidence
+    float* d_wb = nullptr;
+    float* d_wm = nullptr;
+    float* d_wc = nullptr;
+    int32_t* d_labels = nullptr; // speckle components
+    uint32_t* d_sizes = nullptr;
+    uint8_t* d_full = nullptr;   // full-resolution pair (2 * stereo_w x 2 * stereo_h each) for the detail refinement
     size_t bytes = 0;
     sgm_timings timings{};
     double pending_wall_ms = 0;
@@ -235,11 +252,122 @@ void sgm_destroy(sgm_context* ctx) {
     for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
                     (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol, (void*)ctx->d_maps,
-                    (void*)ctx->d_raw})
+                    (void*)ctx->d_raw, (void*)ctx->d_pyr, (void*)ctx->d_wa, (void*)ctx->d_wb, (void*)ctx->d_wm,
+                    (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_sizes, (void*)ctx->d_full})
         dfree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
+
+}  // extern "C"
+
+// The SGM-only configuration sgm_create runs: compute_disparity is then sgm_disparity itself.
+static sgm_stereo_params sgm_only(const sgm_params& p) {
+    sgm_stereo_params sp{};
+    sp.sgm = p;
+    sp.pyramid_levels = 0;
+    sp.refine_search_radius = 2;
+    sp.refine_zncc_radius = 3;
+    sp.refine_min_zncc = 0.5f;
+    sp.refine_reject_edge_winners = 1;
+    sp.detail_enabled = 0;
+    sp.detail_radius = 5;
+    sp.detail_search = 1.0f;
+    sp.detail_step = 0.5f;
+    sp.detail_min_zncc = 0.5f;
+    sp.filter_occlusion = 0;
+    sp.filter_occlusion_tolerance = 1.0f;
+    sp.filter_spikes = 0;
+    sp.filter_spike_threshold = 1.0f;
+    sp.speckle_max_region_size = 0;
+    sp.speckle_max_diff = 1.0f;
+    return sp;
+}
+
+static bool has_post(const sgm_stereo_params& sp) {
+    return sp.pyramid_levels > 0 || sp.detail_enabled || sp.filter_occlusion || sp.filter_spikes ||
+           sp.speckle_max_region_size > 1;
+}
+
+// Limits of the refinement stages, as MetalStereo::create checks them (metal_stereo.cpp:145-149).
+static int validate_stereo(const sgm_stereo_params* sp, std::string& err) {
+    if (sp->pyramid_levels < 0 || sp->pyramid_levels > kMaxLevels) {
+        err = "pyramid_levels must be in [0, " + std::to_string(kMaxLevels) + "]";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (sp->pyramid_levels > 0 && (sp->refine_search_radius < 1 || sp->refine_search_radius > 4)) {
+        err = "refine search radius must be 1..4";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (sp->pyramid_levels > 0 && (sp->refine_zncc_radius < 1 || sp->refine_zncc_radius > 7)) {
+        err = "refine zncc_radius must be 1..7";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (sp->detail_enabled && (!(sp->detail_step > 0) || std::lround(sp->detail_search / sp->detail_step) > 8 ||
+                               sp->detail_radius < 1 || sp->detail_radius > 16))
+        return err = "detail search must be at most 8 steps either side (step > 0, radius 1..16)", SGM_ERR_INVALID_ARGUMENT;
+    return SGM_OK;
+}
+
+// A context for SGM on images up to sgm_w x sgm_h, run inside compute_disparity on stereo_w x
+// stereo_h images with the stages of sp.
+static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int stereo_w, int stereo_h,
+                          const sgm_stereo_params& sp, sgm_context** out) {
+    auto* ctx = new sgm_context;
+    ctx->p = *params;
+    ctx->consts = make_consts(*params);
+    ctx->max_w = sgm_w;
+    ctx->max_h = sgm_h;
+    ctx->stereo_w = stereo_w;
+    ctx->stereo_h = stereo_h;
+    ctx->sp = sp;
+    ctx->levels = sp.pyramid_levels;
+    ctx->ndirs = params->eight_paths ? 8 : 4;
+    const size_t npx = (size_t)sgm_w * (size_t)sgm_h;
+    const size_t spx = (size_t)stereo_w * (size_t)stereo_h;
+    const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int r = 0; e == hipSuccess && r < kEventRing; ++r)
+        for (int i = 0; e == hipSuccess && i < kStageEvents; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
+    size_t& b = ctx->bytes;
+    if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * std::max(npx, spx), b);
+    ctx->census_half = census_words(sgm_w, sgm_h, params->min_disparity, params->num_disparities);
+    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_cvol, npx * dp, b);  // cost-volume mode is the default
+    ctx->use_cvol = true;
+    if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_conf, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_dint, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_status, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_rdisp, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_best, npx, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_second, npx, b);
+    if (has_post(sp)) {
+        size_t pyr = 0;
+        for (int i = 1, lw = stereo_w, lh = stereo_h; i <= sp.pyramid_levels; ++i) {
+            lw /= 2;
+            lh /= 2;
+            pyr += 2 * (size_t)lw * lh;
+        }
+        if (e == hipSuccess && pyr) e = dalloc(&ctx->d_pyr, pyr, b);
+        for (float** w : {&ctx->d_wa, &ctx->d_wb, &ctx->d_wm, &ctx->d_wc})
+            if (e == hipSuccess) e = dalloc(w, spx, b);
+        if (e == hipSuccess) e = dalloc(&ctx->d_labels, spx, b);
+        if (e == hipSuccess) e = dalloc(&ctx->d_sizes, spx, b);
+        if (e == hipSuccess && sp.detail_enabled) e = dalloc(&ctx->d_full, 8 * spx, b);
+    }
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_create: device allocation", e);
+        sgm_destroy(ctx);
+        return SGM_ERR_IO;
+    }
+    *out = ctx;
+    return SGM_OK;
+}
+
+extern "C" {
 
 int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_context** out) {
     if (!out) {
@@ -256,39 +384,41 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
         g_last_error = "max_width must be in [1, " + std::to_string(kMaxWidth) + "] and max_height >= 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
-    auto* ctx = new sgm_context;
-    ctx->p = *params;
-    ctx->consts = make_consts(*params);
-    ctx->max_w = max_width;
-    ctx->max_h = max_height;
-    ctx->ndirs = params->eight_paths ? 8 : 4;
-    const size_t npx = (size_t)max_width * (size_t)max_height;
-    const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
-    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int r = 0; e == hipSuccess && r < kEventRing; ++r)
-        for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&ctx->ring[r].ev[i]);
-    size_t& b = ctx->bytes;
-    if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * npx, b);
-    ctx->census_half = census_words(max_width, max_height, params->min_disparity, params->num_disparities);
-    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_cvol, npx * dp, b);  // cost-volume mode is the default
-    ctx->use_cvol = true;
-    if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_conf, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_dint, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_status, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_rdisp, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_best, npx, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_second, npx, b);
-    if (e != hipSuccess) {
-        g_last_error = hip_msg("sgm_create: device allocation", e);
-        sgm_destroy(ctx);
-        return SGM_ERR_IO;
+    return create_context(params, max_width, max_height, max_width, max_height, sgm_only(*params), out);
+}
+
+void sgm_stereo_params_default(sgm_stereo_params* p) {
+    if (!p) return;
+    *p = sgm_only(sgm_params{});
+    sgm_params_default(&p->sgm);
+    p->pyramid_levels = 2;                 // stereo.hpp:82-89 and the stage defaults of :42-79
+    p->detail_enabled = 1;
+    p->filter_occlusion = 1;
+    p->filter_spikes = 1;
+    p->speckle_max_region_size = 200;
+}
+
+int sgm_create_stereo(const sgm_stereo_params* sp, int width, int height, sgm_context** out) {
+    if (!out || !sp) {
+        g_last_error = "params and out must not be NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
     }
-    *out = ctx;
-    return SGM_OK;
+    *out = nullptr;
+    std::string err;
+    if (int rc = validate_params(&sp->sgm, err); rc != SGM_OK) {
+        g_last_error = err;
+        return rc;
+    }
+    if (int rc = validate_stereo(sp, err); rc != SGM_OK) {
+        g_last_error = err;
+        return rc;
+    }
+    const int sw = width >> sp->pyramid_levels, sh = height >> sp->pyramid_levels;
+    if (width < 1 || height < 1 || width > kMaxWidth || sw < 1 || sh < 1) {
+        g_last_error = "width must be in [1, " + std::to_string(kMaxWidth) + "] and the SGM level at least 1x1";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    return create_context(&sp->sgm, sw, sh, width, height, *sp, out);
 }
 
 size_t sgm_device_bytes(const sgm_context* ctx) { return ctx ? ctx->bytes : 0; }
@@ -445,11 +575,17 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
 
 // Adds one completed event set to the running totals; the newest also becomes last_timings.
 static void account(sgm_context* ctx, Stages& st, bool newest) {
-    float ms[4] = {0, 0, 0, 0}, all = 0, rect = 0;
+    float ms[4] = {0, 0, 0, 0}, all = 0, rect = 0, refine = 0, filter = 0;
     for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], st.ev[i], st.ev[i + 1]);
-    if (st.rectified) (void)hipEventElapsedTime(&rect, st.ev[5], st.ev[0]);
-    (void)hipEventElapsedTime(&all, st.rectified ? st.ev[5] : st.ev[0], st.ev[4]);
+    if (st.front) (void)hipEventElapsedTime(&rect, st.ev[5], st.ev[0]);
+    if (st.post) {
+        (void)hipEventElapsedTime(&refine, st.ev[6], st.ev[7]);
+        (void)hipEventElapsedTime(&filter, st.ev[7], st.ev[8]);
+    }
+    (void)hipEventElapsedTime(&all, st.front ? st.ev[5] : st.ev[0], st.post ? st.ev[8] : st.ev[4]);
     ctx->totals.rectify_ms += rect;
+    ctx->totals.refine_ms += refine;
+    ctx->totals.filter_ms += filter;
     ctx->totals.census_ms += ms[0];
     ctx->totals.paths_ms += ms[1];
     ctx->totals.rows_ms += ms[2];
@@ -464,9 +600,12 @@ static void account(sgm_context* ctx, Stages& st, bool newest) {
         ctx->timings.gpu_ms = all;
         ctx->timings.total_ms = ctx->pending_wall_ms > 0 ? ctx->pending_wall_ms : all;
         ctx->timings.rectify_ms = rect;
+        ctx->timings.refine_ms = refine;
+        ctx->timings.filter_ms = filter;
     }
     st.recorded = false;
-    st.rectified = false;
+    st.front = false;
+    st.post = false;
 }
 
 // Reads every recorded set (the stream has been synchronised).
@@ -482,7 +621,7 @@ static void read_timings(sgm_context* ctx) {
 static Stages* next_stages(sgm_context* ctx) {
     if (ctx->ring_pending == kEventRing) {
         const int i = (ctx->ring_head - ctx->ring_pending + kEventRing) % kEventRing;
-        (void)hipEventSynchronize(ctx->ring[i].ev[4]);
+        (void)hipEventSynchronize(ctx->ring[i].ev[ctx->ring[i].post ? 8 : 4]);
         --ctx->ring_pending;
         account(ctx, ctx->ring[i], false);
     }
@@ -560,9 +699,9 @@ static hipError_t enqueue_rectify(sgm_context* ctx, const uint8_t* raw_l, const 
     r.out_h = ctx->map_h / 2;
     r.out_pitch = r.out_w;
     hipError_t e;
-    if (st) {
+    if (st && !st->front) {
         if ((e = hipEventRecord(st->ev[5], s)) != hipSuccess) return e;
-        st->rectified = true;
+        st->front = true;
     }
     return launch_rectify(r, s);
 }
@@ -681,11 +820,11 @@ int sgm_set_rectification(sgm_context* ctx, const float* left_map_x, const float
         return SGM_ERR_INVALID_ARGUMENT;
     }
     if (!left_map_x || !left_map_y || !right_map_x || !right_map_y) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null map");
-    if (map_w < 2 || map_h < 2 || map_w / 2 > ctx->max_w || map_h / 2 > ctx->max_h)
+    if (map_w < 2 || map_h < 2 || map_w / 2 > ctx->stereo_w || map_h / 2 > ctx->stereo_h)
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "remap resolution must be twice the stereo resolution (stereo size " +
                                                        std::to_string(map_w / 2) + "x" + std::to_string(map_h / 2) +
-                                                       " outside the context's " + std::to_string(ctx->max_w) + "x" +
-                                                       std::to_string(ctx->max_h) + ")");
+                                                       " outside the context's " + std::to_string(ctx->stereo_w) + "x" +
+                                                       std::to_string(ctx->stereo_h) + ")");
     // Raw sizes stay below 2^24 so that the kernel's float bounds test is exact.
     if (raw_w < 1 || raw_h < 1 || raw_w >= (1 << 24) || raw_h >= (1 << 24))
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "raw size must be in [1, 2^24)");
@@ -747,6 +886,7 @@ int sgm_compute_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* ra
     }
     if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
     if (!raw_left || !raw_right || raw_stride < raw_w) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    if (int rc = check_size(ctx, ctx->map_w / 2, ctx->map_h / 2); rc != SGM_OK) return rc;
     return compute_host(ctx, HostFrame{raw_left, raw_right, raw_stride, true, rect_left, rect_right}, ctx->map_w / 2,
                         ctx->map_h / 2, out, "sgm_compute_raw");
 }
@@ -763,6 +903,7 @@ int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const ui
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
     const int width = ctx->map_w / 2, height = ctx->map_h / 2;
+    if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
     uint8_t* dl = d_rect_left ? d_rect_left : ctx->d_img;
     uint8_t* dr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)ctx->max_w * ctx->max_h;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
@@ -772,6 +913,306 @@ int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const ui
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, d_out);
     if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines)) != hipSuccess)
         return ctx->hip_fail("sgm_compute_raw_device: launch", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = 0;
+    return SGM_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------- compute_disparity
+// stereo.cpp:496-536: pyramid, SGM at the top level, refine_level down to level 0, refine_detail on
+// the full-resolution pair, filter_disparity, remove_speckles, confidence. fl / fr are null or the
+// 2w x 2h full-resolution pair.
+
+// out_disp / out_conf null: the results stay in the context's buffers, returned in *res_disp /
+// *res_conf (the finalize kernel reads and writes each pixel in one thread, so it may work in place).
+static hipError_t enqueue_disparity(sgm_context* ctx, const uint8_t* dl, const uint8_t* dr, int64_t pitch, int w, int h,
+                                    const uint8_t* fl, const uint8_t* fr, int64_t fpitch, float* out_disp,
+                                    float* out_conf, hipStream_t s, Stages* st, float** res_disp = nullptr,
+                                    float** res_conf = nullptr) {
+    const sgm_stereo_params& sp = ctx->sp;
+    const int L = ctx->levels;
+    int lw[kMaxLevels + 1], lh[kMaxLevels + 1];
+    const uint8_t* il[kMaxLevels + 1];
+    const uint8_t* ir[kMaxLevels + 1];
+    int64_t ip[kMaxLevels + 1];
+    lw[0] = w;
+    lh[0] = h;
+    il[0] = dl;
+    ir[0] = dr;
+    ip[0] = pitch;
+    hipError_t e;
+    if (st && L > 0 && !st->front) {
+        if ((e = hipEventRecord(st->ev[5], s)) != hipSuccess) return e;
+        st->front = true;
+    }
+    size_t off = 0;
+    for (int i = 1; i <= L; ++i) {
+        lw[i] = lw[i - 1] / 2;
+        lh[i] = lh[i - 1] / 2;
+        uint8_t* a = ctx->d_pyr + off;
+        uint8_t* b = a + (size_t)lw[i] * lh[i];
+        off += 2 * (size_t)lw[i] * lh[i];
+        if ((e = launch_downsample2(il[i - 1], ir[i - 1], ip[i - 1], lw[i - 1], lh[i - 1], a, b, lw[i], s)) != hipSuccess)
+            return e;
+        il[i] = a;
+        ir[i] = b;
+        ip[i] = lw[i];
+    }
+    sgm_outputs o{};
+    o.disparity = ctx->d_disp;
+    o.disparity_median = L > 0 ? ctx->d_wm : nullptr;  // refine_level's median3 of the coarse disparity
+    PairDesc d = context_desc(ctx, il[L], ir[L], ip[L], lw[L], lh[L], &o);
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines)) != hipSuccess) return e;
+    if (st) {
+        if ((e = hipEventRecord(st->ev[6], s)) != hipSuccess) return e;
+        st->post = true;
+    }
+    float* cur = ctx->d_disp;
+    const float* conf_src = nullptr;
+    for (int l = L - 1; l >= 0; --l) {
+        RefineDesc R{};
+        R.left = il[l];
+        R.right = ir[l];
+        R.pitch = ip[l];
+        R.w = lw[l];
+        R.h = lh[l];
+        R.coarse = ctx->d_wm;
+        R.cw = lw[l + 1];
+        R.ch = lh[l + 1];
+        R.disp = ctx->d_wa;
+        R.conf = ctx->d_wc;
+        R.radius = sp.refine_zncc_radius;
+        R.search_radius = sp.refine_search_radius;
+        R.reject_edge = sp.refine_reject_edge_winners ? 1 : 0;
+        R.subpixel = ctx->p.subpixel ? 1 : 0;
+        R.min_zncc = sp.refine_min_zncc;
+        if ((e = launch_refine(R, s)) != hipSuccess) return e;
+        if (l > 0 && (e = launch_median3(ctx->d_wa, ctx->d_wm, lw[l], lh[l], s)) != hipSuccess) return e;
+        cur = ctx->d_wa;
+        conf_src = ctx->d_wc;
+    }
+    // Ping-pong partner of the current map (d_disp, d_wa and d_wb hold disparity maps).
+    auto other = [&](const float* c) { return c == ctx->d_wa ? ctx->d_wb : ctx->d_wa; };
+    if (sp.detail_enabled && fl && fr) {
+        DetailDesc D{};
+        D.fl = fl;
+        D.fr = fr;
+        D.fpitch = fpitch;
+        D.W = 2 * w;
+        D.H = 2 * h;
+        D.first = cur;
+        D.out = other(cur);
+        D.w = w;
+        D.h = h;
+        D.radius = sp.detail_radius;
+        D.K = std::min(16, (int)std::lround(sp.detail_search / sp.detail_step));
+        D.subpixel = ctx->p.subpixel ? 1 : 0;
+        D.step = sp.detail_step;
+        D.min_zncc = sp.detail_min_zncc;
+        if ((e = launch_detail(D, s)) != hipSuccess) return e;
+        cur = D.out;
+    }
+    if (st && (e = hipEventRecord(st->ev[7], s)) != hipSuccess) return e;
+    if (sp.filter_occlusion && (e = launch_occlusion(cur, w, h, sp.filter_occlusion_tolerance, s)) != hipSuccess) return e;
+    if (sp.filter_spikes) {
+        float* next = other(cur);
+        if ((e = launch_spikes(cur, next, w, h, sp.filter_spike_threshold, s)) != hipSuccess) return e;
+        cur = next;
+    }
+    const bool speckle = sp.speckle_max_region_size > 1;
+    if (speckle && (e = launch_ccl(cur, ctx->d_labels, ctx->d_sizes, w, h, sp.speckle_max_diff, s)) != hipSuccess) return e;
+    if (!out_disp) out_disp = cur;
+    if (!out_conf && res_conf) out_conf = conf_src ? ctx->d_wc : (ctx->d_wc ? ctx->d_wc : ctx->d_conf);
+    if ((e = launch_finalize(cur, speckle ? ctx->d_labels : nullptr, ctx->d_sizes, sp.speckle_max_region_size, conf_src,
+                             out_disp, out_conf, w, h, s)) != hipSuccess)
+        return e;
+    if (res_disp) *res_disp = out_disp;
+    if (res_conf) *res_conf = out_conf;
+    if (st && (e = hipEventRecord(st->ev[8], s)) != hipSuccess) return e;
+    if (st) st->recorded = true;
+    return hipSuccess;
+}
+
+static int check_stereo_size(sgm_context* ctx, int width, int height) {
+    if (width < 1 || height < 1 || width > ctx->stereo_w || height > ctx->stereo_h || (width >> ctx->levels) < 1 ||
+        (height >> ctx->levels) < 1)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "image size " + std::to_string(width) + "x" + std::to_string(height) +
+                                                       " outside the context's " + std::to_string(ctx->stereo_w) + "x" +
+                                                       std::to_string(ctx->stereo_h));
+    return SGM_OK;
+}
+
+// Raw frames (device) through the resident maps into the stereo-resolution pair rl / rr (pitch w),
+// and, when the detail refinement runs, into the full-resolution pair in d_full first.
+static hipError_t enqueue_raw_front(sgm_context* ctx, const uint8_t* raw_l, const uint8_t* raw_r, int64_t raw_pitch,
+                                    uint8_t* rl, uint8_t* rr, hipStream_t s, Stages* st, const uint8_t** fl,
+                                    const uint8_t** fr) {
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    *fl = *fr = nullptr;
+    if (!(ctx->sp.detail_enabled && ctx->d_full)) return enqueue_rectify(ctx, raw_l, raw_r, raw_pitch, rl, rr, s, st);
+    hipError_t e;
+    if (st && !st->front) {
+        if ((e = hipEventRecord(st->ev[5], s)) != hipSuccess) return e;
+        st->front = true;
+    }
+    RectifyDesc r{};
+    const size_t n = (size_t)ctx->map_w * ctx->map_h;
+    r.raw[0] = raw_l;
+    r.raw[1] = raw_r;
+    r.map[0] = ctx->d_maps;
+    r.map[1] = ctx->d_maps + 2 * n;
+    r.raw_pitch = raw_pitch;
+    r.raw_w = ctx->raw_w;
+    r.raw_h = ctx->raw_h;
+    r.map_w = ctx->map_w;
+    uint8_t* f0 = ctx->d_full;
+    uint8_t* f1 = ctx->d_full + (size_t)4 * w * h;
+    if ((e = launch_rectify_full(r, f0, f1, 2 * w, 2 * w, 2 * h, s)) != hipSuccess) return e;
+    if ((e = launch_downsample2(f0, f1, 2 * w, 2 * w, 2 * h, rl, rr, w, s)) != hipSuccess) return e;
+    *fl = f0;
+    *fr = f1;
+    return hipSuccess;
+}
+
+// Host frames in, host maps out, blocking: sgm_compute_disparity (raw == false) and
+// sgm_compute_disparity_raw (raw == true: left / right are raw frames).
+static int disparity_host(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h,
+                          const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride, bool full, bool raw,
+                          float* disparity, float* confidence, uint8_t* rect_l, uint8_t* rect_r, const char* who) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::string nm(who);
+    hipStream_t s = ctx->stream;
+    const size_t npx = (size_t)w * h;
+    uint8_t* dl = ctx->d_img;
+    uint8_t* dr = ctx->d_img + npx;
+    const uint8_t* fl = nullptr;
+    const uint8_t* fr = nullptr;
+    Stages* st = next_stages(ctx);
+    hipError_t e = hipSuccess;
+    if (raw) {
+        const size_t rpx = (size_t)ctx->raw_w * ctx->raw_h;
+        e = hipMemcpy2DAsync(ctx->d_raw, ctx->raw_w, left, stride, ctx->raw_w, ctx->raw_h, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(ctx->d_raw + rpx, ctx->raw_w, right, stride, ctx->raw_w, ctx->raw_h, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return ctx->hip_fail((nm + ": upload").c_str(), e);
+        if ((e = enqueue_raw_front(ctx, ctx->d_raw, ctx->d_raw + rpx, ctx->raw_w, dl, dr, s, st, &fl, &fr)) != hipSuccess)
+            return ctx->hip_fail((nm + ": rectify").c_str(), e);
+    } else {
+        e = hipMemcpy2DAsync(dl, w, left, stride, w, h, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpy2DAsync(dr, w, right, stride, w, h, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && full) {
+            uint8_t* f0 = ctx->d_full;
+            uint8_t* f1 = ctx->d_full + 4 * npx;
+            e = hipMemcpy2DAsync(f0, 2 * w, full_left, full_stride, 2 * w, 2 * h, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpy2DAsync(f1, 2 * w, full_right, full_stride, 2 * w, 2 * h, hipMemcpyHostToDevice, s);
+            fl = f0;
+            fr = f1;
+        }
+        if (e != hipSuccess) return ctx->hip_fail((nm + ": upload").c_str(), e);
+    }
+    float* dd = nullptr;
+    float* dc = nullptr;
+    if ((e = enqueue_disparity(ctx, dl, dr, w, w, h, fl, fr, 2 * w, nullptr, nullptr, s, st, &dd, &dc)) != hipSuccess)
+        return ctx->hip_fail((nm + ": launch").c_str(), e);
+    if ((e = hipMemcpyAsync(disparity, dd, npx * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return ctx->hip_fail((nm + ": download").c_str(), e);
+    if (confidence && (e = hipMemcpyAsync(confidence, dc, npx * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return ctx->hip_fail((nm + ": download").c_str(), e);
+    if (rect_l && (e = hipMemcpyAsync(rect_l, dl, npx, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return ctx->hip_fail((nm + ": download").c_str(), e);
+    if (rect_r && (e = hipMemcpyAsync(rect_r, dr, npx, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return ctx->hip_fail((nm + ": download").c_str(), e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail((nm + ": execution").c_str(), e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    read_timings(ctx);
+    return SGM_OK;
+}
+
+extern "C" {
+
+int sgm_compute_disparity(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int width,
+                          int height, const uint8_t* full_left, const uint8_t* full_right, ptrdiff_t full_stride,
+                          int full_width, int full_height, float* disparity, float* confidence) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_stereo_size(ctx, width, height); rc != SGM_OK) return rc;
+    if (!left || !right || stride < width || !disparity)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or stride < width");
+    // The detail refinement needs the images left and right were downsampled from (stereo.cpp:524).
+    const bool full = ctx->sp.detail_enabled && ctx->d_full && full_left && full_right && full_width == 2 * width &&
+                      full_height == 2 * height;
+    if (full && full_stride < full_width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "full_stride < full_width");
+    return disparity_host(ctx, left, right, stride, width, height, full_left, full_right, full_stride, full, false,
+                          disparity, confidence, nullptr, nullptr, "sgm_compute_disparity");
+}
+
+int sgm_compute_disparity_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch,
+                                 int width, int height, const uint8_t* d_full_left, const uint8_t* d_full_right,
+                                 ptrdiff_t full_pitch, int full_width, int full_height, float* d_disparity,
+                                 float* d_confidence, void* stream) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_stereo_size(ctx, width, height); rc != SGM_OK) return rc;
+    if (!d_left || !d_right || pitch < width || !d_disparity)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or pitch < width");
+    const bool full = ctx->sp.detail_enabled && d_full_left && d_full_right && full_width == 2 * width &&
+                      full_height == 2 * height;
+    if (full && full_pitch < full_width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "full_pitch < full_width");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    const hipError_t e = enqueue_disparity(ctx, d_left, d_right, pitch, width, height, full ? d_full_left : nullptr,
+                                           full ? d_full_right : nullptr, full_pitch, d_disparity, d_confidence, s,
+                                           next_stages(ctx));
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_disparity_device: launch", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = 0;
+    return SGM_OK;
+}
+
+int sgm_compute_disparity_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride,
+                              int raw_w, int raw_h, float* disparity, float* confidence, uint8_t* rect_left,
+                              uint8_t* rect_right) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!raw_left || !raw_right || raw_stride < raw_w || !disparity)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or stride < width");
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
+    return disparity_host(ctx, raw_left, raw_right, raw_stride, w, h, nullptr, nullptr, 0, false, true, disparity,
+                          confidence, rect_left, rect_right, "sgm_compute_disparity_raw");
+}
+
+int sgm_compute_disparity_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right,
+                                     ptrdiff_t raw_pitch, int raw_w, int raw_h, float* d_disparity, float* d_confidence,
+                                     uint8_t* d_rect_left, uint8_t* d_rect_right, void* stream) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!d_raw_left || !d_raw_right || raw_pitch < raw_w || !d_disparity)
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or pitch < width");
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    uint8_t* rl = d_rect_left ? d_rect_left : ctx->d_img;
+    uint8_t* rr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)w * h;
+    Stages* st = next_stages(ctx);
+    const uint8_t* fl = nullptr;
+    const uint8_t* fr = nullptr;
+    hipError_t e = enqueue_raw_front(ctx, d_raw_left, d_raw_right, raw_pitch, rl, rr, s, st, &fl, &fr);
+    if (e == hipSuccess)
+        e = enqueue_disparity(ctx, rl, rr, w, w, h, fl, fr, 2 * w, d_disparity, d_confidence, s, st);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_disparity_raw_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
     return SGM_OK;

@@ -15,6 +15,7 @@ constexpr int kLinesPerBlock = kBlock / kGroup;
 constexpr int kMaxBatch = 64;                 // pairs per launch table
 constexpr int kMaxDirs = 8;
 constexpr int kMaxWidth = 4096;               // the row kernel keeps 9 bytes of LDS per column
+constexpr int kMaxLevels = 4;                 // pyramid levels below the full resolution
 
 // Device pointers and geometry of one rectified pair. Every output pointer may be null except
 // disp, which the left/right check and median read back.
@@ -79,6 +80,33 @@ struct RectifyDesc {
     int32_t out_w, out_h;  // map_w / 2, map_h / 2
 };
 
+// refine_level at one pyramid level (refine_kernels.hip).
+struct RefineDesc {
+    const uint8_t* left;
+    const uint8_t* right;
+    int64_t pitch;        // bytes, both images
+    int32_t w, h;
+    const float* coarse;  // median3 of the next coarser level's disparity, cw x ch
+    int32_t cw, ch;
+    float* disp;          // w x h outputs
+    float* conf;
+    int32_t radius, search_radius, reject_edge, subpixel;
+    float min_zncc;
+};
+
+// refine_detail of a w x h disparity against the W x H full-resolution pair.
+struct DetailDesc {
+    const uint8_t* fl;
+    const uint8_t* fr;
+    int64_t fpitch;
+    int32_t W, H;
+    const float* first;   // input disparity
+    float* out;           // refined copy
+    int32_t w, h;
+    int32_t radius, K, subpixel;
+    float step, min_zncc;
+};
+
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
@@ -88,6 +116,20 @@ hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 hipError_t launch_rectify(const RectifyDesc& r, hipStream_t stream);
+
+// Host-side launchers of the stages around SGM (refine_kernels.hip).
+hipError_t launch_downsample2(const uint8_t* s0, const uint8_t* s1, int64_t spitch, int sw, int sh, uint8_t* d0, uint8_t* d1,
+                              int64_t dpitch, hipStream_t stream);
+hipError_t launch_median3(const float* in, float* out, int w, int h, hipStream_t stream);
+hipError_t launch_refine(const RefineDesc& r, hipStream_t stream);
+hipError_t launch_detail(const DetailDesc& d, hipStream_t stream);
+hipError_t launch_occlusion(float* disp, int w, int h, float tol, hipStream_t stream);
+hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, hipStream_t stream);
+hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* sizes, int w, int h, float max_diff, hipStream_t stream);
+hipError_t launch_finalize(const float* disp, const int32_t* labels, const uint32_t* sizes, int max_region,
+                           const float* conf_src, float* out_disp, float* out_conf, int w, int h, hipStream_t stream);
+hipError_t launch_rectify_full(const RectifyDesc& r, uint8_t* f0, uint8_t* f1, int64_t fpitch, int fw, int fh,
+                               hipStream_t stream);
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);

@@ -61,12 +61,12 @@ class CSgmOutputs(C.Structure):
 
 class CSgmTimings(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms",
-                                          "rectify_ms")]
+                                          "rectify_ms", "refine_ms", "filter_ms")]
 
 
 class CSgmStageTotals(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")] + \
-               [("count", C.c_int64), ("rectify_ms", C.c_double)]
+               [("count", C.c_int64), ("rectify_ms", C.c_double), ("refine_ms", C.c_double), ("filter_ms", C.c_double)]
 
 
 class CSgmServerConfig(C.Structure):
@@ -131,6 +131,47 @@ class SgmParams:
                           float(self.uniqueness), self.lr_max_diff, int(self.subpixel))
 
 
+_STEREO_FIELDS = [
+    ("pyramid_levels", C.c_int32), ("refine_search_radius", C.c_int32), ("refine_zncc_radius", C.c_int32),
+    ("refine_min_zncc", C.c_float), ("refine_reject_edge_winners", C.c_int32), ("detail_enabled", C.c_int32),
+    ("detail_radius", C.c_int32), ("detail_search", C.c_float), ("detail_step", C.c_float),
+    ("detail_min_zncc", C.c_float), ("filter_occlusion", C.c_int32), ("filter_occlusion_tolerance", C.c_float),
+    ("filter_spikes", C.c_int32), ("filter_spike_threshold", C.c_float), ("speckle_max_region_size", C.c_int32),
+    ("speckle_max_diff", C.c_float),
+]
+
+
+class CSgmStereoParams(C.Structure):
+    _fields_ = [("sgm", CSgmParams)] + _STEREO_FIELDS
+
+
+@dataclass
+class StereoParams:
+    """depth::StereoParams (libs/depth/include/einstar/depth/stereo.hpp:42-89) minus sgm/subpixel, with
+    the reference defaults: two pyramid levels and every stage on."""
+
+    pyramid_levels: int = 2
+    refine_search_radius: int = 2
+    refine_zncc_radius: int = 3
+    refine_min_zncc: float = 0.5
+    refine_reject_edge_winners: bool = True
+    detail_enabled: bool = True
+    detail_radius: int = 5
+    detail_search: float = 1.0
+    detail_step: float = 0.5
+    detail_min_zncc: float = 0.5
+    filter_occlusion: bool = True
+    filter_occlusion_tolerance: float = 1.0
+    filter_spikes: bool = True
+    filter_spike_threshold: float = 1.0
+    speckle_max_region_size: int = 200
+    speckle_max_diff: float = 1.0
+
+    def to_c(self, sgm: "SgmParams") -> CSgmStereoParams:
+        vals = [float(getattr(self, f)) if t is C.c_float else int(getattr(self, f)) for f, t in _STEREO_FIELDS]
+        return CSgmStereoParams(sgm.to_c(), *vals)
+
+
 _lib = None
 
 
@@ -152,6 +193,14 @@ def lib():
         L.sgm_set_rectification.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32]
         L.sgm_compute_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp, vp]
         L.sgm_compute_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmOutputs), vp, vp, vp]
+        SP = P(CSgmStereoParams)
+        L.sgm_stereo_params_default.argtypes = [SP]
+        L.sgm_create_stereo.argtypes = [SP, i32, i32, P(vp)]
+        L.sgm_compute_disparity.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, C.c_ssize_t, i32, i32, vp, vp]
+        L.sgm_compute_disparity_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, C.c_ssize_t, i32, i32, vp,
+                                                   vp, vp]
+        L.sgm_compute_disparity_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, vp, vp]
+        L.sgm_compute_disparity_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, vp, vp, vp]
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
         L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
@@ -267,6 +316,61 @@ class HipStereo:
         cp = params.to_c()
         _check(lib().sgm_create(C.byref(cp), width, height, C.byref(h)))
         return cls(h.value, params, width, height)
+
+    @classmethod
+    def create_stereo(cls, params: SgmParams, stereo: StereoParams, width: int, height: int) -> "HipStereo":
+        """MetalStereo::create with the full StereoParams: compute_disparity on width x height images,
+        SGM at (width >> levels) x (height >> levels)."""
+        h = C.c_void_p()
+        cp = stereo.to_c(params)
+        _check(lib().sgm_create_stereo(C.byref(cp), width, height, C.byref(h)))
+        obj = cls(h.value, params, width, height)
+        obj.stereo = stereo
+        return obj
+
+    def compute_disparity(self, left, right, full_left=None, full_right=None) -> dict:
+        """depth::compute_disparity (stereo.cpp:496-536): disparity and confidence at left's size."""
+        left, right = np.ascontiguousarray(left, np.uint8), np.ascontiguousarray(right, np.uint8)
+        h, w = left.shape
+        fl = None if full_left is None else np.ascontiguousarray(full_left, np.uint8)
+        fr = None if full_right is None else np.ascontiguousarray(full_right, np.uint8)
+        fh, fw = (0, 0) if fl is None else fl.shape
+        res = {"disparity": np.empty((h, w), np.float32), "confidence": np.empty((h, w), np.float32)}
+        _check(lib().sgm_compute_disparity(self._h, _ptr(left), _ptr(right), w, w, h, _ptr(fl), _ptr(fr), fw, fw, fh,
+                                           _ptr(res["disparity"]), _ptr(res["confidence"])), ctx=self._h)
+        return res
+
+    def compute_disparity_device(self, d_left: int, d_right: int, pitch: int, width: int, height: int,
+                                 d_disparity: int, d_confidence: int | None = None, d_full_left: int | None = None,
+                                 d_full_right: int | None = None, full_pitch: int = 0, stream: int | None = None):
+        """Device-resident compute_disparity (full pair, when given, is 2*width x 2*height); no wait."""
+        fw, fh = (2 * width, 2 * height) if d_full_left else (0, 0)
+        _check(lib().sgm_compute_disparity_device(self._h, d_left, d_right, pitch, width, height, d_full_left,
+                                                  d_full_right, full_pitch or fw, fw, fh, d_disparity, d_confidence,
+                                                  stream), ctx=self._h)
+
+    def compute_disparity_raw(self, raw_left, raw_right, rectified: bool = False) -> dict:
+        """MetalStereo::compute_raw with the full StereoParams: rectify, then compute_disparity."""
+        raw_left, raw_right = np.ascontiguousarray(raw_left, np.uint8), np.ascontiguousarray(raw_right, np.uint8)
+        rh, rw = raw_left.shape
+        w, h = getattr(self, "rect_size", (1, 1))
+        res = {"disparity": np.empty((h, w), np.float32), "confidence": np.empty((h, w), np.float32)}
+        rl = rr = None
+        if rectified:
+            res["rect_left"] = rl = np.empty((h, w), np.uint8)
+            res["rect_right"] = rr = np.empty((h, w), np.uint8)
+        _check(lib().sgm_compute_disparity_raw(self._h, _ptr(raw_left), _ptr(raw_right), rw, rw, rh,
+                                               _ptr(res["disparity"]), _ptr(res["confidence"]), _ptr(rl), _ptr(rr)),
+               ctx=self._h)
+        return res
+
+    def compute_disparity_raw_device(self, d_raw_left: int, d_raw_right: int, pitch: int, raw_width: int,
+                                     raw_height: int, d_disparity: int, d_confidence: int | None = None,
+                                     d_rect_left: int | None = None, d_rect_right: int | None = None,
+                                     stream: int | None = None):
+        _check(lib().sgm_compute_disparity_raw_device(self._h, d_raw_left, d_raw_right, pitch, raw_width, raw_height,
+                                                      d_disparity, d_confidence, d_rect_left, d_rect_right, stream),
+               ctx=self._h)
 
     def close(self):
         if self._h:

@@ -58,3 +58,21 @@ def test_invalid_sizes_are_rejected():
         with pytest.raises(sgm.SgmError) as e:
             sgm.HipStereo.create(sgm.SgmParams(), w, h)
         assert e.value.errc == "invalid_argument"
+
+
+@pytest.mark.parametrize("bad", [dict(pyramid_levels=5), dict(refine_search_radius=5), dict(refine_search_radius=0),
+                                 dict(detail_search=5.0), dict(detail_step=0.0), dict(refine_zncc_radius=8)])
+def test_invalid_stereo_parameters_are_rejected_before_any_gpu_work(bad):
+    """MetalStereo::create's limits on the refinement stages (metal_stereo.cpp:145-149)."""
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.HipStereo.create_stereo(sgm.SgmParams(num_disparities=16), sgm.StereoParams(**bad), 320, 256)
+    assert e.value.errc == "invalid_argument"
+
+
+def test_stereo_defaults_match_reference_stereo_params():
+    import oracle as O
+    p = sgm.CSgmStereoParams()
+    sgm.lib().sgm_stereo_params_default(C.byref(p))
+    ref = O.StereoParams()  # stereo.hpp:42-89
+    for f, _ in sgm._STEREO_FIELDS:
+        assert getattr(p, f) == pytest.approx(float(getattr(ref, f))), f
