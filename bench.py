@@ -117,6 +117,77 @@ def raw_serving_inputs(sgm, speckle_pair, pool):
     return (full, roi), maps
 
 
+def downsample2(img):
+    """(a + b + c + d + 2) / 4 over 2x2 blocks: input preparation for the chain below."""
+    a = img[: img.shape[0] // 2 * 2, : img.shape[1] // 2 * 2].astype(np.uint16)
+    return ((a[0::2, 0::2] + a[0::2, 1::2] + a[1::2, 0::2] + a[1::2, 1::2] + 2) // 4).astype(np.uint8)
+
+
+def stereo_chain(sgm, speckle_pair, steps, warmup, verify=True, cpu=True):
+    """depth::compute_disparity with the reference defaults on the served frame size: 1280x1024
+    rectified pair plus its 2560x2048 full-resolution pair; SGM at 320x256 (D=32), two ZNCC
+    refinement levels, detail refinement, occlusion/spike filters, speckle removal. Device-resident
+    inputs, K back-to-back calls."""
+    W0, H0 = 1280, 1024
+    full = speckle_pair(2 * W0, 2 * H0, 256, seed=11, min_cells=3, max_cells=6)
+    left, right = downsample2(full.left), downsample2(full.right)
+    sg = dict(PARAMS, min_disparity=full.min_disparity // 8 - 1, num_disparities=32)
+    st = sgm.HipStereo.create_stereo(sgm.SgmParams(**sg), sgm.StereoParams(), W0, H0)
+    img, fimg = sgm.DeviceBuffer(2 * W0 * H0), sgm.DeviceBuffer(8 * W0 * H0)
+    img.upload(left, 0)
+    img.upload(right, W0 * H0)
+    fimg.upload(full.left, 0)
+    fimg.upload(full.right, 4 * W0 * H0)
+    dd, dc = sgm.DeviceBuffer(4 * W0 * H0), sgm.DeviceBuffer(4 * W0 * H0)
+    try:
+        def step():
+            st.compute_disparity_device(img.ptr, img.ptr + W0 * H0, W0, W0, H0, dd.ptr, dc.ptr, fimg.ptr,
+                                        fimg.ptr + 4 * W0 * H0, 2 * W0)
+
+        for _ in range(warmup):
+            step()
+        st.synchronize()
+        st.reset_stage_totals()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        st.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        tot = st.stage_totals()
+        n = max(1, tot["count"])
+        stages = {k: round(tot[k] / n, 4) for k in ("rectify_ms", "census_ms", "paths_ms", "rows_ms", "median_ms",
+                                                    "refine_ms", "filter_ms", "gpu_ms")}
+        out = {"workload": "compute_disparity 1280x1024 + 2560x2048 full pair, reference StereoParams defaults "
+                           "(SGM 320x256 D=32 8 paths, 2 refinement levels, detail, filters, speckles)",
+               "ms_per_frame": round(ms, 4), "frames_per_s": round(1e3 / ms, 1), "stage_ms": stages}
+        got = {"disparity": dd.download((H0, W0), np.float32), "confidence": dc.download((H0, W0), np.float32)}
+        out["valid_fraction"] = round(float((got["disparity"] > -1e8).mean()), 4)
+        if verify or cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            threads = min(len(os.sched_getaffinity(0)), 64)
+            runs = []
+            ref = None
+            for _ in range(3 if cpu else 1):
+                t = time.perf_counter()
+                ref = O.compute_disparity(left, right, O.SgmParams(**sg), O.StereoParams(), full.left, full.right,
+                                          threads=threads)
+                runs.append(time.perf_counter() - t)
+            if verify:
+                out["bit_exact"] = all(np.array_equal(ref[f].view(np.uint32), got[f].view(np.uint32))
+                                       for f in ("disparity", "confidence"))
+            if cpu:
+                med = float(np.median(runs[1:] if len(runs) > 1 else runs))
+                out["cpu_baseline"] = {"ms_per_frame": round(med * 1e3, 1), "cores": threads, "kind": "port",
+                                       "sample": f"oracle compute_disparity on the same frame, median of "
+                                                 f"{max(1, len(runs) - 1)} runs after 1 warm-up"}
+        return out
+    finally:
+        for b in (img, fimg, dd, dc):
+            b.free()
+        st.close()
+
+
 def large_batched(sgm, speckle_pair, resident=16, total=64, distinct=8, verify=True):
     """64 pairs of 2048x1536 with 256 disparities, 8 paths, streamed through a resident batch of
     `resident` pairs (about 116 GB of path costs and scratch): per chunk, the next pairs' images are
@@ -198,6 +269,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
     ap.add_argument("--no-large", action="store_true", help="skip the 64 x 2048x1536x256 streamed batch")
+    ap.add_argument("--no-chain", action="store_true", help="skip the full compute_disparity chain")
     ap.add_argument("--serve-frames", type=int, default=600, help="frames of the 60 Hz served stream (0: skip)")
     ap.add_argument("--serve-sweep", type=str, default="240,480,720,960",
                     help="arrival rates (Hz) of the saturation sweep, 300 frames each ('' to skip)")
@@ -368,6 +440,11 @@ def main():
         log(f"batched: {batched}")
         b.close()
 
+    chain = None
+    if not args.no_chain:
+        chain = stereo_chain(sgm, speckle_pair, args.steps, args.warmup, verify=not args.no_verify, cpu=not args.no_cpu)
+        log(f"compute_disparity chain: {chain}")
+
     large = None
     if not args.no_large:
         large = large_batched(sgm, speckle_pair, verify=not args.no_verify)
@@ -422,6 +499,7 @@ def main():
         "batched": batched,
         "served": served,
         "large_batched": large,
+        "compute_disparity": chain,
     }
     print(json.dumps(line, ensure_ascii=False), flush=True)
     for bbuf in outs.values():

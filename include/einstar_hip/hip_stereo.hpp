@@ -1,8 +1,8 @@
 #pragma once
 
 // einstar::depth_hip::HipStereo -- C++ replacement of einstar::depth_metal::MetalStereo
-// (libs/depth_metal/include/einstar/depth_metal/metal_stereo.hpp:67-107) for its SGM stage, on the
-// gfx950 library behind include/sgm_hip.h. Header-only: it only calls the C ABI.
+// (libs/depth_metal/include/einstar/depth_metal/metal_stereo.hpp:67-107) for its stereo matching,
+// on the gfx950 library behind include/sgm_hip.h. Header-only: it only calls the C ABI.
 //
 // The value types restate the reference's (C++20 here; the reference's need C++23 <expected>):
 //   ImageView / Image            libs/core/include/einstar/core/image.hpp:16-102 (stride in elements)
@@ -10,11 +10,13 @@
 //   SgmParams ... StereoParams   libs/depth/include/einstar/depth/stereo.hpp:26-88
 //   StereoResult, StereoTimings  stereo.hpp:90-93, metal_stereo.hpp:61-65
 //
-// Scope: create / compute / last_timings as MetalStereo has them, for a configuration whose whole
-// depth::compute_disparity is the SGM stage (pyramid_levels == 0, occlusion and spike filters off,
-// speckle removal off). Anything else answers Errc::unsupported, naming the missing stage.
-// compute_sgm additionally returns every per-pixel output of the SGM stage (integer disparity,
-// status bits, right disparity, best/second costs, median3).
+// Scope: create / set_rectification / compute / compute_raw / last_timings as MetalStereo has them,
+// running depth::compute_disparity with every stage of StereoParams (pyramid, SGM, ZNCC refinement,
+// detail refinement, occlusion and spike filters, speckle removal). compute_full is
+// compute_disparity's second form. compute_sgm additionally returns every per-pixel output of the
+// SGM stage (integer disparity, status bits, right disparity, best/second costs, median3) when SGM
+// runs at the configured size (pyramid_levels == 0). The point-cloud and blob stages of MetalStereo
+// (set_point_params, compute_frame*) are not part of this library.
 
 #include <cstddef>
 #include <cstdint>
@@ -211,14 +213,49 @@ inline sgm_params to_c(const StereoParams& p) {
     return c;
 }
 
+inline sgm_stereo_params to_c_stereo(const StereoParams& p) {
+    sgm_stereo_params c{};
+    c.sgm = to_c(p);
+    c.pyramid_levels = p.pyramid_levels;
+    c.refine_search_radius = p.refine.search_radius;
+    c.refine_zncc_radius = p.refine.zncc_radius;
+    c.refine_min_zncc = p.refine.min_zncc;
+    c.refine_reject_edge_winners = p.refine.reject_edge_winners ? 1 : 0;
+    c.detail_enabled = p.detail.enabled ? 1 : 0;
+    c.detail_radius = p.detail.radius;
+    c.detail_search = p.detail.search;
+    c.detail_step = p.detail.step;
+    c.detail_min_zncc = p.detail.min_zncc;
+    c.filter_occlusion = p.filter.occlusion ? 1 : 0;
+    c.filter_occlusion_tolerance = p.filter.occlusion_tolerance;
+    c.filter_spikes = p.filter.spikes ? 1 : 0;
+    c.filter_spike_threshold = p.filter.spike_threshold;
+    c.speckle_max_region_size = p.speckle.max_region_size;
+    c.speckle_max_diff = p.speckle.max_diff;
+    return c;
+}
+
+// True when depth::compute_disparity with these parameters is the SGM stage alone.
+inline bool sgm_only(const StereoParams& p) {
+    return p.pyramid_levels == 0 && !p.detail.enabled && !p.filter.occlusion && !p.filter.spikes &&
+           p.speckle.max_region_size <= 1;
+}
+
 class HipStereo {
 public:
     // MetalStereo::create (metal_stereo.hpp:70-71) without the Metal context argument: the current
-    // HIP device is used. `width`/`height`: the resolution the disparity is computed at.
+    // HIP device is used. `width`/`height`: the resolution the disparity is computed at (the finest
+    // level); SGM runs at (width >> pyramid_levels) x (height >> pyramid_levels).
     static Result<std::unique_ptr<HipStereo>> create(const StereoParams& params, int width, int height) {
-        const sgm_params c = to_c(params);
         sgm_context* ctx = nullptr;
-        const int rc = sgm_create(&c, width, height, &ctx);
+        int rc;
+        if (sgm_only(params)) {
+            const sgm_params c = to_c(params);
+            rc = sgm_create(&c, width, height, &ctx);
+        } else {
+            const sgm_stereo_params c = to_c_stereo(params);
+            rc = sgm_create_stereo(&c, width, height, &ctx);
+        }
         if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(nullptr));
         return std::unique_ptr<HipStereo>(new HipStereo(ctx, params, width, height));
     }
@@ -226,28 +263,35 @@ public:
     HipStereo(const HipStereo&) = delete;
     HipStereo& operator=(const HipStereo&) = delete;
 
-    // MetalStereo::compute (metal_stereo.hpp:78): depth::compute_disparity on rectified images at the
-    // configured size. Supported when that is the SGM stage alone (see the header comment).
+    // MetalStereo::compute (metal_stereo.hpp:77-78): depth::compute_disparity (stereo.cpp:496-536) on
+    // rectified images at the configured size, every stage of the StereoParams.
     Result<StereoResult> compute(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right) {
-        if (params_.pyramid_levels != 0)
-            return make_error(Errc::unsupported, "coarse-to-fine ZNCC refinement (pyramid_levels > 0) is not implemented");
-        if (params_.filter.occlusion || params_.filter.spikes)
-            return make_error(Errc::unsupported, "occlusion and spike filters are not implemented");
-        if (params_.speckle.max_region_size > 1)
-            return make_error(Errc::unsupported, "speckle removal is not implemented");
+        return compute_full(left, right, {}, {});
+    }
+
+    // depth::compute_disparity's second form (stereo.hpp:96-98): with the full-resolution images the
+    // pair was box-downsampled from, for the detail refinement (used when they are twice the size).
+    Result<StereoResult> compute_full(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right,
+                                      ImageView<const std::uint8_t> full_left, ImageView<const std::uint8_t> full_right) {
         if (auto r = check(left, right); !r) return r.error();
+        if (full_left.data && full_left.stride != full_right.stride)
+            return make_error(Errc::invalid_argument, "full-resolution images must share a stride");
         StereoResult res;
         res.disparity = ImageF32(w_, h_);
         res.confidence = ImageF32(w_, h_);
-        sgm_outputs o{};
-        o.disparity = res.disparity.data();
-        o.confidence = res.confidence.data();
-        if (auto r = run(left, right, o); !r) return r.error();
+        const int rc = sgm_compute_disparity(ctx_, left.data, right.data, left.stride, w_, h_, full_left.data,
+                                             full_right.data, full_left.stride, full_left.width, full_left.height,
+                                             res.disparity.data(), res.confidence.data());
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        read_timings();
         return res;
     }
 
-    // The SGM stage's full output set (sgm_disparity plus median3 and the integer intermediates).
+    // The SGM stage's full output set (sgm_disparity plus median3 and the integer intermediates) at the
+    // configured size, for configurations that run SGM at that size (pyramid_levels == 0).
     Result<SgmResult> compute_sgm(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right) {
+        if (params_.pyramid_levels != 0)
+            return make_error(Errc::unsupported, "compute_sgm needs pyramid_levels 0 (SGM at the configured size)");
         if (auto r = check(left, right); !r) return r.error();
         SgmResult res;
         res.disparity = ImageF32(w_, h_);
@@ -286,9 +330,6 @@ public:
     // and run stereo; optionally returns the rectified pair.
     Result<StereoResult> compute_raw(ImageView<const std::uint8_t> raw_left, ImageView<const std::uint8_t> raw_right,
                                      ImageU8* rect_left = nullptr, ImageU8* rect_right = nullptr) {
-        if (params_.pyramid_levels != 0 || params_.filter.occlusion || params_.filter.spikes ||
-            params_.speckle.max_region_size > 1)
-            return make_error(Errc::unsupported, "only the SGM stage is implemented (pyramid_levels 0, filters off)");
         if (raw_w_ == 0) return make_error(Errc::invalid_argument, "set_rectification() not called");
         if (raw_left.width != raw_w_ || raw_left.height != raw_h_ || raw_right.width != raw_w_ || raw_right.height != raw_h_)
             return make_error(Errc::invalid_argument, "raw size mismatch");
@@ -299,11 +340,10 @@ public:
         res.confidence = ImageF32(w_, h_);
         if (rect_left) *rect_left = ImageU8(w_, h_);
         if (rect_right) *rect_right = ImageU8(w_, h_);
-        sgm_outputs o{};
-        o.disparity = res.disparity.data();
-        o.confidence = res.confidence.data();
-        const int rc = sgm_compute_raw(ctx_, raw_left.data, raw_right.data, raw_left.stride, raw_w_, raw_h_, &o,
-                                       rect_left ? rect_left->data() : nullptr, rect_right ? rect_right->data() : nullptr);
+        const int rc = sgm_compute_disparity_raw(ctx_, raw_left.data, raw_right.data, raw_left.stride, raw_w_, raw_h_,
+                                                 res.disparity.data(), res.confidence.data(),
+                                                 rect_left ? rect_left->data() : nullptr,
+                                                 rect_right ? rect_right->data() : nullptr);
         if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
         read_timings();
         return res;

@@ -322,26 +322,34 @@ __global__ __launch_bounds__(kTw* kTh) void spike_kernel(const float* in, float*
 // ------------------------------------------------------------------------------- speckles
 // remove_speckles (stereo.cpp:465-494) labels the connected components of the graph whose edges
 // join 4-neighbours that are both valid with |d_a - d_b| <= max_diff; a flood fill and a union-find
-// find the same components. Labels are pixel indices; every label points at a smaller or equal
-// index, so every find ends, and unions link the larger root under the smaller by atomic min.
+// find the same components. Labels are pixel indices and every label points at a smaller or equal
+// index, so every find ends; unions link the larger root under the smaller by atomic min.
+//
+// Three passes: (1) each 16x16 tile is labelled in LDS, every pixel then points at its tile-local
+// root and each local root holds its local pixel count; (2) edges that cross tile borders unite the
+// local roots in global memory (chains are at most a few tiles long); (3) each local root adds its
+// count to its global root and points straight at it, so a pixel's global root is L[L[i]].
 
-__device__ __forceinline__ int32_t load_label(const int32_t* L, int32_t i) {
-    return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <typename T>
+__device__ __forceinline__ T atomic_ld(const T* p, int scope) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, scope);
 }
 
+template <int Scope>
 __device__ __forceinline__ int32_t find_root(const int32_t* L, int32_t a) {
-    int32_t p = load_label(L, a);
+    int32_t p = __hip_atomic_load(L + a, __ATOMIC_RELAXED, Scope);
     while (p != a) {
         a = p;
-        p = load_label(L, a);
+        p = __hip_atomic_load(L + a, __ATOMIC_RELAXED, Scope);
     }
     return a;
 }
 
+template <int Scope>
 __device__ void unite(int32_t* L, int32_t a, int32_t b) {
     for (;;) {
-        a = find_root(L, a);
-        b = find_root(L, b);
+        a = find_root<Scope>(L, a);
+        b = find_root<Scope>(L, b);
         if (a == b) return;
         if (a < b) {
             const int32_t old = atomicMin(L + b, a);
@@ -355,38 +363,62 @@ __device__ void unite(int32_t* L, int32_t a, int32_t b) {
     }
 }
 
-__global__ __launch_bounds__(kTw* kTh) void ccl_init_kernel(const float* disp, int32_t* L, uint32_t* sizes, int w, int h) {
-    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
-    if (x >= w || y >= h) return;
+__device__ __forceinline__ bool joined(float d, float e, float max_diff) {
+    return valid_d(e) && !(fabsf(e - d) > max_diff);
+}
+
+__global__ __launch_bounds__(kTw* kTh) void ccl_local_kernel(const float* disp, int32_t* L, uint32_t* local, uint32_t* sizes,
+                                                             int w, int h, float max_diff) {
+    __shared__ int32_t ll[kTw * kTh];
+    __shared__ uint32_t cnt[kTw * kTh];
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * kTw + tx;
+    const int x = (int)blockIdx.x * kTw + tx, y = (int)blockIdx.y * kTh + ty;
+    const bool in = x < w && y < h;
     const int32_t i = y * w + x;
-    L[i] = valid_d(disp[i]) ? i : -1;
+    const float d = in ? disp[i] : kInvalid;
+    const bool v = valid_d(d);
+    ll[t] = v ? t : -1;
+    cnt[t] = 0;
+    __syncthreads();
+    if (v) {
+        if (tx + 1 < kTw && x + 1 < w && joined(d, disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + 1);
+        if (ty + 1 < kTh && y + 1 < h && joined(d, disp[i + w], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + kTw);
+    }
+    __syncthreads();
+    int32_t r = -1;
+    if (v) {
+        r = find_root<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t);
+        atomicAdd(&cnt[r], 1u);
+    }
+    __syncthreads();
+    if (!in) return;
+    const int32_t root = v ? ((int32_t)blockIdx.y * kTh + r / kTw) * w + (int32_t)blockIdx.x * kTw + r % kTw : -1;
+    L[i] = root;
+    local[i] = v && r == t ? cnt[t] : 0u;
     sizes[i] = 0;
 }
 
-__global__ __launch_bounds__(kTw* kTh) void ccl_merge_kernel(const float* disp, int32_t* L, int w, int h, float max_diff) {
+__global__ __launch_bounds__(kTw* kTh) void ccl_border_kernel(const float* disp, int32_t* L, int w, int h, float max_diff) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
     if (x >= w || y >= h) return;
+    const bool right = (threadIdx.x == kTw - 1) && x + 1 < w, down = (threadIdx.y == kTh - 1) && y + 1 < h;
+    if (!right && !down) return;
     const int32_t i = y * w + x;
     const float d = disp[i];
     if (!valid_d(d)) return;
-    if (x + 1 < w) {
-        const float e = disp[i + 1];
-        if (valid_d(e) && !(fabsf(e - d) > max_diff)) unite(L, i, i + 1);
-    }
-    if (y + 1 < h) {
-        const float e = disp[i + w];
-        if (valid_d(e) && !(fabsf(e - d) > max_diff)) unite(L, i, i + w);
-    }
+    if (right && joined(d, disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_AGENT>(L, i, i + 1);
+    if (down && joined(d, disp[i + w], max_diff)) unite<__HIP_MEMORY_SCOPE_AGENT>(L, i, i + w);
 }
 
-__global__ __launch_bounds__(kTw* kTh) void ccl_count_kernel(int32_t* L, uint32_t* sizes, int w, int h) {
+__global__ __launch_bounds__(kTw* kTh) void ccl_accum_kernel(int32_t* L, const uint32_t* local, uint32_t* sizes, int w, int h) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
     if (x >= w || y >= h) return;
     const int32_t i = y * w + x;
-    if (L[i] < 0) return;
-    const int32_t root = find_root(L, i);
-    L[i] = root;
-    atomicAdd(sizes + root, 1u);
+    const uint32_t n = local[i];
+    if (n == 0) return;  // not a tile-local root
+    const int32_t root = find_root<__HIP_MEMORY_SCOPE_AGENT>(L, i);
+    atomicAdd(sizes + root, n);
+    __hip_atomic_store(L + i, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The end of compute_disparity (stereo.cpp:527-534): speckle regions removed, confidence 0 where
@@ -398,7 +430,7 @@ __global__ __launch_bounds__(kTw* kTh) void finalize_kernel(const float* disp, c
     if (x >= w || y >= h) return;
     const int32_t i = y * w + x;
     float d = disp[i];
-    if (L && valid_d(d) && sizes[L[i]] < (uint32_t)max_region) d = kInvalid;
+    if (L && valid_d(d) && sizes[L[L[i]]] < (uint32_t)max_region) d = kInvalid;  // L[i]: local root
     out_disp[i] = d;
     if (out_conf) out_conf[i] = valid_d(d) ? (conf_src ? conf_src[i] : 1.0f) : 0.0f;
 }
@@ -469,14 +501,15 @@ hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, h
     return hipGetLastError();
 }
 
-hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* sizes, int w, int h, float max_diff, hipStream_t stream) {
+hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* local, uint32_t* sizes, int w, int h, float max_diff,
+                      hipStream_t stream) {
     if (w <= 0 || h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ccl_init_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, sizes, w, h);
+    hipLaunchKernelGGL(ccl_local_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, local, sizes, w, h, max_diff);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ccl_merge_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, w, h, max_diff);
+    hipLaunchKernelGGL(ccl_border_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, w, h, max_diff);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(ccl_count_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, labels, sizes, w, h);
+    hipLaunchKernelGGL(ccl_accum_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, labels, local, sizes, w, h);
     return hipGetLastError();
 }
 

@@ -207,7 +207,8 @@ struct sgm_context {
     float* d_wb = nullptr;
     float* d_wm = nullptr;
     float* d_wc = nullptr;
-    int32_t* d_labels = nullptr; // speckle components
+    int32_t* d_labels = nullptr; // speckle components: labels, tile-local counts, component sizes
+    uint32_t* d_local = nullptr;
     uint32_t* d_sizes = nullptr;
     uint8_t* d_full = nullptr;   // full-resolution pair (2 * stereo_w x 2 * stereo_h each) for the detail refinement
     size_t bytes = 0;
@@ -253,7 +254,8 @@ void sgm_destroy(sgm_context* ctx) {
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
                     (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol, (void*)ctx->d_maps,
                     (void*)ctx->d_raw, (void*)ctx->d_pyr, (void*)ctx->d_wa, (void*)ctx->d_wb, (void*)ctx->d_wm,
-                    (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_sizes, (void*)ctx->d_full})
+                    (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_local, (void*)ctx->d_sizes,
+                    (void*)ctx->d_full})
         dfree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -355,6 +357,7 @@ static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int st
         for (float** w : {&ctx->d_wa, &ctx->d_wb, &ctx->d_wm, &ctx->d_wc})
             if (e == hipSuccess) e = dalloc(w, spx, b);
         if (e == hipSuccess) e = dalloc(&ctx->d_labels, spx, b);
+        if (e == hipSuccess) e = dalloc(&ctx->d_local, spx, b);
         if (e == hipSuccess) e = dalloc(&ctx->d_sizes, spx, b);
         if (e == hipSuccess && sp.detail_enabled) e = dalloc(&ctx->d_full, 8 * spx, b);
     }
@@ -1022,7 +1025,7 @@ static hipError_t enqueue_disparity(sgm_context* ctx, const uint8_t* dl, const u
         cur = next;
     }
     const bool speckle = sp.speckle_max_region_size > 1;
-    if (speckle && (e = launch_ccl(cur, ctx->d_labels, ctx->d_sizes, w, h, sp.speckle_max_diff, s)) != hipSuccess) return e;
+    if (speckle && (e = launch_ccl(cur, ctx->d_labels, ctx->d_local, ctx->d_sizes, w, h, sp.speckle_max_diff, s)) != hipSuccess) return e;
     if (!out_disp) out_disp = cur;
     if (!out_conf && res_conf) out_conf = conf_src ? ctx->d_wc : (ctx->d_wc ? ctx->d_wc : ctx->d_conf);
     if ((e = launch_finalize(cur, speckle ? ctx->d_labels : nullptr, ctx->d_sizes, sp.speckle_max_region_size, conf_src,

@@ -125,7 +125,8 @@ hipError_t launch_refine(const RefineDesc& r, hipStream_t stream);
 hipError_t launch_detail(const DetailDesc& d, hipStream_t stream);
 hipError_t launch_occlusion(float* disp, int w, int h, float tol, hipStream_t stream);
 hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, hipStream_t stream);
-hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* sizes, int w, int h, float max_diff, hipStream_t stream);
+hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* local, uint32_t* sizes, int w, int h, float max_diff,
+                      hipStream_t stream);
 hipError_t launch_finalize(const float* disp, const int32_t* labels, const uint32_t* sizes, int max_region,
                            const float* conf_src, float* out_disp, float* out_conf, int w, int h, hipStream_t stream);
 hipError_t launch_rectify_full(const RectifyDesc& r, uint8_t* f0, uint8_t* f1, int64_t fpitch, int fw, int fh,

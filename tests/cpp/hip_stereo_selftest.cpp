@@ -68,6 +68,7 @@ int main() {
     sp.filter.occlusion = false;
     sp.filter.spikes = false;
     sp.speckle.max_region_size = 0;
+    sp.detail.enabled = false;  // compute_disparity is then sgm_disparity itself
     sp.sgm.num_disparities = 32;
     sp.sgm.census_radius_x = 4;
     sp.sgm.census_radius_y = 3;
@@ -185,12 +186,55 @@ int main() {
     ImageU8 small(w - 1, h);
     auto bad = (*st)->compute(small.view(), r.view());
     expect(!bad && bad.error().code == Errc::invalid_argument, "size mismatch -> invalid_argument");
-    StereoParams dflt;  // pyramid_levels = 2, filters on
+    // The reference defaults: two pyramid levels and every stage on (stereo.hpp:42-89), against the
+    // oracle's compute_disparity, without and with the full-resolution pair (detail refinement).
+    StereoParams dflt;
+    dflt.sgm = sp.sgm;
+    dflt.sgm.num_disparities = 16;  // at the quarter-resolution SGM level
     auto st2 = HipStereo::create(dflt, w, h);
     expect(st2.has_value(), "create with reference defaults");
     if (st2) {
+        ImageU8 fl(2 * w, 2 * h), fr(2 * w, 2 * h);  // pixel-replicated: downsample2 gives back (l, r)
+        for (int y = 0; y < 2 * h; ++y)
+            for (int x = 0; x < 2 * w; ++x) {
+                fl(x, y) = l(x / 2, y / 2);
+                fr(x, y) = r(x / 2, y / 2);
+            }
+        const orc_params op2 = oracle_params(dflt);
+        orc_stereo_params os{};
+        os.pyramid_levels = dflt.pyramid_levels;
+        os.refine_search_radius = dflt.refine.search_radius;
+        os.refine_zncc_radius = dflt.refine.zncc_radius;
+        os.refine_min_zncc = dflt.refine.min_zncc;
+        os.refine_reject_edge_winners = dflt.refine.reject_edge_winners;
+        os.detail_enabled = dflt.detail.enabled;
+        os.detail_radius = dflt.detail.radius;
+        os.detail_search = dflt.detail.search;
+        os.detail_step = dflt.detail.step;
+        os.detail_min_zncc = dflt.detail.min_zncc;
+        os.filter_occlusion = dflt.filter.occlusion;
+        os.filter_occlusion_tolerance = dflt.filter.occlusion_tolerance;
+        os.filter_spikes = dflt.filter.spikes;
+        os.filter_spike_threshold = dflt.filter.spike_threshold;
+        os.speckle_max_region_size = dflt.speckle.max_region_size;
+        os.speckle_max_diff = dflt.speckle.max_diff;
+        std::vector<float> d1(n), c1(n), d2(n), c2(n);
+        expect(orc_compute_disparity(l.data(), r.data(), w, w, h, nullptr, nullptr, 0, 0, 0, &op2, &os, d1.data(),
+                                     c1.data(), 0) == 0 &&
+                   orc_compute_disparity(l.data(), r.data(), w, w, h, fl.data(), fr.data(), 2 * w, 2 * w, 2 * h, &op2,
+                                         &os, d2.data(), c2.data(), 0) == 0,
+               "oracle compute_disparity");
         auto u = (*st2)->compute(l.view(), r.view());
-        expect(!u && u.error().code == Errc::unsupported, "refinement stages -> unsupported");
+        expect(u.has_value() && same_bytes(u->disparity, d1) && same_bytes(u->confidence, c1),
+               "compute == compute_disparity (pyramid, refinement, filters, speckles)");
+        auto v = (*st2)->compute_full(l.view(), r.view(), fl.view(), fr.view());
+        expect(v.has_value() && same_bytes(v->disparity, d2) && same_bytes(v->confidence, c2),
+               "compute_full == compute_disparity with the detail refinement");
+        int valid = 0;
+        for (std::size_t i = 0; i < n; ++i) valid += valid_disparity(d2[i]);
+        expect(valid > static_cast<int>(n / 2), "most pixels valid after every stage");
+        auto s3 = (*st2)->compute_sgm(l.view(), r.view());
+        expect(!s3 && s3.error().code == Errc::unsupported, "compute_sgm with pyramid levels -> unsupported");
     }
     StereoParams big = sp;
     big.sgm.num_disparities = 600;
