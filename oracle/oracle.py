@@ -117,6 +117,37 @@ class StereoParams:
                                                   for f, t in OrcStereoParams._fields_)])
 
 
+class OrcGeometry(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("f", "cx", "cy", "baseline", "cx_offset")]
+
+
+class OrcPointParams(C.Structure):
+    _fields_ = [(f, C.c_float) for f in ("min_depth", "max_depth", "max_depth_jump", "edge_weight",
+                                         "weight_reference_depth")]
+
+
+@dataclass
+class Geometry:
+    """depth::RectifiedGeometry (libs/depth/include/einstar/depth/point_image.hpp:12-26)."""
+
+    f: float
+    cx: float
+    cy: float
+    baseline: float
+    cx_offset: float = 0.0
+
+
+@dataclass
+class PointParams:
+    """depth::PointImageParams (point_image.hpp:43-49) with the reference defaults."""
+
+    min_depth: float = 150.0
+    max_depth: float = 700.0
+    max_depth_jump: float = 4.0
+    edge_weight: float = 0.25
+    weight_reference_depth: float = 0.0
+
+
 _lib = None
 
 
@@ -155,6 +186,10 @@ def lib():
         L.orc_compute_disparity.argtypes = [vp, vp, C.c_ssize_t, i32, i32, vp, vp, C.c_ssize_t, i32, i32,
                                             C.POINTER(OrcParams), SP, vp, vp, i32]
         L.orc_compute_disparity.restype = i32
+        L.orc_disparity_to_points.argtypes = [vp, vp, i32, i32, C.POINTER(OrcGeometry), C.POINTER(OrcPointParams), vp, vp,
+                                              vp, i32]
+        L.orc_pack_depth.argtypes = [vp, vp, i32, i32, vp]
+        L.orc_unpack_depth.argtypes = [vp, i32, i32, vp, vp]
         _lib = L
     return _lib
 
@@ -381,3 +416,36 @@ def compute_disparity(left, right, sgm_params: SgmParams, sp: StereoParams, full
     if rc != 0:
         raise MemoryError("oracle compute_disparity failed")
     return {"disparity": disp, "confidence": conf}
+
+
+def disparity_to_points(disparity, confidence, g: Geometry, p: PointParams = PointParams(), threads: int = 0) -> dict:
+    """disparity_to_points (libs/depth/src/point_image.cpp:10-55): points, normals [h, w, 3], weights."""
+    d = np.ascontiguousarray(disparity, dtype=np.float32)
+    c = None if confidence is None else np.ascontiguousarray(confidence, dtype=np.float32)
+    h, w = d.shape
+    out = {"points": np.empty((h, w, 3), np.float32), "normals": np.empty((h, w, 3), np.float32),
+           "weights": np.empty((h, w), np.float32)}
+    cg = OrcGeometry(g.f, g.cx, g.cy, g.baseline, g.cx_offset)
+    cp = OrcPointParams(p.min_depth, p.max_depth, p.max_depth_jump, p.edge_weight, p.weight_reference_depth)
+    lib().orc_disparity_to_points(_ptr(d), _ptr(c), w, h, C.byref(cg), C.byref(cp), _ptr(out["points"]),
+                                  _ptr(out["normals"]), _ptr(out["weights"]), threads)
+    return out
+
+
+def pack_depth(points, weights) -> np.ndarray:
+    """The session file's uncompressed depth layout (libs/session/src/session.cpp:143-160): 3*w*h bytes."""
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    h, w = pts.shape[:2]
+    out = np.empty(3 * w * h, np.uint8)
+    lib().orc_pack_depth(_ptr(pts), _ptr(wt), w, h, _ptr(out))
+    return out
+
+
+def unpack_depth(packed, w: int, h: int):
+    """decode_images without zstd (session.cpp:164-185): (depth mm, confidence)."""
+    b = np.ascontiguousarray(packed, dtype=np.uint8)
+    depth = np.empty((h, w), np.float32)
+    conf = np.empty((h, w), np.float32)
+    lib().orc_unpack_depth(_ptr(b), w, h, _ptr(depth), _ptr(conf))
+    return depth, conf

@@ -917,3 +917,142 @@ done:
     free(finer);
     return rc;
 }
+
+/* ---------------------------------------------------------------- points (point_image.cpp:10-55) */
+
+typedef struct {
+    const float *disp, *conf;
+    int w, h;
+    const orc_geometry* g;
+    const orc_point_params* p;
+    float *points, *normals, *weights;
+} pts_ctx;
+
+static void points_rows(void* vctx, int y0, int y1) {
+    const pts_ctx* c = (const pts_ctx*)vctx;
+    const orc_geometry* g = c->g;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < c->w; ++x) {
+            const size_t i = (size_t)y * c->w + x;
+            const float d = c->disp[i];
+            if (!valid_d(d) || d + g->cx_offset <= 0.0) continue;
+            const double z = g->f * g->baseline / (d + g->cx_offset); /* depth_from_disparity */
+            if (z < c->p->min_depth || z > c->p->max_depth) continue;
+            c->points[3 * i] = (float)((x - g->cx) * z / g->f);
+            c->points[3 * i + 1] = (float)((y - g->cy) * z / g->f);
+            c->points[3 * i + 2] = (float)z;
+            c->weights[i] = c->conf ? c->conf[i] : 1.0f;
+        }
+}
+
+/* Eigen's Vector3f operations as the reference spells them: dot = (x x' + y y') + z z',
+ * norm = sqrt of the dot with itself, normalized = v / norm, cross component by component. */
+static inline float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static float depth_noise_weight(float z, float reference) { /* point_image.hpp:51-55 */
+    if (reference <= 0.0f) return 1.0f;
+    const float r = reference / z;
+    return r * r > 1.0f ? 1.0f : (r * r < 0.1f ? 0.1f : r * r);
+}
+
+static void normals_rows(void* vctx, int y0, int y1) {
+    const pts_ctx* c = (const pts_ctx*)vctx;
+    const int w = c->w, h = c->h;
+    const float* P = c->points;
+    const float rc[3] = {(float)c->g->baseline, 0.0f, 0.0f};
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const float* p = P + 3 * i;
+            if (p[2] == 0.0f) continue;
+            float view = c->p->edge_weight;
+            if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
+                const float* l = P + 3 * (i - 1);
+                const float* r = P + 3 * (i + 1);
+                const float* u = P + 3 * (i - w);
+                const float* dn = P + 3 * (i + w);
+                if (l[2] != 0 && r[2] != 0 && u[2] != 0 && dn[2] != 0 && fabsf(l[2] - r[2]) <= c->p->max_depth_jump &&
+                    fabsf(u[2] - dn[2]) <= c->p->max_depth_jump) {
+                    const float a[3] = {r[0] - l[0], r[1] - l[1], r[2] - l[2]};
+                    const float b[3] = {dn[0] - u[0], dn[1] - u[1], dn[2] - u[2]};
+                    float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+                    const float len = sqrtf(dot3(n, n));
+                    if (len >= 1e-9f) {
+                        n[0] /= len;
+                        n[1] /= len;
+                        n[2] /= len;
+                        if (dot3(n, p) > 0) {
+                            n[0] = -n[0];
+                            n[1] = -n[1];
+                            n[2] = -n[2];
+                        }
+                        c->normals[3 * i] = n[0];
+                        c->normals[3 * i + 1] = n[1];
+                        c->normals[3 * i + 2] = n[2];
+                        const float pn = sqrtf(dot3(p, p));
+                        const float ph[3] = {p[0] / pn, p[1] / pn, p[2] / pn};
+                        const float q[3] = {p[0] - rc[0], p[1] - rc[1], p[2] - rc[2]};
+                        const float qn = sqrtf(dot3(q, q));
+                        const float qh[3] = {q[0] / qn, q[1] / qn, q[2] / qn};
+                        const float v1 = -dot3(n, ph), v2 = -dot3(n, qh);
+                        const float m = v2 < v1 ? v2 : v1; /* std::min(v1, v2) */
+                        view = 0.0f < m ? m : 0.0f;        /* std::max(0.0f, m) */
+                    }
+                }
+            }
+            c->weights[i] *= view * depth_noise_weight(p[2], c->p->weight_reference_depth);
+        }
+}
+
+void orc_disparity_to_points(const float* disparity, const float* confidence, int w, int h, const orc_geometry* g,
+                             const orc_point_params* p, float* points, float* normals, float* weights, int threads) {
+    const size_t n = (size_t)w * (size_t)h;
+    memset(points, 0, sizeof(float) * 3 * n);
+    memset(normals, 0, sizeof(float) * 3 * n);
+    memset(weights, 0, sizeof(float) * n);
+    pts_ctx c = {disparity, confidence, w, h, g, p, points, normals, weights};
+    parallel_for(h, threads, points_rows, &c);
+    parallel_for(h, threads, normals_rows, &c);
+}
+
+/* ---------------------------------------------------------------- depth packing (session.cpp:143-185) */
+
+void orc_pack_depth(const float* points, const float* weights, int w, int h, uint8_t* out) {
+    const size_t n = (size_t)w * (size_t)h;
+    uint16_t* d16 = (uint16_t*)out; /* out is at least 2-byte aligned */
+    for (int y = 0; y < h; ++y) {
+        uint16_t prev = 0;
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            float z = points[3 * i + 2];
+            if (!(z > 0.0f)) z = 0.0f; /* capture_depth: max(0, z) */
+            long q = 0;
+            if (z > 0) {
+                q = lround(z * 50.0); /* kDepthScale = 50.0 (double) */
+                q = q < 1 ? 1 : (q > 65535 ? 65535 : q);
+            }
+            d16[i] = (uint16_t)((uint16_t)q - prev);
+            prev = (uint16_t)q;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const float c = weights ? weights[i] : 1.0f;
+        long b = lroundf(c * 255.0f);
+        b = b < 0 ? 0 : (b > 255 ? 255 : b);
+        out[2 * n + i] = (uint8_t)b;
+    }
+}
+
+void orc_unpack_depth(const uint8_t* in, int w, int h, float* depth, float* confidence) {
+    const size_t n = (size_t)w * (size_t)h;
+    const uint16_t* d16 = (const uint16_t*)in;
+    for (int y = 0; y < h; ++y) {
+        uint16_t acc = 0;
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            acc = (uint16_t)(acc + d16[i]);
+            depth[i] = acc ? (float)(acc / 50.0) : 0.0f;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) confidence[i] = in[2 * n + i] / 255.0f;
+}

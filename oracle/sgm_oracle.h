@@ -163,6 +163,30 @@ int orc_compute_disparity(const uint8_t* left, const uint8_t* right, ptrdiff_t s
                           int full_h, const orc_params* sgm, const orc_stereo_params* sp, float* disparity,
                           float* confidence, int threads);
 
+/* depth::RectifiedGeometry (libs/depth/include/einstar/depth/point_image.hpp:12-26). */
+typedef struct {
+    double f, cx, cy, baseline, cx_offset;
+} orc_geometry;
+
+/* depth::PointImageParams (point_image.hpp:43-49). */
+typedef struct {
+    float min_depth, max_depth, max_depth_jump, edge_weight, weight_reference_depth;
+} orc_point_params;
+
+/* disparity_to_points, libs/depth/src/point_image.cpp:10-55: points and normals as [y][x][3] floats
+ * (zero where invalid), weights [y][x]. confidence may be NULL (weight 1). */
+void orc_disparity_to_points(const float* disparity, const float* confidence, int w, int h, const orc_geometry* g,
+                             const orc_point_params* p, float* points, float* normals, float* weights, int threads);
+
+/* The session file's uncompressed depth layout (libs/session/src/session.cpp:143-160, with
+ * capture_depth's depth = max(0, z) and confidence = weight, :296-313): w*h u16 row-delta codes of
+ * lround(z * 50) clamped to [1, 65535] (0 where z <= 0), then w*h u8 lround(weight * 255) clamped to
+ * [0, 255]. z is read from points [y][x][3] (index 2); weights may be NULL (then 1). out: 3*w*h bytes. */
+void orc_pack_depth(const float* points, const float* weights, int w, int h, uint8_t* out);
+
+/* decode_images without the zstd step (session.cpp:164-185): depth in mm and confidence. */
+void orc_unpack_depth(const uint8_t* in, int w, int h, float* depth, float* confidence);
+
 /* sgm_disparity (stereo.cpp:361-463) followed by median3; fills whatever outputs are non-NULL.
  * Returns 0, or -1 when an allocation fails. */
 int orc_sgm(const uint8_t* left, const uint8_t* right, ptrdiff_t stride, int w, int h, const orc_params* p,
