@@ -97,6 +97,7 @@ typedef struct {
     double rectify_ms; /* rectification (raw calls) and pyramid (compute_disparity), else 0 */
     double refine_ms;  /* compute_disparity: ZNCC refinement of every level and the detail refinement */
     double filter_ms;  /* compute_disparity: occlusion and spike filters, speckle removal, confidence */
+    double points_ms;  /* compute_frame: points, normals and weights */
 } sgm_timings;
 
 /* Per-stage GPU times summed over every call completed since the last reset (count = calls). */
@@ -110,6 +111,7 @@ typedef struct {
     double rectify_ms;
     double refine_ms;
     double filter_ms;
+    double points_ms;
 } sgm_stage_totals;
 
 typedef struct sgm_context sgm_context;
@@ -224,6 +226,65 @@ int sgm_compute_disparity_raw(sgm_context* ctx, const uint8_t* raw_left, const u
 int sgm_compute_disparity_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right,
                                      ptrdiff_t raw_pitch, int raw_w, int raw_h, float* d_disparity, float* d_confidence,
                                      uint8_t* d_rect_left, uint8_t* d_rect_right, void* stream);
+
+/* depth::RectifiedGeometry (libs/depth/include/einstar/depth/point_image.hpp:12-26) at the disparity's
+ * resolution: focal length and left principal point in pixels, baseline in mm, right cx - left cx. */
+typedef struct {
+    double f, cx, cy, baseline, cx_offset;
+} sgm_rectified_geometry;
+
+/* depth::PointImageParams (point_image.hpp:43-49). */
+typedef struct {
+    float min_depth, max_depth;   /* mm */
+    float max_depth_jump;         /* mm between the neighbours used for a normal */
+    float edge_weight;            /* view weight of a point without a normal */
+    float weight_reference_depth; /* mm; 0 = no depth-noise term */
+} sgm_point_params;
+
+/* Fills the reference defaults (point_image.hpp:43-49). */
+void sgm_point_params_default(sgm_point_params* p);
+
+/* An organised point frame, w*h each: points and normals as (x, y, z, valid) float4s in the
+ * rectified left camera frame (zero where invalid; a normal's fourth value is 1 where it exists),
+ * weights as floats. As MetalFrameData holds them (points_xyzw, normals_xyzw, weights). */
+typedef struct {
+    float* points_xyzw;
+    float* normals_xyzw;
+    float* weights;
+} sgm_frame;
+
+/* Replaces depth::disparity_to_points (point_image.hpp:57-58, point_image.cpp:10-55) on device data:
+ * d_confidence may be NULL (weight 1). Enqueued on `stream` (NULL = the null stream), no wait;
+ * allocates nothing. Points are formed in double as the CPU forms them. */
+int sgm_disparity_to_points_device(const float* d_disparity, const float* d_confidence, int width, int height,
+                                   const sgm_rectified_geometry* geometry, const sgm_point_params* params,
+                                   const sgm_frame* d_out, void* stream);
+
+/* Replaces depth_metal::DepthPacker::pack (libs/depth_metal/include/einstar/depth_metal/depth_packer.hpp:17-38,
+ * depth_packer.cpp:12-26): a device point frame into the session file's uncompressed depth layout
+ * (libs/session/src/session.cpp:143-160): w*h u16 row-delta codes of lround(max(0, z) * 50) clamped
+ * to [1, 65535] (0 where z <= 0), then w*h u8 lround(weight * 255) clamped to [0, 255]; 3*w*h bytes
+ * at d_out. Quantised in double as the session writer does (the Metal packer uses float and may
+ * differ by one unit). d_weights may be NULL (confidence 255). No wait; allocates nothing. */
+int sgm_pack_depth_device(const float* d_points_xyzw, const float* d_weights, int width, int height, uint8_t* d_out,
+                          void* stream);
+
+/* Replaces MetalStereo::set_point_params (metal_stereo.hpp:83-85): the geometry and limits the frame
+ * entry points convert disparity with; allocates the context's point buffers. */
+int sgm_set_point_params(sgm_context* ctx, const sgm_rectified_geometry* geometry, const sgm_point_params* params);
+
+/* Replaces MetalStereo::compute_frame_raw (metal_stereo.hpp:86-90, metal_stereo.cpp:250-279): raw
+ * frames -> rectified pair -> compute_disparity -> points, normals, weights. Host outputs (any of the
+ * three may be NULL) and rectified pair (may be NULL); blocking. Errors as the reference:
+ * "set_point_params() not called", "set_rectification() not called", "raw size mismatch". */
+int sgm_compute_frame_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride,
+                          int raw_w, int raw_h, const sgm_frame* out, uint8_t* rect_left, uint8_t* rect_right);
+
+/* The same on device-resident raw frames and outputs (NULL outputs: the context's buffers), on
+ * `stream` without waiting. */
+int sgm_compute_frame_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right,
+                                 ptrdiff_t raw_pitch, int raw_w, int raw_h, const sgm_frame* d_out, uint8_t* d_rect_left,
+                                 uint8_t* d_rect_right, void* stream);
 
 /* Waits for the work enqueued by this context and records its timings. */
 int sgm_synchronize(sgm_context* ctx);

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wextra", "-Wno-unused-parameter",
 ]
 
-HIP_SOURCES = ["sgm_kernels.hip", "refine_kernels.hip", "sgm_api.hip", "sgm_server.hip"]
+HIP_SOURCES = ["sgm_kernels.hip", "refine_kernels.hip", "frame_kernels.hip", "sgm_api.hip", "sgm_server.hip"]
 
 
 def _run(cmd, cwd=None):

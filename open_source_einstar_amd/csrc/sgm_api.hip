@@ -20,13 +20,15 @@ thread_local std::string g_last_error;
 
 // ev[0..4] bracket census, paths, rows and median; ev[5] marks the start of the front end
 // (rectification and pyramid) when the call had one (front); ev[6..8] bracket the refinement and the
-// filters of compute_disparity (post).
-constexpr int kStageEvents = 9;
+// filters of compute_disparity (post); ev[9] ends the point image of a frame call (points).
+constexpr int kStageEvents = 10;
 struct Stages {
     hipEvent_t ev[kStageEvents] = {};
     bool recorded = false;
     bool front = false;
     bool post = false;
+    bool points = false;
+    int last() const { return points ? 9 : (post ? 8 : 4); }
 };
 
 // Stage events of the calls in flight: a ring, so that per-stage times can be summed over many
@@ -211,6 +213,13 @@ struct sgm_context {
     uint32_t* d_local = nullptr;
     uint32_t* d_sizes = nullptr;
     uint8_t* d_full = nullptr;   // full-resolution pair (2 * stereo_w x 2 * stereo_h each) for the detail refinement
+    // Point frames (sgm_set_point_params).
+    bool have_points = false;
+    sgm_rectified_geometry geom{};
+    sgm_point_params pparams{};
+    float* d_pts = nullptr;      // float4 per pixel
+    float* d_nrm = nullptr;      // float4 per pixel
+    float* d_wts = nullptr;
     size_t bytes = 0;
     sgm_timings timings{};
     double pending_wall_ms = 0;
@@ -255,7 +264,7 @@ void sgm_destroy(sgm_context* ctx) {
                     (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol, (void*)ctx->d_maps,
                     (void*)ctx->d_raw, (void*)ctx->d_pyr, (void*)ctx->d_wa, (void*)ctx->d_wb, (void*)ctx->d_wm,
                     (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_local, (void*)ctx->d_sizes,
-                    (void*)ctx->d_full})
+                    (void*)ctx->d_full, (void*)ctx->d_pts, (void*)ctx->d_nrm, (void*)ctx->d_wts})
         dfree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -578,14 +587,16 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
 
 // Adds one completed event set to the running totals; the newest also becomes last_timings.
 static void account(sgm_context* ctx, Stages& st, bool newest) {
-    float ms[4] = {0, 0, 0, 0}, all = 0, rect = 0, refine = 0, filter = 0;
+    float ms[4] = {0, 0, 0, 0}, all = 0, rect = 0, refine = 0, filter = 0, pts = 0;
     for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], st.ev[i], st.ev[i + 1]);
     if (st.front) (void)hipEventElapsedTime(&rect, st.ev[5], st.ev[0]);
     if (st.post) {
         (void)hipEventElapsedTime(&refine, st.ev[6], st.ev[7]);
         (void)hipEventElapsedTime(&filter, st.ev[7], st.ev[8]);
     }
-    (void)hipEventElapsedTime(&all, st.front ? st.ev[5] : st.ev[0], st.post ? st.ev[8] : st.ev[4]);
+    if (st.points) (void)hipEventElapsedTime(&pts, st.ev[8], st.ev[9]);
+    (void)hipEventElapsedTime(&all, st.front ? st.ev[5] : st.ev[0], st.ev[st.last()]);
+    ctx->totals.points_ms += pts;
     ctx->totals.rectify_ms += rect;
     ctx->totals.refine_ms += refine;
     ctx->totals.filter_ms += filter;
@@ -605,10 +616,12 @@ static void account(sgm_context* ctx, Stages& st, bool newest) {
         ctx->timings.rectify_ms = rect;
         ctx->timings.refine_ms = refine;
         ctx->timings.filter_ms = filter;
+        ctx->timings.points_ms = pts;
     }
     st.recorded = false;
     st.front = false;
     st.post = false;
+    st.points = false;
 }
 
 // Reads every recorded set (the stream has been synchronised).
@@ -624,7 +637,7 @@ static void read_timings(sgm_context* ctx) {
 static Stages* next_stages(sgm_context* ctx) {
     if (ctx->ring_pending == kEventRing) {
         const int i = (ctx->ring_head - ctx->ring_pending + kEventRing) % kEventRing;
-        (void)hipEventSynchronize(ctx->ring[i].ev[ctx->ring[i].post ? 8 : 4]);
+        (void)hipEventSynchronize(ctx->ring[i].ev[ctx->ring[i].last()]);
         --ctx->ring_pending;
         account(ctx, ctx->ring[i], false);
     }
@@ -1216,6 +1229,196 @@ int sgm_compute_disparity_raw_device(sgm_context* ctx, const uint8_t* d_raw_left
     if (e == hipSuccess)
         e = enqueue_disparity(ctx, rl, rr, w, w, h, fl, fr, 2 * w, d_disparity, d_confidence, s, st);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_disparity_raw_device: launch", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = 0;
+    return SGM_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------- point frames
+
+static PointsDesc points_desc(const sgm_rectified_geometry& g, const sgm_point_params& pp, const float* disp,
+                              const float* conf, int w, int h, const sgm_frame& out) {
+    PointsDesc P{};
+    P.disp = disp;
+    P.conf = conf;
+    P.points = out.points_xyzw;
+    P.normals = out.normals_xyzw;
+    P.weights = out.weights;
+    P.w = w;
+    P.h = h;
+    P.f = g.f;
+    P.cx = g.cx;
+    P.cy = g.cy;
+    P.baseline = g.baseline;
+    P.cx_offset = g.cx_offset;
+    P.min_depth = pp.min_depth;
+    P.max_depth = pp.max_depth;
+    P.max_depth_jump = pp.max_depth_jump;
+    P.edge_weight = pp.edge_weight;
+    P.weight_reference_depth = pp.weight_reference_depth;
+    P.rc_x = (float)g.baseline;
+    return P;
+}
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Raw frames (device) -> rectified pair -> compute_disparity -> point frame in `out` (device).
+static hipError_t enqueue_frame(sgm_context* ctx, const uint8_t* raw_l, const uint8_t* raw_r, int64_t raw_pitch,
+                                uint8_t* rl, uint8_t* rr, const sgm_frame& out, hipStream_t s, Stages* st) {
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    const uint8_t* fl = nullptr;
+    const uint8_t* fr = nullptr;
+    hipError_t e = enqueue_raw_front(ctx, raw_l, raw_r, raw_pitch, rl, rr, s, st, &fl, &fr);
+    float* dd = nullptr;
+    float* dc = nullptr;
+    if (e == hipSuccess) e = enqueue_disparity(ctx, rl, rr, w, w, h, fl, fr, 2 * w, nullptr, nullptr, s, st, &dd, &dc);
+    if (e == hipSuccess) e = launch_points(points_desc(ctx->geom, ctx->pparams, dd, dc, w, h, out), s);
+    if (e == hipSuccess && st) {
+        e = hipEventRecord(st->ev[9], s);
+        st->points = true;
+    }
+    return e;
+}
+
+static int check_frame(sgm_context* ctx) {
+    if (!ctx->have_points) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "set_point_params() not called");
+    return SGM_OK;
+}
+
+extern "C" {
+
+void sgm_point_params_default(sgm_point_params* p) {
+    if (!p) return;
+    p->min_depth = 150.0f;
+    p->max_depth = 700.0f;
+    p->max_depth_jump = 4.0f;
+    p->edge_weight = 0.25f;
+    p->weight_reference_depth = 0.0f;
+}
+
+int sgm_disparity_to_points_device(const float* d_disparity, const float* d_confidence, int width, int height,
+                                   const sgm_rectified_geometry* geometry, const sgm_point_params* params,
+                                   const sgm_frame* d_out, void* stream) {
+    if (!d_disparity || !geometry || !params || !d_out || !d_out->points_xyzw || !d_out->normals_xyzw || !d_out->weights ||
+        width < 1 || height < 1 || !aligned16(d_out->points_xyzw) || !aligned16(d_out->normals_xyzw)) {
+        g_last_error = "sgm_disparity_to_points_device: null pointer, size < 1 or float4 arrays not 16-byte aligned";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    const hipError_t e = launch_points(points_desc(*geometry, *params, d_disparity, d_confidence, width, height, *d_out),
+                                       static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_disparity_to_points_device", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_pack_depth_device(const float* d_points_xyzw, const float* d_weights, int width, int height, uint8_t* d_out,
+                          void* stream) {
+    if (!d_points_xyzw || !d_out || width < 1 || height < 1 || !aligned16(d_points_xyzw) ||
+        (reinterpret_cast<uintptr_t>(d_out) & 1u)) {
+        g_last_error = "sgm_pack_depth_device: null pointer, size < 1 or misaligned buffers";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    const hipError_t e = launch_pack(d_points_xyzw, d_weights, d_out, width, height, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_pack_depth_device", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_set_point_params(sgm_context* ctx, const sgm_rectified_geometry* geometry, const sgm_point_params* params) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (!geometry || !params || !(geometry->f > 0)) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null parameters or f <= 0");
+    if (!ctx->d_pts) {
+        const size_t spx = (size_t)ctx->stereo_w * ctx->stereo_h;
+        size_t got = 0;
+        hipError_t e = dalloc(&ctx->d_pts, 4 * spx, got);
+        if (e == hipSuccess) e = dalloc(&ctx->d_nrm, 4 * spx, got);
+        if (e == hipSuccess) e = dalloc(&ctx->d_wts, spx, got);
+        ctx->bytes += got;
+        if (e != hipSuccess) return ctx->hip_fail("sgm_set_point_params: allocation", e);
+    }
+    ctx->geom = *geometry;
+    ctx->pparams = *params;
+    ctx->have_points = true;
+    return SGM_OK;
+}
+
+int sgm_compute_frame_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8_t* raw_right, ptrdiff_t raw_stride,
+                          int raw_w, int raw_h, const sgm_frame* out, uint8_t* rect_left, uint8_t* rect_right) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_frame(ctx); rc != SGM_OK) return rc;
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!raw_left || !raw_right || raw_stride < raw_w) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or stride < width");
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx->stream;
+    const size_t npx = (size_t)w * h, rpx = (size_t)raw_w * raw_h;
+    uint8_t* rl = ctx->d_img;
+    uint8_t* rr = ctx->d_img + npx;
+    Stages* st = next_stages(ctx);
+    hipError_t e = hipMemcpy2DAsync(ctx->d_raw, raw_w, raw_left, raw_stride, raw_w, raw_h, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpy2DAsync(ctx->d_raw + rpx, raw_w, raw_right, raw_stride, raw_w, raw_h, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_frame_raw: upload", e);
+    const sgm_frame dev{ctx->d_pts, ctx->d_nrm, ctx->d_wts};
+    if ((e = enqueue_frame(ctx, ctx->d_raw, ctx->d_raw + rpx, raw_w, rl, rr, dev, s, st)) != hipSuccess)
+        return ctx->hip_fail("sgm_compute_frame_raw: launch", e);
+    struct Copy {
+        void* host;
+        const void* dev;
+        size_t bytes;
+    };
+    const Copy copies[] = {{out ? out->points_xyzw : nullptr, ctx->d_pts, npx * 16},
+                           {out ? out->normals_xyzw : nullptr, ctx->d_nrm, npx * 16},
+                           {out ? out->weights : nullptr, ctx->d_wts, npx * 4},
+                           {rect_left, rl, npx},
+                           {rect_right, rr, npx}};
+    for (const Copy& c : copies)
+        if (c.host && (e = hipMemcpyAsync(c.host, c.dev, c.bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return ctx->hip_fail("sgm_compute_frame_raw: download", e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return ctx->hip_fail("sgm_compute_frame_raw: execution", e);
+    ctx->last_stream = s;
+    ctx->pending_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    read_timings(ctx);
+    return SGM_OK;
+}
+
+int sgm_compute_frame_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const uint8_t* d_raw_right,
+                                 ptrdiff_t raw_pitch, int raw_w, int raw_h, const sgm_frame* d_out, uint8_t* d_rect_left,
+                                 uint8_t* d_rect_right, void* stream) {
+    if (!ctx) {
+        g_last_error = "ctx is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = check_frame(ctx); rc != SGM_OK) return rc;
+    if (int rc = check_raw(ctx, raw_w, raw_h); rc != SGM_OK) return rc;
+    if (!d_raw_left || !d_raw_right || raw_pitch < raw_w) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
+    const int w = ctx->map_w / 2, h = ctx->map_h / 2;
+    if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
+    sgm_frame dev{ctx->d_pts, ctx->d_nrm, ctx->d_wts};
+    if (d_out) {
+        if (d_out->points_xyzw) dev.points_xyzw = d_out->points_xyzw;
+        if (d_out->normals_xyzw) dev.normals_xyzw = d_out->normals_xyzw;
+        if (d_out->weights) dev.weights = d_out->weights;
+    }
+    if (!aligned16(dev.points_xyzw) || !aligned16(dev.normals_xyzw))
+        return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "point and normal arrays must be 16-byte aligned");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    uint8_t* rl = d_rect_left ? d_rect_left : ctx->d_img;
+    uint8_t* rr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)w * h;
+    const hipError_t e = enqueue_frame(ctx, d_raw_left, d_raw_right, raw_pitch, rl, rr, dev, s, next_stages(ctx));
+    if (e != hipSuccess) return ctx->hip_fail("sgm_compute_frame_raw_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
     return SGM_OK;

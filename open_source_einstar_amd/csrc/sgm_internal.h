@@ -107,6 +107,20 @@ struct DetailDesc {
     float step, min_zncc;
 };
 
+// disparity_to_points over one w x h disparity (frame_kernels.hip): points and normals as float4
+// (x, y, z, 1 where valid; zero elsewhere), weights.
+struct PointsDesc {
+    const float* disp;
+    const float* conf;     // may be null: weight 1
+    float* points;         // float4 [y][x]
+    float* normals;        // float4 [y][x]
+    float* weights;
+    int32_t w, h;
+    double f, cx, cy, baseline, cx_offset;
+    float min_depth, max_depth, max_depth_jump, edge_weight, weight_reference_depth;
+    float rc_x;            // the right camera centre's x as a float (the baseline)
+};
+
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
@@ -131,6 +145,10 @@ hipError_t launch_finalize(const float* disp, const int32_t* labels, const uint3
                            const float* conf_src, float* out_disp, float* out_conf, int w, int h, hipStream_t stream);
 hipError_t launch_rectify_full(const RectifyDesc& r, uint8_t* f0, uint8_t* f1, int64_t fpitch, int fw, int fh,
                                hipStream_t stream);
+
+// Host-side launchers of the point image and the depth packing (frame_kernels.hip).
+hipError_t launch_points(const PointsDesc& p, hipStream_t stream);
+hipError_t launch_pack(const float* points, const float* weights, uint8_t* out, int w, int h, hipStream_t stream);
 
 // Per-pair block counts for each stage (the launchers' grid geometry).
 int census_blocks(int w, int h);

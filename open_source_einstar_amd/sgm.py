@@ -61,12 +61,12 @@ class CSgmOutputs(C.Structure):
 
 class CSgmTimings(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms", "total_ms",
-                                          "rectify_ms", "refine_ms", "filter_ms")]
+                                          "rectify_ms", "refine_ms", "filter_ms", "points_ms")]
 
 
 class CSgmStageTotals(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")] + \
-               [("count", C.c_int64), ("rectify_ms", C.c_double), ("refine_ms", C.c_double), ("filter_ms", C.c_double)]
+               [("count", C.c_int64)] + [(f, C.c_double) for f in ("rectify_ms", "refine_ms", "filter_ms", "points_ms")]
 
 
 class CSgmServerConfig(C.Structure):
@@ -172,6 +172,48 @@ class StereoParams:
         return CSgmStereoParams(sgm.to_c(), *vals)
 
 
+class CSgmRectifiedGeometry(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("f", "cx", "cy", "baseline", "cx_offset")]
+
+
+class CSgmPointParams(C.Structure):
+    _fields_ = [(f, C.c_float) for f in ("min_depth", "max_depth", "max_depth_jump", "edge_weight",
+                                         "weight_reference_depth")]
+
+
+class CSgmFrame(C.Structure):
+    _fields_ = [("points_xyzw", C.c_void_p), ("normals_xyzw", C.c_void_p), ("weights", C.c_void_p)]
+
+
+@dataclass
+class RectifiedGeometry:
+    """depth::RectifiedGeometry (libs/depth/include/einstar/depth/point_image.hpp:12-26)."""
+
+    f: float
+    cx: float
+    cy: float
+    baseline: float
+    cx_offset: float = 0.0
+
+    def to_c(self) -> CSgmRectifiedGeometry:
+        return CSgmRectifiedGeometry(self.f, self.cx, self.cy, self.baseline, self.cx_offset)
+
+
+@dataclass
+class PointParams:
+    """depth::PointImageParams (point_image.hpp:43-49) with the reference defaults."""
+
+    min_depth: float = 150.0
+    max_depth: float = 700.0
+    max_depth_jump: float = 4.0
+    edge_weight: float = 0.25
+    weight_reference_depth: float = 0.0
+
+    def to_c(self) -> CSgmPointParams:
+        return CSgmPointParams(self.min_depth, self.max_depth, self.max_depth_jump, self.edge_weight,
+                               self.weight_reference_depth)
+
+
 _lib = None
 
 
@@ -201,6 +243,13 @@ def lib():
                                                    vp, vp]
         L.sgm_compute_disparity_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, vp, vp]
         L.sgm_compute_disparity_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, vp, vp, vp, vp, vp]
+        L.sgm_point_params_default.argtypes = [P(CSgmPointParams)]
+        L.sgm_disparity_to_points_device.argtypes = [vp, vp, i32, i32, P(CSgmRectifiedGeometry), P(CSgmPointParams),
+                                                     P(CSgmFrame), vp]
+        L.sgm_pack_depth_device.argtypes = [vp, vp, i32, i32, vp, vp]
+        L.sgm_set_point_params.argtypes = [vp, P(CSgmRectifiedGeometry), P(CSgmPointParams)]
+        L.sgm_compute_frame_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp]
+        L.sgm_compute_frame_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp, vp]
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
         L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
@@ -371,6 +420,34 @@ class HipStereo:
         _check(lib().sgm_compute_disparity_raw_device(self._h, d_raw_left, d_raw_right, pitch, raw_width, raw_height,
                                                       d_disparity, d_confidence, d_rect_left, d_rect_right, stream),
                ctx=self._h)
+
+    def set_point_params(self, geometry: RectifiedGeometry, params: PointParams = PointParams()):
+        """MetalStereo::set_point_params (metal_stereo.hpp:83-85)."""
+        g, p = geometry.to_c(), params.to_c()
+        _check(lib().sgm_set_point_params(self._h, C.byref(g), C.byref(p)), ctx=self._h)
+
+    def compute_frame_raw(self, raw_left, raw_right, rectified: bool = False) -> dict:
+        """MetalStereo::compute_frame_raw: raw frames -> points/normals (h, w, 4) and weights (h, w)."""
+        raw_left, raw_right = np.ascontiguousarray(raw_left, np.uint8), np.ascontiguousarray(raw_right, np.uint8)
+        rh, rw = raw_left.shape
+        w, h = getattr(self, "rect_size", (1, 1))
+        res = {"points": np.empty((h, w, 4), np.float32), "normals": np.empty((h, w, 4), np.float32),
+               "weights": np.empty((h, w), np.float32)}
+        rl = rr = None
+        if rectified:
+            res["rect_left"] = rl = np.empty((h, w), np.uint8)
+            res["rect_right"] = rr = np.empty((h, w), np.uint8)
+        fr = CSgmFrame(_ptr(res["points"]), _ptr(res["normals"]), _ptr(res["weights"]))
+        _check(lib().sgm_compute_frame_raw(self._h, _ptr(raw_left), _ptr(raw_right), rw, rw, rh, C.byref(fr), _ptr(rl),
+                                           _ptr(rr)), ctx=self._h)
+        return res
+
+    def compute_frame_raw_device(self, d_raw_left: int, d_raw_right: int, pitch: int, raw_width: int, raw_height: int,
+                                 d_points: int | None = None, d_normals: int | None = None,
+                                 d_weights: int | None = None, stream: int | None = None):
+        fr = CSgmFrame(d_points, d_normals, d_weights)
+        _check(lib().sgm_compute_frame_raw_device(self._h, d_raw_left, d_raw_right, pitch, raw_width, raw_height,
+                                                  C.byref(fr), None, None, stream), ctx=self._h)
 
     def close(self):
         if self._h:
@@ -732,3 +809,48 @@ class Server:
             self.stop()
         except Exception:
             pass
+
+
+def disparity_to_points(disparity, confidence, geometry: RectifiedGeometry, params: PointParams = PointParams()) -> dict:
+    """depth::disparity_to_points on the GPU (sgm_disparity_to_points_device) for host arrays:
+    points and normals (h, w, 4) as (x, y, z, valid), weights (h, w)."""
+    d = np.ascontiguousarray(disparity, np.float32)
+    h, w = d.shape
+    bufs = [DeviceBuffer(d.nbytes), DeviceBuffer(d.nbytes) if confidence is not None else None,
+            DeviceBuffer(16 * w * h), DeviceBuffer(16 * w * h), DeviceBuffer(4 * w * h)]
+    try:
+        bufs[0].upload(d, 0)
+        if confidence is not None:
+            bufs[1].upload(np.ascontiguousarray(confidence, np.float32), 0)
+        fr = CSgmFrame(bufs[2].ptr, bufs[3].ptr, bufs[4].ptr)
+        g, p = geometry.to_c(), params.to_c()
+        rc = lib().sgm_disparity_to_points_device(bufs[0].ptr, bufs[1].ptr if bufs[1] else None, w, h, C.byref(g),
+                                                  C.byref(p), C.byref(fr), None)
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
+        return {"points": bufs[2].download((h, w, 4), np.float32), "normals": bufs[3].download((h, w, 4), np.float32),
+                "weights": bufs[4].download((h, w), np.float32)}
+    finally:
+        for b in bufs:
+            if b is not None:
+                b.free()
+
+
+def pack_depth(points_xyzw, weights=None) -> np.ndarray:
+    """DepthPacker::pack on the GPU (sgm_pack_depth_device) for host arrays: 3*w*h bytes."""
+    pts = np.ascontiguousarray(points_xyzw, np.float32)
+    h, w = pts.shape[:2]
+    bp, bo = DeviceBuffer(pts.nbytes), DeviceBuffer(3 * w * h)
+    bw = DeviceBuffer(4 * w * h) if weights is not None else None
+    try:
+        bp.upload(pts, 0)
+        if bw is not None:
+            bw.upload(np.ascontiguousarray(weights, np.float32), 0)
+        rc = lib().sgm_pack_depth_device(bp.ptr, bw.ptr if bw else None, w, h, bo.ptr, None)
+        if rc != 0:
+            raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
+        return bo.download((3 * w * h,), np.uint8)
+    finally:
+        for b in (bp, bo, bw):
+            if b is not None:
+                b.free()

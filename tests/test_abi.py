@@ -76,3 +76,25 @@ def test_stereo_defaults_match_reference_stereo_params():
     ref = O.StereoParams()  # stereo.hpp:42-89
     for f, _ in sgm._STEREO_FIELDS:
         assert getattr(p, f) == pytest.approx(float(getattr(ref, f))), f
+
+
+def test_struct_sizes_match_the_header():
+    """The ctypes mirrors have the C structs' sizes (a field added to one side only would shift every
+    later field)."""
+    import subprocess
+    import tempfile
+    names = ["sgm_params", "sgm_outputs", "sgm_timings", "sgm_stage_totals", "sgm_stereo_params", "sgm_request",
+             "sgm_server_config", "sgm_completion", "sgm_server_stats", "sgm_pair", "sgm_batch_tuning",
+             "sgm_batch_plan", "sgm_rectified_geometry", "sgm_point_params", "sgm_frame"]
+    src = '#include <stdio.h>\n#include "sgm_hip.h"\nint main(void) {\n' + "".join(
+        f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "s.c"), os.path.join(d, "s")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        sizes = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    mirrors = [sgm.CSgmParams, sgm.CSgmOutputs, sgm.CSgmTimings, sgm.CSgmStageTotals, sgm.CSgmStereoParams,
+               sgm.CSgmRequest, sgm.CSgmServerConfig, sgm.CSgmCompletion, sgm.CSgmServerStats, sgm.CSgmPair,
+               sgm.CSgmBatchTuning, sgm.CSgmBatchPlan, sgm.CSgmRectifiedGeometry, sgm.CSgmPointParams, sgm.CSgmFrame]
+    for n, m, sz in zip(names, mirrors, sizes):
+        assert C.sizeof(m) == sz, (n, C.sizeof(m), sz)

@@ -1,0 +1,104 @@
+"""Point frames and depth packing on the GPU against the oracle's disparity_to_points
+(libs/depth/src/point_image.cpp:10-55) and session packing (libs/session/src/session.cpp:143-160):
+points, normals and weights bit-identical (points formed in double, the float expressions in the
+CPU's order), packed bytes identical; and the whole raw -> frame path of compute_frame_raw."""
+import numpy as np
+import pytest
+
+import oracle as O
+from open_source_einstar_amd import sgm
+from open_source_einstar_amd.synth import rectification_maps, speckle_pair
+
+pytestmark = pytest.mark.gpu
+
+SGM = dict(census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=True)
+
+
+def assert_frame(ref, got, tag):
+    valid = got["points"][..., 3] > 0
+    np.testing.assert_array_equal(valid, ref["points"][..., 2] > 0, err_msg=f"{tag} validity")
+    np.testing.assert_array_equal(got["points"][..., :3].view(np.uint32), ref["points"].view(np.uint32),
+                                  err_msg=f"{tag} points")
+    np.testing.assert_array_equal(got["normals"][..., :3].view(np.uint32), ref["normals"].view(np.uint32),
+                                  err_msg=f"{tag} normals")
+    np.testing.assert_array_equal(got["weights"].view(np.uint32), ref["weights"].view(np.uint32),
+                                  err_msg=f"{tag} weights")
+
+
+def planes_with_holes(h, w, seed):
+    rng = np.random.default_rng(seed)
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float32)
+    d = (50 + 0.05 * xs - 0.03 * ys).astype(np.float32)
+    d[:, w // 2:] += np.float32(6)                      # a depth jump
+    d += rng.normal(0, 0.05, d.shape).astype(np.float32)
+    d[rng.random(d.shape) < 0.05] = np.float32(-1e9)    # holes
+    d[0:3, 0:5] = np.float32(2000)                       # too near
+    return d
+
+
+@pytest.mark.parametrize("g,p", [
+    (O.Geometry(600.0, 159.5, 119.5, 50.0, 0.0), O.PointParams()),
+    (O.Geometry(580.25, 150.0, 121.75, 63.5, 12.5), O.PointParams(min_depth=100, max_depth=900, max_depth_jump=2.0,
+                                                                  edge_weight=0.4, weight_reference_depth=320.0)),
+], ids=["defaults", "offset-and-depth-weight"])
+def test_disparity_to_points_bit_exact(g, p):
+    h, w = 240, 320
+    d = planes_with_holes(h, w, 3)
+    conf = np.random.default_rng(4).uniform(0.3, 1, (h, w)).astype(np.float32)
+    ref = O.disparity_to_points(d, conf, g, p)
+    got = sgm.disparity_to_points(d, conf, sgm.RectifiedGeometry(g.f, g.cx, g.cy, g.baseline, g.cx_offset),
+                                  sgm.PointParams(p.min_depth, p.max_depth, p.max_depth_jump, p.edge_weight,
+                                                  p.weight_reference_depth))
+    assert_frame(ref, got, "points")
+    assert (got["points"][..., 3] > 0).mean() > 0.5 and (got["normals"][..., 3] > 0).mean() > 0.4
+    # Without a confidence map the weight starts at 1.
+    ref1 = O.disparity_to_points(d, None, g, p)
+    got1 = sgm.disparity_to_points(d, None, sgm.RectifiedGeometry(g.f, g.cx, g.cy, g.baseline, g.cx_offset),
+                                   sgm.PointParams(p.min_depth, p.max_depth, p.max_depth_jump, p.edge_weight,
+                                                   p.weight_reference_depth))
+    assert_frame(ref1, got1, "no confidence")
+
+
+def test_pack_depth_bytes_identical():
+    rng = np.random.default_rng(5)
+    h, w = 120, 161
+    pts = np.zeros((h, w, 4), np.float32)
+    pts[..., 2] = rng.uniform(-5, 1400, (h, w)).astype(np.float32)
+    pts[rng.random((h, w)) < 0.2, 2] = 0
+    pts[0, :8, 2] = np.array([1.25, 1.01, 0.0104, 1310.7, 2000, -1, 0.01, 0.02], np.float32)
+    wt = rng.uniform(-0.2, 1.2, (h, w)).astype(np.float32)
+    wt[0, :4] = np.array([0.5 / 255, 0.5, 1.5 / 255, 2.5 / 255], np.float32)
+    ref = O.pack_depth(pts[..., :3], wt)
+    np.testing.assert_array_equal(sgm.pack_depth(pts, wt), ref)
+    np.testing.assert_array_equal(sgm.pack_depth(pts, None), O.pack_depth(pts[..., :3], None))
+
+
+def test_compute_frame_raw_matches_the_cpu_chain():
+    mw, mh, rw, rh = 640, 512, 660, 530
+    m = rectification_maps(mw, mh, rw, rh, 15)
+    # One camera's map for both: the raw pair stays a stereo pair after rectification, so the frame
+    # has many valid points (independent maps would rotate the cameras apart).
+    maps = type(m)(m.left_x, m.left_y, m.left_x, m.left_y)
+    raw = speckle_pair(rw, rh, 64, 16, min_cells=2, max_cells=4)
+    kw = dict(min_disparity=raw.min_disparity // 8 - 1, num_disparities=24, **SGM)
+    sp = O.StereoParams(speckle_max_region_size=50)
+    geom = O.Geometry(420.0, 159.5, 127.5, 60.0, 0.0)
+    pp = O.PointParams(min_depth=50, max_depth=5000)
+    fl, fr = O.remap(raw.left, maps.left_x, maps.left_y), O.remap(raw.right, maps.right_x, maps.right_y)
+    hl, hr = O.downsample2(fl), O.downsample2(fr)
+    disp = O.compute_disparity(hl, hr, O.SgmParams(**kw), sp, fl, fr)
+    ref = O.disparity_to_points(disp["disparity"], disp["confidence"], geom, pp)
+    with sgm.HipStereo.create_stereo(sgm.SgmParams(**kw), sgm.StereoParams(speckle_max_region_size=50), mw // 2,
+                                     mh // 2) as ctx:
+        with pytest.raises(sgm.SgmError) as e:
+            ctx.compute_frame_raw(raw.left, raw.right)
+        assert "set_point_params() not called" in str(e.value)
+        ctx.set_point_params(sgm.RectifiedGeometry(420.0, 159.5, 127.5, 60.0, 0.0), sgm.PointParams(50, 5000))
+        ctx.set_rectification(maps.left_x, maps.left_y, maps.right_x, maps.right_y, rw, rh)
+        got = ctx.compute_frame_raw(raw.left, raw.right, rectified=True)
+        t = ctx.last_timings()
+    np.testing.assert_array_equal(got["rect_left"], hl)
+    assert_frame(ref, got, "frame")
+    assert (got["points"][..., 3] > 0).mean() > 0.3
+    assert t["points_ms"] > 0 and t["gpu_ms"] > t["points_ms"]
+    np.testing.assert_array_equal(sgm.pack_depth(got["points"], got["weights"]), O.pack_depth(ref["points"], ref["weights"]))
