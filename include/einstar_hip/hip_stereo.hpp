@@ -15,8 +15,9 @@
 // detail refinement, occlusion and spike filters, speckle removal). compute_full is
 // compute_disparity's second form. compute_sgm additionally returns every per-pixel output of the
 // SGM stage (integer disparity, status bits, right disparity, best/second costs, median3) when SGM
-// runs at the configured size (pyramid_levels == 0). The point-cloud and blob stages of MetalStereo
-// (set_point_params, compute_frame*) are not part of this library.
+// runs at the configured size (pyramid_levels == 0). set_point_params / compute_frame_raw produce the
+// point frame (points, normals, weights) and pack_depth the recording layout; MetalStereo's
+// marker-blob search (compute_frame's FrameRequest) is not part of this library.
 
 #include <cstddef>
 #include <cstdint>
@@ -196,6 +197,27 @@ struct SgmResult {
     ImageU16 second_cost;
 };
 
+// depth::RectifiedGeometry and the point limits MetalStereo::set_point_params takes
+// (point_image.hpp:12-26, 43-49; metal_stereo.hpp:83-85).
+struct RectifiedGeometry {
+    double f = 0, cx = 0, cy = 0, baseline = 0, cx_offset = 0;
+};
+struct PointImageParams {
+    float min_depth = 150.0f;
+    float max_depth = 700.0f;
+    float max_depth_jump = 4.0f;
+    float edge_weight = 0.25f;
+    float weight_reference_depth = 0.0f;
+};
+
+// A point frame as MetalFrameData exposes it (points_xyzw, normals_xyzw, weights), on the host.
+struct FrameResult {
+    int width = 0, height = 0;
+    std::vector<float> points_xyzw;   // 4 per pixel, zero where invalid
+    std::vector<float> normals_xyzw;  // 4 per pixel, w = 1 where a normal exists
+    std::vector<float> weights;
+};
+
 inline sgm_params to_c(const StereoParams& p) {
     sgm_params c{};
     c.min_disparity = p.sgm.min_disparity;
@@ -349,6 +371,46 @@ public:
         return res;
     }
 
+    // MetalStereo::set_point_params (metal_stereo.hpp:83-85).
+    Result<void> set_point_params(const RectifiedGeometry& g, float min_depth_mm, float max_depth_mm,
+                                  float max_depth_jump_mm = 4.0f) {
+        const sgm_rectified_geometry cg{g.f, g.cx, g.cy, g.baseline, g.cx_offset};
+        sgm_point_params pp{};
+        sgm_point_params_default(&pp);
+        pp.min_depth = min_depth_mm;
+        pp.max_depth = max_depth_mm;
+        pp.max_depth_jump = max_depth_jump_mm;
+        const int rc = sgm_set_point_params(ctx_, &cg, &pp);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        return {};
+    }
+
+    // MetalStereo::compute_frame_raw (metal_stereo.hpp:86-90): raw pair -> rectified -> stereo ->
+    // points, normals, weights; the frame comes back to the host (the reference keeps it on the GPU).
+    Result<FrameResult> compute_frame_raw(ImageView<const std::uint8_t> raw_left, ImageView<const std::uint8_t> raw_right,
+                                          ImageU8* rect_left = nullptr, ImageU8* rect_right = nullptr) {
+        if (raw_left.stride != raw_right.stride)
+            return make_error(Errc::invalid_argument, "left and right images must share a stride");
+        if (raw_w_ && (raw_left.width != raw_w_ || raw_left.height != raw_h_))
+            return make_error(Errc::invalid_argument, "raw size mismatch");
+        FrameResult f;
+        f.width = w_;
+        f.height = h_;
+        const auto n = static_cast<std::size_t>(w_) * static_cast<std::size_t>(h_);
+        f.points_xyzw.resize(4 * n);
+        f.normals_xyzw.resize(4 * n);
+        f.weights.resize(n);
+        if (rect_left) *rect_left = ImageU8(w_, h_);
+        if (rect_right) *rect_right = ImageU8(w_, h_);
+        const sgm_frame out{f.points_xyzw.data(), f.normals_xyzw.data(), f.weights.data()};
+        const int rc = sgm_compute_frame_raw(ctx_, raw_left.data, raw_right.data, raw_left.stride, raw_left.width,
+                                             raw_left.height, &out, rect_left ? rect_left->data() : nullptr,
+                                             rect_right ? rect_right->data() : nullptr);
+        if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), sgm_last_error(ctx_));
+        read_timings();
+        return f;
+    }
+
     // MetalStereo::last_timings (metal_stereo.hpp:97).
     [[nodiscard]] const StereoTimings& last_timings() const { return timings_; }
 
@@ -385,5 +447,32 @@ private:
     int raw_w_ = 0, raw_h_ = 0;
     StereoTimings timings_;
 };
+
+// depth_metal::DepthPacker::pack (libs/depth_metal/include/einstar/depth_metal/depth_packer.hpp:17-38)
+// for a host frame: the session file's uncompressed depth layout, 3 * width * height bytes, packed on
+// the GPU (sgm_pack_depth_device) through temporary device buffers.
+inline Result<std::vector<std::uint8_t>> pack_depth(const FrameResult& f) {
+    const auto n = static_cast<std::size_t>(f.width) * static_cast<std::size_t>(f.height);
+    if (n == 0 || f.points_xyzw.size() != 4 * n || (!f.weights.empty() && f.weights.size() != n))
+        return make_error(Errc::invalid_argument, "frame arrays do not match its size");
+    void* dp = nullptr;
+    void* dw = nullptr;
+    void* dout = nullptr;
+    std::vector<std::uint8_t> bytes(3 * n);
+    int rc = sgm_dev_malloc(16 * n, &dp);
+    if (rc == SGM_OK && !f.weights.empty()) rc = sgm_dev_malloc(4 * n, &dw);
+    if (rc == SGM_OK) rc = sgm_dev_malloc(3 * n, &dout);
+    if (rc == SGM_OK) rc = sgm_dev_upload(dp, f.points_xyzw.data(), 16 * n);
+    if (rc == SGM_OK && dw) rc = sgm_dev_upload(dw, f.weights.data(), 4 * n);
+    if (rc == SGM_OK)
+        rc = sgm_pack_depth_device(static_cast<const float*>(dp), static_cast<const float*>(dw), f.width, f.height,
+                                   static_cast<std::uint8_t*>(dout), nullptr);
+    if (rc == SGM_OK) rc = sgm_dev_download(bytes.data(), dout, 3 * n);
+    const std::string err = rc == SGM_OK ? std::string() : sgm_last_error(nullptr);
+    for (void* p : {dp, dw, dout})
+        if (p) sgm_dev_free(p);
+    if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), err);
+    return bytes;
+}
 
 }  // namespace einstar::depth_hip

@@ -286,6 +286,11 @@ int sgm_compute_frame_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, co
                                  ptrdiff_t raw_pitch, int raw_w, int raw_h, const sgm_frame* d_out, uint8_t* d_rect_left,
                                  uint8_t* d_rect_right, void* stream);
 
+/* The context's own stream (a hipStream_t), on which calls with stream == NULL are enqueued, so that a
+ * caller can order its own work (e.g. sgm_pack_depth_device) after them. No counterpart in the
+ * reference (its command buffers are internal). */
+void* sgm_get_stream(sgm_context* ctx);
+
 /* Waits for the work enqueued by this context and records its timings. */
 int sgm_synchronize(sgm_context* ctx);
 

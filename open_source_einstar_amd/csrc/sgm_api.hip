@@ -667,6 +667,8 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     return SGM_OK;
 }
 
+void* sgm_get_stream(sgm_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
 int sgm_synchronize(sgm_context* ctx) {
     if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
     hipStream_t s = ctx->last_stream ? ctx->last_stream : ctx->stream;

@@ -250,6 +250,8 @@ def lib():
         L.sgm_set_point_params.argtypes = [vp, P(CSgmRectifiedGeometry), P(CSgmPointParams)]
         L.sgm_compute_frame_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp]
         L.sgm_compute_frame_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp, vp]
+        L.sgm_get_stream.argtypes = [vp]
+        L.sgm_get_stream.restype = vp
         L.sgm_synchronize.argtypes = [vp]
         L.sgm_last_timings.argtypes = [vp, P(CSgmTimings)]
         L.sgm_get_stage_totals.argtypes = [vp, P(CSgmStageTotals)]
@@ -540,6 +542,10 @@ class HipStereo:
 
     def synchronize(self):
         _check(lib().sgm_synchronize(self._h), ctx=self._h)
+
+    def stream(self) -> int:
+        """The context's own stream handle (hipStream_t), to order other work after its calls."""
+        return lib().sgm_get_stream(self._h)
 
     def last_timings(self) -> dict:
         """MetalStereo::last_timings (metal_stereo.hpp:97)."""
@@ -834,6 +840,14 @@ def disparity_to_points(disparity, confidence, geometry: RectifiedGeometry, para
         for b in bufs:
             if b is not None:
                 b.free()
+
+
+def pack_depth_device(d_points: int, d_weights: int | None, width: int, height: int, d_out: int,
+                      stream: int | None = None):
+    """DepthPacker::pack on device data (sgm_pack_depth_device), enqueued on `stream`; no wait."""
+    rc = lib().sgm_pack_depth_device(d_points, d_weights, width, height, d_out, stream)
+    if rc != 0:
+        raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
 
 
 def pack_depth(points_xyzw, weights=None) -> np.ndarray:

@@ -182,6 +182,41 @@ int main() {
         expect(!mism && mism.error().message == "raw size mismatch", "raw size mismatch");
     }
 
+    // Point frames (compute_frame_raw) and packing against disparity_to_points and the session layout,
+    // through the SGM-only context with the rectification above.
+    {
+        ImageU8 raw_l, raw_r;
+        const int rw = 2 * w + 30, rh = 2 * h + 20;
+        make_pair(rw, rh, 18, raw_l, raw_r);
+        auto none = (*st)->compute_frame_raw(raw_l.view(), raw_r.view());
+        expect(!none && none.error().message == "set_point_params() not called", "compute_frame_raw before set_point_params");
+        const RectifiedGeometry g{420.0, w / 2.0, h / 2.0, 60.0, 0.0};
+        expect((*st)->set_point_params(g, 50.0f, 20000.0f).has_value(), "set_point_params");
+        auto fr = (*st)->compute_frame_raw(raw_l.view(), raw_r.view());
+        auto dr = (*st)->compute_raw(raw_l.view(), raw_r.view());
+        expect(fr.has_value() && dr.has_value(), "compute_frame_raw");
+        if (fr && dr) {
+            std::vector<float> pts(3 * n), nrm(3 * n), wts(n);
+            const orc_geometry og{g.f, g.cx, g.cy, g.baseline, g.cx_offset};
+            const orc_point_params opp{50.0f, 20000.0f, 4.0f, 0.25f, 0.0f};
+            orc_disparity_to_points(dr->disparity.data(), dr->confidence.data(), w, h, &og, &opp, pts.data(), nrm.data(),
+                                    wts.data(), 0);
+            bool same = true;
+            int valid = 0;
+            for (std::size_t i = 0; i < n; ++i) {
+                for (int c = 0; c < 3; ++c)
+                    same &= fr->points_xyzw[4 * i + c] == pts[3 * i + c] && fr->normals_xyzw[4 * i + c] == nrm[3 * i + c];
+                same &= fr->weights[i] == wts[i];
+                valid += pts[3 * i + 2] > 0;
+            }
+            expect(same && valid > static_cast<int>(n / 4), "compute_frame_raw == disparity_to_points of compute_raw");
+            auto packed = pack_depth(*fr);
+            std::vector<std::uint8_t> want(3 * n);
+            orc_pack_depth(pts.data(), wts.data(), w, h, want.data());
+            expect(packed.has_value() && *packed == want, "pack_depth == the session layout");
+        }
+    }
+
     // Error behaviour of MetalStereo (metal_stereo.cpp:115-120, 235-236).
     ImageU8 small(w - 1, h);
     auto bad = (*st)->compute(small.view(), r.view());
