@@ -115,12 +115,62 @@ __device__ float zncc_slanted(const uint8_t* a, const uint8_t* b, int64_t pitch,
     return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
 }
 
+// Fixed window widths: the samples of a window row are unrolled, so their loads are issued together
+// and waited for once per row; a sample outside the right image clears a flag instead of leaving the
+// loop (its load is clamped into the row and the whole score is discarded), so lanes do not diverge.
+// Rows stay a loop (fully unrolled windows keep every load in flight and run out of registers). The
+// sums run in the same order as above.
+
+// The bilinear sample at xb of row rb (clamped into [0, bw - 2]); ok &= xb is a valid position.
+__device__ __forceinline__ double sample(const uint8_t* rb, int bw, float xb, bool& ok) {
+    ok = ok && xb >= 0.0f && xb < (float)(bw - 1);
+    const float fl = __builtin_floorf(xb);
+    const int x0 = min(max((int)fl, 0), bw - 2);
+    const float f = xb - fl;
+    return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
+}
+
+__device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, double sbb, double sab, double n) {
+    const double va = saa - sa * sa / n;
+    const double vb = sbb - sb * sb / n;
+    if (va < 1e-6 || vb < 1e-6) return -1.0f;
+    return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
+}
+
+// zncc_slanted (stereo.cpp:101-125) for radius R; gxu[u + R] = gx * u.
+template <int R>
+__device__ float zncc_slanted_w(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x, int y, float d,
+                                const float* gxu, float gy, double sa, double saa) {
+    constexpr int N = 2 * R + 1;
+    double sb = 0, sbb = 0, sab = 0;
+    bool ok = true;
+#pragma unroll 1
+    for (int v = -R; v <= R; ++v) {
+        const uint8_t* ra = a + (int64_t)(y + v) * pitch + x;
+        const uint8_t* rb = b + (int64_t)(y + v) * pitch;
+        const float gyv = gy * (float)v;
+#pragma unroll
+        for (int u = -R; u <= R; ++u) {
+            const float xb = (float)(x + u) - ((d + gxu[u + R]) + gyv);
+            const double ib = sample(rb, bw, xb, ok);
+            const double ia = ra[u];
+            sb += ib;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    if (!ok) return -2.0f;
+    return zncc_finish(sa, saa, sb, sbb, sab, (double)(N * N));
+}
+
+// RR > 0: a fixed radius; RR == 0: any radius (R.radius) through the loop form above.
+template <int RR>
 __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
     if (x >= R.w || y >= R.h) return;
     const int64_t o = (int64_t)y * R.w + x;
     float out_d = kInvalid, out_c = 0.0f;
-    const int r = R.radius;
+    const int r = RR > 0 ? RR : R.radius;
     if (y >= r && y < R.h - r && x >= r && x < R.w - r) {
         const int cy = min(y / 2, R.ch - 1), cx = min(x / 2, R.cw - 1);
         const float* C = R.coarse;
@@ -153,6 +203,11 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
                     saa += ia * ia;
                 }
             }
+            float gxu[RR > 0 ? 2 * RR + 1 : 1];
+            if constexpr (RR > 0) {
+#pragma unroll
+                for (int u = -RR; u <= RR; ++u) gxu[u + RR] = gx * (float)u;
+            }
             const float kStep = 0.5f;
             const float center = 2.0f * d0;
             const int nd = min(127, (int)((float)(2 * R.search_radius) / kStep) + 1);
@@ -162,7 +217,11 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
             int best_k = -1;
             for (int k = 0; k < nd; ++k) {
                 const float d = lo + kStep * (float)k;
-                const float s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
+                float s;
+                if constexpr (RR > 0)
+                    s = zncc_slanted_w<RR>(R.left, R.right, R.pitch, R.w, x, y, d, gxu, gy, sa, saa);
+                else
+                    s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
                 if (s > best) {
                     best = s;
                     best_k = k;
@@ -225,6 +284,35 @@ __device__ __forceinline__ float detail_grad(float l, float c, float r) {
     return clampf(0.5f * (r - l), -0.9f, 0.9f);
 }
 
+// zncc_window (stereo.cpp:211-239) for an N x N window with the left window in registers;
+// gxx[i] = gx * (x0 + i - cx).
+template <int N>
+__device__ float zncc_window_w(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x0, int y0, float cy,
+                               float D, const float* gxx, float gy, double sa, double saa) {
+    double sb = 0, sbb = 0, sab = 0;
+    bool ok = true;
+#pragma unroll 1
+    for (int j = 0; j < N; ++j) {
+        const int Y = y0 + j;
+        const uint8_t* ra = a + (int64_t)Y * pitch + x0;
+        const uint8_t* rb = b + (int64_t)Y * pitch;
+        const float dy = gy * ((float)Y - cy);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float xb = (float)(x0 + i) - ((D + gxx[i]) + dy);
+            const double ib = sample(rb, bw, xb, ok);
+            const double ia = ra[i];
+            sb += ib;
+            sbb += ib * ib;
+            sab += ia * ib;
+        }
+    }
+    if (!ok) return -2.0f;
+    return zncc_finish(sa, saa, sb, sbb, sab, (double)N * N);
+}
+
+// NW > 0: a fixed window (2 * radius); NW == 0: any radius through the loop form above.
+template <int NW>
 __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
     if (x >= D.w || y >= D.h) return;
@@ -232,8 +320,8 @@ __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
     const int w = D.w;
     const float d = F[(int64_t)y * w + x];
     float out = d;
-    const int n = 2 * D.radius, K = D.K, nc = 2 * K + 1;
-    const int y0 = 2 * y - D.radius + 1, x0 = 2 * x - D.radius + 1;
+    const int n = NW > 0 ? NW : 2 * D.radius, K = D.K, nc = 2 * K + 1;
+    const int y0 = 2 * y - n / 2 + 1, x0 = 2 * x - n / 2 + 1;
     if (y >= 1 && y + 1 < D.h && y0 >= 0 && y0 + n <= D.H && x >= 1 && x + 1 < w && valid_d(d) && x0 >= 0 &&
         x0 + n <= D.W) {
         const float gx = detail_grad(F[(int64_t)y * w + x - 1], d, F[(int64_t)y * w + x + 1]);
@@ -249,11 +337,20 @@ __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
                 saa += ia * ia;
             }
         }
+        float gxx[NW > 0 ? NW : 1];
+        if constexpr (NW > 0) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) gxx[i] = gx * ((float)(x0 + i) - cx);
+        }
         float best = -2.0f, prev = -3.0f, best_prev = -3.0f, best_next = -3.0f;
         int best_k = -1;
         for (int k = 0; k < nc; ++k) {
-            const float s = zncc_window(D.fl, D.fr, D.fpitch, D.W, x0, y0, n, cx, cy, D0 + D.step * (float)(k - K), gx, gy,
-                                        sa, saa);
+            const float Dk = D0 + D.step * (float)(k - K);
+            float s;
+            if constexpr (NW > 0)
+                s = zncc_window_w<NW>(D.fl, D.fr, D.fpitch, D.W, x0, y0, cy, Dk, gxx, gy, sa, saa);
+            else
+                s = zncc_window(D.fl, D.fr, D.fpitch, D.W, x0, y0, n, cx, cy, Dk, gx, gy, sa, saa);
             if (s > best) {
                 best = s;
                 best_k = k;
@@ -479,13 +576,30 @@ hipError_t launch_median3(const float* in, float* out, int w, int h, hipStream_t
 
 hipError_t launch_refine(const RefineDesc& r, hipStream_t stream) {
     if (r.w <= 0 || r.h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(refine_kernel, grid2d(r.w, r.h), dim3(kTw, kTh), 0, stream, r);
+    const dim3 g = grid2d(r.w, r.h), b(kTw, kTh);
+    switch (r.radius) {  // the window in registers up to 11 x 11
+        case 1: hipLaunchKernelGGL(refine_kernel<1>, g, b, 0, stream, r); break;
+        case 2: hipLaunchKernelGGL(refine_kernel<2>, g, b, 0, stream, r); break;
+        case 3: hipLaunchKernelGGL(refine_kernel<3>, g, b, 0, stream, r); break;
+        case 4: hipLaunchKernelGGL(refine_kernel<4>, g, b, 0, stream, r); break;
+        case 5: hipLaunchKernelGGL(refine_kernel<5>, g, b, 0, stream, r); break;
+        default: hipLaunchKernelGGL(refine_kernel<0>, g, b, 0, stream, r); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_detail(const DetailDesc& d, hipStream_t stream) {
     if (d.w <= 0 || d.h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(detail_kernel, grid2d(d.w, d.h), dim3(kTw, kTh), 0, stream, d);
+    const dim3 g = grid2d(d.w, d.h), b(kTw, kTh);
+    switch (d.radius) {  // the window in registers up to 12 x 12
+        case 1: hipLaunchKernelGGL(detail_kernel<2>, g, b, 0, stream, d); break;
+        case 2: hipLaunchKernelGGL(detail_kernel<4>, g, b, 0, stream, d); break;
+        case 3: hipLaunchKernelGGL(detail_kernel<6>, g, b, 0, stream, d); break;
+        case 4: hipLaunchKernelGGL(detail_kernel<8>, g, b, 0, stream, d); break;
+        case 5: hipLaunchKernelGGL(detail_kernel<10>, g, b, 0, stream, d); break;
+        case 6: hipLaunchKernelGGL(detail_kernel<12>, g, b, 0, stream, d); break;
+        default: hipLaunchKernelGGL(detail_kernel<0>, g, b, 0, stream, d); break;
+    }
     return hipGetLastError();
 }
 
