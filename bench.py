@@ -344,8 +344,9 @@ def main():
     def step():
         st.compute_device(img.ptr, img.ptr + W * H, W, W, H, d_out)
 
-    def timed(cost_volume):
+    def timed(cost_volume, group):
         st.set_option(sgm.OPT_COST_VOLUME, cost_volume)
+        st.set_option(sgm.OPT_PATH_GROUP, group)
         for _ in range(args.warmup):
             step()
         st.synchronize()
@@ -358,19 +359,22 @@ def main():
         totals = st.stage_totals()
         return elapsed * 1e3 / args.steps, totals
 
-    # Both cost modes (they give the same bytes); the faster one is the configuration reported.
+    # Both cost modes and both scanline widths (they give the same bytes); the fastest is reported.
     modes = {}
     for cv in (0, 1):
-        ms, totals = timed(cv)
-        modes[cv] = (ms, totals)
-        log(f"cost_volume={cv}: {ms:.3f} ms/step, paths {totals['paths_ms'] / max(1, totals['count']):.3f} ms")
-    best_cv = min(modes, key=lambda k: modes[k][0])
-    ms_per_step, totals = timed(best_cv)
+        for group in (16, 32):
+            ms, totals = timed(cv, group)
+            modes[(cv, group)] = (ms, totals)
+            log(f"cost_volume={cv} lanes/line={group}: {ms:.3f} ms/step, paths "
+                f"{totals['paths_ms'] / max(1, totals['count']):.3f} ms")
+    best_cv, best_group = min(modes, key=lambda k: modes[k][0])
+    ms_per_step, totals = timed(best_cv, best_group)
     pxd = W * H * D
     value = pxd / (ms_per_step * 1e-3) / 1e6
     paths_ms = totals["paths_ms"] / max(1, totals["count"])
     stage_ms = {k: totals[k] / max(1, totals["count"]) for k in ("census_ms", "paths_ms", "rows_ms", "median_ms", "gpu_ms")}
-    log(f"timed {args.steps} steps (cost_volume={best_cv}): {ms_per_step:.3f} ms/step, {value:.0f} Mpix*disp/s, stages {stage_ms}")
+    log(f"timed {args.steps} steps (cost_volume={best_cv}, lanes/line={best_group}): {ms_per_step:.3f} ms/step, "
+        f"{value:.0f} Mpix*disp/s, stages {stage_ms}")
 
     # Single synchronised steps: latency from launch to host-visible completion.
     lat = []
@@ -445,7 +449,7 @@ def main():
 
     agg_bytes = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
     achieved = agg_bytes / (paths_ms * 1e-3) / 1e9
-    kernel = f"paths_kernel<8, false, {'true' if best_cv else 'false'}>"
+    kernel = (f"paths_kernel<{8 if best_group == 16 else 4}, false, {'true' if best_cv else 'false'}, {best_group}>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel),
                 "kernel": kernel, "kernel_ms": round(paths_ms, 4), "algorithmic_bytes": agg_bytes}
@@ -533,7 +537,8 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "rectify": rectify,
         "cost_mode": "volume" if best_cv else "recompute",
-        "cost_modes_ms_per_step": {"recompute": round(modes[0][0], 4), "volume": round(modes[1][0], 4)},
+        "lanes_per_scanline": best_group,
+        "modes_ms_per_step": {f"{'volume' if cv else 'recompute'}/{g}": round(v[0], 4) for (cv, g), v in modes.items()},
         "batched": batched,
         "served": served,
         "large_batched": large,

@@ -95,7 +95,13 @@ void dfree(void* p) {
 }
 
 // Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
-hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 16) {
+hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 16,
+                        int path_group = 16) {
+    // A 32-lane line needs dpl % 4 == 0 and at most 8 lines per 256-thread block.
+    if (path_group == 32 && !paths_group_ok(d.dpl, 32, path_lines)) {
+        if (d.dpl % 4 == 0 && path_lines == 16) path_lines = 8;
+        else path_group = 16;
+    }
     LaunchTable t{};
     t.one = d;
     t.n = 1;
@@ -113,7 +119,7 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
     }
     if (st && (e = hipEventRecord(st->ev[1], s)) != hipSuccess) return e;
     t.total_blocks = path_blocks(d, path_lines);
-    if ((e = launch_paths(d.dpl, padded, t, c, path_lines, s)) != hipSuccess) return e;
+    if ((e = launch_paths(d.dpl, padded, t, c, path_lines, path_group, s)) != hipSuccess) return e;
     if (st && (e = hipEventRecord(st->ev[2], s)) != hipSuccess) return e;
     t.total_blocks = row_blocks(d);
     if ((e = launch_rows(d.dpl, padded, t, c, d.w, s)) != hipSuccess) return e;
@@ -191,6 +197,7 @@ struct sgm_context {
     uint8_t* d_cvol = nullptr;   // u8 matching costs when the cost-volume mode is on
     bool use_cvol = false;
     int path_lines = 16;         // scanlines per path-kernel block
+    int path_group = 32;         // lanes per scanline in the path kernel (32 where the lane layout allows)
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     // Resident rectification (sgm_set_rectification): both maps as interleaved (x, y) floats, and
@@ -452,6 +459,11 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
                 return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "path lines per block must be 4, 8 or 16");
             ctx->path_lines = (int)value;
             return SGM_OK;
+        case SGM_OPT_PATH_GROUP:
+            if (value != 16 && value != 32)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 16 or 32");
+            ctx->path_group = (int)value;
+            return SGM_OK;
         default:
             return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));
     }
@@ -660,7 +672,7 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
     const bool padded = d.dp != d.nd;
-    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx), ctx->path_lines);
+    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx), ctx->path_lines, ctx->path_group);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
@@ -782,7 +794,7 @@ static int compute_host(sgm_context* ctx, const HostFrame& f, int width, int hei
     o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
     o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
-    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines);
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group);
     if (e != hipSuccess) return ctx->hip_fail((w + ": launch").c_str(), e);
     struct Copy {
         void* host;
@@ -929,7 +941,7 @@ int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const ui
     hipError_t e = enqueue_rectify(ctx, d_raw_left, d_raw_right, raw_pitch, dl, dr, s, st);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_raw_device: rectify", e);
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, d_out);
-    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines)) != hipSuccess)
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group)) != hipSuccess)
         return ctx->hip_fail("sgm_compute_raw_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
@@ -982,7 +994,7 @@ static hipError_t enqueue_disparity(sgm_context* ctx, const uint8_t* dl, const u
     o.disparity = ctx->d_disp;
     o.disparity_median = L > 0 ? ctx->d_wm : nullptr;  // refine_level's median3 of the coarse disparity
     PairDesc d = context_desc(ctx, il[L], ir[L], ip[L], lw[L], lh[L], &o);
-    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines)) != hipSuccess) return e;
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group)) != hipSuccess) return e;
     if (st) {
         if ((e = hipEventRecord(st->ev[6], s)) != hipSuccess) return e;
         st->post = true;
@@ -1601,7 +1613,7 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         const Group& g = groups[gi];
         hipStream_t gs = (nbr == 1 || gi == 0) ? s : b->branch[gi - 1];
         if (plan.cost_volume) le = launch_cost8(g.dpl, g.cost, b->consts, gs);
-        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, gs);
+        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, 16, gs);
         if (le == hipSuccess) le = launch_rows(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
     }
     for (size_t i = 1; le == hipSuccess && nbr > 1 && i < nbr; ++i) {

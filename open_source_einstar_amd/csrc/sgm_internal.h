@@ -124,7 +124,10 @@ struct PointsDesc {
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
-hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, hipStream_t stream);
+// group: lanes per scanline, 16 (any dpl, 4/8/16 lines per block) or 32 (dpl % 4 == 0, 4/8 lines).
+bool paths_group_ok(int dpl, int group, int lines_per_block);
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, int group,
+                        hipStream_t stream);
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);

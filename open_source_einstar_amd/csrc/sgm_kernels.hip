@@ -51,6 +51,8 @@ constexpr int kRowHalfMirror = 0x141;
 constexpr int kRowMirror = 0x140;
 constexpr int kRowShl1 = 0x101;       // lane i reads lane i+1 of its row
 constexpr int kRowShr1 = 0x111;       // lane i reads lane i-1 of its row
+constexpr int kWaveShl1 = 0x130;      // lane i reads lane i+1 of the wave
+constexpr int kWaveShr1 = 0x138;      // lane i reads lane i-1 of the wave
 
 template <int Ctrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
@@ -334,16 +336,30 @@ __device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint3
     }
 }
 
-// One step of the recurrence (stereo.cpp:62-81) on the lane's packed pairs.
-template <int DPL, bool PAD>
+// Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value into the
+// line's end lanes, which keep = 0 / sent = sentinel replace. (A select would let the compiler run the
+// shift with the end lanes disabled, and a shift never reads a disabled lane's value.)
+struct LineEdge {
+    uint32_t keep_lo, sent_lo, keep_hi, sent_hi;
+};
+
+// One step of the recurrence (stereo.cpp:62-81) on the lane's packed pairs. A line spans G = 16
+// lanes (one DPP row) or 32 lanes (two rows, exchanging across the row boundary by wave shifts).
+template <int DPL, bool PAD, int G>
 __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint32_t (&cc)[DPL / 2], uint32_t pm, int p2,
                                           uint32_t p1p1, uint32_t nslant, const uint32_t (&padm)[DPL / 2],
-                                          uint32_t& mn) {
+                                          uint32_t& mn, const LineEdge& edge) {
     constexpr int R = DPL / 2;
     const uint32_t pmpm = pm * 0x10001u;
     const uint32_t jj = (pm + (uint32_t)p2) * 0x10001u;
-    const uint32_t nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);  // k0-2, k0-1
-    const uint32_t nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);      // k0+DPL, k0+DPL+1
+    uint32_t nb_lo, nb_hi;  // (k0-2, k0-1) and (k0+DPL, k0+DPL+1)
+    if constexpr (G == 16) {
+        nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);
+        nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);
+    } else {
+        nb_lo = (dpp_or<kWaveShr1>(prev[R - 1], kSentinel) & edge.keep_lo) | edge.sent_lo;
+        nb_hi = (dpp_or<kWaveShl1>(prev[0], kSentinel) & edge.keep_hi) | edge.sent_hi;
+    }
     uint32_t cur[R];
     mn = 0xFFFFFFFFu;
 #pragma unroll
@@ -364,13 +380,23 @@ __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint3
     for (int r = 0; r < R; ++r) prev[r] = cur[r];
 }
 
-__device__ __forceinline__ uint32_t pair_min_row(uint32_t mn) { return row_min(min(mn & 0xFFFFu, mn >> 16)); }
+// Minimum over the line's lanes of both halves of mn; every lane of the line receives it. Two rows
+// (G = 32) pair up as 0-1 and 2-3 through permlane16_swap.
+template <int G>
+__device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
+    uint32_t v = row_min(min(mn & 0xFFFFu, mn >> 16));
+    if constexpr (G == 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min((uint32_t)r[0], (uint32_t)r[1]);
+    }
+    return v;
+}
 
 // Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
-template <int DPL, bool PAD, bool CV>
+template <int DPL, bool PAD, bool CV, int G>
 __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const SGM_LDS int32_t* p2q) {
     constexpr int R = DPL / 2;
-    const int line = lb * (int)(blockDim.x >> 4) + (threadIdx.x >> 4);  // 4, 8 or 16 lines per block
+    const int line = lb * (int)(blockDim.x / G) + (int)(threadIdx.x / G);  // 4, 8 or 16 lines per block
     if (line >= P.total_lines) return;
     int dir = 0, off = 0;
 #pragma unroll
@@ -404,7 +430,12 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     if (dy > 0) n = min(n, H - y);
     if (dy < 0) n = min(n, y + 1);
 
-    const int g = threadIdx.x & (kGroup - 1);
+    const int g = threadIdx.x & (G - 1);
+    LineEdge edge;
+    edge.keep_lo = g == 0 ? 0u : 0xFFFFFFFFu;
+    edge.sent_lo = g == 0 ? kSentinel : 0u;
+    edge.keep_hi = g == G - 1 ? 0u : 0xFFFFFFFFu;
+    edge.sent_hi = g == G - 1 ? kSentinel : 0u;
     const int k0 = g * DPL;
     const int nd = P.nd, min_d = P.min_d;
     const uint32_t inv = (uint32_t)c.invalid_cost;
@@ -461,7 +492,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             mn = pk_min(mn, prev[r]);
         }
         store_bytes<DPL>(lv, prev);
-        pm = pair_min_row(mn);
+        pm = pair_min_line<G>(mn);
     }
     int Iprev = A.I;
     auto step = [&](const StepIn<DPL, CV>& S) {
@@ -473,9 +504,9 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         uint32_t cc[R];
         step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
         uint32_t mn;
-        path_step<DPL, PAD>(prev, cc, pm, p2, p1p1, nslant, padm, mn);
+        path_step<DPL, PAD, G>(prev, cc, pm, p2, p1p1, nslant, padm, mn, edge);
         store_bytes<DPL>(lv, prev);
-        pm = pair_min_row(mn);
+        pm = pair_min_line<G>(mn);
     };
     for (int i = 1; i < n; i += 2) {
         load_step<DPL, CV>(A, wp);  // pixel i+1 (prefetch)
@@ -495,7 +526,7 @@ __device__ __forceinline__ void load_p2q(const SgmConsts& c, SGM_LDS int32_t* p2
     }
 }
 
-template <int DPL, bool PAD, bool CV>
+template <int DPL, bool PAD, bool CV, int G>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
     __shared__ int32_t p2q[32];
     load_p2q(c, lds(p2q));
@@ -503,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    paths_body<DPL, PAD, CV>(P, c, lb, lds(p2q));
+    paths_body<DPL, PAD, CV, G>(P, c, lb, lds(p2q));
 }
 
 // -------------------------------------------------------------------------------- cost volume
@@ -890,27 +921,52 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, hipStream_t stream) {
+bool paths_group_ok(int dpl, int group, int lines_per_block) {
+    if (group == 16) return lines_per_block == 4 || lines_per_block == 8 || lines_per_block == 16;
+    return group == 32 && dpl % 4 == 0 && (lines_per_block == 4 || lines_per_block == 8);
+}
+
+hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, int group,
+                        hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
-    if (lines_per_block != 4 && lines_per_block != 8 && lines_per_block != 16) return hipErrorInvalidValue;
-    const unsigned threads = (unsigned)(lines_per_block * kGroup);
+    if (!paths_group_ok(dpl, group, lines_per_block)) return hipErrorInvalidValue;
+    const unsigned threads = (unsigned)(lines_per_block * group);
     const bool cv = (t.n == 1 ? t.one.cvol : nullptr) != nullptr || t.cost_volume;
-    switch (dpl) {
-#define SGM_CASE(D)                                                                                                  \
-    case D:                                                                                                          \
-        if (cv) {                                                                                                    \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);  \
-            else hipLaunchKernelGGL((paths_kernel<D, false, true>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);        \
-        } else {                                                                                                     \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c); \
-            else hipLaunchKernelGGL((paths_kernel<D, false, false>), dim3(t.total_blocks), dim3(threads), 0, stream, t, c);       \
-        }                                                                                                            \
+    const dim3 grid(t.total_blocks);
+#define SGM_LAUNCH(D, G)                                                                                       \
+    do {                                                                                                       \
+        if (cv) {                                                                                              \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true, G>), grid, dim3(threads), 0, stream, t, c);   \
+            else hipLaunchKernelGGL((paths_kernel<D, false, true, G>), grid, dim3(threads), 0, stream, t, c);         \
+        } else {                                                                                               \
+            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false, G>), grid, dim3(threads), 0, stream, t, c);  \
+            else hipLaunchKernelGGL((paths_kernel<D, false, false, G>), grid, dim3(threads), 0, stream, t, c);        \
+        }                                                                                                      \
+    } while (0)
+    if (group == 16) {
+        switch (dpl) {
+#define SGM_CASE(D) \
+    case D:         \
+        SGM_LAUNCH(D, 16); \
         break;
-        SGM_FOR_DPL(SGM_CASE)
+            SGM_FOR_DPL(SGM_CASE)
 #undef SGM_CASE
-        default:
-            return hipErrorInvalidValue;
+            default:
+                return hipErrorInvalidValue;
+        }
+    } else {
+        // 32 lanes per line: each lane holds half the disparities a 16-lane line would.
+        switch (dpl) {
+            case 4: SGM_LAUNCH(2, 32); break;
+            case 8: SGM_LAUNCH(4, 32); break;
+            case 12: SGM_LAUNCH(6, 32); break;
+            case 16: SGM_LAUNCH(8, 32); break;
+            case 24: SGM_LAUNCH(12, 32); break;
+            case 32: SGM_LAUNCH(16, 32); break;
+            default: return hipErrorInvalidValue;
+        }
     }
+#undef SGM_LAUNCH
     return hipGetLastError();
 }
 

@@ -172,3 +172,28 @@ def test_cost_volume_mode_bit_exact(w, h, nd, seed, kw):
         again = st.compute(pair.left, pair.right, outputs=FIELDS)
     assert_same(ref, got, FIELDS, tag=f"cv {w}x{h}x{nd}")
     assert_same(ref, again, FIELDS, tag=f"recompute after cv {w}x{h}x{nd}")
+
+
+G32_CASES = [(96, 64, 64, 81, dict()), (80, 48, 100, 82, dict(eight_paths=True)), (64, 40, 128, 83, dict(eight_paths=True)),
+             (120, 30, 192, 84, dict(eight_paths=True, min_disparity=-10)), (300, 20, 256, 85, dict(eight_paths=True)),
+             (420, 16, 384, 86, dict(eight_paths=True)), (560, 12, 512, 87, dict(eight_paths=True)),
+             (40, 30, 56, 88, dict(min_disparity=-20, eight_paths=True, census_radius_x=4, census_radius_y=3, p1=10, p2=120))]
+
+
+@pytest.mark.parametrize("cv", [0, 1], ids=["recompute", "volume"])
+@pytest.mark.parametrize("w,h,nd,seed,kw", G32_CASES, ids=[f"g32-{c[0]}x{c[1]}x{c[2]}" for c in G32_CASES])
+def test_32_lane_scanlines_bit_exact(w, h, nd, seed, kw, cv):
+    """Scanlines spread over 32 lanes (two DPP rows, exchanging across the row boundary) give the
+    same bytes as the oracle, for every lane layout that allows them, padded or not."""
+    pair = speckle_pair(w, h, max(nd, 6), seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd)
+    params.update(kw)
+    print(f"g32 {w}x{h} D={nd} cv={cv} {params}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, cv)
+        st.set_option(sgm.OPT_PATH_GROUP, 32)
+        for lines in (16, 8, 4):
+            st.set_option(sgm.OPT_PATH_LINES, lines)
+            got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+            assert_same(ref, got, FIELDS + VOLUMES, tag=f"g32 {w}x{h}x{nd} lines {lines}")
