@@ -357,6 +357,8 @@ typedef struct {
     int32_t cost_volume; /* 0: costs recomputed in the path kernel; 1: written once as bytes */
     int32_t path_lines;  /* scanlines per path-kernel block: 4, 8 or 16 */
     int32_t branches;    /* 0: one stream; 1: a graph branch per lane class; 2: two per class */
+    int32_t path_group;  /* lanes per scanline: 16, or 32 (with at most 8 lines; lane classes that do not
+                            allow it use 16) */
 } sgm_batch_tuning;
 
 /* The plan a batch runs and how it was chosen. */
@@ -365,6 +367,7 @@ typedef struct {
     int32_t candidates_timed; /* plans timed at creation (0 when taken from the cache) */
     double best_ms, worst_ms; /* replay time of the fastest and slowest candidate */
     int32_t from_cache;
+    int32_t path_group;       /* lanes per scanline (16 or 32) */
 } sgm_batch_plan;
 
 /* Plans a batch of n pairs: allocates the scratch of all pairs, times every candidate plan on the

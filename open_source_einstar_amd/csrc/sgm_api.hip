@@ -165,7 +165,7 @@ hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, 
     Stages st;
     if (ev5)
         for (int i = 0; i < 5; ++i) st.ev[i] = ev5[i];
-    return enqueue_pair(d, c, padded, s, ev5 ? &st : nullptr);
+    return enqueue_pair(d, c, padded, s, ev5 ? &st : nullptr, 16, 32);  // 32-lane scanlines where allowed
 }
 void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) { set_geometry(d, w, h, min_d, nd, ndirs); }
 size_t census_words_ext(int w, int h, int min_d, int nd) { return census_words(w, h, min_d, nd); }
@@ -1455,7 +1455,11 @@ struct BatchPlan {
     int cost_volume = 0;  // 0: recompute costs in the path kernel, 1: cost volume
     int path_lines = 16;  // scanlines per path-kernel block
     int branches = 1;     // 0: one stream, 1: a branch per lane class, 2: two per class
+    int path_group = 16;  // lanes per scanline: 16, or 32 (at most 8 lines) for classes that allow it
 };
+
+// Lanes per scanline of a class under a plan: 32 only where the lane layout allows it.
+int class_group(const BatchPlan& pl, int dpl) { return paths_group_ok(dpl, pl.path_group, pl.path_lines) ? pl.path_group : 16; }
 
 std::mutex g_plan_mu;
 std::map<std::string, BatchPlan> g_plan_cache;
@@ -1613,7 +1617,7 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         const Group& g = groups[gi];
         hipStream_t gs = (nbr == 1 || gi == 0) ? s : b->branch[gi - 1];
         if (plan.cost_volume) le = launch_cost8(g.dpl, g.cost, b->consts, gs);
-        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, 16, gs);
+        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, class_group(plan, g.dpl), gs);
         if (le == hipSuccess) le = launch_rows(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
     }
     for (size_t i = 1; le == hipSuccess && nbr > 1 && i < nbr; ++i) {
@@ -1691,11 +1695,13 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         g_last_error = "batch size must be in [1, " + std::to_string(kMaxBatch) + "]";
         return SGM_ERR_INVALID_ARGUMENT;
     }
-    sgm_batch_tuning tn{-1, -1, -1};
+    sgm_batch_tuning tn{-1, -1, -1, -1};
     if (tuning) tn = *tuning;
     if ((tn.cost_volume < -1 || tn.cost_volume > 1) || (tn.path_lines != -1 && tn.path_lines != 4 && tn.path_lines != 8 && tn.path_lines != 16) ||
-        (tn.branches < -1 || tn.branches > 2)) {
-        g_last_error = "tuning: cost_volume in {-1,0,1}, path_lines in {-1,4,8,16}, branches in {-1,0,1,2}";
+        (tn.branches < -1 || tn.branches > 2) || (tn.path_group != -1 && tn.path_group != 16 && tn.path_group != 32) ||
+        (tn.path_group == 32 && tn.path_lines == 16)) {
+        g_last_error = "tuning: cost_volume in {-1,0,1}, path_lines in {-1,4,8,16}, branches in {-1,0,1,2}, "
+                       "path_group in {-1,16,32} (32 with at most 8 lines)";
         return SGM_ERR_INVALID_ARGUMENT;
     }
     for (int i = 0; i < n; ++i) {
@@ -1766,13 +1772,15 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
     std::vector<BatchPlan> cands;
     for (int cv : {0, 1})
         for (int pl : {4, 8, 16})
-            for (int br : {0, 1, 2}) {
-                if ((tn.cost_volume >= 0 && cv != tn.cost_volume) || (tn.path_lines > 0 && pl != tn.path_lines) ||
-                    (tn.branches >= 0 && br != tn.branches))
-                    continue;
-                if (tn.branches < 0 && br == 0) continue;  // one stream only when asked for
-                cands.push_back({cv, pl, br});
-            }
+            for (int br : {0, 1, 2})
+                for (int pg : {16, 32}) {
+                    if ((tn.cost_volume >= 0 && cv != tn.cost_volume) || (tn.path_lines > 0 && pl != tn.path_lines) ||
+                        (tn.branches >= 0 && br != tn.branches) || (tn.path_group > 0 && pg != tn.path_group))
+                        continue;
+                    if (tn.branches < 0 && br == 0) continue;  // one stream only when asked for
+                    if (pg == 32 && pl == 16) continue;           // 32-lane lines: at most 8 per block
+                    cands.push_back({cv, pl, br, pg});
+                }
     b->signature = batch_signature(*params, n, pairs);
     BatchPlan best = cands.front();
     float best_ms = 0, worst_ms = 0;
@@ -1812,6 +1820,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
     }
     b->info.cost_volume = best.cost_volume;
     b->info.path_lines = best.path_lines;
+    b->info.path_group = best.path_group;
     b->info.branches = best.branches;
     b->info.candidates_timed = timed;
     b->info.best_ms = best_ms;
@@ -1840,7 +1849,8 @@ int sgm_batch_run_plain(sgm_batch* b, void* stream) {
     for (PairDesc d : b->desc) {
         // The same kernels and knobs as the graph, one pair after another: only batching differs.
         if (!b->plan.cost_volume) d.cvol = nullptr;
-        const hipError_t e = enqueue_pair(d, b->consts, d.dp != d.nd, s, nullptr, b->plan.path_lines);
+        const hipError_t e = enqueue_pair(d, b->consts, d.dp != d.nd, s, nullptr, b->plan.path_lines,
+                                          class_group(b->plan, d.dpl));
         if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_run_plain", e));
     }
     b->last_stream = s;

@@ -94,13 +94,14 @@ class CSgmServerStats(C.Structure):
 
 
 class CSgmBatchTuning(C.Structure):
-    _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32)]
+    _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32),
+                ("path_group", C.c_int32)]
 
 
 class CSgmBatchPlan(C.Structure):
     _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32),
                 ("candidates_timed", C.c_int32), ("best_ms", C.c_double), ("worst_ms", C.c_double),
-                ("from_cache", C.c_int32)]
+                ("from_cache", C.c_int32), ("path_group", C.c_int32)]
 
 
 class CSgmPair(C.Structure):
@@ -619,7 +620,8 @@ class Batch:
         h = C.c_void_p()
         cp = params.to_c()
         t = tuning or {}
-        tn = CSgmBatchTuning(t.get("cost_volume", -1), t.get("path_lines", -1), t.get("branches", -1))
+        tn = CSgmBatchTuning(t.get("cost_volume", -1), t.get("path_lines", -1), t.get("branches", -1),
+                             t.get("path_group", -1))
         rc = lib().sgm_batch_create_tuned(C.byref(cp), len(pairs), cps, C.byref(tn), C.byref(h))
         if rc != 0:
             raise SgmError(rc, (lib().sgm_batch_last_error(None) or b"").decode())

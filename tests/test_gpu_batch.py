@@ -55,9 +55,12 @@ def test_single_pair_batch():
         b.close()
 
 
-@pytest.mark.parametrize("tuning", [dict(cost_volume=0, path_lines=4, branches=0), dict(cost_volume=1, path_lines=8, branches=2),
-                                    dict(cost_volume=1, path_lines=16, branches=1)],
-                         ids=["recompute-4-onestream", "volume-8-split", "volume-16-perclass"])
+@pytest.mark.parametrize("tuning", [dict(cost_volume=0, path_lines=4, branches=0, path_group=16),
+                                    dict(cost_volume=1, path_lines=8, branches=2, path_group=32),
+                                    dict(cost_volume=1, path_lines=16, branches=1, path_group=16),
+                                    dict(cost_volume=0, path_lines=8, branches=1, path_group=32)],
+                         ids=["recompute-4-onestream", "volume-8-split-32lanes", "volume-16-perclass",
+                              "recompute-8-perclass-32lanes"])
 def test_every_plan_gives_the_same_bytes(tuning):
     sizes = [(96, 64, 64), (130, 70, 96), (64, 48, 128), (77, 91, 64), (40, 30, 24)]
     specs, refs = make_specs(sizes, 3000)
@@ -81,11 +84,12 @@ def test_autotuner_times_candidates_and_caches_the_choice():
     b1 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     p1 = b1.plan()
     b1.close()
-    assert p1["candidates_timed"] == 12 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
+    assert p1["candidates_timed"] == 20 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
     b2 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     try:
         p2 = b2.plan()
-        assert p2["from_cache"] == 1 and all(p2[k] == p1[k] for k in ("cost_volume", "path_lines", "branches"))
+        assert p2["from_cache"] == 1 and all(p2[k] == p1[k] for k in ("cost_volume", "path_lines", "branches",
+                                                                        "path_group"))
         b2.run()
         b2.synchronize()
         for i in range(len(sizes)):
