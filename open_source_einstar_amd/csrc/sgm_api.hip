@@ -352,8 +352,9 @@ static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int st
     ctx->census_half = census_words(sgm_w, sgm_h, params->min_disparity, params->num_disparities);
     if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_cvol, npx * dp, b);  // cost-volume mode is the default
-    ctx->use_cvol = true;
+    // Cost-volume mode is the default wherever the volume fits one buffer view.
+    ctx->use_cvol = npx * dp <= kMaxViewBytes;
+    if (e == hipSuccess && ctx->use_cvol) e = dalloc(&ctx->d_cvol, npx * dp, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_disp, npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_median, npx, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_conf, npx, b);
@@ -403,6 +404,10 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
         g_last_error = "max_width must be in [1, " + std::to_string(kMaxWidth) + "] and max_height >= 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if (census_words(max_width, max_height, params->min_disparity, params->num_disparities) * 8 > kMaxViewBytes) {
+        g_last_error = "image too large: the census words of one image exceed 4 GiB - 1";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     return create_context(params, max_width, max_height, max_width, max_height, sgm_only(*params), out);
 }
 
@@ -437,6 +442,10 @@ int sgm_create_stereo(const sgm_stereo_params* sp, int width, int height, sgm_co
         g_last_error = "width must be in [1, " + std::to_string(kMaxWidth) + "] and the SGM level at least 1x1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if (census_words(sw, sh, sp->sgm.min_disparity, sp->sgm.num_disparities) * 8 > kMaxViewBytes) {
+        g_last_error = "image too large: the census words of one image exceed 4 GiB - 1";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     return create_context(&sp->sgm, sw, sh, width, height, *sp, out);
 }
 
@@ -446,8 +455,10 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
     if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
     switch (option) {
         case SGM_OPT_COST_VOLUME: {
+            const size_t n = (size_t)ctx->max_w * ctx->max_h * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
+            if (value && n > kMaxViewBytes)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "cost volume of " + std::to_string(n) + " bytes exceeds 4 GiB - 1");
             if (value && !ctx->d_cvol) {
-                const size_t n = (size_t)ctx->max_w * ctx->max_h * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
                 const hipError_t e = dalloc(&ctx->d_cvol, n, ctx->bytes);
                 if (e != hipSuccess) return ctx->hip_fail("sgm_set_option: cost volume allocation", e);
             }
@@ -570,6 +581,9 @@ static int check_size(sgm_context* ctx, int width, int height) {
     return SGM_OK;
 }
 
+// A device image the path kernel reads through a buffer view (its guide intensities).
+static bool pitch_fits(int64_t pitch, int w, int h) { return (uint64_t)pitch * (uint64_t)(h - 1) + (uint64_t)w <= kMaxViewBytes; }
+
 // Builds the descriptor for one pair over the context's scratch; outputs default to the context's.
 static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t* dr, int64_t pitch, int w, int h,
                              const sgm_outputs* o) {
@@ -667,6 +681,7 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     }
     if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
     if (!d_left || !d_right || pitch < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
+    if (!pitch_fits(pitch, width, height)) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "pitch x height exceeds 4 GiB - 1");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     PairDesc d = context_desc(ctx, d_left, d_right, pitch, width, height, d_out);
     if (d_out && (d_out->census_left || d_out->census_right))
@@ -1192,6 +1207,7 @@ int sgm_compute_disparity_device(sgm_context* ctx, const uint8_t* d_left, const 
     if (int rc = check_stereo_size(ctx, width, height); rc != SGM_OK) return rc;
     if (!d_left || !d_right || pitch < width || !d_disparity)
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or pitch < width");
+    if (!pitch_fits(pitch, width, height)) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "pitch x height exceeds 4 GiB - 1");
     const bool full = ctx->sp.detail_enabled && d_full_left && d_full_right && full_width == 2 * width &&
                       full_height == 2 * height;
     if (full && full_pitch < full_width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "full_pitch < full_width");
@@ -1578,6 +1594,7 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         }
         if (e == hipSuccess && plan.cost_volume) e = make_table(b, g.ds, cb, g.cost);
         if (e == hipSuccess) e = make_table(b, g.ds, pb, g.paths);
+        g.paths.cost_volume = plan.cost_volume ? 1 : 0;  // the path kernel reads the costs the cost launch wrote
         if (e == hipSuccess) e = make_table(b, g.ds, rb, g.rows);
     }
     LaunchTable census{}, median{};
@@ -1718,6 +1735,19 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
             g_last_error = "pair " + std::to_string(i) + ": census and cost-volume outputs are only available from sgm_compute";
             return SGM_ERR_UNSUPPORTED;
         }
+        if (census_words(q.width, q.height, q.min_disparity, q.num_disparities) * 8 > kMaxViewBytes ||
+            !pitch_fits(q.pitch, q.width, q.height)) {
+            g_last_error = "pair " + std::to_string(i) + ": census words or pitch x height exceed 4 GiB - 1";
+            return SGM_ERR_INVALID_ARGUMENT;
+        }
+    }
+    // The cost-volume mode needs every pair's volume within one buffer view.
+    bool cv_fits = true;
+    for (int i = 0; i < n; ++i)
+        cv_fits = cv_fits && (size_t)pairs[i].width * pairs[i].height * dpl_for(pairs[i].num_disparities) * kGroup <= kMaxViewBytes;
+    if (tn.cost_volume == 1 && !cv_fits) {
+        g_last_error = "tuning: a pair's cost volume exceeds 4 GiB - 1";
+        return SGM_ERR_INVALID_ARGUMENT;
     }
     auto* b = new sgm_batch;
     b->p = *params;
@@ -1734,7 +1764,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 + 255) &
                  ~(size_t)255;
         off_cvol[i] = total;
-        total += (npx * dp + 255) & ~(size_t)255;
+        if (cv_fits) total += (npx * dp + 255) & ~(size_t)255;
         off_lvol[i] = total;
         total += ((size_t)b->ndirs * npx * dp + 255) & ~(size_t)255;
     }
@@ -1753,7 +1783,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         d.pitch = q.pitch;
         d.census_l = reinterpret_cast<uint64_t*>(b->d_scratch + off_census[i]);
         d.census_r = d.census_l + census_words(q.width, q.height, q.min_disparity, q.num_disparities);
-        d.cvol = b->d_scratch + off_cvol[i];
+        d.cvol = cv_fits ? b->d_scratch + off_cvol[i] : nullptr;
         d.lvol = b->d_scratch + off_lvol[i];
         set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);
         d.disp = q.out.disparity;
@@ -1779,6 +1809,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
                         continue;
                     if (tn.branches < 0 && br == 0) continue;  // one stream only when asked for
                     if (pg == 32 && pl == 16) continue;           // 32-lane lines: at most 8 per block
+                    if (cv == 1 && !cv_fits) continue;
                     cands.push_back({cv, pl, br, pg});
                 }
     b->signature = batch_signature(*params, n, pairs);

@@ -15,6 +15,10 @@ constexpr int kLinesPerBlock = kBlock / kGroup;
 constexpr int kMaxBatch = 64;                 // pairs per launch table
 constexpr int kMaxDirs = 8;
 constexpr int kMaxWidth = 4096;               // the row kernel keeps 9 bytes of LDS per column
+// The path kernel reads the census rows, the cost volume and the guide image through buffer views
+// addressed by 32-bit byte offsets, so each of them must stay within 4 GiB - 1. Census rows and
+// guide images beyond that are refused; a cost volume beyond it is not used (costs are recomputed).
+constexpr uint64_t kMaxViewBytes = 0xFFFFFFFFull;
 constexpr int kMaxLevels = 4;                 // pyramid levels below the full resolution
 
 // Device pointers and geometry of one rectified pair. Every output pointer may be null except

@@ -4,10 +4,11 @@
 // 361-463); the layout and the parallel decomposition are this library's own:
 //
 //   census   one thread per pixel, LDS tile with halo                  -> census words u64
-//   paths    one 16-lane DPP row per scanline, all directions of all   -> L_dir u8 [dir][y][x][dp]
-//            pairs in one launch; a lane holds `dpl` consecutive
-//            disparities as packed u16 pairs; the costs are recomputed
-//            from the census words on the fly (no cost volume in HBM)
+//   paths    one 16-lane DPP row (or two joined rows) per scanline,    -> L_dir u8 [dir][y][x][dp]
+//            all directions of all pairs in one launch; a lane holds
+//            consecutive disparities as packed u16 pairs; the costs are
+//            read from a u8 cost volume or recomputed from the census
+//            words on the fly
 //   rows     one block per image row: S = sum of the L_dir, left WTA   -> disparity, status, costs
 //            with uniqueness + subpixel, right WTA by LDS atomic-min
 //            of packed (S << 16 | k) keys, then the left/right check
@@ -64,6 +65,12 @@ __device__ __forceinline__ uint32_t dpp_or(uint32_t v, uint32_t edge) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, Ctrl, 0xF, 0xF, false);
 }
 
+// Neighbour lane's value, or 0 where the shift has no source lane (callers replace those lanes).
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_zero(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Ctrl, 0xF, 0xF, true);
+}
+
 // Minimum over the 16 lanes of a DPP row, returned to all of them. The DPP moves use the min's
 // identity as their old value so the compiler can fold them into v_min_u32_dpp.
 template <int Ctrl>
@@ -94,7 +101,19 @@ __device__ __forceinline__ void locate(const LaunchTable& t, int b, int& pi, int
     lb = b - t.first_block[lo];
 }
 
-__device__ __forceinline__ PairDesc load_desc(const LaunchTable& t, int pi) { return t.n == 1 ? t.one : t.many[pi]; }
+// Descriptors are read through the constant address space (the kernel arguments or the batch's
+// device array, both written before the launch), so the block-uniform fields load into scalar
+// registers.
+#define SGM_CONST __attribute__((address_space(4)))
+__device__ __forceinline__ PairDesc load_desc(const LaunchTable& t, int pi) {
+    static_assert(sizeof(PairDesc) % 4 == 0, "descriptors are copied as words");
+    const SGM_CONST uint32_t* src = t.n == 1 ? (const SGM_CONST uint32_t*)&t.one : (const SGM_CONST uint32_t*)(t.many + pi);
+    PairDesc d;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&d);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(PairDesc) / 4); ++i) dst[i] = src[i];
+    return d;
+}
 
 // LDS pointers carry their address space so that stage bodies compiled out of line still use
 // ds_* instructions.
@@ -178,8 +197,9 @@ __global__ __launch_bounds__(kBlock) void cost_volume_kernel(LaunchTable t, SgmC
 // Lane g of a line owns disparities k0..k0+DPL-1 (k0 = g*DPL) as DPL/2 packed u16 pairs
 // (low half = even k). Per step:
 //   L(k) = C(k) + min(L'(k), L'(k+-1) + P1, [L'(k+-2) + P1], pm + p2) - pm,   pm = min_k L'(k)
-// The +-1/+-2 neighbours across lane boundaries come from the adjacent lane by DPP row shifts,
-// and pm by a 4-step DPP butterfly; a line never leaves its DPP row.
+// The +-1/+-2 neighbours across lane boundaries come from the adjacent lane by DPP row (or wave)
+// shifts, and pm by a 4-step DPP butterfly (joined across two rows by permlane16_swap for 32-lane
+// lines).
 
 #define SGM_GLOBAL __attribute__((address_space(1)))
 template <typename T>
@@ -232,6 +252,23 @@ __device__ __forceinline__ void add_raw(const uint32_t (&w)[(DPL + 3) / 4], uint
     }
 }
 
+// Raw buffer views: a wave-uniform base and 32-bit byte offsets. A load at or beyond the view's size
+// returns zero instead of touching memory, so a walk can run its prefetch past its last pixel
+// without a bounds test per step. (Hosts keep every viewed buffer below 4 GiB.)
+typedef __amdgpu_buffer_rsrc_t BufView;
+constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+// Base and size are block-uniform; reading them from the first lane keeps the view in scalar
+// registers wherever the compiler computed them on the vector unit.
+// (readfirstlane returns a signed int: each half goes back to uint32_t before it is widened, or a
+// low half with bit 31 set would sign-extend over the high half.)
+__device__ __forceinline__ uint32_t first_lane(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ BufView buffer_view(const void* base, uint64_t bytes) {
+    const uint32_t n = first_lane(bytes < 0xFFFFFFFFull ? (uint32_t)bytes : 0xFFFFFFFFu);
+    const uint64_t a = (uint64_t)base;
+    const uint64_t b = ((uint64_t)first_lane((uint32_t)(a >> 32)) << 32) | (uint64_t)first_lane((uint32_t)a);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)n, kRawBufferWord3);
+}
+
 // Everything one step of a path reads. Recomputed costs (CV = false): the left census word, the
 // lane's DPL right census words (lowest address first, i.e. disparity k0+DPL-1 first) and the guide
 // intensity. Cost volume (CV = true): the lane's DPL cost bytes and the guide intensity.
@@ -241,32 +278,67 @@ template <int DPL>
 struct StepIn<DPL, false> {
     uint64_t a;
     uint64_t w[DPL];
-    int I;
+    uint32_t I;
 };
 template <int DPL>
 struct StepIn<DPL, true> {
     uint32_t c[(DPL + 3) / 4];
-    int I;
+    uint32_t I;
 };
 
-// Load pointers of a walk; census pointers for recomputed costs, the cost-volume pointer otherwise.
-struct WalkPtrs {
-    const SGM_GLOBAL uint64_t* cl;
-    const SGM_GLOBAL uint64_t* cr;
-    const SGM_GLOBAL uint8_t* cv;
-    const SGM_GLOBAL uint8_t* gd;
+// A walk's load positions (byte offsets into the views) and their per-pixel steps.
+struct Walk {
+    BufView cv, cl, cr, gd;
+    uint32_t o_cv, o_cl, o_cr, o_gd;
+    uint32_t s_cv, s_c, s_gd;
 };
 
+__device__ __forceinline__ uint64_t u64_of(u32x2 v) { return (uint64_t)v.x | ((uint64_t)v.y << 32); }
+
+// Loads one pixel's inputs at the walk's position and moves the walk to the next pixel.
 template <int DPL, bool CV>
-__device__ __forceinline__ void load_step(StepIn<DPL, CV>& s, const WalkPtrs& p) {
+__device__ __forceinline__ void load_step(StepIn<DPL, CV>& s, Walk& wk) {
     if constexpr (CV) {
-        load_raw<DPL>(p.cv, s.c);
-    } else {
-        s.a = *p.cl;
+        if constexpr (DPL % 16 == 0) {
 #pragma unroll
-        for (int j = 0; j < DPL; ++j) s.w[j] = p.cr[j];
+            for (int i = 0; i < DPL / 16; ++i) {
+                const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(wk.cv, wk.o_cv + 16 * i, 0, 0);
+                s.c[4 * i] = q.x;
+                s.c[4 * i + 1] = q.y;
+                s.c[4 * i + 2] = q.z;
+                s.c[4 * i + 3] = q.w;
+            }
+        } else if constexpr (DPL % 8 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 8; ++i) {
+                const u32x2 q = __builtin_amdgcn_raw_buffer_load_b64(wk.cv, wk.o_cv + 8 * i, 0, 0);
+                s.c[2 * i] = q.x;
+                s.c[2 * i + 1] = q.y;
+            }
+        } else if constexpr (DPL % 4 == 0) {
+#pragma unroll
+            for (int i = 0; i < DPL / 4; ++i) s.c[i] = __builtin_amdgcn_raw_buffer_load_b32(wk.cv, wk.o_cv + 4 * i, 0, 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < DPL / 2; ++i) {
+                const uint32_t h = __builtin_amdgcn_raw_buffer_load_b16(wk.cv, wk.o_cv + 2 * i, 0, 0);
+                s.c[i / 2] = (i & 1) ? (s.c[i / 2] | (h << 16)) : h;
+            }
+        }
+        wk.o_cv += wk.s_cv;
+    } else {
+        s.a = u64_of(__builtin_amdgcn_raw_buffer_load_b64(wk.cl, wk.o_cl, 0, 0));
+#pragma unroll
+        for (int j = 0; j < DPL; j += 2) {
+            const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(wk.cr, wk.o_cr + 8 * j, 0, 0);
+            s.w[j] = (uint64_t)q.x | ((uint64_t)q.y << 32);
+            s.w[j + 1] = (uint64_t)q.z | ((uint64_t)q.w << 32);
+        }
+        wk.o_cl += wk.s_c;
+        wk.o_cr += wk.s_c;
     }
-    s.I = *p.gd;
+    s.I = __builtin_amdgcn_raw_buffer_load_b8(wk.gd, wk.o_gd, 0, 0);
+    wk.o_gd += wk.s_gd;
 }
 
 // Packed costs of the lane's disparities k0 + 2r, k0 + 2r + 1 (low, high half). Word w[DPL-1-j]
@@ -336,8 +408,8 @@ __device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint3
     }
 }
 
-// Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value into the
-// line's end lanes, which keep = 0 / sent = sentinel replace. (A select would let the compiler run the
+// Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value (or zero) into
+// the line's end lanes, which keep = 0 / sent = sentinel replace. (A select would let the compiler run the
 // shift with the end lanes disabled, and a shift never reads a disabled lane's value.)
 struct LineEdge {
     uint32_t keep_lo, sent_lo, keep_hi, sent_hi;
@@ -345,46 +417,61 @@ struct LineEdge {
 
 // One step of the recurrence (stereo.cpp:62-81) on the lane's packed pairs. A line spans G = 16
 // lanes (one DPP row) or 32 lanes (two rows, exchanging across the row boundary by wave shifts).
-template <int DPL, bool PAD, int G>
-__device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint32_t (&cc)[DPL / 2], uint32_t pm, int p2,
-                                          uint32_t p1p1, uint32_t nslant, const uint32_t (&padm)[DPL / 2],
-                                          uint32_t& mn, const LineEdge& edge) {
+// pmpm holds the previous step's minimum in both halves, jj = pmpm + p2 in both halves.
+//
+// With slant steps the P1 neighbourhood of k is {k-2, k-1, k+1, k+2}, which is min(a(k-2), a(k+1))
+// for the pairwise minima a(j) = min(L'(j), L'(j+1)): one pairwise pass, one shifted pass. An
+// excluded neighbour is the sentinel, which never wins a minimum against a real value.
+template <int DPL, bool PAD, int G, bool SLANT>
+__device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint32_t (&cc)[DPL / 2], uint32_t pmpm,
+                                          uint32_t jj, uint32_t p1p1, const uint32_t (&padm)[DPL / 2], uint32_t& mn,
+                                          const LineEdge& edge) {
     constexpr int R = DPL / 2;
-    const uint32_t pmpm = pm * 0x10001u;
-    const uint32_t jj = (pm + (uint32_t)p2) * 0x10001u;
     uint32_t nb_lo, nb_hi;  // (k0-2, k0-1) and (k0+DPL, k0+DPL+1)
     if constexpr (G == 16) {
         nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);
         nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);
     } else {
-        nb_lo = (dpp_or<kWaveShr1>(prev[R - 1], kSentinel) & edge.keep_lo) | edge.sent_lo;
-        nb_hi = (dpp_or<kWaveShl1>(prev[0], kSentinel) & edge.keep_hi) | edge.sent_hi;
+        nb_lo = (dpp_zero<kWaveShr1>(prev[R - 1]) & edge.keep_lo) | edge.sent_lo;
+        nb_hi = (dpp_zero<kWaveShl1>(prev[0]) & edge.keep_hi) | edge.sent_hi;
     }
-    uint32_t cur[R];
+    uint32_t nbr[R];  // packed minimum over the P1 neighbourhood
+    if constexpr (SLANT) {
+        uint32_t A[R + 2];  // A[r + 1] = (a(k0 + 2r), a(k0 + 2r + 1)), r = -1 .. R
+        A[0] = pk_min(nb_lo, __builtin_amdgcn_alignbit(prev[0], nb_lo, 16));
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            A[r + 1] = pk_min(prev[r], __builtin_amdgcn_alignbit(r + 1 < R ? prev[r + 1] : nb_hi, prev[r], 16));
+        const u16x2 h = __builtin_bit_cast(u16x2, nb_hi);
+        A[R + 1] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(h, h.yx));  // low half: a(k0 + DPL)
+#pragma unroll
+        for (int r = 0; r < R; ++r) nbr[r] = pk_min(A[r], __builtin_amdgcn_alignbit(A[r + 2], A[r + 1], 16));
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t lo = r == 0 ? nb_lo : prev[r - 1];
+            const uint32_t hi = r == R - 1 ? nb_hi : prev[r + 1];
+            nbr[r] = pk_min(__builtin_amdgcn_alignbit(prev[r], lo, 16), __builtin_amdgcn_alignbit(hi, prev[r], 16));
+        }
+    }
     mn = 0xFFFFFFFFu;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const uint32_t lo = r == 0 ? nb_lo : prev[r - 1];
-        const uint32_t hi = r == R - 1 ? nb_hi : prev[r + 1];
-        const uint32_t km1 = __builtin_amdgcn_alignbit(prev[r], lo, 16);  // (k-1) pairs
-        const uint32_t kp1 = __builtin_amdgcn_alignbit(hi, prev[r], 16);  // (k+1) pairs
-        uint32_t m = pk_min(pk_min(km1, kp1), pk_min(lo, hi) | nslant);
-        m = pk_add(m, p1p1);
-        const uint32_t best = pk_min(pk_min(prev[r], m), jj);
+        const uint32_t best = pk_min(pk_min(prev[r], pk_add(nbr[r], p1p1)), jj);
         uint32_t v = pk_add(pk_sub(best, pmpm), cc[r]);
         if (PAD) v = pk_max(v, padm[r]);
-        cur[r] = v;
+        prev[r] = v;
         mn = pk_min(mn, v);
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) prev[r] = cur[r];
 }
 
-// Minimum over the line's lanes of both halves of mn; every lane of the line receives it. Two rows
-// (G = 32) pair up as 0-1 and 2-3 through permlane16_swap.
+// Minimum over the line's lanes of both halves of mn, in both halves of the result; every lane of
+// the line receives it. (With equal halves, u32 order is the order of the halves.) Two rows (G = 32)
+// pair up as 0-1 and 2-3 through permlane16_swap.
 template <int G>
 __device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
-    uint32_t v = row_min(min(mn & 0xFFFFu, mn >> 16));
+    const u16x2 h = __builtin_bit_cast(u16x2, mn);
+    uint32_t v = row_min(__builtin_bit_cast(uint32_t, __builtin_elementwise_min(h, h.yx)));
     if constexpr (G == 32) {
         const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
         v = min((uint32_t)r[0], (uint32_t)r[1]);
@@ -392,9 +479,13 @@ __device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
     return v;
 }
 
+// The step penalty for every |dI| in [0, 255], in both halves: max(P1+1, P2/(|dI|/8 + 1)) when
+// adaptive (stereo.cpp:62-65), P2 otherwise.
+constexpr int kP2Entries = 256;
+
 // Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
-template <int DPL, bool PAD, bool CV, int G>
-__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const SGM_LDS int32_t* p2q) {
+template <int DPL, bool PAD, bool CV, int G, bool SLANT>
+__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const SGM_LDS uint32_t* p2tab) {
     constexpr int R = DPL / 2;
     const int line = lb * (int)(blockDim.x / G) + (int)(threadIdx.x / G);  // 4, 8 or 16 lines per block
     if (line >= P.total_lines) return;
@@ -437,54 +528,50 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     edge.keep_hi = g == G - 1 ? 0u : 0xFFFFFFFFu;
     edge.sent_hi = g == G - 1 ? kSentinel : 0u;
     const int k0 = g * DPL;
-    const int nd = P.nd, min_d = P.min_d;
+    const int min_d = P.min_d;
     const uint32_t inv = (uint32_t)c.invalid_cost;
     const uint32_t p1p1 = (uint32_t)c.p1 * 0x10001u;
-    const uint32_t nslant = c.slant_steps ? 0u : 0xFFFFFFFFu;
-    const bool adaptive = c.adaptive_p2 != 0;
-    const int p2_fixed = c.p2;
     // The whole group's window [x - min_d - (dp-1), x - min_d] lies in [0, W) for these x.
     const int full_lo = min_d + P.dp - 1, full_hi = W - 1 + min_d;
     uint32_t padm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        padm[r] = PAD ? ((k0 + 2 * r >= nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= nd ? 0x7FFF0000u : 0u)) : 0u;
+        padm[r] = PAD ? ((k0 + 2 * r >= P.nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= P.nd ? 0x7FFF0000u : 0u)) : 0u;
 
-    // Walk pointers (the census rows carry pads, so no window load needs clamping).
+    // Load positions (the census rows carry pads, so no window word leaves its row).
+    Walk wk;
     const int64_t cp = P.census_pitch;
-    const int64_t step_c = (int64_t)dy * cp + dx;
-    const int64_t step_g = (int64_t)dy * P.pitch + dx;
+    if constexpr (CV) {
+        wk.cv = buffer_view(P.cvol, (uint64_t)W * H * P.dp);
+        wk.o_cv = (uint32_t)(((int64_t)y * W + x) * P.dp + k0);
+        wk.s_cv = (uint32_t)(((int64_t)dy * W + dx) * P.dp);
+    } else {
+        wk.cl = buffer_view(P.census_l, (uint64_t)cp * H * 8);
+        wk.cr = buffer_view(P.census_r, (uint64_t)cp * H * 8);
+        wk.o_cl = (uint32_t)(((int64_t)y * cp + P.census_padl + x) * 8);
+        wk.o_cr = (uint32_t)(((int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1))) * 8);
+        wk.s_c = (uint32_t)(((int64_t)dy * cp + dx) * 8);
+    }
+    wk.gd = buffer_view(P.left, (uint64_t)P.pitch * (H - 1) + W);
+    wk.o_gd = (uint32_t)((int64_t)y * P.pitch + x);
+    wk.s_gd = (uint32_t)((int64_t)dy * P.pitch + dx);
     const int64_t step_l = ((int64_t)dy * W + dx) * P.dp;
-    WalkPtrs wp;
-    wp.cl = gptr(P.census_l) + (int64_t)y * cp + P.census_padl + x;
-    wp.cr = gptr(P.census_r) + (int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1));
-    wp.cv = gptr(P.cvol) + ((int64_t)y * W + x) * P.dp + k0;
-    wp.gd = gptr(P.left) + (int64_t)y * P.pitch + x;
     SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + (int64_t)dir * P.lvol_dir_stride + ((int64_t)y * W + x) * P.dp + k0;
 
-    StepIn<DPL, CV> A, B;
-    load_step<DPL, CV>(A, wp);
-    // Advance the load pointers while pixels remain (the last pixel is reloaded, harmlessly).
-    auto advance = [&](int i) {
-        if (i + 1 < n) {
-            if constexpr (CV) {
-                wp.cv += step_l;
-            } else {
-                wp.cl += step_c;
-                wp.cr += step_c;
-            }
-            wp.gd += step_g;
-        }
-    };
-    advance(0);
-    load_step<DPL, CV>(B, wp);
-    advance(1);
+    // Pixels in flight per walk: a ring of K loaded pixels, refilled K pixels ahead as each is used.
+    // Loads past the line's end read other pixels or zeros and are never used.
+    constexpr int K = CV ? 4 : 2;
+    StepIn<DPL, CV> ring[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) load_step<DPL, CV>(ring[j], wk);
 
-    uint32_t prev[R], pm;
+    uint32_t prev[R], pmpm, Iprev;
     int xc = x;
     {
+        const StepIn<DPL, CV> S = ring[0];
+        load_step<DPL, CV>(ring[0], wk);  // pixel K
         uint32_t cc[R];
-        step_costs<DPL, CV>(A, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+        step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -492,49 +579,44 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             mn = pk_min(mn, prev[r]);
         }
         store_bytes<DPL>(lv, prev);
-        pm = pair_min_line<G>(mn);
-    }
-    int Iprev = A.I;
-    auto step = [&](const StepIn<DPL, CV>& S) {
-        xc += dx;
-        lv += step_l;
-        const int dI = S.I > Iprev ? S.I - Iprev : Iprev - S.I;  // stereo.cpp:62-65, |dI|/8 <= 31
-        const int p2 = adaptive ? p2q[dI >> 3] : p2_fixed;
+        pmpm = pair_min_line<G>(mn);
         Iprev = S.I;
-        uint32_t cc[R];
-        step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
-        uint32_t mn;
-        path_step<DPL, PAD, G>(prev, cc, pm, p2, p1p1, nslant, padm, mn, edge);
-        store_bytes<DPL>(lv, prev);
-        pm = pair_min_line<G>(mn);
-    };
-    for (int i = 1; i < n; i += 2) {
-        load_step<DPL, CV>(A, wp);  // pixel i+1 (prefetch)
-        advance(i + 1);
-        step(B);                    // pixel i
-        if (i + 1 >= n) break;
-        load_step<DPL, CV>(B, wp);  // pixel i+2
-        advance(i + 2);
-        step(A);                    // pixel i+1
     }
-}
-
-__device__ __forceinline__ void load_p2q(const SgmConsts& c, SGM_LDS int32_t* p2q) {
-    if (threadIdx.x == 0) {
+    // Pixel i lives in ring slot i % K; i runs from 1 in strides of K, so slot (1 + j) % K holds i + j.
+    for (int i = 1; i < n; i += K) {
 #pragma unroll
-        for (int q = 0; q < 32; ++q) p2q[q] = c.p2_by_q[q];
+        for (int j = 0; j < K; ++j) {
+            if (i + j >= n) break;
+            const StepIn<DPL, CV> S = ring[(1 + j) % K];
+            load_step<DPL, CV>(ring[(1 + j) % K], wk);  // pixel i + j + K
+            xc += dx;
+            lv += step_l;
+            const uint32_t p2p2 = p2tab[__builtin_amdgcn_sad_u8(S.I, Iprev, 0u)];  // |dI|
+            Iprev = S.I;
+            uint32_t cc[R];
+            step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+            uint32_t mn;
+            path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
+            store_bytes<DPL>(lv, prev);
+            pmpm = pair_min_line<G>(mn);
+        }
     }
 }
 
-template <int DPL, bool PAD, bool CV, int G>
+__device__ __forceinline__ void load_p2tab(const SgmConsts& c, SGM_LDS uint32_t* p2tab) {
+    for (int i = threadIdx.x; i < kP2Entries; i += blockDim.x)
+        p2tab[i] = (uint32_t)(c.adaptive_p2 ? c.p2_by_q[i >> 3] : c.p2) * 0x10001u;
+}
+
+template <int DPL, bool PAD, bool CV, int G, bool SLANT>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
-    __shared__ int32_t p2q[32];
-    load_p2q(c, lds(p2q));
+    __shared__ uint32_t p2tab[kP2Entries];
+    load_p2tab(c, lds(p2tab));
     __syncthreads();
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    paths_body<DPL, PAD, CV, G>(P, c, lb, lds(p2q));
+    paths_body<DPL, PAD, CV, G, SLANT>(P, c, lb, lds(p2tab));
 }
 
 // -------------------------------------------------------------------------------- cost volume
@@ -932,16 +1014,22 @@ hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmCon
     if (!paths_group_ok(dpl, group, lines_per_block)) return hipErrorInvalidValue;
     const unsigned threads = (unsigned)(lines_per_block * group);
     const bool cv = (t.n == 1 ? t.one.cvol : nullptr) != nullptr || t.cost_volume;
+    const bool slant = c.slant_steps != 0;
     const dim3 grid(t.total_blocks);
-#define SGM_LAUNCH(D, G)                                                                                       \
-    do {                                                                                                       \
-        if (cv) {                                                                                              \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, true, G>), grid, dim3(threads), 0, stream, t, c);   \
-            else hipLaunchKernelGGL((paths_kernel<D, false, true, G>), grid, dim3(threads), 0, stream, t, c);         \
-        } else {                                                                                               \
-            if (padded) hipLaunchKernelGGL((paths_kernel<D, true, false, G>), grid, dim3(threads), 0, stream, t, c);  \
-            else hipLaunchKernelGGL((paths_kernel<D, false, false, G>), grid, dim3(threads), 0, stream, t, c);        \
-        }                                                                                                      \
+#define SGM_LAUNCH_S(D, PD, CV, G)                                                                          \
+    do {                                                                                                    \
+        if (slant) hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, true>), grid, dim3(threads), 0, stream, t, c);  \
+        else hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, false>), grid, dim3(threads), 0, stream, t, c);       \
+    } while (0)
+#define SGM_LAUNCH(D, G)                                       \
+    do {                                                       \
+        if (cv) {                                              \
+            if (padded) SGM_LAUNCH_S(D, true, true, G);        \
+            else SGM_LAUNCH_S(D, false, true, G);              \
+        } else {                                               \
+            if (padded) SGM_LAUNCH_S(D, true, false, G);       \
+            else SGM_LAUNCH_S(D, false, false, G);             \
+        }                                                      \
     } while (0)
     if (group == 16) {
         switch (dpl) {
@@ -967,6 +1055,7 @@ hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmCon
         }
     }
 #undef SGM_LAUNCH
+#undef SGM_LAUNCH_S
     return hipGetLastError();
 }
 

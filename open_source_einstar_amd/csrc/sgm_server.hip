@@ -60,8 +60,8 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px) {
     o = align_up(o + 2 * npx);
     f.census = o;
     o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8);
-    f.cvol = o;  // matching costs as bytes: the path kernel's cost-volume mode
-    o = align_up(o + npx * dp);
+    f.cvol = o;  // matching costs as bytes: the path kernel's cost-volume mode, where the volume fits a view
+    if (npx * dp <= kMaxViewBytes) o = align_up(o + npx * dp);
     f.lvol = o;
     o = align_up(o + (size_t)ndirs * npx * dp);
     f.disp = o;
@@ -197,7 +197,7 @@ void sgm_server::worker(int si) {
         d.census_l = reinterpret_cast<uint64_t*>(base + f.census);
         d.census_r = d.census_l + census_words_ext(r.w, r.h, r.min_d, r.nd);
         d.lvol = base + f.lvol;
-        d.cvol = base + f.cvol;
+        d.cvol = npx * (size_t)dpl_for(r.nd) * kGroup <= kMaxViewBytes ? base + f.cvol : nullptr;
         set_geometry_ext(d, r.w, r.h, r.min_d, r.nd, ndirs);
         d.disp = reinterpret_cast<float*>(base + f.disp);
         d.median = r.out.disparity_median ? reinterpret_cast<float*>(base + f.median) : nullptr;
@@ -385,7 +385,8 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
     const int in_w = ms.d_maps ? ms.raw_w : rq->width, in_h = ms.d_maps ? ms.raw_h : rq->height;
     if (validate_params_ext(&pp, err) != SGM_OK || bad_rect || rq->width < 1 || rq->height < 1 || rq->width > sv->max_w ||
         rq->height > sv->max_h || rq->num_disparities > sv->max_nd || !rq->left || !rq->right || rq->stride < in_w ||
-        !rq->out.disparity || rq->out.sum_volume || rq->out.cost_volume || rq->out.census_left || rq->out.census_right) {
+        !rq->out.disparity || rq->out.sum_volume || rq->out.cost_volume || rq->out.census_left || rq->out.census_right ||
+        census_words_ext(rq->width, rq->height, rq->min_disparity, rq->num_disparities) * 8 > kMaxViewBytes) {
         std::lock_guard lk(sv->mu);
         sv->err = "sgm_server_submit: size, disparity range, rectification, pointers or outputs not servable" +
                   (err.empty() ? "" : ": " + err);
