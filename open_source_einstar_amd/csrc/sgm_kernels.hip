@@ -683,6 +683,9 @@ __device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {
     return v < -0.5f ? -0.5f : (0.5f < v ? 0.5f : v);
 }
 
+// Pixels per 16-lane group whose path costs are loaded ahead: three where the registers allow.
+constexpr int row_ahead(int dpl) { return dpl <= 8 ? 3 : (dpl <= 16 ? 2 : 1); }
+
 template <int DPL>
 struct RowIn {
     uint32_t w[kMaxDirs][(DPL + 3) / 4];
@@ -817,17 +820,22 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         }
     };
 
-    // Pixels grp, grp + 16, ...: the next pixel's path costs load while this one is processed.
-    RowIn<DPL> A, B;
-    int x = grp;
-    if (x < W) load_row_in<DPL>(A, lrow + (int64_t)x * dp, dstride, ndirs);
-    for (; x < W; x += 2 * kLinesPerBlock) {
-        const int x1 = x + kLinesPerBlock;
-        load_row_in<DPL>(B, lrow + (int64_t)min(x1, W - 1) * dp, dstride, ndirs);
-        process(A, x);
-        if (x1 >= W) break;
-        load_row_in<DPL>(A, lrow + (int64_t)min(x1 + kLinesPerBlock, W - 1) * dp, dstride, ndirs);
-        process(B, x1);
+    // Pixels grp, grp + 16, ...: a ring of kRowAhead pixels' path costs is in flight while one is
+    // processed (reads past the row's end are clamped to its last pixel and never used).
+    constexpr int kRowAhead = row_ahead(DPL);
+    RowIn<DPL> ring[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j)
+        load_row_in<DPL>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * dp, dstride, ndirs);
+    for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
+#pragma unroll
+        for (int j = 0; j < kRowAhead; ++j) {
+            const int xj = x + j * kLinesPerBlock;
+            if (xj >= W) break;
+            const RowIn<DPL> S = ring[j];
+            load_row_in<DPL>(ring[j], lrow + (int64_t)min(xj + kRowAhead * kLinesPerBlock, W - 1) * dp, dstride, ndirs);
+            process(S, xj);
+        }
     }
     __syncthreads();
     // Left/right consistency, stereo.cpp:450-461 (lround: halves away from zero).
