@@ -78,16 +78,25 @@ def build_serve_core_test() -> str:
     return out
 
 
+def build_path_lines_test() -> str:
+    out = os.path.join(ROOT, "tests", "cpp", "path_lines_test")
+    _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-Werror", f"-I{CSRC}", "-o", out,
+          os.path.join(ROOT, "tests", "cpp", "path_lines_test.cpp")])
+    return out
+
+
 def build_all() -> None:
     build_oracle()
     build_synth()
     build_hip()
     build_selftest()
     build_serve_core_test()
+    build_path_lines_test()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
-         "selftest": build_selftest, "serve_core": build_serve_core_test}[w]()
+         "selftest": build_selftest, "serve_core": build_serve_core_test,
+         "path_lines": build_path_lines_test}[w]()

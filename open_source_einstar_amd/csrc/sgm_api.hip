@@ -70,6 +70,7 @@ SgmConsts make_consts(const sgm_params& p) {
     c.p2 = p.p2;
     c.adaptive_p2 = p.adaptive_p2 ? 1 : 0;
     c.slant_steps = p.slant_steps ? 1 : 0;
+    c.ndirs = p.eight_paths ? 8 : 4;
     c.census_rx = p.census_radius_x;
     c.census_ry = p.census_radius_y;
     c.invalid_cost = (2 * p.census_radius_x + 1) * (2 * p.census_radius_y + 1);  // stereo.cpp:366
@@ -143,7 +144,7 @@ void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) {
     d.dpl = dpl_for(nd);
     d.dp = d.dpl * kGroup;
     d.ndirs = ndirs;
-    d.lvol_dir_stride = (int64_t)w * h * d.dp;
+    d.lvol_px_stride = (int64_t)ndirs * d.dp;
     census_layout(w, min_d, d.dp, d.census_padl, d.census_pitch);
     fill_lines(d);
 }

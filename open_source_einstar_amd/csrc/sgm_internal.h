@@ -28,7 +28,7 @@ struct PairDesc {
     const uint8_t* right;
     uint64_t* census_l;
     uint64_t* census_r;
-    uint8_t* lvol;           // [dir][y][x][dp] per-direction path costs L (u8)
+    uint8_t* lvol;           // [y][x][dir][dp] per-direction path costs L (u8)
     uint8_t* cvol;           // [y][x][dp] matching costs C (u8), or null: the path kernel recomputes them
     float* disp;             // sgm_disparity output (post left/right check)
     float* median;           // median3(disp)
@@ -43,7 +43,7 @@ struct PairDesc {
     int64_t pitch;           // input row pitch in bytes
     int32_t census_pitch;    // census words per row: pads, then the w words of the row, then pads
     int32_t census_padl;     // words before x = 0 (every disparity window stays inside the row)
-    int64_t lvol_dir_stride; // bytes between direction planes of lvol
+    int64_t lvol_px_stride;  // bytes per pixel of lvol: ndirs * dp
     int32_t w, h;
     int32_t min_d, nd, dpl, dp;
     int32_t ndirs;
@@ -54,6 +54,7 @@ struct PairDesc {
 // Parameters shared by every pair of a launch (depth::SgmParams minus the disparity range).
 struct SgmConsts {
     int32_t p1, p2, adaptive_p2, slant_steps;
+    int32_t ndirs;  // 4 or 8 path directions
     int32_t census_rx, census_ry, invalid_cost;
     int32_t lr_max_diff, subpixel;
     float uniqueness;

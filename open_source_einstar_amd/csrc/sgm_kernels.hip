@@ -4,7 +4,7 @@
 // 361-463); the layout and the parallel decomposition are this library's own:
 //
 //   census   one thread per pixel, LDS tile with halo                  -> census words u64
-//   paths    one 16-lane DPP row (or two joined rows) per scanline,    -> L_dir u8 [dir][y][x][dp]
+//   paths    one 16-lane DPP row (or two joined rows) per scanline,    -> L u8 [y][x][dir][dp]
 //            all directions of all pairs in one launch; a lane holds
 //            consecutive disparities as packed u16 pairs; the costs are
 //            read from a u8 cost volume or recomputed from the census
@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "path_lines.h"
 #include "sgm_internal.h"
 
 namespace sgm {
@@ -383,29 +384,44 @@ __device__ __forceinline__ void step_costs(const StepIn<DPL, CV>& s, bool full, 
     }
 }
 
-// Low bytes of the DPL/2 packed pairs -> DPL bytes at dst (aligned to DPL's power-of-two part).
+// A pixel's DPL path-cost bytes, packed from the lane's DPL/2 u16 pairs (the low byte of each half).
 template <int DPL>
-__device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
-    if constexpr (DPL % 4 == 0) {
-        uint32_t b[DPL / 4];
+struct Packed {
+    uint32_t w[(DPL + 3) / 4];
+};
+template <int DPL>
+__device__ __forceinline__ Packed<DPL> pack_bytes(const uint32_t (&v)[DPL / 2]) {
+    Packed<DPL> o;
 #pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) b[i] = __builtin_amdgcn_perm(v[2 * i + 1], v[2 * i], 0x06040200u);
-        if constexpr (DPL % 16 == 0) {
+    for (int i = 0; i < DPL / 4; ++i) o.w[i] = __builtin_amdgcn_perm(v[2 * i + 1], v[2 * i], 0x06040200u);
+    if constexpr (DPL % 4 == 2) o.w[DPL / 4] = (v[DPL / 2 - 1] & 0xFFu) | ((v[DPL / 2 - 1] >> 8) & 0xFF00u);
+    return o;
+}
+// DPL bytes at dst (aligned to DPL's power-of-two part).
+template <int DPL>
+__device__ __forceinline__ void store_packed(SGM_GLOBAL uint8_t* dst, const Packed<DPL>& b) {
+    if constexpr (DPL % 16 == 0) {
 #pragma unroll
-            for (int i = 0; i < DPL / 16; ++i)
-                reinterpret_cast<SGM_GLOBAL u32x4*>(dst)[i] = u32x4{b[4 * i], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]};
-        } else if constexpr (DPL % 8 == 0) {
+        for (int i = 0; i < DPL / 16; ++i)
+            reinterpret_cast<SGM_GLOBAL u32x4*>(dst)[i] = u32x4{b.w[4 * i], b.w[4 * i + 1], b.w[4 * i + 2], b.w[4 * i + 3]};
+    } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
-            for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<SGM_GLOBAL u32x2*>(dst)[i] = u32x2{b[2 * i], b[2 * i + 1]};
-        } else {
+        for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<SGM_GLOBAL u32x2*>(dst)[i] = u32x2{b.w[2 * i], b.w[2 * i + 1]};
+    } else if constexpr (DPL % 4 == 0) {
 #pragma unroll
-            for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<SGM_GLOBAL uint32_t*>(dst)[i] = b[i];
-        }
+        for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<SGM_GLOBAL uint32_t*>(dst)[i] = b.w[i];
     } else {
 #pragma unroll
-        for (int i = 0; i < DPL / 2; ++i)
-            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[i] = (uint16_t)((v[i] & 0xFFu) | ((v[i] >> 8) & 0xFF00u));
+        for (int i = 0; i < DPL / 4; ++i) {
+            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[2 * i] = (uint16_t)b.w[i];
+            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[2 * i + 1] = (uint16_t)(b.w[i] >> 16);
+        }
+        reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[DPL / 2 - 1] = (uint16_t)b.w[DPL / 4];
     }
+}
+template <int DPL>
+__device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
+    store_packed<DPL>(dst, pack_bytes<DPL>(v));
 }
 
 // Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value (or zero) into
@@ -497,29 +513,9 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             off = P.line_off[d];
         }
     const int li = line - off;
-    // Direction order of stereo.cpp:387-388.
-    const int dx = (dir == 0 || dir == 4 || dir == 6) ? 1 : ((dir == 1 || dir == 5 || dir == 7) ? -1 : 0);
-    const int dy = (dir == 2 || dir == 4 || dir == 7) ? 1 : ((dir == 3 || dir == 5 || dir == 6) ? -1 : 0);
+    const int dx = path_dx(dir), dy = path_dy(dir);
     const int W = P.w, H = P.h;
-    int x, y;  // start pixel, in the order of stereo.cpp:391-402
-    {
-        const int ncol = dx != 0 ? H : 0;
-        if (li < ncol) {
-            x = dx > 0 ? 0 : W - 1;
-            y = li;
-        } else {
-            int j = li - ncol;
-            if (dx != 0 && j >= (dx > 0 ? 0 : W - 1)) ++j;
-            x = j;
-            y = dy > 0 ? 0 : H - 1;
-        }
-    }
-    // Pixels on the line until it leaves the image.
-    int n = 0x7FFFFFFF;
-    if (dx > 0) n = W - x;
-    if (dx < 0) n = x + 1;
-    if (dy > 0) n = min(n, H - y);
-    if (dy < 0) n = min(n, y + 1);
+    const PathLine ln = path_line(dir, li, W, H);
 
     const int g = threadIdx.x & (G - 1);
     LineEdge edge;
@@ -538,6 +534,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     for (int r = 0; r < R; ++r)
         padm[r] = PAD ? ((k0 + 2 * r >= P.nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= P.nd ? 0x7FFF0000u : 0u)) : 0u;
 
+    const int x = ln.x, y = ln.y, n = ln.n;
     // Load positions (the census rows carry pads, so no window word leaves its row).
     Walk wk;
     const int64_t cp = P.census_pitch;
@@ -555,23 +552,29 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     wk.gd = buffer_view(P.left, (uint64_t)P.pitch * (H - 1) + W);
     wk.o_gd = (uint32_t)((int64_t)y * P.pitch + x);
     wk.s_gd = (uint32_t)((int64_t)dy * P.pitch + dx);
-    const int64_t step_l = ((int64_t)dy * W + dx) * P.dp;
-    SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + (int64_t)dir * P.lvol_dir_stride + ((int64_t)y * W + x) * P.dp + k0;
+    // Path costs are stored [y][x][dir][dp]: the row kernel then reads one pixel's directions as one
+    // contiguous run, and each store here is still a whole dp-byte run.
+    const int64_t step_l = ((int64_t)dy * W + dx) * P.lvol_px_stride;
+    SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + ((int64_t)y * W + x) * P.lvol_px_stride + (int64_t)dir * P.dp + k0;
 
-    // Pixels in flight per walk: a ring of K loaded pixels, refilled K pixels ahead as each is used.
-    // Loads past the line's end read other pixels or zeros and are never used.
+    // Loads and stores. Two banks of K pixels alternate: while one bank's pixels are computed, the
+    // other's loads are in flight. A bank is refilled (2K pixels further on) right after its
+    // pixels are computed and before their path costs are stored, so on this hardware, where
+    // loads and stores retire through one in-order counter, the wait for a bank's loads never
+    // waits for a recent store. Loads past the line's end read other pixels or zeros and are
+    // never used.
     constexpr int K = CV ? 4 : 2;
-    StepIn<DPL, CV> ring[K];
+    StepIn<DPL, CV> first, bank_a[K], bank_b[K];
+    load_step<DPL, CV>(first, wk);  // pixel 0
 #pragma unroll
-    for (int j = 0; j < K; ++j) load_step<DPL, CV>(ring[j], wk);
+    for (int j = 0; j < K; ++j) load_step<DPL, CV>(bank_a[j], wk);  // pixels 1 .. K
+#pragma unroll
+    for (int j = 0; j < K; ++j) load_step<DPL, CV>(bank_b[j], wk);  // pixels K + 1 .. 2K
 
     uint32_t prev[R], pmpm, Iprev;
-    int xc = x;
     {
-        const StepIn<DPL, CV> S = ring[0];
-        load_step<DPL, CV>(ring[0], wk);  // pixel K
         uint32_t cc[R];
-        step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+        step_costs<DPL, CV>(first, x >= full_lo && x <= full_hi, x - min_d - k0, W, inv, cc);
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -580,26 +583,37 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         }
         store_bytes<DPL>(lv, prev);
         pmpm = pair_min_line<G>(mn);
-        Iprev = S.I;
+        Iprev = first.I;
     }
-    // Pixel i lives in ring slot i % K; i runs from 1 in strides of K, so slot (1 + j) % K holds i + j.
-    for (int i = 1; i < n; i += K) {
+    // Pixels base .. base + K - 1 from bank bk (those before the line's end), then the refill,
+    // then the stores. lvh addresses pixel base.
+    SGM_GLOBAL uint8_t* lvh = lv + step_l;
+    auto half = [&](StepIn<DPL, CV>(&bk)[K], const int base) {
+        Packed<DPL> out[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            if (i + j >= n) break;
-            const StepIn<DPL, CV> S = ring[(1 + j) % K];
-            load_step<DPL, CV>(ring[(1 + j) % K], wk);  // pixel i + j + K
-            xc += dx;
-            lv += step_l;
-            const uint32_t p2p2 = p2tab[__builtin_amdgcn_sad_u8(S.I, Iprev, 0u)];  // |dI|
-            Iprev = S.I;
-            uint32_t cc[R];
-            step_costs<DPL, CV>(S, xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
-            uint32_t mn;
-            path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
-            store_bytes<DPL>(lv, prev);
-            pmpm = pair_min_line<G>(mn);
+            if (base + j < n) {
+                const int xc = x + (base + j) * dx;
+                const uint32_t p2p2 = p2tab[__builtin_amdgcn_sad_u8(bk[j].I, Iprev, 0u)];  // |dI|
+                Iprev = bk[j].I;
+                uint32_t cc[R];
+                step_costs<DPL, CV>(bk[j], xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+                uint32_t mn;
+                path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
+                out[j] = pack_bytes<DPL>(prev);
+                pmpm = pair_min_line<G>(mn);
+            }
         }
+#pragma unroll
+        for (int j = 0; j < K; ++j) load_step<DPL, CV>(bk[j], wk);  // pixels base + 2K ..
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
+        lvh += K * step_l;
+    };
+    for (int i = 1; i < n; i += 2 * K) {
+        half(bank_a, i);
+        half(bank_b, i + K);
     }
 }
 
@@ -691,11 +705,12 @@ struct RowIn {
     uint32_t w[kMaxDirs][(DPL + 3) / 4];
 };
 
-template <int DPL>
-__device__ __forceinline__ void load_row_in(RowIn<DPL>& in, const SGM_GLOBAL uint8_t* src, int64_t dir_stride, int ndirs) {
+// The direction count is a template parameter so that every load is unconditional: the compiler can
+// then wait for each ring slot alone instead of for every load in flight.
+template <int DPL, int NDIRS>
+__device__ __forceinline__ void load_row_in(RowIn<DPL>& in, const SGM_GLOBAL uint8_t* src, int64_t dir_stride) {
 #pragma unroll
-    for (int d = 0; d < kMaxDirs; ++d)
-        if (d < ndirs) load_raw<DPL>(src + d * dir_stride, in.w[d]);
+    for (int d = 0; d < NDIRS; ++d) load_raw<DPL>(src + d * dir_stride, in.w[d]);
 }
 
 __device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
@@ -710,7 +725,7 @@ __device__ __forceinline__ uint32_t row_max(uint32_t v) {
 
 // Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
 // 9 bytes per column.
-template <int DPL, bool PAD>
+template <int DPL, bool PAD, int NDIRS>
 __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
     const int W = P.w, y = lb;
     SGM_LDS uint32_t* rkey = smem;
@@ -723,18 +738,16 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     const int k0 = g * DPL;
     const int nd = P.nd, min_d = P.min_d;
     const int64_t dp = P.dp;
-    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * dp + k0;
+    const int64_t pxs = P.lvol_px_stride;  // [y][x][dir][dp]: a pixel's directions are contiguous
+    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + k0;
     const int64_t row0 = (int64_t)y * W;
-    const int ndirs = P.ndirs;
-    const int64_t dstride = P.lvol_dir_stride;
 
     auto process = [&](const RowIn<DPL>& in, int x) {
         uint32_t s[DPL / 2];
 #pragma unroll
         for (int r = 0; r < DPL / 2; ++r) s[r] = 0;
 #pragma unroll
-        for (int d = 0; d < kMaxDirs; ++d)
-            if (d < ndirs) add_raw<DPL>(in.w[d], s);
+        for (int d = 0; d < NDIRS; ++d) add_raw<DPL>(in.w[d], s);
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -826,15 +839,15 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     RowIn<DPL> ring[kRowAhead];
 #pragma unroll
     for (int j = 0; j < kRowAhead; ++j)
-        load_row_in<DPL>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * dp, dstride, ndirs);
+        load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * pxs, dp);
     for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
             const int xj = x + j * kLinesPerBlock;
             if (xj >= W) break;
-            const RowIn<DPL> S = ring[j];
-            load_row_in<DPL>(ring[j], lrow + (int64_t)min(xj + kRowAhead * kLinesPerBlock, W - 1) * dp, dstride, ndirs);
-            process(S, xj);
+            // Refilled after use, so the slot keeps its registers and no load is waited for early.
+            process(ring[j], xj);
+            load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(xj + kRowAhead * kLinesPerBlock, W - 1) * pxs, dp);
         }
     }
     __syncthreads();
@@ -860,13 +873,13 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     }
 }
 
-template <int DPL, bool PAD>
+template <int DPL, bool PAD, int NDIRS>
 __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
     extern __shared__ uint32_t smem[];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    rows_body<DPL, PAD>(P, c, lb, lds(smem));
+    rows_body<DPL, PAD, NDIRS>(P, c, lb, lds(smem));
 }
 
 // ------------------------------------------------------------------------------------ median
@@ -980,15 +993,10 @@ int dpl_for(int nd) {
 }
 
 void fill_lines(PairDesc& d) {
-    static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}, {1, -1}, {-1, 1}};
     int total = 0;
     for (int i = 0; i < kMaxDirs; ++i) {
         d.line_off[i] = total;
-        if (i >= d.ndirs) continue;
-        int n = 0;
-        if (dirs[i][0] != 0) n += d.h;
-        if (dirs[i][1] != 0) n += dirs[i][0] != 0 ? d.w - 1 : d.w;
-        total += n;
+        if (i < d.ndirs) total += path_lines(i, d.w, d.h);
     }
     d.total_lines = total;
 }
@@ -1084,15 +1092,22 @@ hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipSt
 
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
+    if (c.ndirs != 4 && c.ndirs != 8) return hipErrorInvalidValue;
     const size_t lds = (size_t)max_w * 9 + 16;
     switch (dpl) {
-#define SGM_CASE(D)                                                                                                \
-    case D:                                                                                                        \
-        if (padded) hipLaunchKernelGGL((rows_kernel<D, true>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);    \
-        else hipLaunchKernelGGL((rows_kernel<D, false>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);          \
+#define SGM_ROWS(D, PD)                                                                                                  \
+    do {                                                                                                                 \
+        if (c.ndirs == 8) hipLaunchKernelGGL((rows_kernel<D, PD, 8>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);   \
+        else hipLaunchKernelGGL((rows_kernel<D, PD, 4>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);                \
+    } while (0)
+#define SGM_CASE(D)                  \
+    case D:                          \
+        if (padded) SGM_ROWS(D, true); \
+        else SGM_ROWS(D, false);     \
         break;
         SGM_FOR_DPL(SGM_CASE)
 #undef SGM_CASE
+#undef SGM_ROWS
         default:
             return hipErrorInvalidValue;
     }
