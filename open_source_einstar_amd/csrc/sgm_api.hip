@@ -351,7 +351,8 @@ static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int st
     size_t& b = ctx->bytes;
     if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * std::max(npx, spx), b);
     ctx->census_half = census_words(sgm_w, sgm_h, params->min_disparity, params->num_disparities);
-    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half, b);
+    // Census words of both images, then the left image's step-penalty words (4 bytes per pixel).
+    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half + (npx + 1) / 2, b);
     if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
     // Cost-volume mode is the default wherever the volume fits one buffer view.
     ctx->use_cvol = npx * dp <= kMaxViewBytes;
@@ -582,9 +583,6 @@ static int check_size(sgm_context* ctx, int width, int height) {
     return SGM_OK;
 }
 
-// A device image the path kernel reads through a buffer view (its guide intensities).
-static bool pitch_fits(int64_t pitch, int w, int h) { return (uint64_t)pitch * (uint64_t)(h - 1) + (uint64_t)w <= kMaxViewBytes; }
-
 // Builds the descriptor for one pair over the context's scratch; outputs default to the context's.
 static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t* dr, int64_t pitch, int w, int h,
                              const sgm_outputs* o) {
@@ -594,6 +592,7 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
     d.pitch = pitch;
     d.census_l = ctx->d_census;
     d.census_r = ctx->d_census + ctx->census_half;
+    d.edge = reinterpret_cast<uint32_t*>(ctx->d_census + 2 * ctx->census_half);
     d.lvol = ctx->d_lvol;
     d.cvol = ctx->use_cvol ? ctx->d_cvol : nullptr;
     set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs);
@@ -682,7 +681,6 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     }
     if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
     if (!d_left || !d_right || pitch < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
-    if (!pitch_fits(pitch, width, height)) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "pitch x height exceeds 4 GiB - 1");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     PairDesc d = context_desc(ctx, d_left, d_right, pitch, width, height, d_out);
     if (d_out && (d_out->census_left || d_out->census_right))
@@ -1208,7 +1206,6 @@ int sgm_compute_disparity_device(sgm_context* ctx, const uint8_t* d_left, const 
     if (int rc = check_stereo_size(ctx, width, height); rc != SGM_OK) return rc;
     if (!d_left || !d_right || pitch < width || !d_disparity)
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or output, or pitch < width");
-    if (!pitch_fits(pitch, width, height)) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "pitch x height exceeds 4 GiB - 1");
     const bool full = ctx->sp.detail_enabled && d_full_left && d_full_right && full_width == 2 * width &&
                       full_height == 2 * height;
     if (full && full_pitch < full_width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "full_pitch < full_width");
@@ -1736,9 +1733,8 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
             g_last_error = "pair " + std::to_string(i) + ": census and cost-volume outputs are only available from sgm_compute";
             return SGM_ERR_UNSUPPORTED;
         }
-        if (census_words(q.width, q.height, q.min_disparity, q.num_disparities) * 8 > kMaxViewBytes ||
-            !pitch_fits(q.pitch, q.width, q.height)) {
-            g_last_error = "pair " + std::to_string(i) + ": census words or pitch x height exceed 4 GiB - 1";
+        if (census_words(q.width, q.height, q.min_disparity, q.num_disparities) * 8 > kMaxViewBytes) {
+            g_last_error = "pair " + std::to_string(i) + ": the census words of one image exceed 4 GiB - 1";
             return SGM_ERR_INVALID_ARGUMENT;
         }
     }
@@ -1762,8 +1758,8 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         const size_t npx = (size_t)pairs[i].width * pairs[i].height;
         const size_t dp = (size_t)dpl_for(pairs[i].num_disparities) * kGroup;
         off_census[i] = total;
-        total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 + 255) &
-                 ~(size_t)255;
+        total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 +
+                  npx * 4 + 255) & ~(size_t)255;  // census words of both images, then the step-penalty words
         off_cvol[i] = total;
         if (cv_fits) total += (npx * dp + 255) & ~(size_t)255;
         off_lvol[i] = total;
@@ -1784,6 +1780,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         d.pitch = q.pitch;
         d.census_l = reinterpret_cast<uint64_t*>(b->d_scratch + off_census[i]);
         d.census_r = d.census_l + census_words(q.width, q.height, q.min_disparity, q.num_disparities);
+        d.edge = reinterpret_cast<uint32_t*>(d.census_r + census_words(q.width, q.height, q.min_disparity, q.num_disparities));
         d.cvol = cv_fits ? b->d_scratch + off_cvol[i] : nullptr;
         d.lvol = b->d_scratch + off_lvol[i];
         set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);

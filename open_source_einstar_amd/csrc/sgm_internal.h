@@ -15,9 +15,10 @@ constexpr int kLinesPerBlock = kBlock / kGroup;
 constexpr int kMaxBatch = 64;                 // pairs per launch table
 constexpr int kMaxDirs = 8;
 constexpr int kMaxWidth = 4096;               // the row kernel keeps 9 bytes of LDS per column
-// The path kernel reads the census rows, the cost volume and the guide image through buffer views
-// addressed by 32-bit byte offsets, so each of them must stay within 4 GiB - 1. Census rows and
-// guide images beyond that are refused; a cost volume beyond it is not used (costs are recomputed).
+// The path kernel reads the census rows, the step penalties and the cost volume through buffer
+// views addressed by 32-bit byte offsets, so each of them must stay within 4 GiB - 1. Census rows
+// beyond that are refused (the penalties are half their size); a cost volume beyond it is not used
+// (costs are recomputed).
 constexpr uint64_t kMaxViewBytes = 0xFFFFFFFFull;
 constexpr int kMaxLevels = 4;                 // pyramid levels below the full resolution
 
@@ -28,6 +29,7 @@ struct PairDesc {
     const uint8_t* right;
     uint64_t* census_l;
     uint64_t* census_r;
+    uint32_t* edge;          // [y][x] left-image step penalties of the four path axes, one byte each
     uint8_t* lvol;           // [y][x][dir][dp] per-direction path costs L (u8)
     uint8_t* cvol;           // [y][x][dp] matching costs C (u8), or null: the path kernel recomputes them
     float* disp;             // sgm_disparity output (post left/right check)

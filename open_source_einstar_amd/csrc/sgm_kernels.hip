@@ -19,6 +19,7 @@
 // per-direction split of the sum do not change any result.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "path_lines.h"
 #include "sgm_internal.h"
@@ -127,14 +128,21 @@ __device__ __forceinline__ SGM_LDS T* lds(T* p) {
 // ------------------------------------------------------------------------------------ census
 // census_transform, stereo.cpp:328-346: rows outer, columns inner, centre skipped, each compare
 // shifted in at bit 0 (the first compare ends at bit cells-2); border pixels are 0.
+//
+// The left image's tiles also write each pixel's step penalties (stereo.cpp:62-65) for the four
+// path axes r = (1, 0), (0, 1), (1, 1), (1, -1) as four bytes: byte k is the penalty of the step
+// into this pixel from p - r_k, max(P1 + 1, P2 / (|dI|/8 + 1)) when adaptive, P2 otherwise (0
+// where p - r_k leaves the image; no path steps from there). A step along -r_k uses byte k of the
+// pixel it leaves. Every penalty fits a byte (cells + penalty <= 255 is enforced).
 
 constexpr int kTile = 16;
 constexpr int kMaxRx = 4, kMaxRy = 3;
 constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
 
 // One 16x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
-// the block calls it (it synchronises once).
-__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint8_t* tile) {
+// the block calls it (it synchronises once). The tile keeps a halo of at least one pixel.
+__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint8_t* tile,
+                                            SGM_LDS int32_t* p2q) {
     const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
     const int side = lb / (tx_n * ty_n);
     const int tl = lb - side * tx_n * ty_n;
@@ -142,36 +150,47 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
     const uint8_t* img = side ? P.right : P.left;
     uint64_t* out = side ? P.census_r : P.census_l;
     const int rx = c.census_rx, ry = c.census_ry;
-    const int tw = kTile + 2 * rx, th = kTile + 2 * ry;
+    const int hx = max(rx, 1), hy = max(ry, 1);
+    const int tw = kTile + 2 * hx, th = kTile + 2 * hy;
     for (int i = threadIdx.x; i < tw * th; i += kBlock) {
         const int ly = i / tw, lx = i - ly * tw;
-        const int gx = x0 - rx + lx, gy = y0 - ry + ly;
+        const int gx = x0 - hx + lx, gy = y0 - hy + ly;
         tile[ly * kTileW + lx] = (gx >= 0 && gy >= 0 && gx < P.w && gy < P.h) ? img[(int64_t)gy * P.pitch + gx] : 0;
     }
+    if (threadIdx.x < 32) p2q[threadIdx.x] = c.adaptive_p2 ? c.p2_by_q[threadIdx.x] : c.p2;
     __syncthreads();
     const int lx = threadIdx.x & (kTile - 1), ly = threadIdx.x >> 4;
     const int x = x0 + lx, y = y0 + ly;
     if (x >= P.w || y >= P.h) return;
+    const SGM_LDS uint8_t* ctr = &tile[(ly + hy) * kTileW + lx + hx];
     uint64_t bits = 0;
     if (x >= rx && x < P.w - rx && y >= ry && y < P.h - ry) {
-        const uint8_t center = tile[(ly + ry) * kTileW + lx + rx];
-        for (int v = 0; v <= 2 * ry; ++v) {
-            const SGM_LDS uint8_t* row = &tile[(ly + v) * kTileW + lx];
+        const uint8_t center = *ctr;
+        for (int v = -ry; v <= ry; ++v) {
+            const SGM_LDS uint8_t* row = ctr + v * kTileW - rx;
             for (int u = 0; u <= 2 * rx; ++u) {
-                if (u == rx && v == ry) continue;
+                if (u == rx && v == 0) continue;
                 bits = (bits << 1) | (uint64_t)(row[u] < center);
             }
         }
     }
     out[(int64_t)y * P.census_pitch + P.census_padl + x] = bits;
+    if (side == 0) {
+        const int I = *ctr;
+        auto pen = [&](int q, bool inside) { return inside ? (uint32_t)p2q[abs(I - q) >> 3] : 0u; };
+        const uint32_t e = pen(ctr[-1], x > 0) | (pen(ctr[-kTileW], y > 0) << 8) | (pen(ctr[-kTileW - 1], x > 0 && y > 0) << 16) |
+                           (pen(ctr[kTileW - 1], x > 0 && y + 1 < P.h) << 24);
+        P.edge[(int64_t)y * P.w + x] = e;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
     __shared__ uint8_t tile[kTileH * kTileW];
+    __shared__ int32_t p2q[32];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    census_body(P, c, lb, lds(tile));
+    census_body(P, c, lb, lds(tile), lds(p2q));
 }
 
 // ------------------------------------------------------------------------------- cost volume
@@ -270,28 +289,29 @@ __device__ __forceinline__ BufView buffer_view(const void* base, uint64_t bytes)
     return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)n, kRawBufferWord3);
 }
 
-// Everything one step of a path reads. Recomputed costs (CV = false): the left census word, the
-// lane's DPL right census words (lowest address first, i.e. disparity k0+DPL-1 first) and the guide
-// intensity. Cost volume (CV = true): the lane's DPL cost bytes and the guide intensity.
+// Everything one step of a path reads. Recomputed costs (CV = false): the left census word and the
+// lane's DPL right census words (lowest address first, i.e. disparity k0+DPL-1 first). Cost volume
+// (CV = true): the lane's DPL cost bytes. Both: the step-penalty word of the pixel whose byte
+// holds this step's penalty.
 template <int DPL, bool CV>
 struct StepIn;
 template <int DPL>
 struct StepIn<DPL, false> {
     uint64_t a;
     uint64_t w[DPL];
-    uint32_t I;
+    uint32_t E;  // step penalties of the pixel's axes (census kernel)
 };
 template <int DPL>
 struct StepIn<DPL, true> {
     uint32_t c[(DPL + 3) / 4];
-    uint32_t I;
+    uint32_t E;
 };
 
 // A walk's load positions (byte offsets into the views) and their per-pixel steps.
 struct Walk {
-    BufView cv, cl, cr, gd;
-    uint32_t o_cv, o_cl, o_cr, o_gd;
-    uint32_t s_cv, s_c, s_gd;
+    BufView cv, cl, cr, ed;
+    uint32_t o_cv, o_cl, o_cr, o_e;
+    uint32_t s_cv, s_c, s_e;
 };
 
 __device__ __forceinline__ uint64_t u64_of(u32x2 v) { return (uint64_t)v.x | ((uint64_t)v.y << 32); }
@@ -338,8 +358,8 @@ __device__ __forceinline__ void load_step(StepIn<DPL, CV>& s, Walk& wk) {
         wk.o_cl += wk.s_c;
         wk.o_cr += wk.s_c;
     }
-    s.I = __builtin_amdgcn_raw_buffer_load_b8(wk.gd, wk.o_gd, 0, 0);
-    wk.o_gd += wk.s_gd;
+    s.E = __builtin_amdgcn_raw_buffer_load_b32(wk.ed, wk.o_e, 0, 0);
+    wk.o_e += wk.s_e;
 }
 
 // Packed costs of the lane's disparities k0 + 2r, k0 + 2r + 1 (low, high half). Word w[DPL-1-j]
@@ -495,13 +515,9 @@ __device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
     return v;
 }
 
-// The step penalty for every |dI| in [0, 255], in both halves: max(P1+1, P2/(|dI|/8 + 1)) when
-// adaptive (stereo.cpp:62-65), P2 otherwise.
-constexpr int kP2Entries = 256;
-
 // Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
 template <int DPL, bool PAD, bool CV, int G, bool SLANT>
-__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb, const SGM_LDS uint32_t* p2tab) {
+__device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb) {
     constexpr int R = DPL / 2;
     const int line = lb * (int)(blockDim.x / G) + (int)(threadIdx.x / G);  // 4, 8 or 16 lines per block
     if (line >= P.total_lines) return;
@@ -549,9 +565,15 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         wk.o_cr = (uint32_t)(((int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1))) * 8);
         wk.s_c = (uint32_t)(((int64_t)dy * cp + dx) * 8);
     }
-    wk.gd = buffer_view(P.left, (uint64_t)P.pitch * (H - 1) + W);
-    wk.o_gd = (uint32_t)((int64_t)y * P.pitch + x);
-    wk.s_gd = (uint32_t)((int64_t)dy * P.pitch + dx);
+    // Step penalties: byte dir/2 of the word of the pixel stepped into (directions 0, 2, 4, 6 run
+    // along their axis) or of the pixel stepped from (1, 3, 5, 7 run against it), selected into
+    // both halves.
+    const bool along = (dir & 1) == 0;
+    const uint32_t axis = (uint32_t)(dir >> 1);
+    const uint32_t pen_sel = 0x0C000C00u | (axis << 16) | axis;
+    wk.ed = buffer_view(P.edge, (uint64_t)W * H * 4);
+    wk.o_e = (uint32_t)((((int64_t)y * W + x) - (along ? 0 : ((int64_t)dy * W + dx))) * 4);
+    wk.s_e = (uint32_t)(((int64_t)dy * W + dx) * 4);
     // Path costs are stored [y][x][dir][dp]: the row kernel then reads one pixel's directions as one
     // contiguous run, and each store here is still a whole dp-byte run.
     const int64_t step_l = ((int64_t)dy * W + dx) * P.lvol_px_stride;
@@ -571,7 +593,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
 #pragma unroll
     for (int j = 0; j < K; ++j) load_step<DPL, CV>(bank_b[j], wk);  // pixels K + 1 .. 2K
 
-    uint32_t prev[R], pmpm, Iprev;
+    uint32_t prev[R], pmpm;
     {
         uint32_t cc[R];
         step_costs<DPL, CV>(first, x >= full_lo && x <= full_hi, x - min_d - k0, W, inv, cc);
@@ -583,19 +605,22 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         }
         store_bytes<DPL>(lv, prev);
         pmpm = pair_min_line<G>(mn);
-        Iprev = first.I;
     }
-    // Pixels base .. base + K - 1 from bank bk (those before the line's end), then the refill,
-    // then the stores. lvh addresses pixel base.
-    SGM_GLOBAL uint8_t* lvh = lv + step_l;
-    auto half = [&](StepIn<DPL, CV>(&bk)[K], const int base) {
+    // Pixels base .. base + K - 1 from bank bk, then the bank's refill, then the stores. In the main
+    // loop every pixel of a half lies on the line (full), so its stores are unconditional and each
+    // wait counts exactly the loads and stores issued since; the tail half tests each pixel.
+    SGM_GLOBAL uint8_t* lvh = lv + step_l;  // pixel base
+    auto half = [&](auto full, StepIn<DPL, CV>(&bk)[K], const int base) {
+        constexpr bool kFull = decltype(full)::value;
+        // Keeps the scheduler from hoisting the other bank's unpacking (and with it the wait for
+        // loads still in flight) into this half.
+        __builtin_amdgcn_sched_barrier(0);
         Packed<DPL> out[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            if (base + j < n) {
+            if (kFull || base + j < n) {
                 const int xc = x + (base + j) * dx;
-                const uint32_t p2p2 = p2tab[__builtin_amdgcn_sad_u8(bk[j].I, Iprev, 0u)];  // |dI|
-                Iprev = bk[j].I;
+                const uint32_t p2p2 = __builtin_amdgcn_perm(0u, bk[j].E, pen_sel);
                 uint32_t cc[R];
                 step_costs<DPL, CV>(bk[j], xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
                 uint32_t mn;
@@ -604,33 +629,29 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
                 pmpm = pair_min_line<G>(mn);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < K; ++j) load_step<DPL, CV>(bk[j], wk);  // pixels base + 2K ..
 #pragma unroll
         for (int j = 0; j < K; ++j)
-            if (base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
+            if (kFull || base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
         lvh += K * step_l;
     };
-    for (int i = 1; i < n; i += 2 * K) {
-        half(bank_a, i);
-        half(bank_b, i + K);
+    int i = 1;
+    for (; i + 2 * K <= n; i += 2 * K) {
+        half(std::true_type{}, bank_a, i);
+        half(std::true_type{}, bank_b, i + K);
     }
-}
-
-__device__ __forceinline__ void load_p2tab(const SgmConsts& c, SGM_LDS uint32_t* p2tab) {
-    for (int i = threadIdx.x; i < kP2Entries; i += blockDim.x)
-        p2tab[i] = (uint32_t)(c.adaptive_p2 ? c.p2_by_q[i >> 3] : c.p2) * 0x10001u;
+    if (i < n) half(std::false_type{}, bank_a, i);
+    if (i + K < n) half(std::false_type{}, bank_b, i + K);
 }
 
 template <int DPL, bool PAD, bool CV, int G, bool SLANT>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
-    __shared__ uint32_t p2tab[kP2Entries];
-    load_p2tab(c, lds(p2tab));
-    __syncthreads();
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    paths_body<DPL, PAD, CV, G, SLANT>(P, c, lb, lds(p2tab));
+    paths_body<DPL, PAD, CV, G, SLANT>(P, c, lb);
 }
 
 // -------------------------------------------------------------------------------- cost volume

@@ -59,7 +59,7 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px) {
     f.img = o;
     o = align_up(o + 2 * npx);
     f.census = o;
-    o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8);
+    o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8 + npx * 4);  // census words, then step-penalty words
     f.cvol = o;  // matching costs as bytes: the path kernel's cost-volume mode, where the volume fits a view
     if (npx * dp <= kMaxViewBytes) o = align_up(o + npx * dp);
     f.lvol = o;
@@ -196,6 +196,7 @@ void sgm_server::worker(int si) {
         d.pitch = r.w;
         d.census_l = reinterpret_cast<uint64_t*>(base + f.census);
         d.census_r = d.census_l + census_words_ext(r.w, r.h, r.min_d, r.nd);
+        d.edge = reinterpret_cast<uint32_t*>(d.census_r + census_words_ext(r.w, r.h, r.min_d, r.nd));
         d.lvol = base + f.lvol;
         d.cvol = npx * (size_t)dpl_for(r.nd) * kGroup <= kMaxViewBytes ? base + f.cvol : nullptr;
         set_geometry_ext(d, r.w, r.h, r.min_d, r.nd, ndirs);
