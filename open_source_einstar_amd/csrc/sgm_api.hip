@@ -477,6 +477,11 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
                 return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 16 or 32");
             ctx->path_group = (int)value;
             return SGM_OK;
+        case SGM_OPT_RECOMPUTE_DIRS:
+            if (value < 0 || value > 8)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "directions recomputing their costs must be 0..8");
+            ctx->consts.cv_from_dir = (int32_t)value;
+            return SGM_OK;
         default:
             return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));
     }

@@ -57,6 +57,7 @@ struct PairDesc {
 struct SgmConsts {
     int32_t p1, p2, adaptive_p2, slant_steps;
     int32_t ndirs;  // 4 or 8 path directions
+    int32_t cv_from_dir;  // cost-volume launches: directions below this recompute their costs
     int32_t census_rx, census_ry, invalid_cost;
     int32_t lr_max_diff, subpixel;
     float uniqueness;

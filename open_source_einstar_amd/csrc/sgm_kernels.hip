@@ -551,99 +551,112 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         padm[r] = PAD ? ((k0 + 2 * r >= P.nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= P.nd ? 0x7FFF0000u : 0u)) : 0u;
 
     const int x = ln.x, y = ln.y, n = ln.n;
-    // Load positions (the census rows carry pads, so no window word leaves its row).
-    Walk wk;
-    const int64_t cp = P.census_pitch;
-    if constexpr (CV) {
-        wk.cv = buffer_view(P.cvol, (uint64_t)W * H * P.dp);
-        wk.o_cv = (uint32_t)(((int64_t)y * W + x) * P.dp + k0);
-        wk.s_cv = (uint32_t)(((int64_t)dy * W + dx) * P.dp);
-    } else {
-        wk.cl = buffer_view(P.census_l, (uint64_t)cp * H * 8);
-        wk.cr = buffer_view(P.census_r, (uint64_t)cp * H * 8);
-        wk.o_cl = (uint32_t)(((int64_t)y * cp + P.census_padl + x) * 8);
-        wk.o_cr = (uint32_t)(((int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1))) * 8);
-        wk.s_c = (uint32_t)(((int64_t)dy * cp + dx) * 8);
-    }
     // Step penalties: byte dir/2 of the word of the pixel stepped into (directions 0, 2, 4, 6 run
     // along their axis) or of the pixel stepped from (1, 3, 5, 7 run against it), selected into
     // both halves.
     const bool along = (dir & 1) == 0;
     const uint32_t axis = (uint32_t)(dir >> 1);
     const uint32_t pen_sel = 0x0C000C00u | (axis << 16) | axis;
-    wk.ed = buffer_view(P.edge, (uint64_t)W * H * 4);
-    wk.o_e = (uint32_t)((((int64_t)y * W + x) - (along ? 0 : ((int64_t)dy * W + dx))) * 4);
-    wk.s_e = (uint32_t)(((int64_t)dy * W + dx) * 4);
     // Path costs are stored [y][x][dir][dp]: the row kernel then reads one pixel's directions as one
     // contiguous run, and each store here is still a whole dp-byte run.
     const int64_t step_l = ((int64_t)dy * W + dx) * P.lvol_px_stride;
     SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + ((int64_t)y * W + x) * P.lvol_px_stride + (int64_t)dir * P.dp + k0;
 
-    // Loads and stores. Two banks of K pixels alternate: while one bank's pixels are computed, the
-    // other's loads are in flight. A bank is refilled (2K pixels further on) right after its
-    // pixels are computed and before their path costs are stored, so on this hardware, where
-    // loads and stores retire through one in-order counter, the wait for a bank's loads never
-    // waits for a recent store. Loads past the line's end read other pixels or zeros and are
-    // never used.
-    constexpr int K = CV ? 4 : 2;
-    StepIn<DPL, CV> first, bank_a[K], bank_b[K];
-    load_step<DPL, CV>(first, wk);  // pixel 0
-#pragma unroll
-    for (int j = 0; j < K; ++j) load_step<DPL, CV>(bank_a[j], wk);  // pixels 1 .. K
-#pragma unroll
-    for (int j = 0; j < K; ++j) load_step<DPL, CV>(bank_b[j], wk);  // pixels K + 1 .. 2K
+    // The walk, for one source of matching costs: the u8 cost volume (CW) or recomputation from the
+    // census words. A cost-volume launch may still recompute the costs of its first cv_from_dir
+    // directions: that trades vector work, which this kernel has to spare, for cost-volume reads,
+    // the bandwidth it is bound by.
+    auto run = [&](auto cw) {
+        constexpr bool CW = decltype(cw)::value;
+        // Load positions (the census rows carry pads, so no window word leaves its row).
+        Walk wk;
+        const int64_t cp = P.census_pitch;
+        if constexpr (CW) {
+            wk.cv = buffer_view(P.cvol, (uint64_t)W * H * P.dp);
+            wk.o_cv = (uint32_t)(((int64_t)y * W + x) * P.dp + k0);
+            wk.s_cv = (uint32_t)(((int64_t)dy * W + dx) * P.dp);
+        } else {
+            wk.cl = buffer_view(P.census_l, (uint64_t)cp * H * 8);
+            wk.cr = buffer_view(P.census_r, (uint64_t)cp * H * 8);
+            wk.o_cl = (uint32_t)(((int64_t)y * cp + P.census_padl + x) * 8);
+            wk.o_cr = (uint32_t)(((int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1))) * 8);
+            wk.s_c = (uint32_t)(((int64_t)dy * cp + dx) * 8);
+        }
+        wk.ed = buffer_view(P.edge, (uint64_t)W * H * 4);
+        wk.o_e = (uint32_t)((((int64_t)y * W + x) - (along ? 0 : ((int64_t)dy * W + dx))) * 4);
+        wk.s_e = (uint32_t)(((int64_t)dy * W + dx) * 4);
+        // Loads and stores. Two banks of K pixels alternate: while one bank's pixels are computed, the
+        // other's loads are in flight. A bank is refilled (2K pixels further on) right after its
+        // pixels are computed and before their path costs are stored, so on this hardware, where
+        // loads and stores retire through one in-order counter, the wait for a bank's loads never
+        // waits for a recent store. Loads past the line's end read other pixels or zeros and are
+        // never used.
+        constexpr int K = CW ? 4 : 2;
+        StepIn<DPL, CW> first, bank_a[K], bank_b[K];
+        load_step<DPL, CW>(first, wk);  // pixel 0
+    #pragma unroll
+        for (int j = 0; j < K; ++j) load_step<DPL, CW>(bank_a[j], wk);  // pixels 1 .. K
+    #pragma unroll
+        for (int j = 0; j < K; ++j) load_step<DPL, CW>(bank_b[j], wk);  // pixels K + 1 .. 2K
 
-    uint32_t prev[R], pmpm;
-    {
-        uint32_t cc[R];
-        step_costs<DPL, CV>(first, x >= full_lo && x <= full_hi, x - min_d - k0, W, inv, cc);
-        uint32_t mn = 0xFFFFFFFFu;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            prev[r] = PAD ? pk_max(cc[r], padm[r]) : cc[r];
-            mn = pk_min(mn, prev[r]);
-        }
-        store_bytes<DPL>(lv, prev);
-        pmpm = pair_min_line<G>(mn);
-    }
-    // Pixels base .. base + K - 1 from bank bk, then the bank's refill, then the stores. In the main
-    // loop every pixel of a half lies on the line (full), so its stores are unconditional and each
-    // wait counts exactly the loads and stores issued since; the tail half tests each pixel.
-    SGM_GLOBAL uint8_t* lvh = lv + step_l;  // pixel base
-    auto half = [&](auto full, StepIn<DPL, CV>(&bk)[K], const int base) {
-        constexpr bool kFull = decltype(full)::value;
-        // Keeps the scheduler from hoisting the other bank's unpacking (and with it the wait for
-        // loads still in flight) into this half.
-        __builtin_amdgcn_sched_barrier(0);
-        Packed<DPL> out[K];
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            if (kFull || base + j < n) {
-                const int xc = x + (base + j) * dx;
-                const uint32_t p2p2 = __builtin_amdgcn_perm(0u, bk[j].E, pen_sel);
-                uint32_t cc[R];
-                step_costs<DPL, CV>(bk[j], xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
-                uint32_t mn;
-                path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
-                out[j] = pack_bytes<DPL>(prev);
-                pmpm = pair_min_line<G>(mn);
+        uint32_t prev[R], pmpm;
+        {
+            uint32_t cc[R];
+            step_costs<DPL, CW>(first, x >= full_lo && x <= full_hi, x - min_d - k0, W, inv, cc);
+            uint32_t mn = 0xFFFFFFFFu;
+    #pragma unroll
+            for (int r = 0; r < R; ++r) {
+                prev[r] = PAD ? pk_max(cc[r], padm[r]) : cc[r];
+                mn = pk_min(mn, prev[r]);
             }
+            store_bytes<DPL>(lv, prev);
+            pmpm = pair_min_line<G>(mn);
         }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < K; ++j) load_step<DPL, CV>(bk[j], wk);  // pixels base + 2K ..
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (kFull || base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
-        lvh += K * step_l;
+        // Pixels base .. base + K - 1 from bank bk, then the bank's refill, then the stores. In the main
+        // loop every pixel of a half lies on the line (full), so its stores are unconditional and each
+        // wait counts exactly the loads and stores issued since; the tail half tests each pixel.
+        SGM_GLOBAL uint8_t* lvh = lv + step_l;  // pixel base
+        auto half = [&](auto full, StepIn<DPL, CW>(&bk)[K], const int base) {
+            constexpr bool kFull = decltype(full)::value;
+            // Keeps the scheduler from hoisting the other bank's unpacking (and with it the wait for
+            // loads still in flight) into this half.
+            __builtin_amdgcn_sched_barrier(0);
+            Packed<DPL> out[K];
+    #pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (kFull || base + j < n) {
+                    const int xc = x + (base + j) * dx;
+                    const uint32_t p2p2 = __builtin_amdgcn_perm(0u, bk[j].E, pen_sel);
+                    uint32_t cc[R];
+                    step_costs<DPL, CW>(bk[j], xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
+                    uint32_t mn;
+                    path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
+                    out[j] = pack_bytes<DPL>(prev);
+                    pmpm = pair_min_line<G>(mn);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+            for (int j = 0; j < K; ++j) load_step<DPL, CW>(bk[j], wk);  // pixels base + 2K ..
+    #pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (kFull || base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
+            lvh += K * step_l;
+        };
+        int i = 1;
+        for (; i + 2 * K <= n; i += 2 * K) {
+            half(std::true_type{}, bank_a, i);
+            half(std::true_type{}, bank_b, i + K);
+        }
+        if (i < n) half(std::false_type{}, bank_a, i);
+        if (i + K < n) half(std::false_type{}, bank_b, i + K);
     };
-    int i = 1;
-    for (; i + 2 * K <= n; i += 2 * K) {
-        half(std::true_type{}, bank_a, i);
-        half(std::true_type{}, bank_b, i + K);
+    if constexpr (CV) {
+        if (dir >= c.cv_from_dir) run(std::true_type{});
+        else run(std::false_type{});
+    } else {
+        run(std::false_type{});
     }
-    if (i < n) half(std::false_type{}, bank_a, i);
-    if (i + K < n) half(std::false_type{}, bank_b, i + K);
 }
 
 template <int DPL, bool PAD, bool CV, int G, bool SLANT>
