@@ -776,12 +776,20 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + k0;
     const int64_t row0 = (int64_t)y * W;
 
-    auto process = [&](const RowIn<DPL>& in, int x) {
-        uint32_t s[DPL / 2];
+    // One pixel's results between its computation and its commit (right-WTA atomics and stores).
+    struct PxOut {
+        uint32_t s[DPL / 2];  // S of the lane's disparities, packed pairs
+        uint32_t key, sec;    // best (S << 16 | k) over the row's lanes, second best S
+        float d;
+        uint8_t st;
+    };
+    // Sums, left WTA with uniqueness and subpixel (stereo.cpp:412-432); no memory side effects, so
+    // the computations of several pixels can overlap.
+    auto compute = [&](const RowIn<DPL>& in, int x, PxOut& o) {
 #pragma unroll
-        for (int r = 0; r < DPL / 2; ++r) s[r] = 0;
+        for (int r = 0; r < DPL / 2; ++r) o.s[r] = 0;
 #pragma unroll
-        for (int d = 0; d < NDIRS; ++d) add_raw<DPL>(in.w[d], s);
+        for (int d = 0; d < NDIRS; ++d) add_raw<DPL>(in.w[d], o.s);
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -790,13 +798,9 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         uint32_t sec = 0xFFFFu;
         if (!PAD && min_k == 0 && max_k == nd) {
             // Every disparity is a candidate (most pixels): no per-element window tests.
-            SGM_LDS uint32_t* rk = rkey + (x - min_d - k0 - (DPL - 1));  // rk[DPL-1-j]: right pixel of k0+j
 #pragma unroll
-            for (int j = 0; j < DPL; ++j) {
-                const uint32_t kk = ((j & 1) ? (s[j >> 1] & 0xFFFF0000u) : (s[j >> 1] << 16)) | (uint32_t)(k0 + j);
-                key = min(key, kk);
-                __hip_atomic_fetch_min(&rk[DPL - 1 - j], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA
-            }
+            for (int j = 0; j < DPL; ++j)
+                key = min(key, ((j & 1) ? (o.s[j >> 1] & 0xFFFF0000u) : (o.s[j >> 1] << 16)) | (uint32_t)(k0 + j));
             key = row_min(key);
             const int bk = (int)(key & 0xFFFFu);
             // Second best over |k - bk| > 1: the excluded halves are forced to 0xFFFF.
@@ -805,18 +809,14 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             for (int r = 0; r < DPL / 2; ++r) {
                 const int ka = k0 + 2 * r;
                 const uint32_t ex = ((unsigned)(ka - bk + 1) <= 2u ? 0xFFFFu : 0u) | ((unsigned)(ka - bk + 2) <= 2u ? 0xFFFF0000u : 0u);
-                m = pk_min(m, s[r] | ex);
+                m = pk_min(m, o.s[r] | ex);
             }
             sec = min(m & 0xFFFFu, m >> 16);
         } else {
 #pragma unroll
             for (int j = 0; j < DPL; ++j) {
                 const int k = k0 + j;
-                if (k >= min_k && k < max_k) {
-                    const uint32_t kk = (half16(s[j >> 1], j & 1) << 16) | (uint32_t)k;
-                    key = min(key, kk);
-                    __hip_atomic_fetch_min(&rkey[x - min_d - k], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // right WTA, stereo.cpp:434-447
-                }
+                if (k >= min_k && k < max_k) key = min(key, (half16(o.s[j >> 1], j & 1) << 16) | (uint32_t)k);
             }
             key = row_min(key);  // lowest S, then lowest k: the first strict argmin of :422-424
             const int bk0 = key == 0xFFFFFFFFu ? -1 : (int)(key & 0xFFFFu);
@@ -824,7 +824,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             for (int j = 0; j < DPL; ++j) {  // stereo.cpp:425-427
                 const int k = k0 + j;
                 const int dk = k - bk0;
-                if (k >= min_k && k < max_k && (dk > 1 || dk < -1)) sec = min(sec, half16(s[j >> 1], j & 1));
+                if (k >= min_k && k < max_k && (dk > 1 || dk < -1)) sec = min(sec, half16(o.s[j >> 1], j & 1));
             }
         }
         const int bk = key == 0xFFFFFFFFu ? -1 : (int)(key & 0xFFFFu);
@@ -842,7 +842,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
                     uint32_t vm = 0, vp = 0;
 #pragma unroll
                     for (int j = 0; j < DPL; ++j) {
-                        const uint32_t v = half16(s[j >> 1], j & 1);
+                        const uint32_t v = half16(o.s[j >> 1], j & 1);
                         vm = (k0 + j == bk - 1) ? v : vm;
                         vp = (k0 + j == bk + 1) ? v : vp;
                     }
@@ -852,36 +852,71 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
                 }
             }
         }
+        o.key = key;
+        o.sec = sec;
+        o.d = d;
+        o.st = st;
+    };
+    // Right WTA by LDS atomic minimum of the same keys (stereo.cpp:434-447), then the pixel's
+    // outputs.
+    auto commit = [&](const PxOut& o, int x) {
+        const int min_k = max(0, x - min_d - (W - 1));
+        const int max_k = min(nd, x - min_d + 1);
+        if (!PAD && min_k == 0 && max_k == nd) {
+            SGM_LDS uint32_t* rk = rkey + (x - min_d - k0 - (DPL - 1));  // rk[DPL-1-j]: right pixel of k0+j
+#pragma unroll
+            for (int j = 0; j < DPL; ++j) {
+                const uint32_t kk = ((j & 1) ? (o.s[j >> 1] & 0xFFFF0000u) : (o.s[j >> 1] << 16)) | (uint32_t)(k0 + j);
+                __hip_atomic_fetch_min(&rk[DPL - 1 - j], kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < DPL; ++j) {
+                const int k = k0 + j;
+                if (k >= min_k && k < max_k)
+                    __hip_atomic_fetch_min(&rkey[x - min_d - k], (half16(o.s[j >> 1], j & 1) << 16) | (uint32_t)k,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        const int bk = o.key == 0xFFFFFFFFu ? -1 : (int)(o.key & 0xFFFFu);
         if (g == 0) {
-            drow[x] = d;
-            srow[x] = st;
+            drow[x] = o.d;
+            srow[x] = o.st;
             if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
-            if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? bS : 0xFFFFu);
-            if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? sec : 0xFFFFu);
+            if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? o.key >> 16 : 0xFFFFu);
+            if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? o.sec : 0xFFFFu);
         }
         if (P.sum_volume) {
             uint16_t* sv = P.sum_volume + (row0 + x) * nd;
 #pragma unroll
             for (int j = 0; j < DPL; ++j)
-                if (k0 + j < nd) sv[k0 + j] = (uint16_t)half16(s[j >> 1], j & 1);
+                if (k0 + j < nd) sv[k0 + j] = (uint16_t)half16(o.s[j >> 1], j & 1);
         }
     };
 
-    // Pixels grp, grp + 16, ...: a ring of kRowAhead pixels' path costs is in flight while one is
-    // processed (reads past the row's end are clamped to its last pixel and never used).
+    // Pixels grp, grp + 16, ...: a ring of kRowAhead pixels' path costs is in flight while the
+    // previous ring is computed; the ring's pixels are computed together (their work overlaps),
+    // the slots refilled, then the pixels committed. Reads past the row's end are clamped to its
+    // last pixel and never used.
     constexpr int kRowAhead = row_ahead(DPL);
     RowIn<DPL> ring[kRowAhead];
 #pragma unroll
     for (int j = 0; j < kRowAhead; ++j)
         load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * pxs, dp);
     for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
+        PxOut o[kRowAhead];
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
             const int xj = x + j * kLinesPerBlock;
-            if (xj >= W) break;
-            // Refilled after use, so the slot keeps its registers and no load is waited for early.
-            process(ring[j], xj);
-            load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(xj + kRowAhead * kLinesPerBlock, W - 1) * pxs, dp);
+            if (xj < W) compute(ring[j], xj, o[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kRowAhead; ++j)
+            load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(x + (j + kRowAhead) * kLinesPerBlock, W - 1) * pxs, dp);
+#pragma unroll
+        for (int j = 0; j < kRowAhead; ++j) {
+            const int xj = x + j * kLinesPerBlock;
+            if (xj < W) commit(o[j], xj);
         }
     }
     __syncthreads();
