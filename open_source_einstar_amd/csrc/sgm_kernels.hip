@@ -417,26 +417,30 @@ __device__ __forceinline__ Packed<DPL> pack_bytes(const uint32_t (&v)[DPL / 2]) 
     if constexpr (DPL % 4 == 2) o.w[DPL / 4] = (v[DPL / 2 - 1] & 0xFFu) | ((v[DPL / 2 - 1] >> 8) & 0xFF00u);
     return o;
 }
-// DPL bytes at dst (aligned to DPL's power-of-two part).
+// DPL bytes at dst (aligned to DPL's power-of-two part). The stores are non-temporal: path costs
+// are read back once, by the row kernel, while the cost volume is re-read by every direction and
+// should keep its place in the caches.
 template <int DPL>
 __device__ __forceinline__ void store_packed(SGM_GLOBAL uint8_t* dst, const Packed<DPL>& b) {
     if constexpr (DPL % 16 == 0) {
 #pragma unroll
         for (int i = 0; i < DPL / 16; ++i)
-            reinterpret_cast<SGM_GLOBAL u32x4*>(dst)[i] = u32x4{b.w[4 * i], b.w[4 * i + 1], b.w[4 * i + 2], b.w[4 * i + 3]};
+            __builtin_nontemporal_store(u32x4{b.w[4 * i], b.w[4 * i + 1], b.w[4 * i + 2], b.w[4 * i + 3]},
+                                        reinterpret_cast<SGM_GLOBAL u32x4*>(dst) + i);
     } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 8; ++i) reinterpret_cast<SGM_GLOBAL u32x2*>(dst)[i] = u32x2{b.w[2 * i], b.w[2 * i + 1]};
+        for (int i = 0; i < DPL / 8; ++i)
+            __builtin_nontemporal_store(u32x2{b.w[2 * i], b.w[2 * i + 1]}, reinterpret_cast<SGM_GLOBAL u32x2*>(dst) + i);
     } else if constexpr (DPL % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) reinterpret_cast<SGM_GLOBAL uint32_t*>(dst)[i] = b.w[i];
+        for (int i = 0; i < DPL / 4; ++i) __builtin_nontemporal_store(b.w[i], reinterpret_cast<SGM_GLOBAL uint32_t*>(dst) + i);
     } else {
 #pragma unroll
         for (int i = 0; i < DPL / 4; ++i) {
-            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[2 * i] = (uint16_t)b.w[i];
-            reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[2 * i + 1] = (uint16_t)(b.w[i] >> 16);
+            __builtin_nontemporal_store((uint16_t)b.w[i], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i);
+            __builtin_nontemporal_store((uint16_t)(b.w[i] >> 16), reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i + 1);
         }
-        reinterpret_cast<SGM_GLOBAL uint16_t*>(dst)[DPL / 2 - 1] = (uint16_t)b.w[DPL / 4];
+        __builtin_nontemporal_store((uint16_t)b.w[DPL / 4], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + DPL / 2 - 1);
     }
 }
 template <int DPL>
