@@ -227,13 +227,14 @@ __device__ __forceinline__ SGM_GLOBAL T* gptr(T* p) {
     return (SGM_GLOBAL T*)p;
 }
 
-// DPL bytes (path costs or matching costs of one pixel), as (DPL+3)/4 words.
+// DPL path-cost bytes of one pixel, as (DPL+3)/4 words. The loads are non-temporal: each byte is
+// read exactly once.
 template <int DPL>
 __device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t (&w)[(DPL + 3) / 4]) {
     if constexpr (DPL % 16 == 0) {
 #pragma unroll
         for (int i = 0; i < DPL / 16; ++i) {
-            const u32x4 q = reinterpret_cast<const SGM_GLOBAL u32x4*>(src)[i];
+            const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const SGM_GLOBAL u32x4*>(src) + i);
             w[4 * i] = q.x;
             w[4 * i + 1] = q.y;
             w[4 * i + 2] = q.z;
@@ -242,17 +243,17 @@ __device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t
     } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
         for (int i = 0; i < DPL / 8; ++i) {
-            const u32x2 q = reinterpret_cast<const SGM_GLOBAL u32x2*>(src)[i];
+            const u32x2 q = __builtin_nontemporal_load(reinterpret_cast<const SGM_GLOBAL u32x2*>(src) + i);
             w[2 * i] = q.x;
             w[2 * i + 1] = q.y;
         }
     } else if constexpr (DPL % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) w[i] = reinterpret_cast<const SGM_GLOBAL uint32_t*>(src)[i];
+        for (int i = 0; i < DPL / 4; ++i) w[i] = __builtin_nontemporal_load(reinterpret_cast<const SGM_GLOBAL uint32_t*>(src) + i);
     } else {
 #pragma unroll
         for (int i = 0; i < DPL / 2; ++i) {
-            const uint32_t h = reinterpret_cast<const SGM_GLOBAL uint16_t*>(src)[i];
+            const uint32_t h = __builtin_nontemporal_load(reinterpret_cast<const SGM_GLOBAL uint16_t*>(src) + i);
             w[i / 2] = (i & 1) ? (w[i / 2] | (h << 16)) : h;
         }
     }
