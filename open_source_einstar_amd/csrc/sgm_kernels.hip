@@ -141,6 +141,7 @@ constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
 
 // One 16x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
 // the block calls it (it synchronises once). The tile keeps a halo of at least one pixel.
+template <int RX, int RY>  // the census radii when known at compile time, else -1
 __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint8_t* tile,
                                             SGM_LDS int32_t* p2q) {
     const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
@@ -165,12 +166,32 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
     const SGM_LDS uint8_t* ctr = &tile[(ly + hy) * kTileW + lx + hx];
     uint64_t bits = 0;
     if (x >= rx && x < P.w - rx && y >= ry && y < P.h - ry) {
-        const uint8_t center = *ctr;
-        for (int v = -ry; v <= ry; ++v) {
-            const SGM_LDS uint8_t* row = ctr + v * kTileW - rx;
-            for (int u = 0; u <= 2 * rx; ++u) {
-                if (u == rx && v == 0) continue;
-                bits = (bits << 1) | (uint64_t)(row[u] < center);
+        const uint32_t center = *ctr;
+        if constexpr (RX >= 0) {
+            // Compile-time window: compare i lands at bit cells - 2 - i, each compare sets its own
+            // bit, so the compares do not chain through one shift register.
+            constexpr int kCells = (2 * RX + 1) * (2 * RY + 1);
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int v = -RY; v <= RY; ++v) {
+#pragma unroll
+                for (int u = -RX; u <= RX; ++u) {
+                    const int i = (v + RY) * (2 * RX + 1) + (u + RX) - ((v > 0 || (v == 0 && u > 0)) ? 1 : 0);
+                    if (u == 0 && v == 0) continue;
+                    const int pos = kCells - 2 - i;
+                    const uint32_t b = (uint32_t)ctr[v * kTileW + u] < center ? 1u : 0u;
+                    if (pos < 32) lo |= b << pos;
+                    else hi |= b << (pos - 32);
+                }
+            }
+            bits = ((uint64_t)hi << 32) | lo;
+        } else {
+            for (int v = -ry; v <= ry; ++v) {
+                const SGM_LDS uint8_t* row = ctr + v * kTileW - rx;
+                for (int u = 0; u <= 2 * rx; ++u) {
+                    if (u == rx && v == 0) continue;
+                    bits = (bits << 1) | (uint64_t)(row[u] < center);
+                }
             }
         }
     }
@@ -184,13 +205,14 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
     }
 }
 
+template <int RX, int RY>
 __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
     __shared__ uint8_t tile[kTileH * kTileW];
     __shared__ int32_t p2q[32];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    census_body(P, c, lb, lds(tile), lds(p2q));
+    census_body<RX, RY>(P, c, lb, lds(tile), lds(p2q));
 }
 
 // ------------------------------------------------------------------------------- cost volume
@@ -1083,7 +1105,14 @@ int median_blocks(int w, int h) { return ((w + kTile - 1) / kTile) * ((h + kTile
 
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(census_kernel, dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
+    // The windows of the reference's parameter sets (9x7 and 5x5) are compiled in; others run the
+    // generic loop.
+    if (c.census_rx == 4 && c.census_ry == 3)
+        hipLaunchKernelGGL((census_kernel<4, 3>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
+    else if (c.census_rx == 2 && c.census_ry == 2)
+        hipLaunchKernelGGL((census_kernel<2, 2>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
+    else
+        hipLaunchKernelGGL((census_kernel<-1, -1>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);
     return hipGetLastError();
 }
 
