@@ -137,77 +137,143 @@ __device__ __forceinline__ SGM_LDS T* lds(T* p) {
 
 constexpr int kTile = 16;
 constexpr int kMaxRx = 4, kMaxRy = 3;
-constexpr int kTileW = kTile + 2 * kMaxRx, kTileH = kTile + 2 * kMaxRy;
+// A census block covers 64 x 16 pixels; each thread takes four adjacent pixels of one row (16
+// threads a row, a wave four rows). The LDS tile holds the block's pixels with a halo of kMaxRx
+// columns and kMaxRy rows, one u16 per pixel (0 outside the image), so that a 32-bit LDS word holds
+// two horizontally neighbouring pixels.
+constexpr int kCenW = 64, kCenH = 16, kCenPx = 4;
+constexpr int kTileW = kCenW + 2 * kMaxRx, kTileH = kCenH + 2 * kMaxRy;
+static_assert(kCenW / kCenPx * kCenH == kBlock, "one census thread per four pixels");
 
-// One 16x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
-// the block calls it (it synchronises once). The tile keeps a halo of at least one pixel.
+// Bits 8k .. 8k + 3 (k = 0 .. 3) of the census words of pixels 2q and 2q + 1, gathered from byte 1
+// and byte 3 of the octet words t0 .. t3.
+__device__ __forceinline__ void census_gather(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t& even,
+                                              uint32_t& odd) {
+    const uint32_t p01 = __builtin_amdgcn_perm(t1, t0, 0x07030501u);  // t0.b1 t1.b1 t0.b3 t1.b3
+    const uint32_t p23 = __builtin_amdgcn_perm(t3, t2, 0x07030501u);
+    even = __builtin_amdgcn_perm(p23, p01, 0x05040100u);
+    odd = __builtin_amdgcn_perm(p23, p01, 0x07060302u);
+}
+
+// One 64x16 tile of one image (lb counts the left image's tiles, then the right's). Every thread of
+// the block calls it (it synchronises once).
 template <int RX, int RY>  // the census radii when known at compile time, else -1
-__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint8_t* tile,
+__device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint16_t* tile,
                                             SGM_LDS int32_t* p2q) {
-    const int tx_n = (P.w + kTile - 1) / kTile, ty_n = (P.h + kTile - 1) / kTile;
+    const int tx_n = (P.w + kCenW - 1) / kCenW, ty_n = (P.h + kCenH - 1) / kCenH;
     const int side = lb / (tx_n * ty_n);
     const int tl = lb - side * tx_n * ty_n;
-    const int x0 = (tl % tx_n) * kTile, y0 = (tl / tx_n) * kTile;
+    const int x0 = (tl % tx_n) * kCenW, y0 = (tl / tx_n) * kCenH;
     const uint8_t* img = side ? P.right : P.left;
     uint64_t* out = side ? P.census_r : P.census_l;
-    const int rx = c.census_rx, ry = c.census_ry;
-    const int hx = max(rx, 1), hy = max(ry, 1);
-    const int tw = kTile + 2 * hx, th = kTile + 2 * hy;
-    for (int i = threadIdx.x; i < tw * th; i += kBlock) {
-        const int ly = i / tw, lx = i - ly * tw;
-        const int gx = x0 - hx + lx, gy = y0 - hy + ly;
-        tile[ly * kTileW + lx] = (gx >= 0 && gy >= 0 && gx < P.w && gy < P.h) ? img[(int64_t)gy * P.pitch + gx] : 0;
+    // Every halo byte is fetched before the first one is written to LDS.
+    constexpr int kTileN = kTileW * kTileH;
+    constexpr int kLoads = (kTileN + kBlock - 1) / kBlock;
+    uint32_t px[kLoads];
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+        const int i = (int)threadIdx.x + j * kBlock;
+        const int ly = i / kTileW, lx = i - ly * kTileW;
+        const int gx = x0 - kMaxRx + lx, gy = y0 - kMaxRy + ly;
+        px[j] = (i < kTileN && gx >= 0 && gy >= 0 && gx < P.w && gy < P.h) ? img[(int64_t)gy * P.pitch + gx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+        const int i = (int)threadIdx.x + j * kBlock;
+        if (i < kTileN) tile[i] = (uint16_t)px[j];
     }
     if (threadIdx.x < 32) p2q[threadIdx.x] = c.adaptive_p2 ? c.p2_by_q[threadIdx.x] : c.p2;
     __syncthreads();
-    const int lx = threadIdx.x & (kTile - 1), ly = threadIdx.x >> 4;
+    const int lx = (int)(threadIdx.x % (kCenW / kCenPx)) * kCenPx, ly = (int)(threadIdx.x / (kCenW / kCenPx));
     const int x = x0 + lx, y = y0 + ly;
     if (x >= P.w || y >= P.h) return;
-    const SGM_LDS uint8_t* ctr = &tile[(ly + hy) * kTileW + lx + hx];
-    uint64_t bits = 0;
-    if (x >= rx && x < P.w - rx && y >= ry && y < P.h - ry) {
-        const uint32_t center = *ctr;
-        if constexpr (RX >= 0) {
-            // Compile-time window: compare i lands at bit cells - 2 - i, each compare sets its own
-            // bit, so the compares do not chain through one shift register.
-            constexpr int kCells = (2 * RX + 1) * (2 * RY + 1);
-            uint32_t lo = 0, hi = 0;
+    const int rx = c.census_rx, ry = c.census_ry;
+    const SGM_LDS uint16_t* ctr = &tile[(ly + kMaxRy) * kTileW + lx + kMaxRx];  // pixel (x, y)
+    uint64_t bits[kCenPx];
+    if constexpr (RX >= 0) {
+        // Compile-time window, two pixels per 32-bit word: the packed u16 difference neighbour -
+        // centre has its half's high byte 0xFF exactly when the neighbour is below the centre, and
+        // compare i of a pixel lands at bit cells - 2 - i of its word. T[q][k] collects bits 8k ..
+        // 8k + 7 of pixels 2q (byte 1) and 2q + 1 (byte 3).
+        constexpr int kCells = (2 * RX + 1) * (2 * RY + 1);
+        constexpr int kOct = (kCells - 1 + 7) / 8;
+        uint32_t T[2][8] = {};
+        // Word j of a tile row: pixels x - 4 + 2j and x - 3 + 2j (tile column lx + 2j is even).
+        const SGM_LDS uint32_t* row0 = reinterpret_cast<const SGM_LDS uint32_t*>(ctr - kMaxRx);
+        const uint32_t C0 = row0[2], C1 = row0[3];  // centres: pixels (x, x + 1), (x + 2, x + 3)
 #pragma unroll
-            for (int v = -RY; v <= RY; ++v) {
+        for (int v = -RY; v <= RY; ++v) {
+            const SGM_LDS uint32_t* rw = row0 + v * (kTileW / 2);
+            uint32_t wd[6];
 #pragma unroll
-                for (int u = -RX; u <= RX; ++u) {
-                    const int i = (v + RY) * (2 * RX + 1) + (u + RX) - ((v > 0 || (v == 0 && u > 0)) ? 1 : 0);
-                    if (u == 0 && v == 0) continue;
-                    const int pos = kCells - 2 - i;
-                    const uint32_t b = (uint32_t)ctr[v * kTileW + u] < center ? 1u : 0u;
-                    if (pos < 32) lo |= b << pos;
-                    else hi |= b << (pos - 32);
+            for (int j = 0; j < 6; ++j) wd[j] = rw[j];
+#pragma unroll
+            for (int u = -RX; u <= RX; ++u) {
+                if (u == 0 && v == 0) continue;
+                const int i = (v + RY) * (2 * RX + 1) + (u + RX) - ((v > 0 || (v == 0 && u > 0)) ? 1 : 0);
+                const int pos = kCells - 2 - i;
+                const uint32_t m = 0x01000100u << (pos & 7);
+                uint32_t a0, a1;  // neighbours (u, v) of pixels (x, x + 1) and (x + 2, x + 3)
+                if ((u & 1) == 0) {
+                    a0 = wd[(u + 4) / 2];
+                    a1 = wd[(u + 4) / 2 + 1];
+                } else {
+                    a0 = __builtin_amdgcn_alignbit(wd[(u + 5) / 2], wd[(u + 3) / 2], 16);
+                    a1 = __builtin_amdgcn_alignbit(wd[(u + 5) / 2 + 1], wd[(u + 3) / 2 + 1], 16);
                 }
-            }
-            bits = ((uint64_t)hi << 32) | lo;
-        } else {
-            for (int v = -ry; v <= ry; ++v) {
-                const SGM_LDS uint8_t* row = ctr + v * kTileW - rx;
-                for (int u = 0; u <= 2 * rx; ++u) {
-                    if (u == rx && v == 0) continue;
-                    bits = (bits << 1) | (uint64_t)(row[u] < center);
-                }
+                T[0][pos >> 3] |= pk_sub(a0, C0) & m;
+                T[1][pos >> 3] |= pk_sub(a1, C1) & m;
             }
         }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            uint32_t lo_e, lo_o, hi_e = 0, hi_o = 0;
+            census_gather(T[q][0], T[q][1], T[q][2], T[q][3], lo_e, lo_o);
+            if constexpr (kOct > 4) census_gather(T[q][4], T[q][5], T[q][6], T[q][7], hi_e, hi_o);
+            bits[2 * q] = ((uint64_t)hi_e << 32) | lo_e;
+            bits[2 * q + 1] = ((uint64_t)hi_o << 32) | lo_o;
+        }
+    } else {
+        // rows outer, columns inner, each compare shifted in at bit 0
+#pragma unroll
+        for (int p = 0; p < kCenPx; ++p) {
+            const uint32_t center = ctr[p];
+            uint64_t b = 0;
+            for (int v = -ry; v <= ry; ++v) {
+                const SGM_LDS uint16_t* row = ctr + p + v * kTileW - rx;
+                for (int u = 0; u <= 2 * rx; ++u) {
+                    if (u == rx && v == 0) continue;
+                    b = (b << 1) | (uint64_t)(row[u] < center);
+                }
+            }
+            bits[p] = b;
+        }
     }
-    out[(int64_t)y * P.census_pitch + P.census_padl + x] = bits;
+    const int64_t orow = (int64_t)y * P.census_pitch + P.census_padl;
+    const bool row_in = y >= ry && y < P.h - ry;
+#pragma unroll
+    for (int p = 0; p < kCenPx; ++p) {
+        const int xp = x + p;
+        if (xp < P.w) out[orow + xp] = (row_in && xp >= rx && xp < P.w - rx) ? bits[p] : 0ull;
+    }
     if (side == 0) {
-        const int I = *ctr;
-        auto pen = [&](int q, bool inside) { return inside ? (uint32_t)p2q[abs(I - q) >> 3] : 0u; };
-        const uint32_t e = pen(ctr[-1], x > 0) | (pen(ctr[-kTileW], y > 0) << 8) | (pen(ctr[-kTileW - 1], x > 0 && y > 0) << 16) |
-                           (pen(ctr[kTileW - 1], x > 0 && y + 1 < P.h) << 24);
-        P.edge[(int64_t)y * P.w + x] = e;
+#pragma unroll
+        for (int p = 0; p < kCenPx; ++p) {
+            const int xp = x + p;
+            if (xp >= P.w) break;
+            const SGM_LDS uint16_t* q = ctr + p;
+            const int I = q[0];
+            auto pen = [&](int v, bool inside) { return inside ? (uint32_t)p2q[abs(I - v) >> 3] : 0u; };
+            const uint32_t e = pen(q[-1], xp > 0) | (pen(q[-kTileW], y > 0) << 8) | (pen(q[-kTileW - 1], xp > 0 && y > 0) << 16) |
+                               (pen(q[kTileW - 1], xp > 0 && y + 1 < P.h) << 24);
+            P.edge[(int64_t)y * P.w + xp] = e;
+        }
     }
 }
 
 template <int RX, int RY>
 __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
-    __shared__ uint8_t tile[kTileH * kTileW];
+    __shared__ __attribute__((aligned(16))) uint16_t tile[kTileH * kTileW];
     __shared__ int32_t p2q[32];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
@@ -698,9 +764,9 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
 // Matching costs as bytes, [y][x][dp], for the path kernel's cost-volume mode (the cost fill of
 // stereo.cpp:366-380, invalid outside the image). A block covers kCostSeg consecutive pixels of a
 // row: the right-image census words all of them need (dp + kCostSeg - 1) are staged in LDS once;
-// each 16-lane group then takes pixels grp, grp + 16 with the path kernel's lane layout.
+// each 16-lane group then takes pixels grp, grp + 16, ... with the path kernel's lane layout.
 
-constexpr int kCostSeg = 32;
+constexpr int kCostSeg = 128;
 // LDS position of window word i: one spare word after every 8, so that the 16 lanes of a group,
 // whose windows start 8 words apart, read 16 different bank pairs (a 72-byte lane stride).
 __device__ __forceinline__ int win_pos(int i) { return i + (i >> 3); }
@@ -1097,7 +1163,7 @@ void fill_lines(PairDesc& d) {
     d.total_lines = total;
 }
 
-int census_blocks(int w, int h) { return 2 * ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
+int census_blocks(int w, int h) { return 2 * ((w + kCenW - 1) / kCenW) * ((h + kCenH - 1) / kCenH); }
 int path_blocks(const PairDesc& d, int lines_per_block) { return (d.total_lines + lines_per_block - 1) / lines_per_block; }
 int row_blocks(const PairDesc& d) { return d.h; }
 int cost8_blocks(int w, int h) { return h * cost8_segments(w); }
