@@ -684,7 +684,9 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         // loads and stores retire through one in-order counter, the wait for a bank's loads never
         // waits for a recent store. Loads past the line's end read other pixels or zeros and are
         // never used.
-        constexpr int K = CW ? 4 : 2;
+        // Seven cost-volume pixels a bank: the deepest prefetch that keeps three waves per SIMD at
+        // DPL = 8 (160 VGPRs); this walk is bound by the loads in flight, not by occupancy.
+        constexpr int K = CW ? 7 : 2;
         StepIn<DPL, CW> first, bank_a[K], bank_b[K];
         load_step<DPL, CW>(first, wk);  // pixel 0
     #pragma unroll
