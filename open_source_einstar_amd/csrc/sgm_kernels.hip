@@ -853,13 +853,16 @@ __device__ __forceinline__ uint32_t row_max(uint32_t v) {
 }
 
 // Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
-// 9 bytes per column.
+// 15 bytes per column: the right WTA keys, and the left results that the last pass writes out
+// coalesced (keys, second-best sums, disparities, status).
 template <int DPL, bool PAD, int NDIRS>
 __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
     const int W = P.w, y = lb;
     SGM_LDS uint32_t* rkey = smem;
-    SGM_LDS float* drow = (SGM_LDS float*)(smem + W);
-    SGM_LDS uint8_t* srow = (SGM_LDS uint8_t*)(smem + 2 * W);
+    SGM_LDS uint32_t* lkey = smem + W;
+    SGM_LDS float* drow = (SGM_LDS float*)(smem + 2 * W);
+    SGM_LDS uint16_t* lsec = (SGM_LDS uint16_t*)(smem + 3 * W);
+    SGM_LDS uint8_t* srow = (SGM_LDS uint8_t*)(smem + 3 * W) + 2 * W;
     for (int x = threadIdx.x; x < W; x += kBlock) rkey[x] = 0xFFFFFFFFu;
     __syncthreads();
 
@@ -973,13 +976,11 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        const int bk = o.key == 0xFFFFFFFFu ? -1 : (int)(o.key & 0xFFFFu);
         if (g == 0) {
             drow[x] = o.d;
             srow[x] = o.st;
-            if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
-            if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? o.key >> 16 : 0xFFFFu);
-            if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? o.sec : 0xFFFFu);
+            lkey[x] = o.key;
+            lsec[x] = (uint16_t)o.sec;
         }
         if (P.sum_volume) {
             uint16_t* sv = P.sum_volume + (row0 + x) * nd;
@@ -1032,6 +1033,11 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         }
         P.disp[row0 + x] = d;
         if (P.status) P.status[row0 + x] = st;
+        const uint32_t lk = lkey[x];
+        const int bk = lk == 0xFFFFFFFFu ? -1 : (int)(lk & 0xFFFFu);
+        if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
+        if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? lk >> 16 : 0xFFFFu);
+        if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? lsec[x] : 0xFFFFu);
         if (P.rdisp) P.rdisp[row0 + x] = rk == 0xFFFFFFFFu ? INT_MIN : min_d + (int)(rk & 0xFFFFu);
         if (P.conf) P.conf[row0 + x] = d > -1.0e8f ? 1.0f : 0.0f;
     }
@@ -1264,7 +1270,7 @@ hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipSt
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
     if (c.ndirs != 4 && c.ndirs != 8) return hipErrorInvalidValue;
-    const size_t lds = (size_t)max_w * 9 + 16;
+    const size_t lds = (size_t)max_w * 15 + 16;
     switch (dpl) {
 #define SGM_ROWS(D, PD)                                                                                                  \
     do {                                                                                                                 \
