@@ -506,35 +506,39 @@ __device__ __forceinline__ Packed<DPL> pack_bytes(const uint32_t (&v)[DPL / 2]) 
     if constexpr (DPL % 4 == 2) o.w[DPL / 4] = (v[DPL / 2 - 1] & 0xFFu) | ((v[DPL / 2 - 1] >> 8) & 0xFF00u);
     return o;
 }
-// DPL bytes at dst (aligned to DPL's power-of-two part). The stores are non-temporal: path costs
-// are read back once, by the row kernel, while the cost volume is re-read by every direction and
-// should keep its place in the caches.
-template <int DPL>
+// DPL bytes at dst (aligned to DPL's power-of-two part). Path costs are stored non-temporally
+// (NT): they are read back once, by the row kernel, while the cost volume, stored with ordinary
+// stores, is re-read by every direction and should stay in the caches.
+template <bool NT, typename T, typename P>
+__device__ __forceinline__ void put(T v, P* dst) {
+    if constexpr (NT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+}
+template <int DPL, bool NT = true>
 __device__ __forceinline__ void store_packed(SGM_GLOBAL uint8_t* dst, const Packed<DPL>& b) {
     if constexpr (DPL % 16 == 0) {
 #pragma unroll
         for (int i = 0; i < DPL / 16; ++i)
-            __builtin_nontemporal_store(u32x4{b.w[4 * i], b.w[4 * i + 1], b.w[4 * i + 2], b.w[4 * i + 3]},
-                                        reinterpret_cast<SGM_GLOBAL u32x4*>(dst) + i);
+            put<NT>(u32x4{b.w[4 * i], b.w[4 * i + 1], b.w[4 * i + 2], b.w[4 * i + 3]},
+                    reinterpret_cast<SGM_GLOBAL u32x4*>(dst) + i);
     } else if constexpr (DPL % 8 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 8; ++i)
-            __builtin_nontemporal_store(u32x2{b.w[2 * i], b.w[2 * i + 1]}, reinterpret_cast<SGM_GLOBAL u32x2*>(dst) + i);
+        for (int i = 0; i < DPL / 8; ++i) put<NT>(u32x2{b.w[2 * i], b.w[2 * i + 1]}, reinterpret_cast<SGM_GLOBAL u32x2*>(dst) + i);
     } else if constexpr (DPL % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < DPL / 4; ++i) __builtin_nontemporal_store(b.w[i], reinterpret_cast<SGM_GLOBAL uint32_t*>(dst) + i);
+        for (int i = 0; i < DPL / 4; ++i) put<NT>(b.w[i], reinterpret_cast<SGM_GLOBAL uint32_t*>(dst) + i);
     } else {
 #pragma unroll
         for (int i = 0; i < DPL / 4; ++i) {
-            __builtin_nontemporal_store((uint16_t)b.w[i], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i);
-            __builtin_nontemporal_store((uint16_t)(b.w[i] >> 16), reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i + 1);
+            put<NT>((uint16_t)b.w[i], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i);
+            put<NT>((uint16_t)(b.w[i] >> 16), reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + 2 * i + 1);
         }
-        __builtin_nontemporal_store((uint16_t)b.w[DPL / 4], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + DPL / 2 - 1);
+        put<NT>((uint16_t)b.w[DPL / 4], reinterpret_cast<SGM_GLOBAL uint16_t*>(dst) + DPL / 2 - 1);
     }
 }
-template <int DPL>
+template <int DPL, bool NT = true>
 __device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint32_t (&v)[DPL / 2]) {
-    store_packed<DPL>(dst, pack_bytes<DPL>(v));
+    store_packed<DPL, NT>(dst, pack_bytes<DPL>(v));
 }
 
 // Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value (or zero) into
@@ -803,7 +807,7 @@ __device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c
         const bool full = x >= P.min_d + P.dp - 1 && x <= P.w - 1 + P.min_d;
         uint32_t cc[DPL / 2];
         census_costs<DPL>(a, w, full, x - P.min_d - k0, P.w, (uint32_t)c.invalid_cost, cc);
-        store_bytes<DPL>(gptr(P.cvol) + ((int64_t)y * P.w + x) * P.dp + k0, cc);
+        store_bytes<DPL, false>(gptr(P.cvol) + ((int64_t)y * P.w + x) * P.dp + k0, cc);
     }
 }
 
