@@ -14,7 +14,7 @@ constexpr int kBlock = 256;                   // threads per block (4 wave64s)
 constexpr int kLinesPerBlock = kBlock / kGroup;
 constexpr int kMaxBatch = 64;                 // pairs per launch table
 constexpr int kMaxDirs = 8;
-constexpr int kMaxWidth = 4096;               // the row kernel keeps 9 bytes of LDS per column
+constexpr int kMaxWidth = 4096;               // the row kernel keeps 15 bytes of LDS per column
 // The path kernel reads the census rows, the step penalties and the cost volume through buffer
 // views addressed by 32-bit byte offsets, so each of them must stay within 4 GiB - 1. Census rows
 // beyond that are refused (the penalties are half their size); a cost volume beyond it is not used
