@@ -770,7 +770,7 @@ __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts 
 // Matching costs as bytes, [y][x][dp], for the path kernel's cost-volume mode (the cost fill of
 // stereo.cpp:366-380, invalid outside the image). A block covers kCostSeg consecutive pixels of a
 // row: the right-image census words all of them need (dp + kCostSeg - 1) are staged in LDS once;
-// each 16-lane group then takes pixels grp, grp + 16, ... with the path kernel's lane layout.
+// each 16-lane group then takes kCostSeg / 16 consecutive pixels with the path kernel's lane layout.
 
 constexpr int kCostSeg = 128;
 // LDS position of window word i: one spare word after every 8, so that the 16 lanes of a group,
@@ -793,17 +793,26 @@ __device__ __forceinline__ void cost8_body(const PairDesc& P, const SgmConsts& c
         win[win_pos(i)] = (xr >= lo && xr <= hi) ? gptr(P.census_r)[row + xr] : 0ull;
     }
     __syncthreads();
+    // Each 16-lane group takes kCostRun consecutive pixels: from one pixel to the next a lane's window
+    // moves up by one word, so after the first pixel it reads one new word from LDS instead of DPL.
+    constexpr int kCostRun = kCostSeg / kLinesPerBlock;
     const int grp = threadIdx.x >> 4;
     const int k0 = (threadIdx.x & (kGroup - 1)) * DPL;
+    const int xs = x0 + grp * kCostRun;
+    const int i0 = xs - P.min_d - k0 - (DPL - 1) - base;  // win index of the lane's lowest word at xs
+    uint64_t w[DPL];
 #pragma unroll
-    for (int rep = 0; rep < kCostSeg / kLinesPerBlock; ++rep) {
-        const int x = x0 + grp + rep * kLinesPerBlock;
+    for (int j = 0; j < DPL; ++j) w[j] = win[win_pos(i0 + j)];
+#pragma unroll
+    for (int j = 0; j < kCostRun; ++j) {
+        if (j > 0) {
+#pragma unroll
+            for (int t = 0; t + 1 < DPL; ++t) w[t] = w[t + 1];
+            w[DPL - 1] = win[win_pos(i0 + j + DPL - 1)];
+        }
+        const int x = xs + j;
         if (x >= P.w) break;
         const uint64_t a = gptr(P.census_l)[row + x];
-        const int i0 = x - P.min_d - k0 - (DPL - 1) - base;  // win index of the lane's lowest word
-        uint64_t w[DPL];
-#pragma unroll
-        for (int j = 0; j < DPL; ++j) w[j] = win[win_pos(i0 + j)];
         const bool full = x >= P.min_d + P.dp - 1 && x <= P.w - 1 + P.min_d;
         uint32_t cc[DPL / 2];
         census_costs<DPL>(a, w, full, x - P.min_d - k0, P.w, (uint32_t)c.invalid_cost, cc);
