@@ -116,16 +116,22 @@ __device__ float zncc_slanted(const uint8_t* a, const uint8_t* b, int64_t pitch,
 }
 
 // Fixed window widths: the samples of a window row are unrolled, so their loads are issued together
-// and waited for once per row; a sample outside the right image clears a flag instead of leaving the
-// loop (its load is clamped into the row and the whole score is discarded), so lanes do not diverge.
-// Rows stay a loop (fully unrolled windows keep every load in flight and run out of registers). The
-// sums run in the same order as above.
+// and waited for once per row; a window that leaves the right image clears a flag instead of leaving
+// the loop (every load is clamped into the row and the whole score is discarded), so lanes do not
+// diverge. Rows stay a loop (fully unrolled windows keep every load in flight and run out of
+// registers). The sums run in the same order as above.
+//
+// Within a window row the sample position xb strictly increases with the column: each column step
+// adds 1 - gx >= 0.1 (|gx| <= 0.9) while the float roundings of the position's terms change it by a
+// few ulps of values below 2^15 (< 0.01). So every sample of the row lies in [0, bw - 1) exactly when
+// the first is >= 0 and the last is < bw - 1, and only those two are tested; finite slopes and
+// disparities leave no NaN to test for.
 
-// The bilinear sample at xb of row rb (clamped into [0, bw - 2]); ok &= xb is a valid position.
-__device__ __forceinline__ double sample(const uint8_t* rb, int bw, float xb, bool& ok) {
-    ok = ok && xb >= 0.0f && xb < (float)(bw - 1);
+// The bilinear sample at xb of row rb, its left pixel clamped into [0, bw - 2].
+__device__ __forceinline__ double sample(const uint8_t* rb, int bw, float xb) {
     const float fl = __builtin_floorf(xb);
-    const int x0 = min(max((int)fl, 0), bw - 2);
+    const int x0 = (int)__builtin_amdgcn_fmed3f(fl, 0.0f, (float)(bw - 2));
+    __builtin_assume(x0 >= 0);
     const float f = xb - fl;
     return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
 }
@@ -152,7 +158,9 @@ __device__ float zncc_slanted_w(const uint8_t* a, const uint8_t* b, int64_t pitc
 #pragma unroll
         for (int u = -R; u <= R; ++u) {
             const float xb = (float)(x + u) - ((d + gxu[u + R]) + gyv);
-            const double ib = sample(rb, bw, xb, ok);
+            if (u == -R) ok = ok && xb >= 0.0f;
+            if (u == R) ok = ok && xb < (float)(bw - 1);
+            const double ib = sample(rb, bw, xb);
             const double ia = ra[u];
             sb += ib;
             sbb += ib * ib;
@@ -300,7 +308,9 @@ __device__ float zncc_window_w(const uint8_t* a, const uint8_t* b, int64_t pitch
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const float xb = (float)(x0 + i) - ((D + gxx[i]) + dy);
-            const double ib = sample(rb, bw, xb, ok);
+            if (i == 0) ok = ok && xb >= 0.0f;
+            if (i == N - 1) ok = ok && xb < (float)(bw - 1);
+            const double ib = sample(rb, bw, xb);
             const double ia = ra[i];
             sb += ib;
             sbb += ib * ib;
