@@ -15,6 +15,7 @@ namespace {
 
 constexpr float kInvalid = -1.0e9f;
 constexpr int kTw = 16, kTh = 16;  // 2-D blocks of 256 threads
+#define RK_LDS __attribute__((address_space(3)))
 
 __device__ __forceinline__ bool valid_d(float d) { return d > -1.0e8f; }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
@@ -143,16 +144,17 @@ __device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, d
     return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
 }
 
-// zncc_slanted (stereo.cpp:101-125) for radius R; gxu[u + R] = gx * u.
-template <int R>
-__device__ float zncc_slanted_w(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x, int y, float d,
+// zncc_slanted (stereo.cpp:101-125) for radius R; gxu[u + R] = gx * u. The left window comes from the
+// block's LDS tile (ta: the pixel, rows TS values apart), already converted to double.
+template <int R, int TS>
+__device__ float zncc_slanted_w(const RK_LDS double* ta, const uint8_t* b, int64_t pitch, int bw, int x, int y, float d,
                                 const float* gxu, float gy, double sa, double saa) {
     constexpr int N = 2 * R + 1;
     double sb = 0, sbb = 0, sab = 0;
     bool ok = true;
 #pragma unroll 1
     for (int v = -R; v <= R; ++v) {
-        const uint8_t* ra = a + (int64_t)(y + v) * pitch + x;
+        const RK_LDS double* ra = ta + v * TS;
         const uint8_t* rb = b + (int64_t)(y + v) * pitch;
         const float gyv = gy * (float)v;
 #pragma unroll
@@ -175,6 +177,20 @@ __device__ float zncc_slanted_w(const uint8_t* a, const uint8_t* b, int64_t pitc
 template <int RR>
 __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    // Fixed radius: the block's left pixels with an RR halo, as doubles (0 outside the image, where no
+    // scored window reaches).
+    constexpr int TW = kTw + 2 * RR, TH = kTh + 2 * RR;
+    __shared__ double tile[RR > 0 ? TW * TH : 1];
+    if constexpr (RR > 0) {
+        const int bx = (int)blockIdx.x * kTw - RR, by = (int)blockIdx.y * kTh - RR;
+        for (int i = (int)(threadIdx.y * kTw + threadIdx.x); i < TW * TH; i += kTw * kTh) {
+            const int ly = i / TW, lx = i - ly * TW;
+            const int gx = bx + lx, gy = by + ly;
+            tile[i] = (gx >= 0 && gy >= 0 && gx < R.w && gy < R.h) ? (double)R.left[(int64_t)gy * R.pitch + gx] : 0.0;
+        }
+        __syncthreads();
+    }
+    const RK_LDS double* tc = (const RK_LDS double*)tile + (threadIdx.y + RR) * TW + threadIdx.x + RR;
     if (x >= R.w || y >= R.h) return;
     const int64_t o = (int64_t)y * R.w + x;
     float out_d = kInvalid, out_c = 0.0f;
@@ -203,12 +219,22 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
             gx = clampf(gx, -0.9f, 0.9f);
             gy = clampf(gy, -0.9f, 0.9f);
             double sa = 0, saa = 0;
-            for (int v = -r; v <= r; ++v) {
-                const uint8_t* ra = R.left + (int64_t)(y + v) * R.pitch;
-                for (int u = -r; u <= r; ++u) {
-                    const double ia = ra[x + u];
-                    sa += ia;
-                    saa += ia * ia;
+            if constexpr (RR > 0) {
+                for (int v = -RR; v <= RR; ++v)
+#pragma unroll
+                    for (int u = -RR; u <= RR; ++u) {
+                        const double ia = tc[v * TW + u];
+                        sa += ia;
+                        saa += ia * ia;
+                    }
+            } else {
+                for (int v = -r; v <= r; ++v) {
+                    const uint8_t* ra = R.left + (int64_t)(y + v) * R.pitch;
+                    for (int u = -r; u <= r; ++u) {
+                        const double ia = ra[x + u];
+                        sa += ia;
+                        saa += ia * ia;
+                    }
                 }
             }
             float gxu[RR > 0 ? 2 * RR + 1 : 1];
@@ -227,7 +253,7 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
                 const float d = lo + kStep * (float)k;
                 float s;
                 if constexpr (RR > 0)
-                    s = zncc_slanted_w<RR>(R.left, R.right, R.pitch, R.w, x, y, d, gxu, gy, sa, saa);
+                    s = zncc_slanted_w<RR, TW>(tc, R.right, R.pitch, R.w, x, y, d, gxu, gy, sa, saa);
                 else
                     s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
                 if (s > best) {
@@ -292,17 +318,17 @@ __device__ __forceinline__ float detail_grad(float l, float c, float r) {
     return clampf(0.5f * (r - l), -0.9f, 0.9f);
 }
 
-// zncc_window (stereo.cpp:211-239) for an N x N window with the left window in registers;
-// gxx[i] = gx * (x0 + i - cx).
-template <int N>
-__device__ float zncc_window_w(const uint8_t* a, const uint8_t* b, int64_t pitch, int bw, int x0, int y0, float cy,
+// zncc_window (stereo.cpp:211-239) for an N x N window; gxx[i] = gx * (x0 + i - cx). The left window
+// comes from the block's LDS tile (ta: the window's first pixel, rows TS values apart) as doubles.
+template <int N, int TS>
+__device__ float zncc_window_w(const RK_LDS double* ta, const uint8_t* b, int64_t pitch, int bw, int x0, int y0, float cy,
                                float D, const float* gxx, float gy, double sa, double saa) {
     double sb = 0, sbb = 0, sab = 0;
     bool ok = true;
 #pragma unroll 1
     for (int j = 0; j < N; ++j) {
         const int Y = y0 + j;
-        const uint8_t* ra = a + (int64_t)Y * pitch + x0;
+        const RK_LDS double* ra = ta + j * TS;
         const uint8_t* rb = b + (int64_t)Y * pitch;
         const float dy = gy * ((float)Y - cy);
 #pragma unroll
@@ -325,6 +351,20 @@ __device__ float zncc_window_w(const uint8_t* a, const uint8_t* b, int64_t pitch
 template <int NW>
 __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    // Fixed window: the full-resolution left pixels of the block's windows (thread (tx, ty) starts at
+    // tile (2 tx, 2 ty)), as doubles (0 outside the image, where no scored window reaches).
+    constexpr int TW = 2 * kTw - 2 + NW, TH = 2 * kTh - 2 + NW;
+    __shared__ double tile[NW > 0 ? TW * TH : 1];
+    if constexpr (NW > 0) {
+        const int bx = 2 * (int)blockIdx.x * kTw - NW / 2 + 1, by = 2 * (int)blockIdx.y * kTh - NW / 2 + 1;
+        for (int i = (int)(threadIdx.y * kTw + threadIdx.x); i < TW * TH; i += kTw * kTh) {
+            const int ly = i / TW, lx = i - ly * TW;
+            const int gx = bx + lx, gy = by + ly;
+            tile[i] = (gx >= 0 && gy >= 0 && gx < D.W && gy < D.H) ? (double)D.fl[(int64_t)gy * D.fpitch + gx] : 0.0;
+        }
+        __syncthreads();
+    }
+    const RK_LDS double* tw = (const RK_LDS double*)tile + 2 * threadIdx.y * TW + 2 * threadIdx.x;
     if (x >= D.w || y >= D.h) return;
     const float* F = D.first;
     const int w = D.w;
@@ -339,12 +379,22 @@ __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
         const float cx = 2.0f * (float)x + 0.5f, cy = 2.0f * (float)y + 0.5f;
         const float D0 = 2.0f * d;
         double sa = 0, saa = 0;
-        for (int j = 0; j < n; ++j) {
-            const uint8_t* ra = D.fl + (int64_t)(y0 + j) * D.fpitch;
-            for (int i = 0; i < n; ++i) {
-                const double ia = ra[x0 + i];
-                sa += ia;
-                saa += ia * ia;
+        if constexpr (NW > 0) {
+            for (int j = 0; j < NW; ++j)
+#pragma unroll
+                for (int i = 0; i < NW; ++i) {
+                    const double ia = tw[j * TW + i];
+                    sa += ia;
+                    saa += ia * ia;
+                }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                const uint8_t* ra = D.fl + (int64_t)(y0 + j) * D.fpitch;
+                for (int i = 0; i < n; ++i) {
+                    const double ia = ra[x0 + i];
+                    sa += ia;
+                    saa += ia * ia;
+                }
             }
         }
         float gxx[NW > 0 ? NW : 1];
@@ -358,7 +408,7 @@ __global__ __launch_bounds__(kTw* kTh) void detail_kernel(DetailDesc D) {
             const float Dk = D0 + D.step * (float)(k - K);
             float s;
             if constexpr (NW > 0)
-                s = zncc_window_w<NW>(D.fl, D.fr, D.fpitch, D.W, x0, y0, cy, Dk, gxx, gy, sa, saa);
+                s = zncc_window_w<NW, TW>(tw, D.fr, D.fpitch, D.W, x0, y0, cy, Dk, gxx, gy, sa, saa);
             else
                 s = zncc_window(D.fl, D.fr, D.fpitch, D.W, x0, y0, n, cx, cy, Dk, gx, gy, sa, saa);
             if (s > best) {
