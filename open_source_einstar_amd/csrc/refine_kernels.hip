@@ -144,25 +144,55 @@ __device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, d
     return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
 }
 
+// Right-image samples of a fixed-window kernel come from global memory, or, when the block's samples
+// fit, from an LDS copy of the rows they read: entry c - c0 of a copied row holds bytes c and c + 1
+// (low, high), for the columns c0 .. c1 the block can reach. The left pixel is then clamped into
+// [c0, c1 - 1] instead of [0, bw - 2]: every sample inside the image reads the same two bytes, and
+// the others (whose whole window score is discarded) stay inside the copy.
+struct RightRows {
+    const uint8_t* b;  // global image: the window's first row, rows `pitch` apart
+    int64_t pitch;
+    uint32_t t;        // LDS copy: byte address of the window's first row minus 2 c0 (rows kRightCols apart)
+    float lo, hi;      // clamp of the left pixel: [0, bw - 2] or [c0, c1 - 1]
+};
+constexpr int kRightCols = 256;
+
+// The bilinear sample at xb of row `row` (counted from the window's first).
+template <bool TILE>
+__device__ __forceinline__ double sample_row(const RightRows& rr, int row, float xb) {
+    const float fl = __builtin_floorf(xb);
+    const float f = xb - fl;
+    const int x0 = (int)__builtin_amdgcn_fmed3f(fl, rr.lo, rr.hi);
+    __builtin_assume(x0 >= 0);
+    if constexpr (TILE) {
+        const uint32_t a = rr.t + (uint32_t)(row * kRightCols * 2) + 2u * (uint32_t)x0;
+        const uint32_t p = *(const RK_LDS uint16_t*)(uintptr_t)a;
+        return (float)(p & 0xFFu) * (1.0f - f) + (float)(p >> 8) * f;
+    } else {
+        const uint8_t* rb = rr.b + (int64_t)row * rr.pitch;
+        return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
+    }
+}
+
 // zncc_slanted (stereo.cpp:101-125) for radius R; gxu[u + R] = gx * u. The left window comes from the
-// block's LDS tile (ta: the pixel, rows TS values apart), already converted to double.
-template <int R, int TS>
-__device__ float zncc_slanted_w(const RK_LDS double* ta, const uint8_t* b, int64_t pitch, int bw, int x, int y, float d,
-                                const float* gxu, float gy, double sa, double saa) {
+// block's LDS tile (ta: the pixel, rows TS values apart), already converted to double; rr's rows
+// start at the window's first row.
+template <int R, int TS, bool TILE>
+__device__ float zncc_slanted_w(const RK_LDS double* ta, const RightRows& rr, int bw, int x, float d, const float* gxu,
+                                float gy, double sa, double saa) {
     constexpr int N = 2 * R + 1;
     double sb = 0, sbb = 0, sab = 0;
     bool ok = true;
 #pragma unroll 1
     for (int v = -R; v <= R; ++v) {
         const RK_LDS double* ra = ta + v * TS;
-        const uint8_t* rb = b + (int64_t)(y + v) * pitch;
         const float gyv = gy * (float)v;
 #pragma unroll
         for (int u = -R; u <= R; ++u) {
             const float xb = (float)(x + u) - ((d + gxu[u + R]) + gyv);
             if (u == -R) ok = ok && xb >= 0.0f;
             if (u == R) ok = ok && xb < (float)(bw - 1);
-            const double ib = sample(rb, bw, xb);
+            const double ib = sample_row<TILE>(rr, v + R, xb);
             const double ia = ra[u];
             sb += ib;
             sbb += ib * ib;
@@ -177,31 +207,38 @@ __device__ float zncc_slanted_w(const RK_LDS double* ta, const uint8_t* b, int64
 template <int RR>
 __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
+    const int tid = (int)(threadIdx.y * kTw + threadIdx.x);
     // Fixed radius: the block's left pixels with an RR halo, as doubles (0 outside the image, where no
-    // scored window reaches).
+    // scored window reaches), and, when they fit, the right-image columns its samples reach.
     constexpr int TW = kTw + 2 * RR, TH = kTh + 2 * RR;
     __shared__ double tile[RR > 0 ? TW * TH : 1];
+    __shared__ uint16_t rtile[RR > 0 ? TH * kRightCols : 1];
+    __shared__ int rlim[2];
+    const int bx = (int)blockIdx.x * kTw - RR, by = (int)blockIdx.y * kTh - RR;
     if constexpr (RR > 0) {
-        const int bx = (int)blockIdx.x * kTw - RR, by = (int)blockIdx.y * kTh - RR;
-        for (int i = (int)(threadIdx.y * kTw + threadIdx.x); i < TW * TH; i += kTw * kTh) {
+        for (int i = tid; i < TW * TH; i += kTw * kTh) {
             const int ly = i / TW, lx = i - ly * TW;
             const int gx = bx + lx, gy = by + ly;
             tile[i] = (gx >= 0 && gy >= 0 && gx < R.w && gy < R.h) ? (double)R.left[(int64_t)gy * R.pitch + gx] : 0.0;
         }
-        __syncthreads();
+        if (tid == 0) {
+            rlim[0] = INT_MAX;
+            rlim[1] = INT_MIN;
+        }
     }
-    const RK_LDS double* tc = (const RK_LDS double*)tile + (threadIdx.y + RR) * TW + threadIdx.x + RR;
-    if (x >= R.w || y >= R.h) return;
-    const int64_t o = (int64_t)y * R.w + x;
-    float out_d = kInvalid, out_c = 0.0f;
+    const bool inside = x < R.w && y < R.h;
     const int r = RR > 0 ? RR : R.radius;
-    if (y >= r && y < R.h - r && x >= r && x < R.w - r) {
+    const float kStep = 0.5f;
+    const int nd = min(127, (int)((float)(2 * R.search_radius) / kStep) + 1);
+    bool work = false;
+    float d0 = 0.0f, gx = 0.0f, gy = 0.0f;
+    if (inside && y >= r && y < R.h - r && x >= r && x < R.w - r) {
         const int cy = min(y / 2, R.ch - 1), cx = min(x / 2, R.cw - 1);
         const float* C = R.coarse;
-        const float d0 = C[(int64_t)cy * R.cw + cx];
+        d0 = C[(int64_t)cy * R.cw + cx];
         if (valid_d(d0)) {
+            work = true;
             // The slope lambda, stereo.cpp:159-170.
-            float gx = 0.0f, gy = 0.0f;
             if (cx > 0 && cx + 1 < R.cw) {
                 const float l = C[(int64_t)cy * R.cw + cx - 1], rr = C[(int64_t)cy * R.cw + cx + 1];
                 if (valid_d(l) && valid_d(rr)) {
@@ -218,62 +255,99 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
             }
             gx = clampf(gx, -0.9f, 0.9f);
             gy = clampf(gy, -0.9f, 0.9f);
-            double sa = 0, saa = 0;
-            if constexpr (RR > 0) {
-                for (int v = -RR; v <= RR; ++v)
+        }
+    }
+    const float lo = 2.0f * d0 - (float)R.search_radius;
+    bool tiled = false;
+    RightRows rr{R.right + (int64_t)(y - r) * R.pitch, R.pitch, 0u, 0.0f, (float)(R.w - 2)};
+    if constexpr (RR > 0) {
+        __syncthreads();
+        if (work) {
+            // Columns of this pixel's samples, with a margin of two beyond the slopes' reach.
+            const float m = 1.8f * (float)RR + 2.0f;
+            atomicMin(&rlim[0], (int)__builtin_floorf((float)(x - RR) - (lo + kStep * (float)(nd - 1)) - m));
+            atomicMax(&rlim[1], (int)__builtin_floorf((float)(x + RR) - lo + m) + 1);
+        }
+        __syncthreads();
+        const int c0 = min(max(rlim[0], 0), R.w - 2), c1 = min(max(rlim[1], c0 + 1), R.w - 1);
+        tiled = rlim[0] <= rlim[1] && c1 - c0 + 1 <= kRightCols;
+        if (tiled) {
+            const int nc = c1 - c0 + 1;
+            for (int i = tid; i < TH * nc; i += kTw * kTh) {
+                const int ly = i / nc, c = c0 + (i - ly * nc), gy = by + ly;
+                uint32_t p = 0;
+                if (gy >= 0 && gy < R.h) {
+                    const uint8_t* row = R.right + (int64_t)gy * R.pitch;
+                    p = (uint32_t)row[c] | (c + 1 < R.w ? (uint32_t)row[c + 1] << 8 : 0u);
+                }
+                rtile[ly * kRightCols + (c - c0)] = (uint16_t)p;
+            }
+            // (Unsigned arithmetic: the offset by -2 c0 wraps, the sampled addresses stay in the copy.)
+            rr.t = (uint32_t)(uintptr_t)((const RK_LDS uint16_t*)rtile + ((int)threadIdx.y + RR - r) * kRightCols) -
+                   2u * (uint32_t)c0;
+            rr.lo = (float)c0;
+            rr.hi = (float)(c1 - 1);
+        }
+        __syncthreads();
+    }
+    if (!inside) return;
+    const int64_t o = (int64_t)y * R.w + x;
+    float out_d = kInvalid, out_c = 0.0f;
+    if (work) {
+        const RK_LDS double* tc = (const RK_LDS double*)tile + (threadIdx.y + RR) * TW + threadIdx.x + RR;
+        double sa = 0, saa = 0;
+        if constexpr (RR > 0) {
+            for (int v = -RR; v <= RR; ++v)
 #pragma unroll
-                    for (int u = -RR; u <= RR; ++u) {
-                        const double ia = tc[v * TW + u];
-                        sa += ia;
-                        saa += ia * ia;
-                    }
+                for (int u = -RR; u <= RR; ++u) {
+                    const double ia = tc[v * TW + u];
+                    sa += ia;
+                    saa += ia * ia;
+                }
+        } else {
+            for (int v = -r; v <= r; ++v) {
+                const uint8_t* ra = R.left + (int64_t)(y + v) * R.pitch;
+                for (int u = -r; u <= r; ++u) {
+                    const double ia = ra[x + u];
+                    sa += ia;
+                    saa += ia * ia;
+                }
+            }
+        }
+        float gxu[RR > 0 ? 2 * RR + 1 : 1];
+        if constexpr (RR > 0) {
+#pragma unroll
+            for (int u = -RR; u <= RR; ++u) gxu[u + RR] = gx * (float)u;
+        }
+        // Only the winner and its two neighbours' scores matter afterwards.
+        float best = -2.0f, prev = -3.0f, best_prev = -3.0f, best_next = -3.0f;
+        int best_k = -1;
+        for (int k = 0; k < nd; ++k) {
+            const float d = lo + kStep * (float)k;
+            float s;
+            if constexpr (RR > 0) {
+                if (tiled) s = zncc_slanted_w<RR, TW, true>(tc, rr, R.w, x, d, gxu, gy, sa, saa);
+                else s = zncc_slanted_w<RR, TW, false>(tc, rr, R.w, x, d, gxu, gy, sa, saa);
             } else {
-                for (int v = -r; v <= r; ++v) {
-                    const uint8_t* ra = R.left + (int64_t)(y + v) * R.pitch;
-                    for (int u = -r; u <= r; ++u) {
-                        const double ia = ra[x + u];
-                        sa += ia;
-                        saa += ia * ia;
-                    }
-                }
+                s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
             }
-            float gxu[RR > 0 ? 2 * RR + 1 : 1];
-            if constexpr (RR > 0) {
-#pragma unroll
-                for (int u = -RR; u <= RR; ++u) gxu[u + RR] = gx * (float)u;
+            if (s > best) {
+                best = s;
+                best_k = k;
+                best_prev = prev;
+            } else if (k == best_k + 1) {
+                best_next = s;
             }
-            const float kStep = 0.5f;
-            const float center = 2.0f * d0;
-            const int nd = min(127, (int)((float)(2 * R.search_radius) / kStep) + 1);
-            const float lo = center - (float)R.search_radius;
-            // Only the winner and its two neighbours' scores matter afterwards.
-            float best = -2.0f, prev = -3.0f, best_prev = -3.0f, best_next = -3.0f;
-            int best_k = -1;
-            for (int k = 0; k < nd; ++k) {
-                const float d = lo + kStep * (float)k;
-                float s;
-                if constexpr (RR > 0)
-                    s = zncc_slanted_w<RR, TW>(tc, R.right, R.pitch, R.w, x, y, d, gxu, gy, sa, saa);
-                else
-                    s = zncc_slanted(R.left, R.right, R.pitch, R.w, x, y, d, gx, gy, r, sa, saa);
-                if (s > best) {
-                    best = s;
-                    best_k = k;
-                    best_prev = prev;
-                } else if (k == best_k + 1) {
-                    best_next = s;
-                }
-                prev = s;
-            }
-            bool keep = !(best_k < 0 || best < R.min_zncc);
-            if (keep && R.reject_edge && (best_k == 0 || best_k == nd - 1)) keep = false;
-            if (keep) {
-                float d = lo + kStep * (float)best_k;
-                if (R.subpixel && best_k > 0 && best_k + 1 < nd && best_prev > -2.0f && best_next > -2.0f)
-                    d += kStep * subpixel_offset(1.0f - best_prev, 1.0f - best, 1.0f - best_next);
-                out_d = d;
-                out_c = best;
-            }
+            prev = s;
+        }
+        bool keep = !(best_k < 0 || best < R.min_zncc);
+        if (keep && R.reject_edge && (best_k == 0 || best_k == nd - 1)) keep = false;
+        if (keep) {
+            float d = lo + kStep * (float)best_k;
+            if (R.subpixel && best_k > 0 && best_k + 1 < nd && best_prev > -2.0f && best_next > -2.0f)
+                d += kStep * subpixel_offset(1.0f - best_prev, 1.0f - best, 1.0f - best_next);
+            out_d = d;
+            out_c = best;
         }
     }
     R.disp[o] = out_d;
