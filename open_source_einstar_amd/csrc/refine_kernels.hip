@@ -144,15 +144,15 @@ __device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, d
     return (float)((sab - sa * sb / n) / __builtin_sqrt(va * vb));
 }
 
-// Right-image samples of a fixed-window kernel come from global memory, or, when the block's samples
-// fit, from an LDS copy of the rows they read: entry c - c0 of a copied row holds bytes c and c + 1
-// (low, high), for the columns c0 .. c1 the block can reach. The left pixel is then clamped into
+// Right-image samples of the fixed-radius refine kernel come from global memory, or, when the block's
+// samples fit, from an LDS copy of the rows they read: entry c - c0 of a copied row holds pixel c as a
+// float, for the columns c0 .. c1 the block can reach. The left pixel is then clamped into
 // [c0, c1 - 1] instead of [0, bw - 2]: every sample inside the image reads the same two bytes, and
 // the others (whose whole window score is discarded) stay inside the copy.
 struct RightRows {
     const uint8_t* b;  // global image: the window's first row, rows `pitch` apart
     int64_t pitch;
-    uint32_t t;        // LDS copy: byte address of the window's first row minus 2 c0 (rows kRightCols apart)
+    uint32_t t;        // LDS copy: byte address of the window's first row minus 4 c0 (rows kRightCols apart)
     float lo, hi;      // clamp of the left pixel: [0, bw - 2] or [c0, c1 - 1]
 };
 constexpr int kRightCols = 256;
@@ -165,9 +165,8 @@ __device__ __forceinline__ double sample_row(const RightRows& rr, int row, float
     const int x0 = (int)__builtin_amdgcn_fmed3f(fl, rr.lo, rr.hi);
     __builtin_assume(x0 >= 0);
     if constexpr (TILE) {
-        const uint32_t a = rr.t + (uint32_t)(row * kRightCols * 2) + 2u * (uint32_t)x0;
-        const uint32_t p = *(const RK_LDS uint16_t*)(uintptr_t)a;
-        return (float)(p & 0xFFu) * (1.0f - f) + (float)(p >> 8) * f;
+        const RK_LDS float* t = (const RK_LDS float*)(uintptr_t)(rr.t + (uint32_t)(row * kRightCols * 4) + 4u * (uint32_t)x0);
+        return t[0] * (1.0f - f) + t[1] * f;
     } else {
         const uint8_t* rb = rr.b + (int64_t)row * rr.pitch;
         return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
@@ -212,7 +211,7 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
     // scored window reaches), and, when they fit, the right-image columns its samples reach.
     constexpr int TW = kTw + 2 * RR, TH = kTh + 2 * RR;
     __shared__ double tile[RR > 0 ? TW * TH : 1];
-    __shared__ uint16_t rtile[RR > 0 ? TH * kRightCols : 1];
+    __shared__ float rtile[RR > 0 ? TH * kRightCols : 1];
     __shared__ int rlim[2];
     const int bx = (int)blockIdx.x * kTw - RR, by = (int)blockIdx.y * kTh - RR;
     if constexpr (RR > 0) {
@@ -275,16 +274,11 @@ __global__ __launch_bounds__(kTw* kTh) void refine_kernel(RefineDesc R) {
             const int nc = c1 - c0 + 1;
             for (int i = tid; i < TH * nc; i += kTw * kTh) {
                 const int ly = i / nc, c = c0 + (i - ly * nc), gy = by + ly;
-                uint32_t p = 0;
-                if (gy >= 0 && gy < R.h) {
-                    const uint8_t* row = R.right + (int64_t)gy * R.pitch;
-                    p = (uint32_t)row[c] | (c + 1 < R.w ? (uint32_t)row[c + 1] << 8 : 0u);
-                }
-                rtile[ly * kRightCols + (c - c0)] = (uint16_t)p;
+                rtile[ly * kRightCols + (c - c0)] = (gy >= 0 && gy < R.h) ? (float)R.right[(int64_t)gy * R.pitch + c] : 0.0f;
             }
-            // (Unsigned arithmetic: the offset by -2 c0 wraps, the sampled addresses stay in the copy.)
-            rr.t = (uint32_t)(uintptr_t)((const RK_LDS uint16_t*)rtile + ((int)threadIdx.y + RR - r) * kRightCols) -
-                   2u * (uint32_t)c0;
+            // (Unsigned arithmetic: the offset by -4 c0 wraps, the sampled addresses stay in the copy.)
+            rr.t = (uint32_t)(uintptr_t)((const RK_LDS float*)rtile + ((int)threadIdx.y + RR - r) * kRightCols) -
+                   4u * (uint32_t)c0;
             rr.lo = (float)c0;
             rr.hi = (float)(c1 - 1);
         }
