@@ -347,17 +347,32 @@ __device__ __forceinline__ void load_raw(const SGM_GLOBAL uint8_t* src, uint32_t
     }
 }
 
-// Adds the bytes of `w` (disparities k0..k0+DPL-1) to the packed u16 sums.
-template <int DPL>
-__device__ __forceinline__ void add_raw(const uint32_t (&w)[(DPL + 3) / 4], uint32_t (&s)[DPL / 2]) {
+// S = the sum over the NDIRS directions of the path-cost bytes (disparities k0..k0+DPL-1), as packed
+// u16 pairs. Each word's even and odd bytes are unpacked into u16 lanes by one v_perm each and summed
+// with three-input adds: a sum of at most 8 bytes never carries out of its 16-bit lane.
+template <int DPL, int NDIRS>
+__device__ __forceinline__ void sum_dirs(const uint32_t (&w)[kMaxDirs][(DPL + 3) / 4], uint32_t (&s)[DPL / 2]) {
+    static_assert(NDIRS % 2 == 0, "directions come in pairs");
 #pragma unroll
     for (int i = 0; i < DPL / 4; ++i) {
-        s[2 * i] = pk_add(s[2 * i], __builtin_amdgcn_perm(0u, w[i], 0x0C010C00u));
-        s[2 * i + 1] = pk_add(s[2 * i + 1], __builtin_amdgcn_perm(0u, w[i], 0x0C030C02u));
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t sel = h ? 0x0C030C02u : 0x0C010C00u;
+            uint32_t acc = __builtin_amdgcn_perm(0u, w[0][i], sel) + __builtin_amdgcn_perm(0u, w[1][i], sel);
+#pragma unroll
+            for (int d = 2; d < NDIRS; d += 2)
+                acc = acc + __builtin_amdgcn_perm(0u, w[d][i], sel) + __builtin_amdgcn_perm(0u, w[d + 1][i], sel);
+            s[2 * i + h] = acc;
+        }
     }
     if constexpr (DPL % 4 == 2) {
-        const uint32_t q = w[DPL / 4];
-        s[DPL / 2 - 1] = pk_add(s[DPL / 2 - 1], (q & 0xFFu) | ((q & 0xFF00u) << 8));
+        uint32_t acc = 0;
+#pragma unroll
+        for (int d = 0; d < NDIRS; ++d) {
+            const uint32_t q = w[d][DPL / 4];
+            acc += (q & 0xFFu) | ((q & 0xFF00u) << 8);
+        }
+        s[DPL / 2 - 1] = acc;
     }
 }
 
@@ -857,12 +872,23 @@ __device__ __forceinline__ void load_row_in(RowIn<DPL>& in, const SGM_GLOBAL uin
 
 __device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
 
-__device__ __forceinline__ uint32_t row_max(uint32_t v) {
-    v = max(v, dpp_id<kQuadSwap1>(v, 0u));
-    v = max(v, dpp_id<kQuadSwap2>(v, 0u));
-    v = max(v, dpp_id<kRowHalfMirror>(v, 0u));
-    v = max(v, dpp_id<kRowMirror>(v, 0u));
+// OR over the 16 lanes of a DPP row, in every lane.
+__device__ __forceinline__ uint32_t row_or(uint32_t v) {
+    v |= dpp_id<kQuadSwap1>(v, 0u);
+    v |= dpp_id<kQuadSwap2>(v, 0u);
+    v |= dpp_id<kRowHalfMirror>(v, 0u);
+    v |= dpp_id<kRowMirror>(v, 0u);
     return v;
+}
+
+// S of the lane's element e (disparity k0 + e) from its packed pairs, 0 when e is not in [0, DPL).
+template <int DPL>
+__device__ __forceinline__ uint32_t lane_elem(const uint32_t (&s)[DPL / 2], int e) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int r = 0; r < DPL / 2; ++r) w = (e >> 1) == r ? s[r] : w;
+    w = (e & 1) ? (w >> 16) : (w & 0xFFFFu);
+    return (unsigned)e < (unsigned)DPL ? w : 0u;
 }
 
 // Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
@@ -897,10 +923,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     // Sums, left WTA with uniqueness and subpixel (stereo.cpp:412-432); no memory side effects, so
     // the computations of several pixels can overlap.
     auto compute = [&](const RowIn<DPL>& in, int x, PxOut& o) {
-#pragma unroll
-        for (int r = 0; r < DPL / 2; ++r) o.s[r] = 0;
-#pragma unroll
-        for (int d = 0; d < NDIRS; ++d) add_raw<DPL>(in.w[d], o.s);
+        sum_dirs<DPL, NDIRS>(in.w, o.s);
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -914,13 +937,17 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
                 key = min(key, ((j & 1) ? (o.s[j >> 1] & 0xFFFF0000u) : (o.s[j >> 1] << 16)) | (uint32_t)(k0 + j));
             key = row_min(key);
             const int bk = (int)(key & 0xFFFFu);
-            // Second best over |k - bk| > 1: the excluded halves are forced to 0xFFFF.
+            // Second best over |k - bk| > 1: the excluded halves are forced to 0xFFFF. Bit e of xm marks
+            // element e (disparity k0 + e) as one of bk - 1, bk, bk + 1.
+            // (The shift reaches DPL + 3 bits: a 64-bit word where that passes 31.)
+            using Mask = std::conditional_t<(DPL + 3 < 32), uint32_t, uint64_t>;
+            const int t = min(max(bk - k0, -2), DPL + 1);
+            const Mask xm = ((Mask)7 << (t + 2)) >> 3;
             uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
             for (int r = 0; r < DPL / 2; ++r) {
-                const int ka = k0 + 2 * r;
-                const uint32_t ex = ((unsigned)(ka - bk + 1) <= 2u ? 0xFFFFu : 0u) | ((unsigned)(ka - bk + 2) <= 2u ? 0xFFFF0000u : 0u);
-                m = pk_min(m, o.s[r] | ex);
+                const uint32_t lo = 0u - (uint32_t)((xm >> (2 * r)) & 1u), hi = 0u - (uint32_t)((xm >> (2 * r + 1)) & 1u);
+                m = pk_min(m, o.s[r] | (lo & 0xFFFFu) | (hi & 0xFFFF0000u));
             }
             sec = min(m & 0xFFFFu, m >> 16);
         } else {
@@ -949,17 +976,10 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
                 st |= 2;
                 d = (float)(min_d + bk);
                 if (c.subpixel && bk > min_k && bk + 1 < max_k) {  // stereo.cpp:430-431
-                    // S(bk-1) and S(bk+1): the owning lanes contribute them, the others 0.
-                    uint32_t vm = 0, vp = 0;
-#pragma unroll
-                    for (int j = 0; j < DPL; ++j) {
-                        const uint32_t v = half16(o.s[j >> 1], j & 1);
-                        vm = (k0 + j == bk - 1) ? v : vm;
-                        vp = (k0 + j == bk + 1) ? v : vp;
-                    }
-                    vm = row_max(vm);
-                    vp = row_max(vp);
-                    d += subpixel_offset((float)vm, (float)bS, (float)vp);
+                    // S(bk-1) and S(bk+1) in one word: the owning lanes contribute their halves, the
+                    // others 0, and one OR over the row gathers both.
+                    const uint32_t vv = row_or((lane_elem<DPL>(o.s, bk - 1 - k0) << 16) | lane_elem<DPL>(o.s, bk + 1 - k0));
+                    d += subpixel_offset((float)(vv >> 16), (float)bS, (float)(vv & 0xFFFFu));
                 }
             }
         }

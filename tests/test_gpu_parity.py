@@ -98,6 +98,15 @@ def test_every_lane_layout(nd):
              p1=10, p2=120)
 
 
+def test_wide_lanes_second_best_full_window():
+    """512 disparities (32 per lane) on an image whose right part sees every disparity, so the row
+    kernel's full-window branch picks best disparities in every lane slot (1,051 pixels in slots 30
+    and 31, where the second-best exclusion reaches past bit 31 of a 32-bit mask), with a strict
+    uniqueness ratio so that the second-best cost decides many pixels."""
+    run_case(1100, 48, 512, 77, census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=True,
+             uniqueness=0.95)
+
+
 def test_config1_640x400x64_four_paths():
     run_case(640, 400, 64, 1, census_radius_x=4, census_radius_y=3, p1=10, p2=120)
 
