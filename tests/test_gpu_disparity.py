@@ -70,6 +70,15 @@ STAGES = [
 ]
 
 
+def test_refine_blocks_beyond_the_lds_copy():
+    """A 480-disparity scene with many depth edges: some 16x16 refine blocks reach more right-image
+    columns than the kernel's LDS copy holds (26 blocks by the ground truth's spread) and sample global
+    memory instead, in the same launch as blocks that use the copy."""
+    ref, got, t = run(640, 480, 480, 91, dict(OFF, pyramid_levels=1), 240, cells=(12, 24))
+    assert t["refine_ms"] > 0
+    assert (ref["disparity"] > -1e8).mean() > 0.15  # about 64,000 refined pixels compared
+
+
 @pytest.mark.parametrize("name,w,h,nd,stereo,nd_top,full", STAGES, ids=[c[0] for c in STAGES])
 def test_stage_parity(name, w, h, nd, stereo, nd_top, full):
     ref, got, t = run(w, h, nd, 100 + len(name), stereo, nd_top, full)
