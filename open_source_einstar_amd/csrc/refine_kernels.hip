@@ -147,8 +147,9 @@ __device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, d
 // Right-image samples of the fixed-radius refine kernel come from global memory, or, when the block's
 // samples fit, from an LDS copy of the rows they read: entry c - c0 of a copied row holds pixel c as a
 // float, for the columns c0 .. c1 the block can reach. The left pixel is then clamped into
-// [c0, c1 - 1] instead of [0, bw - 2]: every sample inside the image reads the same two bytes, and
-// the others (whose whole window score is discarded) stay inside the copy.
+// [c0, c1 - 1] instead of [0, bw - 2]: every sample inside the image reads the same two pixels (a
+// byte converted to float is exact), and the others (whose whole window score is discarded) stay
+// inside the copy.
 struct RightRows {
     const uint8_t* b;  // global image: the window's first row, rows `pitch` apart
     int64_t pitch;

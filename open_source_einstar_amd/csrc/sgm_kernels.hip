@@ -135,7 +135,7 @@ __device__ __forceinline__ SGM_LDS T* lds(T* p) {
 // where p - r_k leaves the image; no path steps from there). A step along -r_k uses byte k of the
 // pixel it leaves. Every penalty fits a byte (cells + penalty <= 255 is enforced).
 
-constexpr int kTile = 16;
+constexpr int kTile = 16;  // median tiles
 constexpr int kMaxRx = 4, kMaxRy = 3;
 // A census block covers 64 x 16 pixels; each thread takes four adjacent pixels of one row (16
 // threads a row, a wave four rows). The LDS tile holds the block's pixels with a halo of kMaxRx
@@ -145,8 +145,8 @@ constexpr int kCenW = 64, kCenH = 16, kCenPx = 4;
 constexpr int kTileW = kCenW + 2 * kMaxRx, kTileH = kCenH + 2 * kMaxRy;
 static_assert(kCenW / kCenPx * kCenH == kBlock, "one census thread per four pixels");
 
-// Bits 8k .. 8k + 3 (k = 0 .. 3) of the census words of pixels 2q and 2q + 1, gathered from byte 1
-// and byte 3 of the octet words t0 .. t3.
+// 32 bits of the census words of pixels 2q and 2q + 1: byte k of `even` is byte 1 of t_k, byte k of
+// `odd` is byte 3 of t_k (t_k holds bits 8k .. 8k + 7 of both words, relative to the 32-bit half).
 __device__ __forceinline__ void census_gather(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t& even,
                                               uint32_t& odd) {
     const uint32_t p01 = __builtin_amdgcn_perm(t1, t0, 0x07030501u);  // t0.b1 t1.b1 t0.b3 t1.b3
