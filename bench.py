@@ -317,8 +317,8 @@ def main():
                     help="first run one plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the path shards by pair with no exchange between GPUs, and this "
-              f"build runs on single-GPU boxes only (asked for --gpus {args.gpus}).", file=sys.stderr)
+        print(f"bench.py measures one GPU: pairs shard across GPUs with no exchange between them, and "
+              f"multi-GPU runs are not supported (asked for --gpus {args.gpus}).", file=sys.stderr)
         return 2
 
     from open_source_einstar_amd import sgm
