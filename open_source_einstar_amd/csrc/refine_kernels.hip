@@ -551,14 +551,30 @@ __global__ __launch_bounds__(kTw* kTh) void spike_kernel(const float* in, float*
 // find the same components. Labels are pixel indices and every label points at a smaller or equal
 // index, so every find ends; unions link the larger root under the smaller by atomic min.
 //
-// Three passes: (1) each 16x16 tile is labelled in LDS, every pixel then points at its tile-local
-// root and each local root holds its local pixel count; (2) edges that cross tile borders unite the
-// local roots in global memory (chains are at most a few tiles long); (3) each local root adds its
-// count to its global root and points straight at it, so a pixel's global root is L[L[i]].
+// Three passes: (1) each 64x64 tile is labelled in LDS by one block of 1024 threads, every pixel then
+// points at its tile-local root and each local root holds its local pixel count; (2) edges that cross
+// tile borders unite the local roots in global memory (large tiles keep these unions few and their
+// chains short); (3) each local root adds its count to its global root and points straight at it, so
+// a pixel's global root is L[L[i]].
 
 template <typename T>
 __device__ __forceinline__ T atomic_ld(const T* p, int scope) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, scope);
+}
+
+// find_root with path halving: each node passed is pointed at its grandparent by an atomic minimum, so
+// it only ever moves to a smaller ancestor and never undoes a concurrent union. (The tile-local pass
+// uses it; in the global passes the extra atomic per hop cost more than the shorter paths saved.)
+template <int Scope>
+__device__ __forceinline__ int32_t find_root_h(int32_t* L, int32_t a) {
+    for (;;) {
+        const int32_t p = __hip_atomic_load(L + a, __ATOMIC_RELAXED, Scope);
+        if (p == a) return a;
+        const int32_t gp = __hip_atomic_load(L + p, __ATOMIC_RELAXED, Scope);
+        if (gp == p) return p;
+        __hip_atomic_fetch_min(L + a, gp, __ATOMIC_RELAXED, Scope);
+        a = gp;
+    }
 }
 
 template <int Scope>
@@ -574,8 +590,8 @@ __device__ __forceinline__ int32_t find_root(const int32_t* L, int32_t a) {
 template <int Scope>
 __device__ void unite(int32_t* L, int32_t a, int32_t b) {
     for (;;) {
-        a = find_root<Scope>(L, a);
-        b = find_root<Scope>(L, b);
+        a = Scope == __HIP_MEMORY_SCOPE_WORKGROUP ? find_root_h<Scope>(L, a) : find_root<Scope>(L, a);
+        b = Scope == __HIP_MEMORY_SCOPE_WORKGROUP ? find_root_h<Scope>(L, b) : find_root<Scope>(L, b);
         if (a == b) return;
         if (a < b) {
             const int32_t old = atomicMin(L + b, a);
@@ -593,41 +609,61 @@ __device__ __forceinline__ bool joined(float d, float e, float max_diff) {
     return valid_d(e) && !(fabsf(e - d) > max_diff);
 }
 
-__global__ __launch_bounds__(kTw* kTh) void ccl_local_kernel(const float* disp, int32_t* L, uint32_t* local, uint32_t* sizes,
-                                                             int w, int h, float max_diff) {
-    __shared__ int32_t ll[kTw * kTh];
-    __shared__ uint32_t cnt[kTw * kTh];
-    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * kTw + tx;
-    const int x = (int)blockIdx.x * kTw + tx, y = (int)blockIdx.y * kTh + ty;
-    const bool in = x < w && y < h;
-    const int32_t i = y * w + x;
-    const float d = in ? disp[i] : kInvalid;
-    const bool v = valid_d(d);
-    ll[t] = v ? t : -1;
-    cnt[t] = 0;
-    __syncthreads();
-    if (v) {
-        if (tx + 1 < kTw && x + 1 < w && joined(d, disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + 1);
-        if (ty + 1 < kTh && y + 1 < h && joined(d, disp[i + w], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + kTw);
+constexpr int kCcl = 64, kCclThreads = 1024, kCclPx = kCcl * kCcl / kCclThreads;
+
+__global__ __launch_bounds__(kCclThreads) void ccl_local_kernel(const float* disp, int32_t* L, uint32_t* local,
+                                                                uint32_t* sizes, int w, int h, float max_diff) {
+    __shared__ int32_t ll[kCcl * kCcl];
+    __shared__ uint32_t cnt[kCcl * kCcl];
+    const int bx = (int)blockIdx.x * kCcl, by = (int)blockIdx.y * kCcl;
+    // Thread tid takes tile pixels tid + j * 1024: a wave covers 64 consecutive pixels of a row.
+    float d[kCclPx];
+    bool v[kCclPx];
+#pragma unroll
+    for (int j = 0; j < kCclPx; ++j) {
+        const int t = (int)threadIdx.x + j * kCclThreads, x = bx + (t & (kCcl - 1)), y = by + t / kCcl;
+        d[j] = x < w && y < h ? disp[y * w + x] : kInvalid;
+        v[j] = valid_d(d[j]);
+        ll[t] = v[j] ? t : -1;
+        cnt[t] = 0;
     }
     __syncthreads();
-    int32_t r = -1;
-    if (v) {
-        r = find_root<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t);
-        atomicAdd(&cnt[r], 1u);
+#pragma unroll
+    for (int j = 0; j < kCclPx; ++j) {
+        const int t = (int)threadIdx.x + j * kCclThreads, lx = t & (kCcl - 1), ly = t / kCcl;
+        const int x = bx + lx, y = by + ly, i = y * w + x;
+        if (!v[j]) continue;
+        if (lx + 1 < kCcl && x + 1 < w && joined(d[j], disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + 1);
+        if (ly + 1 < kCcl && y + 1 < h && joined(d[j], disp[i + w], max_diff))
+            unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + kCcl);
     }
     __syncthreads();
-    if (!in) return;
-    const int32_t root = v ? ((int32_t)blockIdx.y * kTh + r / kTw) * w + (int32_t)blockIdx.x * kTw + r % kTw : -1;
-    L[i] = root;
-    local[i] = v && r == t ? cnt[t] : 0u;
-    sizes[i] = 0;
+    int32_t r[kCclPx];
+#pragma unroll
+    for (int j = 0; j < kCclPx; ++j) {
+        const int t = (int)threadIdx.x + j * kCclThreads;
+        r[j] = -1;
+        if (v[j]) {
+            r[j] = find_root<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t);
+            atomicAdd(&cnt[r[j]], 1u);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kCclPx; ++j) {
+        const int t = (int)threadIdx.x + j * kCclThreads, x = bx + (t & (kCcl - 1)), y = by + t / kCcl;
+        if (x >= w || y >= h) continue;
+        const int32_t i = y * w + x;
+        L[i] = v[j] ? (by + r[j] / kCcl) * w + bx + (r[j] & (kCcl - 1)) : -1;
+        local[i] = v[j] && r[j] == t ? cnt[t] : 0u;
+        sizes[i] = 0;
+    }
 }
 
 __global__ __launch_bounds__(kTw* kTh) void ccl_border_kernel(const float* disp, int32_t* L, int w, int h, float max_diff) {
     const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
     if (x >= w || y >= h) return;
-    const bool right = (threadIdx.x == kTw - 1) && x + 1 < w, down = (threadIdx.y == kTh - 1) && y + 1 < h;
+    const bool right = (x % kCcl == kCcl - 1) && x + 1 < w, down = (y % kCcl == kCcl - 1) && y + 1 < h;
     if (!right && !down) return;
     const int32_t i = y * w + x;
     const float d = disp[i];
@@ -747,7 +783,8 @@ hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, h
 hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* local, uint32_t* sizes, int w, int h, float max_diff,
                       hipStream_t stream) {
     if (w <= 0 || h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ccl_local_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, local, sizes, w, h, max_diff);
+    const dim3 tiles((unsigned)((w + kCcl - 1) / kCcl), (unsigned)((h + kCcl - 1) / kCcl));
+    hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(kCclThreads), 0, stream, disp, labels, local, sizes, w, h, max_diff);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ccl_border_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, w, h, max_diff);
