@@ -356,7 +356,8 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     const size_t npx = (size_t)cfg->max_width * cfg->max_height;
     for (auto& s : sv->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
-        for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&s.ev[i]);
+        // Timing only (a request completes by a stream synchronisation): no system-scope fence.
+        for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreateWithFlags(&s.ev[i], hipEventDisableSystemFence);
     }
     if (e == hipSuccess) e = sv->add_pinned_class(2 * npx, 2 * cfg->slots + 2);
     if (e != hipSuccess) {
