@@ -65,6 +65,9 @@ STAGES = [
     ("spikes", 256, 192, 48, dict(OFF, pyramid_levels=0, filter_spikes=True), 48, False),
     ("speckles", 256, 192, 48, dict(OFF, pyramid_levels=0, speckle_max_region_size=60), 48, False),
     ("speckles-diff-0.5", 256, 192, 48, dict(OFF, pyramid_levels=1, speckle_max_region_size=200, speckle_max_diff=0.5), 32, False),
+    # sizes that leave partial 64x64 speckle tiles on the right and bottom (546 and 407 pixels removed)
+    ("speckles-250x170", 250, 170, 48, dict(OFF, pyramid_levels=0, speckle_max_region_size=60), 48, False),
+    ("speckles-201x77", 201, 77, 48, dict(OFF, pyramid_levels=0, speckle_max_region_size=60), 48, False),
     ("defaults", 640, 512, 96, dict(), 32, True),
     ("defaults-no-full-pair", 640, 512, 96, dict(), 32, False),
 ]
