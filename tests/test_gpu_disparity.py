@@ -61,6 +61,8 @@ STAGES = [
     ("refine-keep-edge-winners", 256, 192, 48, dict(OFF, pyramid_levels=1, refine_reject_edge_winners=False), 32, False),
     ("detail-only", 256, 192, 48, dict(OFF, pyramid_levels=0, detail_enabled=True), 48, True),
     ("detail-after-refine", 320, 256, 64, dict(OFF, pyramid_levels=1, detail_enabled=True), 40, True),
+    # partial 16x16 refine and detail blocks (and their LDS tiles) at the right and bottom edges
+    ("refine-detail-250x170", 250, 170, 64, dict(OFF, pyramid_levels=1, detail_enabled=True), 40, True),
     ("occlusion", 256, 192, 48, dict(OFF, pyramid_levels=0, filter_occlusion=True), 48, False),
     ("spikes", 256, 192, 48, dict(OFF, pyramid_levels=0, filter_spikes=True), 48, False),
     ("speckles", 256, 192, 48, dict(OFF, pyramid_levels=0, speckle_max_region_size=60), 48, False),
