@@ -44,15 +44,16 @@ __device__ __forceinline__ float median9(const float* in, int w, int h, int x, i
             buf[dy * 3 + dx + 4] = ok ? q : __builtin_inff();  // +inf sorts after every valid value
             n += ok ? 1 : 0;
         }
+    // A 25-comparator sorting network (it sorts every 0/1 input, so every input).
+    constexpr int kNet[25][2] = {{0, 3}, {1, 7}, {2, 5}, {4, 8}, {0, 7}, {2, 4}, {3, 8}, {5, 6}, {0, 2},
+                                 {1, 3}, {4, 5}, {7, 8}, {1, 4}, {3, 6}, {5, 7}, {0, 1}, {2, 4}, {3, 5},
+                                 {6, 8}, {2, 3}, {4, 5}, {6, 7}, {1, 2}, {3, 4}, {5, 6}};
 #pragma unroll
-    for (int round = 0; round < 9; ++round)
-#pragma unroll
-        for (int i = round & 1; i + 1 < 9; i += 2) {
-            const float lo = fminf(buf[i], buf[i + 1]);
-            const float hi = fmaxf(buf[i], buf[i + 1]);
-            buf[i] = lo;
-            buf[i + 1] = hi;
-        }
+    for (int c = 0; c < 25; ++c) {
+        const float lo = fminf(buf[kNet[c][0]], buf[kNet[c][1]]), hi = fmaxf(buf[kNet[c][0]], buf[kNet[c][1]]);
+        buf[kNet[c][0]] = lo;
+        buf[kNet[c][1]] = hi;
+    }
     float sel = buf[0];
 #pragma unroll
     for (int i = 1; i < 9; ++i) sel = (n / 2) == i ? buf[i] : sel;

@@ -1088,39 +1088,51 @@ __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c
 // ------------------------------------------------------------------------------------ median
 // median3, stereo.cpp:129-147: the n/2-th smallest (0-based) of the valid in-image 3x3 values.
 
-__device__ __forceinline__ void median_body(const PairDesc& P, int lb) {
+// Sorts 9 values ascending with a 25-comparator network (depth 9; it sorts every 0/1 input, so every
+// input). Any sorting network gives the same values in the same order.
+__device__ __forceinline__ void sort9(float (&b)[9]) {
+    constexpr int kNet[25][2] = {{0, 3}, {1, 7}, {2, 5}, {4, 8}, {0, 7}, {2, 4}, {3, 8}, {5, 6}, {0, 2},
+                                 {1, 3}, {4, 5}, {7, 8}, {1, 4}, {3, 6}, {5, 7}, {0, 1}, {2, 4}, {3, 5},
+                                 {6, 8}, {2, 3}, {4, 5}, {6, 7}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+    for (int c = 0; c < 25; ++c) {
+        const float lo = fminf(b[kNet[c][0]], b[kNet[c][1]]), hi = fmaxf(b[kNet[c][0]], b[kNet[c][1]]);
+        b[kNet[c][0]] = lo;
+        b[kNet[c][1]] = hi;
+    }
+}
+
+// One 16x16 tile of the disparity map; its pixels and a one-pixel halo are read once into LDS
+// (invalid outside the image). Every thread of the block calls it (it synchronises once).
+__device__ __forceinline__ void median_body(const PairDesc& P, int lb, SGM_LDS float* tile) {
+    constexpr int kT = kTile + 2;
     const int tx_n = (P.w + kTile - 1) / kTile;
-    const int x = (lb % tx_n) * kTile + (threadIdx.x & (kTile - 1));
-    const int y = (lb / tx_n) * kTile + (threadIdx.x >> 4);
-    if (x >= P.w || y >= P.h) return;
+    const int x0 = (lb % tx_n) * kTile, y0 = (lb / tx_n) * kTile;
     const float* in = P.disp;
-    const float v = in[(int64_t)y * P.w + x];
+    for (int i = threadIdx.x; i < kT * kT; i += kBlock) {
+        const int ly = i / kT, lx = i - ly * kT, gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        tile[i] = (gx >= 0 && gy >= 0 && gx < P.w && gy < P.h) ? in[(int64_t)gy * P.w + gx] : kInvalid;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & (kTile - 1), ly = threadIdx.x >> 4;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= P.w || y >= P.h) return;
+    const SGM_LDS float* c = tile + (ly + 1) * kT + lx + 1;
     float out = kInvalid;
-    if (v > -1.0e8f) {
+    if (*c > -1.0e8f) {
         float buf[9];
         int n = 0;
 #pragma unroll
         for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx, yy = y + dy;
-                float q = kInvalid;
-                if (xx >= 0 && yy >= 0 && xx < P.w && yy < P.h) q = in[(int64_t)yy * P.w + xx];
+                const float q = c[dy * kT + dx];
                 // Invalid or outside: +inf sorts after every valid value and is never selected.
                 const bool ok = q > -1.0e8f;
                 buf[dy * 3 + dx + 4] = ok ? q : __builtin_inff();
                 n += ok ? 1 : 0;
             }
-        // Odd-even transposition sort (9 rounds sort 9 values): valid values ascending, +inf last.
-#pragma unroll
-        for (int round = 0; round < 9; ++round)
-#pragma unroll
-            for (int i = round & 1; i + 1 < 9; i += 2) {
-                const float lo = fminf(buf[i], buf[i + 1]);
-                const float hi = fmaxf(buf[i], buf[i + 1]);
-                buf[i] = lo;
-                buf[i + 1] = hi;
-            }
+        sort9(buf);  // valid values ascending, +inf last
         float sel = buf[0];
 #pragma unroll
         for (int i = 1; i < 9; ++i) sel = (n / 2) == i ? buf[i] : sel;
@@ -1130,10 +1142,11 @@ __device__ __forceinline__ void median_body(const PairDesc& P, int lb) {
 }
 
 __global__ __launch_bounds__(kBlock) void median_kernel(LaunchTable t) {
+    __shared__ float tile[(kTile + 2) * (kTile + 2)];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    median_body(P, lb);
+    median_body(P, lb, lds(tile));
 }
 
 // ------------------------------------------------------------------------------ rectification
