@@ -107,6 +107,29 @@ def test_wide_lanes_second_best_full_window():
              uniqueness=0.95)
 
 
+def test_maximum_width_4096_with_512_disparities():
+    """The widest row the library accepts (4096 columns: the row kernel keeps 15 bytes of LDS per
+    column) with the largest disparity count, 8 paths and the 9x7 census, with volumes, then in both
+    cost modes; one column more is refused at create time."""
+    w, h, nd = 4096, 24, 512
+    pair = speckle_pair(w, h, nd, 93)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=True, census_radius_x=4,
+                  census_radius_y=3, p1=10, p2=120)
+    print(f"maximum width {w}x{h} D={nd} {params}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+        assert_same(ref, got, FIELDS + VOLUMES, tag=f"{w}x{h}x{nd}")
+        for cv in (1, 0):
+            st.set_option(sgm.OPT_COST_VOLUME, cv)
+            got = st.compute(pair.left, pair.right, outputs=FIELDS)
+            assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} cost volume {cv}")
+    assert (ref["disparity"] > -1e8).mean() > 0.5
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.HipStereo.create(sgm.SgmParams(**params), w + 1, h)
+    assert "max_width" in str(e.value)
+
+
 def test_config1_640x400x64_four_paths():
     run_case(640, 400, 64, 1, census_radius_x=4, census_radius_y=3, p1=10, p2=120)
 
