@@ -229,3 +229,40 @@ def test_32_lane_scanlines_bit_exact(w, h, nd, seed, kw, cv):
             st.set_option(sgm.OPT_PATH_LINES, lines)
             got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
             assert_same(ref, got, FIELDS + VOLUMES, tag=f"g32 {w}x{h}x{nd} lines {lines}")
+
+
+def sweep_cases(n, seed):
+    """Seeded random cases over the whole accepted parameter space and every tuning knob: sizes
+    1..300 x 1..60, 1..512 disparities, census windows 1x1..9x7, P1/P2 up to the byte bound (P1 > P2
+    included), min_disparity -80..80, uniqueness 0.5..1, L-R tolerance -1..3, each flag on or off."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        rx, ry = int(rng.integers(0, 5)), int(rng.integers(0, 4))
+        cells = (2 * rx + 1) * (2 * ry + 1)
+        nd = int(rng.choice([1, 2, 3, 5, 16, 17, 31, 64, 65, 100, 128, 130, 192, 200, 256, 300, 384, 511, 512]))
+        params = dict(min_disparity=int(rng.integers(-80, 81)), num_disparities=nd, census_radius_x=rx,
+                      census_radius_y=ry, p1=int(rng.integers(0, 255 - cells)), p2=int(rng.integers(0, 256 - cells)),
+                      adaptive_p2=bool(rng.random() < 0.5), eight_paths=bool(rng.random() < 0.6),
+                      slant_steps=bool(rng.random() < 0.5), uniqueness=float(np.float32(rng.uniform(0.5, 1.0))),
+                      lr_max_diff=int(rng.integers(-1, 4)), subpixel=bool(rng.random() < 0.8))
+        knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([4, 8, 16])),
+                 sgm.OPT_PATH_GROUP: int(rng.choice([16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9))}
+        cases.append((int(rng.integers(1, 301)), int(rng.integers(1, 61)), 1000 + i, params, knobs))
+    return cases
+
+
+SWEEP = sweep_cases(48, 2026)
+
+
+@pytest.mark.parametrize("w,h,seed,params,knobs", SWEEP, ids=[f"sweep{i}-{c[0]}x{c[1]}x{c[3]['num_disparities']}"
+                                                               for i, c in enumerate(SWEEP)])
+def test_random_parameter_sweep(w, h, seed, params, knobs):
+    pair = speckle_pair(w, h, max(params["num_disparities"], 6), seed)
+    print(f"sweep {w}x{h} {params} knobs {knobs}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        for k, v in knobs.items():
+            st.set_option(k, v)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+    assert_same(ref, got, FIELDS + VOLUMES, tag=f"sweep {w}x{h}")
