@@ -170,3 +170,51 @@ def test_plain_context_compute_disparity_is_sgm():
         got = ctx.compute_disparity(pair.left, pair.right)
     np.testing.assert_array_equal(got["disparity"], ref["disparity"])
     np.testing.assert_array_equal(got["confidence"], (ref["disparity"] > -1e8).astype(np.float32))
+
+
+def chain_sweep(n, seed):
+    """Seeded random compute_disparity configurations: 0..4 pyramid levels, refine search 1..4 and
+    ZNCC radius 1..7 (the fixed-radius kernels and the loop form), detail refinement on or off with
+    radius 1..16 and steps of 0.25..1 px, each filter on or off with its threshold, speckle sizes
+    0..300, with and without the full-resolution pair."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        levels = int(rng.integers(0, 5))
+        step = float(rng.choice([0.25, 0.5, 1.0]))
+        stereo = dict(pyramid_levels=levels, refine_search_radius=int(rng.integers(1, 5)),
+                      refine_zncc_radius=int(rng.integers(1, 8)), refine_min_zncc=float(np.float32(rng.uniform(-0.2, 0.9))),
+                      refine_reject_edge_winners=bool(rng.random() < 0.5), detail_enabled=bool(rng.random() < 0.6),
+                      detail_radius=int(rng.integers(1, 17)), detail_step=step,
+                      detail_search=float(step * rng.integers(1, 9)), detail_min_zncc=float(np.float32(rng.uniform(-0.2, 0.9))),
+                      filter_occlusion=bool(rng.random() < 0.5),
+                      filter_occlusion_tolerance=float(np.float32(rng.uniform(0.25, 3))),
+                      filter_spikes=bool(rng.random() < 0.5), filter_spike_threshold=float(np.float32(rng.uniform(0.25, 3))),
+                      speckle_max_region_size=int(rng.choice([0, 1, 20, 100, 300])),
+                      speckle_max_diff=float(np.float32(rng.uniform(0.25, 2))))
+        w, h = int(rng.integers(32, 257)), int(rng.integers(24, 161))
+        nd = int(rng.choice([16, 32, 48, 64, 96]))
+        cases.append((w, h, nd, 2000 + i, stereo, bool(rng.random() < 0.7), bool(rng.random() < 0.5)))
+    return cases
+
+
+CHAIN_SWEEP = chain_sweep(24, 77)
+
+
+@pytest.mark.parametrize("w,h,nd,seed,stereo,full,eight", CHAIN_SWEEP,
+                         ids=[f"chain{i}-{c[0]}x{c[1]}-L{c[4]['pyramid_levels']}" for i, c in enumerate(CHAIN_SWEEP)])
+def test_random_compute_disparity_sweep(w, h, nd, seed, stereo, full, eight):
+    pair = speckle_pair(w, h, nd, seed, min_cells=2, max_cells=5)
+    levels = stereo["pyramid_levels"]
+    kw = dict(min_disparity=pair.min_disparity // (1 << levels) - 1, num_disparities=max(nd >> levels, 1) + 4,
+              census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=eight)
+    fl = fr = None
+    if full:
+        fl = np.repeat(np.repeat(pair.left, 2, 0), 2, 1)
+        fr = np.repeat(np.repeat(pair.right, 2, 0), 2, 1)
+    print(f"chain sweep {w}x{h} D={nd} {kw} {stereo} full={full}", flush=True)
+    ref = O.compute_disparity(pair.left, pair.right, O.SgmParams(**kw), O.StereoParams(**stereo), full_left=fl,
+                              full_right=fr)
+    with sgm.HipStereo.create_stereo(sgm.SgmParams(**kw), sgm.StereoParams(**stereo), w, h) as ctx:
+        got = ctx.compute_disparity(pair.left, pair.right, fl, fr)
+    assert_maps(ref, got, f"chain sweep {w}x{h}")
