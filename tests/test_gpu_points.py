@@ -102,3 +102,42 @@ def test_compute_frame_raw_matches_the_cpu_chain():
     assert (got["points"][..., 3] > 0).mean() > 0.3
     assert t["points_ms"] > 0 and t["gpu_ms"] > t["points_ms"]
     np.testing.assert_array_equal(sgm.pack_depth(got["points"], got["weights"]), O.pack_depth(ref["points"], ref["weights"]))
+
+
+def point_sweep(n, seed):
+    """Seeded random geometries, point parameters and disparity maps (slanted planes with steps,
+    noise, holes, zero and negative disparities, values near -cx_offset)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        w, h = int(rng.integers(1, 200)), int(rng.integers(1, 150))
+        g = (float(rng.uniform(100, 2000)), float(rng.uniform(0, w)), float(rng.uniform(0, h)),
+             float(rng.uniform(10, 200)), float(rng.choice([0.0, rng.uniform(-20, 20)])))
+        p = (float(rng.uniform(10, 300)), float(rng.uniform(300, 3000)), float(rng.uniform(0.5, 10)),
+             float(rng.uniform(0, 1)), float(rng.choice([0.0, rng.uniform(100, 800)])))
+        cases.append((w, h, g, p, int(rng.integers(0, 2**31)), bool(rng.random() < 0.7)))
+    return cases
+
+
+POINT_SWEEP = point_sweep(16, 505)
+
+
+@pytest.mark.parametrize("w,h,g,p,seed,with_conf", POINT_SWEEP,
+                         ids=[f"points{i}-{c[0]}x{c[1]}" for i, c in enumerate(POINT_SWEEP)])
+def test_random_point_frames_bit_exact(w, h, g, p, seed, with_conf):
+    rng = np.random.default_rng(seed)
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float32)
+    base = np.float32(g[0] * g[3] / rng.uniform(200, 900))
+    d = (base + np.float32(rng.uniform(-0.1, 0.1)) * xs + np.float32(rng.uniform(-0.1, 0.1)) * ys).astype(np.float32)
+    d[:, int(rng.integers(0, w + 1)):] += np.float32(rng.uniform(-8, 8))
+    d += rng.normal(0, 0.05, d.shape).astype(np.float32)
+    d[rng.random(d.shape) < 0.05] = np.float32(-1e9)
+    d[rng.random(d.shape) < 0.02] = np.float32(0)
+    d[rng.random(d.shape) < 0.02] = np.float32(-g[4])
+    d[rng.random(d.shape) < 0.02] = np.float32(rng.uniform(-5, 0))
+    conf = rng.uniform(0, 1, (h, w)).astype(np.float32) if with_conf else None
+    print(f"random points {w}x{h} g={g} p={p} conf={with_conf}", flush=True)
+    ref = O.disparity_to_points(d, conf, O.Geometry(*g), O.PointParams(*p))
+    got = sgm.disparity_to_points(d, conf, sgm.RectifiedGeometry(*g), sgm.PointParams(*p))
+    assert_frame(ref, got, "random points")
+    np.testing.assert_array_equal(sgm.pack_depth(got["points"], got["weights"]), O.pack_depth(ref["points"], ref["weights"]))

@@ -132,3 +132,53 @@ def test_errors_follow_the_reference():
         assert "raw size mismatch" in str(e.value)
         got = st.compute_raw(raw, raw, rectified=True)  # every sample at (0, 0) is inside: copies raw[0, 0]
         assert got["rect_left"].shape == (48, 64) and not got["rect_left"].any()
+
+
+def remap_sweep(n, seed):
+    """Seeded random maps: smooth rectification-like tables mixed with uniform noise over and past
+    the raw frame, exact integer and half-pixel coordinates, -0.0 and values just inside the last
+    row/column, at random map and raw sizes."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        mw, mh = 2 * int(rng.integers(4, 161)), 2 * int(rng.integers(4, 121))
+        rw, rh = int(rng.integers(2, 341)), int(rng.integers(2, 261))
+        cases.append((mw, mh, rw, rh, int(rng.integers(0, 2**31))))
+    return cases
+
+
+def random_map(mw, mh, rw, rh, rng):
+    ys, xs = np.mgrid[0:mh, 0:mw].astype(np.float32)
+    mx = xs * np.float32(rw / mw) + rng.normal(0, 2, (mh, mw)).astype(np.float32)
+    my = ys * np.float32(rh / mh) + rng.normal(0, 2, (mh, mw)).astype(np.float32)
+    noise = rng.random((mh, mw)) < 0.2
+    mx[noise] = rng.uniform(-3, rw + 3, int(noise.sum())).astype(np.float32)
+    my[noise] = rng.uniform(-3, rh + 3, int(noise.sum())).astype(np.float32)
+    special = np.array([0.0, -0.0, 0.5, 1.0, 1.5, rw - 2.0, rw - 1.5, rw - 1.0,
+                        np.nextafter(np.float32(rw - 1), np.float32(0))], np.float32)
+    pick = rng.random((mh, mw)) < 0.1
+    mx[pick] = rng.choice(special, int(pick.sum()))
+    special_y = np.array([0.0, -0.0, 0.5, rh - 2.0, rh - 1.5, rh - 1.0,
+                          np.nextafter(np.float32(rh - 1), np.float32(0))], np.float32)
+    pick = rng.random((mh, mw)) < 0.1
+    my[pick] = rng.choice(special_y, int(pick.sum()))
+    return mx, my
+
+
+REMAP_SWEEP = remap_sweep(16, 404)
+
+
+@pytest.mark.parametrize("mw,mh,rw,rh,seed", REMAP_SWEEP, ids=[f"remap{i}-{c[0]}x{c[1]}-raw{c[2]}x{c[3]}"
+                                                               for i, c in enumerate(REMAP_SWEEP)])
+def test_random_maps_bit_exact(mw, mh, rw, rh, seed):
+    rng = np.random.default_rng(seed)
+    lx, ly = random_map(mw, mh, rw, rh, rng)
+    rx, ry = random_map(mw, mh, rw, rh, rng)
+    raw_l = rng.integers(0, 256, (rh, rw), dtype=np.uint8)
+    raw_r = rng.integers(0, 256, (rh, rw), dtype=np.uint8)
+    print(f"random maps {mw}x{mh} raw {rw}x{rh} seed={seed}", flush=True)
+    with sgm.HipStereo.create(sgm.SgmParams(num_disparities=16), mw // 2, mh // 2) as st:
+        st.set_rectification(lx, ly, rx, ry, rw, rh)
+        got = st.compute_raw(raw_l, raw_r, rectified=True)
+    np.testing.assert_array_equal(got["rect_left"], O.rectify_half(raw_l, lx, ly))
+    np.testing.assert_array_equal(got["rect_right"], O.rectify_half(raw_r, rx, ry))
