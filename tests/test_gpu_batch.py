@@ -96,3 +96,51 @@ def test_autotuner_times_candidates_and_caches_the_choice():
             assert_same(refs[i], b2.results(i), FIELDS, tag=f"tuned pair {i}")
     finally:
         b2.close()
+
+
+def batch_sweep(n, seed):
+    """Seeded random batches: 2..9 pairs of random sizes (1..300 x 1..120), disparity counts
+    (1..512) and min_disparity each, random shared SGM parameters and a random fixed plan."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        rx, ry = int(rng.integers(0, 5)), int(rng.integers(0, 4))
+        cells = (2 * rx + 1) * (2 * ry + 1)
+        params = dict(census_radius_x=rx, census_radius_y=ry, p1=int(rng.integers(0, 255 - cells)),
+                      p2=int(rng.integers(0, 256 - cells)), adaptive_p2=bool(rng.random() < 0.5),
+                      eight_paths=bool(rng.random() < 0.6), slant_steps=bool(rng.random() < 0.5),
+                      uniqueness=float(np.float32(rng.uniform(0.5, 1.0))), lr_max_diff=int(rng.integers(0, 3)),
+                      subpixel=bool(rng.random() < 0.8))
+        pairs = [(int(rng.integers(1, 301)), int(rng.integers(1, 121)),
+                  int(rng.choice([1, 7, 16, 33, 64, 96, 128, 130, 192, 256, 384, 512])), int(rng.integers(-40, 41)))
+                 for _ in range(int(rng.integers(2, 10)))]
+        tuning = dict(cost_volume=int(rng.integers(0, 2)), path_lines=int(rng.choice([4, 8, 16])),
+                      branches=int(rng.integers(0, 3)), path_group=int(rng.choice([16, 32])))
+        if tuning["path_group"] == 32:
+            tuning["path_lines"] = min(tuning["path_lines"], 8)  # 32-lane scanlines take at most 8 lines
+        cases.append((params, pairs, tuning, 5000 + 100 * i))
+    return cases
+
+
+BATCH_SWEEP = batch_sweep(8, 606)
+
+
+@pytest.mark.parametrize("params,pairs,tuning,seed0", BATCH_SWEEP,
+                         ids=[f"batch{i}-{len(c[1])}pairs" for i, c in enumerate(BATCH_SWEEP)])
+def test_random_batches_bit_exact(params, pairs, tuning, seed0):
+    specs, refs = [], []
+    for i, (w, h, nd, min_d) in enumerate(pairs):
+        pair = speckle_pair(w, h, max(nd, 6), seed0 + i)
+        specs.append(sgm.PairSpec(pair.left, pair.right, min_d, nd, outputs=tuple(FIELDS)))
+        refs.append(O.sgm(pair.left, pair.right, O.SgmParams(min_disparity=min_d, num_disparities=nd, **params)))
+    print(f"random batch {pairs} {params} {tuning}", flush=True)
+    b = sgm.Batch(sgm.SgmParams(**params), specs, tuning=tuning)
+    try:
+        for mode in ("graph", "plain", "graph"):
+            b.clear_outputs(0xEE)
+            b.run() if mode == "graph" else b.run_plain()
+            b.synchronize()
+            for i in range(len(pairs)):
+                assert_same(refs[i], b.results(i), FIELDS, tag=f"{mode} pair {i} {pairs[i]}")
+    finally:
+        b.close()
