@@ -139,3 +139,44 @@ def test_raw_frames_rectified_with_resident_maps():
         rl = O.rectify_half(p.left, m.left_x, m.left_y)
         rr = O.rectify_half(p.right, m.right_x, m.right_y)
         assert_same(O.sgm(rl, rr, O.SgmParams(**kw)), c["outputs"], SERVE_FIELDS, tag=f"raw frame {tag} vs oracle")
+
+
+def test_random_request_mix_bit_exact():
+    """40 seeded random requests (sizes 1..320 x 1..256, D 1..96, min_disparity -20..39) submitted
+    with random gaps while earlier ones run and complete, polled in between: every result equals
+    the oracle's."""
+    rng = np.random.default_rng(808)
+    reqs = []
+    for i in range(40):
+        w, h = int(rng.integers(1, 321)), int(rng.integers(1, 257))
+        nd = int(rng.choice([1, 8, 16, 33, 48, 64, 96]))
+        p = speckle_pair(w, h, max(nd, 6), 900 + i)
+        reqs.append((p.left, p.right, int(rng.integers(-20, 40)), nd))
+    print(f"serving {len(reqs)} random requests", flush=True)
+    got = []
+
+    def collect(timeout_us):
+        for c in sv.poll(64, timeout_us=timeout_us):
+            outs = {f: a.copy() for f, a in (c["outputs"] or {}).items()}
+            sv.release(c)
+            c["outputs"] = outs
+            got.append(c)
+
+    with sgm.Server(sgm.SgmParams(**BASE), 320, 256, 96, slots=4, hbm_budget_bytes=192 << 20) as sv:
+        for i, (l, r, md, nd) in enumerate(reqs):
+            sv.submit(l, r, md, nd, outputs=SERVE_FIELDS, tag=i)
+            if rng.random() < 0.3:
+                time.sleep(float(rng.uniform(0, 0.002)))
+            if rng.random() < 0.3:
+                collect(0)
+        t0 = time.time()
+        while len(got) < len(reqs) and time.time() - t0 < 60:
+            collect(100000)
+        st = sv.stats()
+    assert len(got) == len(reqs) and all(c["status"] == 0 for c in got)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert sorted(c["tag"] for c in got) == list(range(len(reqs)))
+    for c in got:
+        l, r, md, nd = reqs[c["tag"]]
+        kw = dict(BASE, min_disparity=md, num_disparities=nd)
+        assert_same(O.sgm(l, r, O.SgmParams(**kw)), c["outputs"], SERVE_FIELDS, tag=f"request {c['tag']} {l.shape} D={nd}")
