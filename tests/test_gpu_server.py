@@ -8,7 +8,7 @@ import pytest
 import oracle as O
 from open_source_einstar_amd import sgm
 from open_source_einstar_amd.synth import speckle_pair
-from test_gpu_parity import FIELDS, assert_same
+from test_gpu_parity import assert_same
 
 pytestmark = pytest.mark.gpu
 
