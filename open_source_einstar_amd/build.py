@@ -9,6 +9,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -24,12 +25,14 @@ ARCH = "gfx950"
 # -ffp-contract=off: float stages (subpixel, uniqueness, left/right check) round like the reference.
 # -packed-fp32-ops off: no hand-free v_pk_*_f32 (the kernels are integer; the flag guards the float bits).
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wextra", "-Wno-unused-parameter",
 ]
 
-HIP_SOURCES = ["sgm_kernels.hip", "refine_kernels.hip", "frame_kernels.hip", "sgm_api.hip", "sgm_server.hip"]
+HIP_SOURCES = ["sgm_kernels.hip", "paths_g16_lo.hip", "paths_g16_hi.hip", "paths_g32.hip", "rows_kernels.hip",
+               "refine_kernels.hip", "frame_kernels.hip", "sgm_api.hip", "sgm_server.hip"]
+OBJ_DIR = os.path.join(PKG, "_build")
 
 
 def _run(cmd, cwd=None):
@@ -38,9 +41,18 @@ def _run(cmd, cwd=None):
 
 
 def build_hip(verbose: bool = True) -> str:
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Compiles each source to an object in parallel (the template-heavy kernel files dominate),
+    then links the shared library."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    jobs = [[HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(HIP_SOURCES, objs)]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, j) for j in jobs]:
+            f.result()
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

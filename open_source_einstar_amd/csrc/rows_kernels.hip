@@ -1,0 +1,47 @@
+// Row-kernel and byte cost-volume instantiations for every lane layout (sgm_kernels.h).
+#include "sgm_kernels.h"
+
+namespace sgm {
+
+#define SGM_FOR_DPL(X) X(2) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
+
+hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    switch (dpl) {
+#define SGM_CASE(D)                                                                                     \
+    case D:                                                                                             \
+        hipLaunchKernelGGL((cost8_kernel<D>), dim3(t.total_blocks), dim3(kBlock), 0, stream, t, c);     \
+        break;
+        SGM_FOR_DPL(SGM_CASE)
+#undef SGM_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
+    if (t.total_blocks <= 0) return hipSuccess;
+    if (c.ndirs != 4 && c.ndirs != 8) return hipErrorInvalidValue;
+    const size_t lds = (size_t)max_w * 15 + 16;
+    switch (dpl) {
+#define SGM_ROWS(D, PD)                                                                                                  \
+    do {                                                                                                                 \
+        if (c.ndirs == 8) hipLaunchKernelGGL((rows_kernel<D, PD, 8>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);   \
+        else hipLaunchKernelGGL((rows_kernel<D, PD, 4>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);                \
+    } while (0)
+#define SGM_CASE(D)                  \
+    case D:                          \
+        if (padded) SGM_ROWS(D, true); \
+        else SGM_ROWS(D, false);     \
+        break;
+        SGM_FOR_DPL(SGM_CASE)
+#undef SGM_CASE
+#undef SGM_ROWS
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sgm
