@@ -296,8 +296,11 @@ public:
     Result<StereoResult> compute_full(ImageView<const std::uint8_t> left, ImageView<const std::uint8_t> right,
                                       ImageView<const std::uint8_t> full_left, ImageView<const std::uint8_t> full_right) {
         if (auto r = check(left, right); !r) return r.error();
-        if (full_left.data && full_left.stride != full_right.stride)
-            return make_error(Errc::invalid_argument, "full-resolution images must share a stride");
+        if ((full_left.data == nullptr) != (full_right.data == nullptr))
+            return make_error(Errc::invalid_argument, "full-resolution images must be given together");
+        if (full_left.data && (full_left.stride != full_right.stride || full_left.width != full_right.width ||
+                               full_left.height != full_right.height))
+            return make_error(Errc::invalid_argument, "full-resolution images must share a size and stride");
         StereoResult res;
         res.disparity = ImageF32(w_, h_);
         res.confidence = ImageF32(w_, h_);
@@ -391,7 +394,8 @@ public:
                                           ImageU8* rect_left = nullptr, ImageU8* rect_right = nullptr) {
         if (raw_left.stride != raw_right.stride)
             return make_error(Errc::invalid_argument, "left and right images must share a stride");
-        if (raw_w_ && (raw_left.width != raw_w_ || raw_left.height != raw_h_))
+        if (raw_left.width != raw_right.width || raw_left.height != raw_right.height ||
+            (raw_w_ && (raw_left.width != raw_w_ || raw_left.height != raw_h_)))
             return make_error(Errc::invalid_argument, "raw size mismatch");
         FrameResult f;
         f.width = w_;

@@ -12,6 +12,11 @@
  * Threading: a context is not re-entrant, like MetalStereo (the reference serialises it with a
  * mutex, libs/pipeline/src/stereo_frontend.cpp:307). Separate contexts are independent. The
  * server functions are safe to call from any thread.
+ *
+ * Streams: every call on one context (or one batch) uses that context's scratch buffers. A call
+ * enqueued on another stream than the previous call's is made to wait, on the device, for the
+ * previous call's stream, so calls on one context never overlap whatever streams they name.
+ * Outputs written by a call on a caller's stream are ready once that stream has reached them.
  */
 #ifndef SGM_HIP_H
 #define SGM_HIP_H
@@ -23,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SGM_HIP_ABI_VERSION 1
+#define SGM_HIP_ABI_VERSION 2
 
 /* Status codes: the values of einstar::Errc (libs/core/include/einstar/core/error.hpp:8-19). */
 enum {
@@ -374,13 +379,22 @@ typedef struct {
 } sgm_batch_plan;
 
 /* Plans a batch of n pairs: allocates the scratch of all pairs, times every candidate plan on the
- * batch itself (a graph replay each) and keeps the fastest, cached per batch signature for the
- * process; the chosen launches are captured in a hipGraph. The pair images and outputs must stay
- * valid until sgm_batch_destroy; their contents may change between runs. */
+ * batch itself (a graph replay each) and keeps the fastest, or takes the plan cached for its key
+ * (sgm_tune_cache_load below); the chosen launches are captured in a hipGraph. The pair images and
+ * outputs must stay valid until sgm_batch_destroy; their contents may change between runs. */
 int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out);
 int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pairs, const sgm_batch_tuning* tuning,
                            sgm_batch** out);
 int sgm_batch_get_plan(const sgm_batch* b, sgm_batch_plan* out);
+/* The autotuner's plan cache. sgm_batch_create caches the plan of every search that left all knobs
+ * open, keyed by the parameters that select kernel variants (path count, census window, slant steps)
+ * and the pairs' shapes bucketed to 64 pixels with their lane layout; a later batch of the same key
+ * takes the cached plan (from_cache = 1) without timing candidates, where it honours the knobs that
+ * batch fixes. load merges a file written by save into the process's cache (*entries = lines read);
+ * save writes the whole cache (*entries = lines written); entries may be NULL. No counterpart in the
+ * reference. */
+int sgm_tune_cache_load(const char* path, int* entries);
+int sgm_tune_cache_save(const char* path, int* entries);
 /* Replays the graph on `stream` (hipStream_t; NULL = the batch's own stream) without waiting. */
 int sgm_batch_run(sgm_batch* b, void* stream);
 /* The same work launched pair by pair without batching or graphs: the "plain" reference point. */
@@ -408,6 +422,10 @@ typedef struct {
     int32_t max_disparities;       /* largest num_disparities of a request */
     int32_t slots;                 /* requests on the device at once (1..64) */
     int64_t hbm_budget_bytes;      /* arena size; 0 = default */
+    int32_t compute_streams;       /* 0 (default) = 1: requests' kernels run one request at a time, in
+                                      admission order, while other requests' copies overlap them;
+                                      k in 1..slots: k shared compute streams; -1: each slot computes on
+                                      its own stream (requests' kernels share the GPU) */
 } sgm_server_config;
 
 typedef struct {

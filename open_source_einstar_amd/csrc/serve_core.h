@@ -90,7 +90,8 @@ struct Admission {
 
 class AdmissionQueue {
 public:
-    AdmissionQueue(int slots, Arena arena) : arena_(std::move(arena)), slot_busy_(static_cast<size_t>(slots), false) {}
+    AdmissionQueue(int slots, Arena arena)
+        : arena_(std::move(arena)), slot_busy_(static_cast<size_t>(slots), false), slot_retired_(static_cast<size_t>(slots), false) {}
 
     // A request that could never fit the arena is refused at submission.
     [[nodiscard]] bool can_ever_fit(size_t bytes) const { return arena_.round(bytes) <= arena_.capacity(); }
@@ -122,10 +123,26 @@ public:
         arena_.release(offset);
     }
 
+    // A request whose device work could not be confirmed finished: its slot is taken out of service
+    // and its arena range is never handed out again (the device may still write it).
+    void retire(int slot) {
+        if (!slot_retired_[static_cast<size_t>(slot)]) ++retired_;
+        slot_retired_[static_cast<size_t>(slot)] = true;
+        slot_busy_[static_cast<size_t>(slot)] = true;
+    }
+    [[nodiscard]] int live_slots() const { return static_cast<int>(slot_busy_.size()) - retired_; }
+    // Every waiting request, in arrival order, removed from the queue (when no slot is left to run them).
+    std::vector<uint64_t> drop_all() {
+        std::vector<uint64_t> out;
+        for (const auto& p : q_) out.push_back(p.ticket);
+        q_.clear();
+        return out;
+    }
+
     [[nodiscard]] size_t queued() const { return q_.size(); }
     [[nodiscard]] int busy_slots() const {
         int n = 0;
-        for (bool b : slot_busy_) n += b ? 1 : 0;
+        for (size_t i = 0; i < slot_busy_.size(); ++i) n += (slot_busy_[i] && !slot_retired_[i]) ? 1 : 0;
         return n;
     }
     [[nodiscard]] const Arena& arena() const { return arena_; }
@@ -133,7 +150,23 @@ public:
 private:
     Arena arena_;
     std::vector<bool> slot_busy_;
+    std::vector<bool> slot_retired_;
+    int retired_ = 0;
     std::deque<Pending> q_;
 };
+
+// How a request ends once its work is enqueued (enqueue_ok false: some enqueue failed part-way).
+// The streams it used are always drained first (drain() returns true when every wait succeeded),
+// whatever the enqueue status, so no copy or kernel of the request can still touch its arena range
+// or the caller's buffers afterwards. Its memory returns to the arena only when the drain succeeded.
+struct Outcome {
+    bool ok;       // the request's results are complete and valid
+    bool reclaim;  // its slot and arena range may be reused
+};
+template <class Drain>
+Outcome finish_request(bool enqueue_ok, Drain&& drain) {
+    const bool drained = drain();
+    return {enqueue_ok && drained, drained};
+}
 
 }  // namespace sgm::serve

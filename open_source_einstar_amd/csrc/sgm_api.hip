@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -179,6 +180,7 @@ struct sgm_context {
     int max_w = 0, max_h = 0, ndirs = 4;
     hipStream_t stream = nullptr;
     hipStream_t last_stream = nullptr;
+    hipEvent_t order_ev = nullptr;  // orders a call on another stream after the previous call's
     Stages ring[kEventRing];
     int ring_head = 0;     // next set to record
     int ring_pending = 0;  // recorded, not yet read; the oldest is ring_head - ring_pending
@@ -264,6 +266,7 @@ const char* sgm_last_error(const sgm_context* ctx) { return ctx ? ctx->err.c_str
 void sgm_destroy(sgm_context* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->last_stream && ctx->last_stream != ctx->stream) (void)hipStreamSynchronize(ctx->last_stream);
     for (Stages& st : ctx->ring)
         for (hipEvent_t e : st.ev)
             if (e) (void)hipEventDestroy(e);
@@ -274,6 +277,7 @@ void sgm_destroy(sgm_context* ctx) {
                     (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_local, (void*)ctx->d_sizes,
                     (void*)ctx->d_full, (void*)ctx->d_pts, (void*)ctx->d_nrm, (void*)ctx->d_wts})
         dfree(p);
+    if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -346,6 +350,7 @@ static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int st
     const size_t spx = (size_t)stereo_w * (size_t)stereo_h;
     const size_t dp = (size_t)dpl_for(params->num_disparities) * kGroup;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming);
     // Stage events only time the stages (completion is always a stream synchronisation), so they skip
     // the system-scope fence: with it, each record between two kernels cost a cache writeback, about
     // 4 us of the step.
@@ -584,6 +589,15 @@ int sgm_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
     return SGM_OK;
 }
 
+// Every call on a context uses the context's scratch buffers, so a call enqueued on another stream
+// than the previous call waits for that stream's work first (calls on one stream are ordered anyway).
+static hipError_t order_after_last(sgm_context* ctx, hipStream_t s) {
+    if (!ctx->last_stream || ctx->last_stream == s) return hipSuccess;
+    hipError_t e = hipEventRecord(ctx->order_ev, ctx->last_stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->order_ev, 0);
+    return e;
+}
+
 static int check_size(sgm_context* ctx, int width, int height) {
     if (width < 1 || height < 1 || width > ctx->max_w || height > ctx->max_h)
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "image size " + std::to_string(width) + "x" + std::to_string(height) +
@@ -691,6 +705,7 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     if (int rc = check_size(ctx, width, height); rc != SGM_OK) return rc;
     if (!d_left || !d_right || pitch < width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "null image or pitch < width");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     PairDesc d = context_desc(ctx, d_left, d_right, pitch, width, height, d_out);
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
@@ -784,6 +799,7 @@ static int compute_host(sgm_context* ctx, const HostFrame& f, int width, int hei
     const size_t npx = (size_t)width * height;
     const size_t nvol = npx * (size_t)ctx->p.num_disparities;
     hipStream_t s = ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     hipError_t e = hipSuccess;
     if (out->sum_volume && !ctx->d_sum) e = dalloc(&ctx->d_sum, (size_t)ctx->max_w * ctx->max_h * ctx->p.num_disparities, ctx->bytes);
     if (e == hipSuccess && out->cost_volume && !ctx->d_cost)
@@ -881,7 +897,9 @@ int sgm_set_rectification(sgm_context* ctx, const float* left_map_x, const float
     // Raw sizes stay below 2^24 so that the kernel's float bounds test is exact.
     if (raw_w < 1 || raw_h < 1 || raw_w >= (1 << 24) || raw_h >= (1 << 24))
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "raw size must be in [1, 2^24)");
+    // Work still reading the old maps may be on the context's stream or on the last caller stream.
     hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && ctx->last_stream) e = hipStreamSynchronize(ctx->last_stream);
     if (e != hipSuccess) return ctx->hip_fail("sgm_set_rectification: synchronize", e);
     dfree(ctx->d_maps);
     dfree(ctx->d_raw);
@@ -960,6 +978,7 @@ int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const ui
     uint8_t* dl = d_rect_left ? d_rect_left : ctx->d_img;
     uint8_t* dr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)ctx->max_w * ctx->max_h;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     Stages* st = next_stages(ctx);
     hipError_t e = enqueue_rectify(ctx, d_raw_left, d_raw_right, raw_pitch, dl, dr, s, st);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_raw_device: rectify", e);
@@ -1137,6 +1156,7 @@ static int disparity_host(sgm_context* ctx, const uint8_t* left, const uint8_t* 
     const auto t0 = std::chrono::steady_clock::now();
     const std::string nm(who);
     hipStream_t s = ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     const size_t npx = (size_t)w * h;
     uint8_t* dl = ctx->d_img;
     uint8_t* dr = ctx->d_img + npx;
@@ -1219,6 +1239,7 @@ int sgm_compute_disparity_device(sgm_context* ctx, const uint8_t* d_left, const 
                       full_height == 2 * height;
     if (full && full_pitch < full_width) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "full_pitch < full_width");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     const hipError_t e = enqueue_disparity(ctx, d_left, d_right, pitch, width, height, full ? d_full_left : nullptr,
                                            full ? d_full_right : nullptr, full_pitch, d_disparity, d_confidence, s,
                                            next_stages(ctx));
@@ -1257,6 +1278,7 @@ int sgm_compute_disparity_raw_device(sgm_context* ctx, const uint8_t* d_raw_left
     const int w = ctx->map_w / 2, h = ctx->map_h / 2;
     if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     uint8_t* rl = d_rect_left ? d_rect_left : ctx->d_img;
     uint8_t* rr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)w * h;
     Stages* st = next_stages(ctx);
@@ -1401,6 +1423,7 @@ int sgm_compute_frame_raw(sgm_context* ctx, const uint8_t* raw_left, const uint8
     if (int rc = check_stereo_size(ctx, w, h); rc != SGM_OK) return rc;
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     const size_t npx = (size_t)w * h, rpx = (size_t)raw_w * raw_h;
     uint8_t* rl = ctx->d_img;
     uint8_t* rr = ctx->d_img + npx;
@@ -1452,6 +1475,7 @@ int sgm_compute_frame_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, co
     if (!aligned16(dev.points_xyzw) || !aligned16(dev.normals_xyzw))
         return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "point and normal arrays must be 16-byte aligned");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (const hipError_t oe = order_after_last(ctx, s); oe != hipSuccess) return ctx->hip_fail("stream order", oe);
     uint8_t* rl = d_rect_left ? d_rect_left : ctx->d_img;
     uint8_t* rr = d_rect_right ? d_rect_right : ctx->d_img + (size_t)w * h;
     const hipError_t e = enqueue_frame(ctx, d_raw_left, d_raw_right, raw_pitch, rl, rr, dev, s, next_stages(ctx));
@@ -1504,6 +1528,7 @@ struct sgm_batch {
     std::vector<void*> plan_allocs;  // device tables of the current plan's graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t order_ev = nullptr;  // orders a run on another stream after the previous run's
     sgm_batch_plan info{};
     std::string signature;
     hipStream_t last_stream = nullptr;
@@ -1672,11 +1697,22 @@ hipError_t time_plan(sgm_batch* b, int reps, float* ms) {
     return e;
 }
 
+// The plan cache's key: the parameters that select kernel variants (path count, census window,
+// slant steps) and the pairs' shapes bucketed -- width and height rounded up to 64, the lane layout
+// (disparities per lane and whether they are padded), the median flag -- in sorted order. Batches of
+// nearly the same shapes share a plan, which is safe because a plan never changes a result.
 std::string batch_signature(const sgm_params& p, int n, const sgm_pair* pairs) {
-    std::string k = std::to_string(p.eight_paths) + ":" + std::to_string(p.census_radius_x) + "x" + std::to_string(p.census_radius_y);
-    for (int i = 0; i < n; ++i)
-        k += ";" + std::to_string(pairs[i].width) + "x" + std::to_string(pairs[i].height) + "x" + std::to_string(pairs[i].num_disparities) +
-             "@" + std::to_string(pairs[i].min_disparity) + (pairs[i].out.disparity_median ? "m" : "");
+    std::vector<std::string> items;
+    for (int i = 0; i < n; ++i) {
+        const int dpl = dpl_for(pairs[i].num_disparities);
+        items.push_back(std::to_string((pairs[i].width + 63) / 64) + "x" + std::to_string((pairs[i].height + 63) / 64) + "d" +
+                        std::to_string(dpl) + (dpl * kGroup != pairs[i].num_disparities ? "p" : "") +
+                        (pairs[i].out.disparity_median ? "m" : ""));
+    }
+    std::sort(items.begin(), items.end());
+    std::string k = "v2:" + std::to_string(p.eight_paths ? 8 : 4) + ":" + std::to_string(p.census_radius_x) + "x" +
+                    std::to_string(p.census_radius_y) + ":s" + std::to_string(p.slant_steps ? 1 : 0);
+    for (const std::string& it : items) k += ";" + it;
     return k;
 }
 
@@ -1687,7 +1723,9 @@ extern "C" {
 void sgm_batch_destroy(sgm_batch* b) {
     if (!b) return;
     drop_plan(b);
+    if (b->last_stream && b->last_stream != b->stream) (void)hipStreamSynchronize(b->last_stream);
     dfree(b->d_scratch);
+    if (b->order_ev) (void)hipEventDestroy(b->order_ev);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
     for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1823,10 +1861,16 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
     BatchPlan best = cands.front();
     float best_ms = 0, worst_ms = 0;
     bool cached = false;
+    // Only a search over every knob is cached, and a cached plan is taken only where it honours the
+    // knobs this call fixed.
+    const bool all_open = tn.cost_volume < 0 && tn.path_lines < 0 && tn.branches < 0 && tn.path_group < 0;
     if (cands.size() > 1) {
         std::lock_guard lk(g_plan_mu);
         auto it = g_plan_cache.find(b->signature);
-        if (it != g_plan_cache.end()) {
+        if (it != g_plan_cache.end() && std::any_of(cands.begin(), cands.end(), [&](const BatchPlan& c) {
+                return c.cost_volume == it->second.cost_volume && c.path_lines == it->second.path_lines &&
+                       c.branches == it->second.branches && c.path_group == it->second.path_group;
+            })) {
             best = it->second;
             cached = true;
         }
@@ -1845,7 +1889,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
             }
             worst_ms = std::max(worst_ms, ms);
         }
-        if (e == hipSuccess) {
+        if (e == hipSuccess && all_open) {
             std::lock_guard lk(g_plan_mu);
             g_plan_cache[b->signature] = best;
         }
@@ -1868,13 +1912,85 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
     return SGM_OK;
 }
 
+int sgm_tune_cache_load(const char* path, int* entries) {
+    if (!path) {
+        g_last_error = "path is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    FILE* f = std::fopen(path, "r");
+    if (!f) {
+        g_last_error = std::string("sgm_tune_cache_load: cannot open ") + path;
+        return SGM_ERR_NOT_FOUND;
+    }
+    // One entry a line: key, a tab, then cost_volume path_lines branches path_group.
+    std::map<std::string, BatchPlan> got;
+    char line[8192];
+    int rc = SGM_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        std::string ln(line);
+        while (!ln.empty() && (ln.back() == '\n' || ln.back() == '\r')) ln.pop_back();
+        if (ln.empty()) continue;
+        const size_t tab = ln.find('\t');
+        BatchPlan pl;
+        if (tab == std::string::npos || tab == 0 ||
+            std::sscanf(ln.c_str() + tab + 1, "%d %d %d %d", &pl.cost_volume, &pl.path_lines, &pl.branches, &pl.path_group) != 4 ||
+            (pl.cost_volume != 0 && pl.cost_volume != 1) || (pl.path_lines != 4 && pl.path_lines != 8 && pl.path_lines != 16) ||
+            pl.branches < 0 || pl.branches > 2 || (pl.path_group != 16 && pl.path_group != 32)) {
+            g_last_error = "sgm_tune_cache_load: malformed line: " + ln.substr(0, 200);
+            rc = SGM_ERR_PROTOCOL;
+            break;
+        }
+        got[ln.substr(0, tab)] = pl;
+    }
+    std::fclose(f);
+    if (rc != SGM_OK) return rc;
+    std::lock_guard lk(g_plan_mu);
+    for (auto& [k, v] : got) g_plan_cache[k] = v;
+    if (entries) *entries = (int)got.size();
+    return SGM_OK;
+}
+
+int sgm_tune_cache_save(const char* path, int* entries) {
+    if (!path) {
+        g_last_error = "path is NULL";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard lk(g_plan_mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        g_last_error = std::string("sgm_tune_cache_save: cannot open ") + path;
+        return SGM_ERR_IO;
+    }
+    for (const auto& [k, v] : g_plan_cache)
+        std::fprintf(f, "%s\t%d %d %d %d\n", k.c_str(), v.cost_volume, v.path_lines, v.branches, v.path_group);
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) {
+        g_last_error = std::string("sgm_tune_cache_save: write failed: ") + path;
+        return SGM_ERR_IO;
+    }
+    if (entries) *entries = (int)g_plan_cache.size();
+    return SGM_OK;
+}
+
 int sgm_batch_create(const sgm_params* params, int n, const sgm_pair* pairs, sgm_batch** out) {
     return sgm_batch_create_tuned(params, n, pairs, nullptr, out);
+}
+
+// Runs of one batch share its scratch, so a run on another stream than the previous run's waits for
+// that stream's work first.
+static hipError_t batch_order(sgm_batch* b, hipStream_t s) {
+    if (!b->last_stream || b->last_stream == s) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (!b->order_ev) e = hipEventCreateWithFlags(&b->order_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(b->order_ev, b->last_stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, b->order_ev, 0);
+    return e;
 }
 
 int sgm_batch_run(sgm_batch* b, void* stream) {
     if (!b) return SGM_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->stream;
+    if (const hipError_t oe = batch_order(b, s); oe != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("stream order", oe));
     const hipError_t e = hipGraphLaunch(b->exec, s);
     if (e != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("sgm_batch_run", e));
     b->last_stream = s;
@@ -1884,6 +2000,7 @@ int sgm_batch_run(sgm_batch* b, void* stream) {
 int sgm_batch_run_plain(sgm_batch* b, void* stream) {
     if (!b) return SGM_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->stream;
+    if (const hipError_t oe = batch_order(b, s); oe != hipSuccess) return b->fail(SGM_ERR_IO, hip_msg("stream order", oe));
     for (PairDesc d : b->desc) {
         // The same kernels and knobs as the graph, one pair after another: only batching differs.
         if (!b->plan.cost_volume) d.cvol = nullptr;

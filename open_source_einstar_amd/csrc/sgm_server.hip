@@ -1,17 +1,23 @@
 // Serving loop of the gfx950 SGM library (sgm_server_* in include/sgm_hip.h).
 //
 // Requests (host image pairs of any size up to the configured maximum, each with its own disparity
-// range) are admitted in arrival order when a slot (a stream and its worker thread) is free and the
-// request's device footprint fits the HBM arena reserved at start (serve_core.h). Requests that
+// range) are admitted in arrival order when a slot (a worker thread and its copy stream) is free and
+// the request's device footprint fits the HBM arena reserved at start (serve_core.h). Requests that
 // arrive while others run are admitted as soon as slots and memory free up, so several frames are on
-// the device at once on separate streams. Each request runs the same kernels as sgm_compute on its
-// own scratch inside the arena, so its result does not depend on what else is in flight.
+// the device at once. A request's copies run on its slot's stream; its kernels run on a compute
+// stream, handed over by events (upload done -> kernels -> results copied back). With shared compute
+// streams (the default: one), requests' kernels run one request after another in admission order
+// while other requests' uploads and result copies overlap them on the copy engines; with one compute
+// stream per slot, the requests' kernels share the GPU. Each request runs the same kernels as
+// sgm_compute on its own scratch inside the arena, so its result does not depend on what else is in
+// flight.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -151,14 +157,22 @@ struct sgm_server {
     }
 
     struct Slot {
-        hipStream_t stream = nullptr;
+        hipStream_t stream = nullptr;  // the request's copies
         hipEvent_t ev[6] = {};  // census .. median as sgm_compute, [5] = rectification start
+        hipEvent_t ev_up = nullptr, ev_done = nullptr;  // inputs on the device; kernels finished
         std::thread th;
         std::condition_variable cv;
         bool has_job = false;
         uint64_t ticket = 0;
     };
     std::vector<Slot> slots;
+    // Shared compute streams, each with a lock that keeps one request's launches together.
+    struct Compute {
+        hipStream_t stream = nullptr;
+        std::mutex mu;
+    };
+    std::vector<std::unique_ptr<Compute>> compute;
+    bool broken = false;  // a stream could not be drained: nothing new is admitted
 
     void dispatch(const std::vector<serve::Admission>& adm) {  // with mu held
         const int64_t t = now_ns();
@@ -232,10 +246,14 @@ void sgm_server::worker(int si) {
                 ue = hipMemcpy2DAsync(dst + n, in_w, r.borrowed[1], r.borrowed_stride, in_w, in_h, hipMemcpyHostToDevice, s.stream);
             return ue;
         };
+        // Copies on the slot's stream; kernels on the compute stream (the slot's own stream when
+        // there are no shared ones), handed over by events.
+        Compute* cs = compute.empty() ? nullptr : compute[(size_t)si % compute.size()].get();
+        hipStream_t ks = cs ? cs->stream : s.stream;
         hipError_t e;
+        RectifyDesc rd{};
         if (r.rect.d_maps) {
             e = upload(base + f.raw, r.rect.raw_w, r.rect.raw_h);
-            RectifyDesc rd{};
             const size_t nmap = (size_t)r.rect.map_w * r.rect.map_h;
             rd.raw[0] = base + f.raw;
             rd.raw[1] = base + f.raw + raw_px;
@@ -250,42 +268,84 @@ void sgm_server::worker(int si) {
             rd.map_w = r.rect.map_w;
             rd.out_w = r.w;
             rd.out_h = r.h;
-            if (e == hipSuccess) e = hipEventRecord(s.ev[5], s.stream);
-            if (e == hipSuccess) e = launch_rectify(rd, s.stream);
         } else {
             e = upload(base + f.img, r.w, r.h);
         }
-        if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, s.stream, s.ev);
+        bool compute_enqueued = false;
+        if (e == hipSuccess && cs) e = hipEventRecord(s.ev_up, s.stream);
+        if (e == hipSuccess) {
+            std::unique_lock<std::mutex> klk;
+            if (cs) klk = std::unique_lock<std::mutex>(cs->mu);
+            if (cs) e = hipStreamWaitEvent(ks, s.ev_up, 0);
+            compute_enqueued = cs != nullptr;
+            if (e == hipSuccess && r.rect.d_maps) {
+                e = hipEventRecord(s.ev[5], ks);
+                if (e == hipSuccess) e = launch_rectify(rd, ks);
+            }
+            if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, ks, s.ev);
+            if (e == hipSuccess && cs) e = hipEventRecord(s.ev_done, ks);
+        }
+        if (e == hipSuccess && cs) e = hipStreamWaitEvent(s.stream, s.ev_done, 0);
         for (const Copy& c : copies) {
             if (!c.host || e != hipSuccess) continue;
             e = hipMemcpyAsync(c.host, base + c.dev_off, c.bytes, hipMemcpyDeviceToHost, s.stream);
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+        // Drain before anything is released: the slot's stream always, and the compute stream too
+        // when the hand-off back to the slot's stream may not have been enqueued.
+        const serve::Outcome out = serve::finish_request(e == hipSuccess, [&] {
+            bool ok = hipStreamSynchronize(s.stream) == hipSuccess;
+            if (e != hipSuccess && compute_enqueued) ok = hipStreamSynchronize(ks) == hipSuccess && ok;
+            return ok;
+        });
         float gpu_ms = 0;
-        if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, r.rect.d_maps ? s.ev[5] : s.ev[0], s.ev[4]);
+        if (out.ok) (void)hipEventElapsedTime(&gpu_ms, r.rect.d_maps ? s.ev[5] : s.ev[0], s.ev[4]);
         sgm_completion cpl{};
         cpl.ticket = r.ticket;
         cpl.user_tag = r.tag;
-        cpl.status = e == hipSuccess ? SGM_OK : SGM_ERR_IO;
+        cpl.status = out.ok ? SGM_OK : SGM_ERR_IO;
         cpl.t_submit_ns = r.t_submit;
         cpl.t_admit_ns = r.t_admit;
         cpl.t_done_ns = now_ns();
         cpl.gpu_ms = gpu_ms;
         {
             std::lock_guard lk(mu);
-            if (e != hipSuccess) {
-                err = std::string("request ") + std::to_string(r.ticket) + ": " + hipGetErrorString(e);
+            if (!out.ok) {
+                err = std::string("request ") + std::to_string(r.ticket) + ": " +
+                      (e != hipSuccess ? hipGetErrorString(e) : "the device did not finish its work");
                 ++failed;
             }
             ++completed;
             if (r.pinned) pinned_free[r.pinned_class].push_back(r.pinned);
             reqs.erase(r.ticket);
             s.has_job = false;
-            aq.finish(si, r.offset);
             done.push_back(cpl);
+            if (out.reclaim) {
+                aq.finish(si, r.offset);
+            } else {
+                // The device may still write this request's range: keep it, and the slot, out of use.
+                broken = true;
+                aq.retire(si);
+                if (aq.live_slots() == 0) {
+                    for (uint64_t t : aq.drop_all()) {
+                        Request& q = reqs.at(t);
+                        sgm_completion dc{};
+                        dc.ticket = t;
+                        dc.user_tag = q.tag;
+                        dc.status = SGM_ERR_IO;
+                        dc.t_submit_ns = q.t_submit;
+                        dc.t_done_ns = now_ns();
+                        if (q.pinned) pinned_free[q.pinned_class].push_back(q.pinned);
+                        reqs.erase(t);
+                        ++failed;
+                        ++completed;
+                        done.push_back(dc);
+                    }
+                }
+            }
             dispatch(aq.admit());
         }
         cv_done.notify_all();
+        if (!out.reclaim) return;  // a retired slot's worker ends
     }
 }
 
@@ -302,12 +362,18 @@ int sgm_server_stop(sgm_server* sv) {
     }
     for (auto& s : sv->slots)
         if (s.th.joinable()) s.th.join();
+    for (auto& c : sv->compute)
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         for (hipEvent_t e : s.ev)
             if (e) (void)hipEventDestroy(e);
+        if (s.ev_up) (void)hipEventDestroy(s.ev_up);
+        if (s.ev_done) (void)hipEventDestroy(s.ev_done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
+    for (auto& c : sv->compute)
+        if (c->stream) (void)hipStreamDestroy(c->stream);
     for (uint8_t* b : sv->pinned_all) (void)hipHostFree(b);
     for (const MapSet& m : sv->map_sets) (void)hipFree(m.d_maps);
     if (sv->d_arena) (void)hipFree(sv->d_arena);
@@ -329,8 +395,9 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     sgm_params pmax = *params;
     pmax.num_disparities = cfg->max_disparities;
     if (validate_params_ext(&pmax, err) != SGM_OK || cfg->max_width < 1 || cfg->max_width > kMaxWidth || cfg->max_height < 1 ||
-        cfg->slots < 1 || cfg->slots > 64) {
-        set_thread_error("server config: max sizes, max_disparities (1..512) or slots (1..64) out of range" +
+        cfg->slots < 1 || cfg->slots > 64 || cfg->compute_streams < -1 || cfg->compute_streams > cfg->slots) {
+        set_thread_error("server config: max sizes, max_disparities (1..512), slots (1..64) or compute_streams "
+                         "(-1..slots) out of range" +
                          (err.empty() ? std::string() : ": " + err));
         return SGM_ERR_INVALID_ARGUMENT;
     }
@@ -358,6 +425,14 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
         // Timing only (a request completes by a stream synchronisation): no system-scope fence.
         for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreateWithFlags(&s.ev[i], hipEventDisableSystemFence);
+        // Hand-offs between the copy and compute streams of one device.
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ev_up, hipEventDisableTiming | hipEventDisableSystemFence);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming | hipEventDisableSystemFence);
+    }
+    const int ncs = cfg->compute_streams == 0 ? 1 : (cfg->compute_streams < 0 ? 0 : cfg->compute_streams);
+    for (int i = 0; e == hipSuccess && i < ncs; ++i) {
+        sv->compute.push_back(std::make_unique<sgm_server::Compute>());
+        e = hipStreamCreateWithFlags(&sv->compute.back()->stream, hipStreamNonBlocking);
     }
     if (e == hipSuccess) e = sv->add_pinned_class(2 * npx, 2 * cfg->slots + 2);
     if (e != hipSuccess) {
@@ -425,11 +500,16 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
         }
     }
     std::lock_guard lk(sv->mu);
-    if (sv->stopping || !sv->aq.can_ever_fit(r.bytes)) {
+    if (sv->stopping || sv->broken || !sv->aq.can_ever_fit(r.bytes)) {
         if (r.pinned) sv->pinned_free[r.pinned_class].push_back(r.pinned);
         if (sv->stopping) {
             sv->err = "sgm_server_submit: server is stopping";
             return SGM_ERR_DISCONNECTED;
+        }
+        if (sv->broken) {
+            sv->err = "sgm_server_submit: a request's device work could not be confirmed finished; the server "
+                      "takes no new requests (stop it)";
+            return SGM_ERR_IO;
         }
         sv->err = "sgm_server_submit: request needs " + std::to_string(r.bytes) + " bytes, more than the HBM arena";
         return SGM_ERR_BUSY;

@@ -73,7 +73,7 @@ class CSgmStageTotals(C.Structure):
 
 class CSgmServerConfig(C.Structure):
     _fields_ = [("max_width", C.c_int32), ("max_height", C.c_int32), ("max_disparities", C.c_int32),
-                ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
+                ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64), ("compute_streams", C.c_int32)]
 
 
 class CSgmRequest(C.Structure):
@@ -282,6 +282,8 @@ def lib():
         L.sgm_batch_device_bytes.argtypes = [vp]
         L.sgm_batch_device_bytes.restype = C.c_size_t
         L.sgm_batch_destroy.argtypes = [vp]
+        L.sgm_tune_cache_load.argtypes = [C.c_char_p, P(i32)]
+        L.sgm_tune_cache_save.argtypes = [C.c_char_p, P(i32)]
         L.sgm_server_start.argtypes = [P(CSgmParams), P(CSgmServerConfig), P(vp)]
         L.sgm_server_submit.argtypes = [vp, P(CSgmRequest), P(C.c_uint64)]
         L.sgm_server_add_rectification.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, P(C.c_int32)]
@@ -304,6 +306,13 @@ def _check(rc: int, ctx=None, batch=None):
         else:
             msg = L.sgm_last_error(ctx)
         raise SgmError(rc, (msg or b"").decode(errors="replace"))
+
+
+def _same_shape(a: np.ndarray, b: np.ndarray, what: str):
+    """Both images 2-D and of one size: the C calls take one width, height and stride for both, so a
+    smaller second image would be read past its end."""
+    if a.ndim != 2 or a.shape != b.shape:
+        raise SgmError(5, f"{what} must be 2-D images of the same size (got {a.shape} and {b.shape})")
 
 
 def _ptr(a):
@@ -385,9 +394,14 @@ class HipStereo:
     def compute_disparity(self, left, right, full_left=None, full_right=None) -> dict:
         """depth::compute_disparity (stereo.cpp:496-536): disparity and confidence at left's size."""
         left, right = np.ascontiguousarray(left, np.uint8), np.ascontiguousarray(right, np.uint8)
+        _same_shape(left, right, "left and right")
         h, w = left.shape
         fl = None if full_left is None else np.ascontiguousarray(full_left, np.uint8)
         fr = None if full_right is None else np.ascontiguousarray(full_right, np.uint8)
+        if (fl is None) != (fr is None):
+            raise SgmError(5, "full_left and full_right must be given together")
+        if fl is not None:
+            _same_shape(fl, fr, "full_left and full_right")
         fh, fw = (0, 0) if fl is None else fl.shape
         res = {"disparity": np.empty((h, w), np.float32), "confidence": np.empty((h, w), np.float32)}
         _check(lib().sgm_compute_disparity(self._h, _ptr(left), _ptr(right), w, w, h, _ptr(fl), _ptr(fr), fw, fw, fh,
@@ -406,6 +420,7 @@ class HipStereo:
     def compute_disparity_raw(self, raw_left, raw_right, rectified: bool = False) -> dict:
         """MetalStereo::compute_raw with the full StereoParams: rectify, then compute_disparity."""
         raw_left, raw_right = np.ascontiguousarray(raw_left, np.uint8), np.ascontiguousarray(raw_right, np.uint8)
+        _same_shape(raw_left, raw_right, "raw frames")
         rh, rw = raw_left.shape
         w, h = getattr(self, "rect_size", (1, 1))
         res = {"disparity": np.empty((h, w), np.float32), "confidence": np.empty((h, w), np.float32)}
@@ -434,6 +449,7 @@ class HipStereo:
     def compute_frame_raw(self, raw_left, raw_right, rectified: bool = False) -> dict:
         """MetalStereo::compute_frame_raw: raw frames -> points/normals (h, w, 4) and weights (h, w)."""
         raw_left, raw_right = np.ascontiguousarray(raw_left, np.uint8), np.ascontiguousarray(raw_right, np.uint8)
+        _same_shape(raw_left, raw_right, "raw frames")
         rh, rw = raw_left.shape
         w, h = getattr(self, "rect_size", (1, 1))
         res = {"points": np.empty((h, w, 4), np.float32), "normals": np.empty((h, w, 4), np.float32),
@@ -677,6 +693,24 @@ class Batch:
             pass
 
 
+def tune_cache_load(path: str) -> int:
+    """Merges a plan-cache file (tune_cache_save) into the process's autotuner cache; returns its entries."""
+    n = C.c_int(0)
+    rc = lib().sgm_tune_cache_load(os.fsencode(path), C.byref(n))
+    if rc != 0:
+        raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
+    return n.value
+
+
+def tune_cache_save(path: str) -> int:
+    """Writes the process's autotuner plan cache to `path`; returns the number of entries."""
+    n = C.c_int(0)
+    rc = lib().sgm_tune_cache_save(os.fsencode(path), C.byref(n))
+    if rc != 0:
+        raise SgmError(rc, (lib().sgm_last_error(None) or b"").decode())
+    return n.value
+
+
 class PinnedPool:
     """Reusable pinned host arrays (sgm_host_alloc), so device copies into them are direct DMA."""
 
@@ -714,8 +748,10 @@ class Server:
     until `release(completion)` returns them to the pool (or the server stops)."""
 
     def __init__(self, params: SgmParams, max_width: int, max_height: int, max_disparities: int, slots: int = 4,
-                 hbm_budget_bytes: int = 0):
-        cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes)
+                 hbm_budget_bytes: int = 0, compute_streams: int = 0):
+        """compute_streams: 0 = one shared compute stream (requests' kernels in admission order, copies
+        overlapping them), k = k shared streams, -1 = each slot computes on its own stream."""
+        cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes, compute_streams)
         h = C.c_void_p()
         cp = params.to_c()
         rc = lib().sgm_server_start(C.byref(cp), C.byref(cfg), C.byref(h))

@@ -1,7 +1,8 @@
 // CPU test of the serving loop's admission logic (open_source_einstar_amd/csrc/serve_core.h) with a
 // fake device: random request sizes, random completion order; checks that admitted requests never
 // overlap in the arena, never exceed the slots, are admitted in arrival order, and that everything
-// submitted completes. Exit status 0 = pass.
+// submitted completes; that a slot whose drain failed is retired with its memory; that a request's
+// streams are drained before anything is released. Exit status 0 = pass.
 #include <cstdio>
 #include <random>
 #include <set>
@@ -72,6 +73,37 @@ int main() {
             live.erase(live.begin() + static_cast<long>(k));
             ++completed;
         }
+    }
+    {  // a retired slot keeps its arena range and is never admitted to again
+        AdmissionQueue r(2, Arena(8 << 20, 1 << 20));
+        r.push(1, 3 << 20);
+        r.push(2, 3 << 20);
+        auto a = r.admit();
+        expect(a.size() == 2, "two slots admit two requests");
+        r.retire(a[0].slot);
+        r.finish(a[1].slot, a[1].offset);
+        expect(r.live_slots() == 1 && r.busy_slots() == 0 && r.arena().free_bytes() == (5u << 20),
+               "a retired slot's range stays allocated");
+        r.push(3, 5 << 20);
+        r.push(4, 1 << 20);
+        auto b = r.admit();
+        expect(b.size() == 1 && b[0].slot == a[1].slot && b[0].offset != a[0].offset, "only the live slot admits");
+        r.retire(b[0].slot);
+        expect(r.live_slots() == 0 && r.admit().empty(), "no live slot: nothing admitted");
+        const auto dropped = r.drop_all();
+        expect(dropped.size() == 1 && dropped[0] == 4 && r.queued() == 0, "waiting requests are dropped in order");
+    }
+    {  // finish_request drains first, always, and reclaims only after a good drain
+        int drains = 0;
+        auto ok = finish_request(true, [&] { ++drains; return true; });
+        auto enq_fail = finish_request(false, [&] { ++drains; return true; });
+        auto drain_fail = finish_request(true, [&] { ++drains; return false; });
+        auto both = finish_request(false, [&] { ++drains; return false; });
+        expect(drains == 4, "the streams are drained even when the enqueue failed");
+        expect(ok.ok && ok.reclaim, "good request: valid, memory reused");
+        expect(!enq_fail.ok && enq_fail.reclaim, "failed enqueue, drained: error status, memory reused");
+        expect(!drain_fail.ok && !drain_fail.reclaim, "failed drain: error status, memory never reused");
+        expect(!both.ok && !both.reclaim, "both failed: error, memory never reused");
     }
     expect(order, "admission in arrival order");
     expect(!overlap, "admitted blocks never overlap");

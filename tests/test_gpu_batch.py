@@ -78,24 +78,68 @@ def test_every_plan_gives_the_same_bytes(tuning):
         b.close()
 
 
-def test_autotuner_times_candidates_and_caches_the_choice():
+def test_autotuner_times_candidates_and_caches_the_choice(tmp_path):
     sizes = [(96, 64, 64), (64, 48, 128), (40, 30, 24)]
     specs, refs = make_specs(sizes, 4000)
     b1 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     p1 = b1.plan()
     b1.close()
     assert p1["candidates_timed"] == 20 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
+    knobs = ("cost_volume", "path_lines", "branches", "path_group")
     b2 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     try:
         p2 = b2.plan()
-        assert p2["from_cache"] == 1 and all(p2[k] == p1[k] for k in ("cost_volume", "path_lines", "branches",
-                                                                        "path_group"))
+        assert p2["from_cache"] == 1 and all(p2[k] == p1[k] for k in knobs)
         b2.run()
         b2.synchronize()
         for i in range(len(sizes)):
             assert_same(refs[i], b2.results(i), FIELDS, tag=f"tuned pair {i}")
     finally:
         b2.close()
+    # A knob fixed against the cached choice is honoured: the cached plan is not taken, the open knobs
+    # are searched, and the restricted result does not replace the cached plan.
+    other = 1 - p1["cost_volume"]
+    b3 = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning={"cost_volume": other})
+    try:
+        p3 = b3.plan()
+        assert p3["cost_volume"] == other and p3["from_cache"] == 0 and p3["candidates_timed"] > 1
+        b3.run()
+        b3.synchronize()
+        for i in range(len(sizes)):
+            assert_same(refs[i], b3.results(i), FIELDS, tag=f"restricted pair {i}")
+    finally:
+        b3.close()
+    b4 = sgm.Batch(sgm.SgmParams(**BASE), specs)
+    p4 = b4.plan()
+    b4.close()
+    assert p4["from_cache"] == 1 and all(p4[k] == p1[k] for k in knobs)
+    # A knob fixed in agreement with the cached plan takes it from the cache.
+    b5 = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning={"cost_volume": p1["cost_volume"]})
+    p5 = b5.plan()
+    b5.close()
+    assert p5["from_cache"] == 1 and all(p5[k] == p1[k] for k in knobs)
+    # Persisted: the saved file holds the plan; a file loaded later replaces it for the same key.
+    path = tmp_path / "plans.txt"
+    assert sgm.tune_cache_save(str(path)) >= 1
+    # The key: path count, census window, slant flag, then each pair's width and height in 64-pixel
+    # buckets, disparities per lane (p: padded) and median flag, sorted.
+    items = sorted(f"{-(-w // 64)}x{-(-h // 64)}d{-(-nd // 16) if nd > 32 else 2}{'p' if nd % 16 else ''}m"
+                   for w, h, nd in sizes)
+    key = "v2:8:4x3:s1;" + ";".join(items)
+    assert key + "\t" + " ".join(str(p1[k]) for k in knobs) in path.read_text().splitlines(), (key, path.read_text())
+    alt = dict(cost_volume=1 - p1["cost_volume"], path_lines=4, branches=2, path_group=16)
+    path.write_text(key + "\t" + " ".join(str(alt[k]) for k in knobs) + "\n")
+    assert sgm.tune_cache_load(str(path)) == 1
+    b6 = sgm.Batch(sgm.SgmParams(**BASE), specs)
+    try:
+        p6 = b6.plan()
+        assert p6["from_cache"] == 1 and all(p6[k] == alt[k] for k in knobs)
+        b6.run()
+        b6.synchronize()
+        for i in range(len(sizes)):
+            assert_same(refs[i], b6.results(i), FIELDS, tag=f"loaded-plan pair {i}")
+    finally:
+        b6.close()
 
 
 def batch_sweep(n, seed):
