@@ -316,9 +316,14 @@ enum {
     SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 16 or 32 (default; used when the
                                 lane layout allows it -- disparities padded to a multiple of 64 -- with
                                 at most 8 lines per block, else 16) */
-    SGM_OPT_RECOMPUTE_DIRS = 4 /* with the cost volume on: how many of the path directions (in the order
+    SGM_OPT_RECOMPUTE_DIRS = 4, /* with the cost volume on: how many of the path directions (in the order
                                   left-right, right-left, down, up, then the diagonals) recompute their
                                   costs from census words instead of reading them, 0..8 */
+    SGM_OPT_VIEW_BASE_HIGH = 5 /* test aid, value 1: re-places the census words, step penalties and
+                                  cost volume (the buffers the path kernel reads through 32-bit buffer
+                                  views) at addresses whose low 32-bit word has bit 31 set, each inside
+                                  an allocation 4 GiB larger than it, so that a sign extension of that
+                                  word would show up as wrong results */
 };
 int sgm_set_option(sgm_context* ctx, int option, int64_t value);
 

@@ -198,6 +198,9 @@ struct sgm_context {
     uint16_t* d_best = nullptr;
     uint16_t* d_second = nullptr;
     uint8_t* d_cvol = nullptr;   // u8 matching costs when the cost-volume mode is on
+    // SGM_OPT_VIEW_BASE_HIGH: the allocations that d_census and d_cvol were placed inside (else null).
+    void* d_census_block = nullptr;
+    void* d_cvol_block = nullptr;
     bool use_cvol = false;
     int path_lines = 16;         // scanlines per path-kernel block
     int path_group = 32;         // lanes per scanline in the path kernel (32 where the lane layout allows)
@@ -270,9 +273,11 @@ void sgm_destroy(sgm_context* ctx) {
     for (Stages& st : ctx->ring)
         for (hipEvent_t e : st.ev)
             if (e) (void)hipEventDestroy(e);
-    for (void* p : {(void*)ctx->d_img, (void*)ctx->d_census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
+    void* census = ctx->d_census_block ? ctx->d_census_block : (void*)ctx->d_census;
+    void* cvol = ctx->d_cvol_block ? ctx->d_cvol_block : (void*)ctx->d_cvol;
+    for (void* p : {(void*)ctx->d_img, census, (void*)ctx->d_lvol, (void*)ctx->d_disp, (void*)ctx->d_median,
                     (void*)ctx->d_conf, (void*)ctx->d_dint, (void*)ctx->d_status, (void*)ctx->d_rdisp, (void*)ctx->d_best,
-                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, (void*)ctx->d_cvol, (void*)ctx->d_maps,
+                    (void*)ctx->d_second, (void*)ctx->d_sum, (void*)ctx->d_cost, cvol, (void*)ctx->d_maps,
                     (void*)ctx->d_raw, (void*)ctx->d_pyr, (void*)ctx->d_wa, (void*)ctx->d_wb, (void*)ctx->d_wm,
                     (void*)ctx->d_wc, (void*)ctx->d_labels, (void*)ctx->d_local, (void*)ctx->d_sizes,
                     (void*)ctx->d_full, (void*)ctx->d_pts, (void*)ctx->d_nrm, (void*)ctx->d_wts})
@@ -462,9 +467,46 @@ int sgm_create_stereo(const sgm_stereo_params* sp, int width, int height, sgm_co
 
 size_t sgm_device_bytes(const sgm_context* ctx) { return ctx ? ctx->bytes : 0; }
 
+// A buffer of `bytes` inside a new allocation of bytes + 4 GiB, placed where the low 32-bit word of its
+// address is `low`: *block receives the allocation, the return value the placed buffer.
+static hipError_t alloc_at_low_word(size_t bytes, uint32_t low, void** block, void** placed) {
+    *block = *placed = nullptr;
+    hipError_t e = hipMalloc(block, bytes + (size_t(1) << 32));
+    if (e != hipSuccess) return e;
+    const uintptr_t base = reinterpret_cast<uintptr_t>(*block);
+    *placed = reinterpret_cast<void*>(base + (uint32_t)(low - (uint32_t)base));  // distance mod 2^32
+    return hipSuccess;
+}
+
 int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
     if (!ctx) return SGM_ERR_INVALID_ARGUMENT;
     switch (option) {
+        case SGM_OPT_VIEW_BASE_HIGH: {
+            if (value != 1) return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "SGM_OPT_VIEW_BASE_HIGH takes 1");
+            if (ctx->d_census_block) return SGM_OK;
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e == hipSuccess && ctx->last_stream) e = hipStreamSynchronize(ctx->last_stream);
+            if (e != hipSuccess) return ctx->hip_fail("sgm_set_option: synchronize", e);
+            const size_t npx = (size_t)ctx->max_w * ctx->max_h;
+            const size_t census_bytes = (2 * ctx->census_half + (npx + 1) / 2) * 8;
+            const size_t cvol_bytes = npx * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
+            void *cb = nullptr, *cp = nullptr, *vb = nullptr, *vp = nullptr;
+            e = alloc_at_low_word(census_bytes, 0x80000100u, &cb, &cp);
+            if (e == hipSuccess && cvol_bytes <= kMaxViewBytes) e = alloc_at_low_word(cvol_bytes, 0x90000100u, &vb, &vp);
+            if (e != hipSuccess) {
+                dfree(cb);
+                dfree(vb);
+                return ctx->hip_fail("sgm_set_option: placed allocation", e);
+            }
+            dfree(ctx->d_census);
+            dfree(ctx->d_cvol);
+            ctx->d_census_block = cb;
+            ctx->d_census = static_cast<uint64_t*>(cp);
+            ctx->d_cvol_block = vb;
+            ctx->d_cvol = static_cast<uint8_t*>(vp);
+            ctx->bytes += 2 * (size_t(1) << 32);
+            return SGM_OK;
+        }
         case SGM_OPT_COST_VOLUME: {
             const size_t n = (size_t)ctx->max_w * ctx->max_h * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
             if (value && n > kMaxViewBytes)

@@ -266,3 +266,26 @@ def test_random_parameter_sweep(w, h, seed, params, knobs):
             st.set_option(k, v)
         got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
     assert_same(ref, got, FIELDS + VOLUMES, tag=f"sweep {w}x{h}")
+
+
+VIEW_CASES = [(1, 16), (1, 32), (0, 16), (0, 32)]  # (cost volume, lanes per scanline)
+
+
+@pytest.mark.parametrize("cv,group", VIEW_CASES, ids=[f"cv{cv}-g{g}" for cv, g in VIEW_CASES])
+def test_view_bases_with_bit31_set(cv, group):
+    """Regression test of the buffer views' base address: the census words, step penalties and cost
+    volume are placed where the low 32-bit word of the address has bit 31 set (SGM_OPT_VIEW_BASE_HIGH),
+    so that a sign extension of that word while building a view would read the wrong memory."""
+    w, h, nd = 200, 96, 128
+    pair = speckle_pair(w, h, nd, 4242)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=True, census_radius_x=4,
+                  census_radius_y=3, p1=10, p2=120)
+    print(f"view bases bit 31: cv={cv} lanes={group}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_VIEW_BASE_HIGH, 1)
+        st.set_option(sgm.OPT_COST_VOLUME, cv)
+        st.set_option(sgm.OPT_PATH_GROUP, group)
+        for _ in range(2):
+            got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+            assert_same(ref, got, FIELDS + VOLUMES, tag=f"high view bases cv={cv} lanes={group}")
