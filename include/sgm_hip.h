@@ -147,6 +147,26 @@ int sgm_compute(sgm_context* ctx, const uint8_t* left, const uint8_t* right, ptr
 int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch, int width,
                        int height, const sgm_outputs* d_out, void* stream);
 
+/* Bytes of device scratch that sgm_compute_device_noalloc has on each GPU (128 MiB). */
+#define SGM_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* sgm_compute_device without a context and without allocating: no counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * hosts that forbid hipMalloc). Same inputs, outputs, results and stream semantics as
+ * sgm_compute_device (params as sgm_create takes them; d_out may be NULL or leave disparity NULL);
+ * census outputs are refused. The call allocates nothing: its scratch (census words, step
+ * penalties, path costs, the cost volume where it fits, the disparity map when d_out has none) lives
+ * in one static device array of SGM_NOALLOC_SCRATCH_BYTES per GPU, held from when the library's
+ * device code is loaded on that GPU (at the latest, the first call there) to the end of the process. Pairs whose scratch exceeds it are refused with
+ * SGM_ERR_INVALID_ARGUMENT (e.g. 640x512 with 64 disparities and 4 paths fits, 1280x1024x128 with 8
+ * paths does not). Every call on one GPU shares that array: calls whose work overlaps on one GPU
+ * (different streams, or different host threads, without ordering between them) race and give wrong
+ * results; calls on one stream are ordered. Errors: sgm_last_error(NULL) on the calling thread. */
+int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, const uint8_t* d_right, ptrdiff_t pitch,
+                               int width, int height, const sgm_outputs* d_out, void* stream);
+/* SGM_NOALLOC_SCRATCH_BYTES, as built into the library. */
+size_t sgm_noalloc_scratch_bytes(void);
+
 /* Replaces MetalStereo::set_rectification (metal_stereo.hpp:74-75, metal_stereo.cpp:203-231): uploads
  * the left and right calib::RemapTable maps (libs/calib/include/einstar/calib/rectify.hpp:40-43;
  * map_x and map_y as row-major map_w x map_h host arrays of source pixel coordinates) once; they stay

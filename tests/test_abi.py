@@ -98,3 +98,20 @@ def test_struct_sizes_match_the_header():
                sgm.CSgmBatchTuning, sgm.CSgmBatchPlan, sgm.CSgmRectifiedGeometry, sgm.CSgmPointParams, sgm.CSgmFrame]
     for n, m, sz in zip(names, mirrors, sizes):
         assert C.sizeof(m) == sz, (n, C.sizeof(m), sz)
+
+
+def test_noalloc_entry_point_refuses_oversize_pairs_before_any_gpu_work():
+    """sgm_compute_device_noalloc checks the pair against its static scratch before it touches the
+    device (the pointers here are never dereferenced)."""
+    L = sgm.lib()
+    L.sgm_compute_device_noalloc.argtypes = [C.POINTER(sgm.CSgmParams), C.c_void_p, C.c_void_p, C.c_ssize_t, C.c_int,
+                                             C.c_int, C.POINTER(sgm.CSgmOutputs), C.c_void_p]
+    L.sgm_noalloc_scratch_bytes.restype = C.c_size_t
+    assert L.sgm_noalloc_scratch_bytes() == 128 << 20
+    cp = sgm.SgmParams(num_disparities=128, eight_paths=True, census_radius_x=4, census_radius_y=3, p1=10,
+                       p2=120).to_c()
+    rc = L.sgm_compute_device_noalloc(C.byref(cp), 16, 16, 1280, 1280, 1024, C.byref(sgm.CSgmOutputs()), None)
+    assert sgm.ERRC[rc] == "invalid_argument" and b"scratch" in L.sgm_last_error(None)
+    bad = sgm.SgmParams(num_disparities=600).to_c()
+    rc = L.sgm_compute_device_noalloc(C.byref(bad), 16, 16, 64, 64, 48, C.byref(sgm.CSgmOutputs()), None)
+    assert sgm.ERRC[rc] == "invalid_argument"
