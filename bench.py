@@ -8,16 +8,21 @@ calls on one stream between two synchronisations.
 
 Printed as one JSON line:
   value            Mpix*disp/s of the timed steps (W*H*D per step / time per step)
-  roofline         the aggregation kernel against HBM: the canonical per-direction dataflow's bytes
-                   for 8 paths (38 B per pixel*disparity + 8 B per pixel) over the kernel's average
-                   duration from HIP events on its stream; traffic = measured HBM bytes per launch
-                   from the committed rocprofv3 counter summary, or null; moved_gbs / moved_frac =
-                   traffic over the same duration (this dataflow moves fewer bytes than the
-                   canonical one, so frac can exceed 1 while moved_frac cannot)
-  cpu_baseline     the CPU oracle on the host cores, same pair, median of runs
+  roofline         the aggregation (path) kernel against HBM: its own algorithmic bytes per launch --
+                   P*W*H*dp cost bytes read, P*W*H*dp path-cost bytes written, P*4 bytes of step
+                   penalties per pixel (16 B per pixel*disparity + 32 B per pixel at 8 paths) -- over its
+                   average duration from HIP events on its stream; frac <= 1. traffic = measured bytes
+                   per launch from the committed rocprofv3 counter summary (FETCH_SIZE counts reads
+                   served by the Infinity Cache too). vs_canonical = the same time against the canonical
+                   per-direction dataflow (u16 sum volume read-modified-written per direction, 38 B per
+                   pixel*disparity + 8 B per pixel), which this design does not move.
+  cpu_baseline     the CPU oracle on the host cores, same pair, median of runs, with the CPU model
   latency_ms       p50/p99 of single synchronised steps (launch to host-visible completion)
   batched          32 uneven pairs (sizes 320-1280 x 240-1024, D in {64,96,128,192}) in one graph
-                   launch vs the same pairs launched pair by pair
+                   launch vs the same pairs launched pair by pair, with each form's dataflow bytes/s
+  large_batched    64 pairs of 2048x1536x256 streamed through a resident batch of 16
+  served           Poisson 60 Hz stream of 10,000 frames (3:1 mix of 1280x1024x128 and 640x512x64 from raw
+                   frames with resident maps): p50/p99/p99.9 latency; and a sweep of arrival rates
 """
 from __future__ import annotations
 
@@ -42,6 +47,48 @@ AGG_BYTES_PER_PXD = 38   # canonical aggregation dataflow, 8 paths: W*H*D*(5P-2)
 AGG_BYTES_PER_PX = 8
 
 
+def lane_dp(nd):
+    """Disparities a scanline carries (16 lanes x the smallest of 2,4,6,8,12,16,24,32 per lane >= nd/16)."""
+    need = -(-nd // 16)
+    return 16 * next(d for d in (2, 4, 6, 8, 12, 16, 24, 32) if d >= need)
+
+
+def path_kernel_bytes(w, h, nd, paths=8, cost_volume=True):
+    """Algorithmic bytes of one path-kernel launch over one pair: per direction each pixel's dp
+    matching costs read (cost volume; recomputing them reads the left census word and the right one
+    the window slides onto, 16 B), its 4-byte step-penalty word read and its dp path costs written."""
+    dp = lane_dp(nd)
+    per_px_dir = (dp if cost_volume else 16) + 4 + dp
+    return paths * w * h * per_px_dir
+
+
+def dataflow_bytes(w, h, nd, paths=8, cost_volume=True, median=True):
+    """Algorithmic bytes of the whole per-pair pipeline as this library moves them: census (both
+    images read, both census words and the step penalties written), the cost volume (census words
+    read, dp cost bytes written), the paths (above), the rows (P*dp path-cost bytes read; disparity,
+    integer disparity and status written) and the median (disparity read, median written)."""
+    dp = lane_dp(nd)
+    b = w * h * (2 + 16 + 4)
+    if cost_volume:
+        b += w * h * (16 + dp)
+    b += path_kernel_bytes(w, h, nd, paths, cost_volume)
+    b += w * h * (paths * dp + 4 + 4 + 1)
+    if median:
+        b += w * h * 8
+    return b
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -54,51 +101,86 @@ def batch_sizes(n=32, seed=3):
     return [(int(w), int(h), int(d)) for w, h, d in zip(ws, hs, ds)]
 
 
-def serve_stream(sgm, rate_hz, n_frames, pairs, seed=7, slots=4, maps=None):
+def serve_stream(sgm, rate_hz, n_frames, pairs, seed=7, slots=4, maps=None, compute_streams=0, warmup=24,
+                 progress_every=0):
     """Poisson arrivals at rate_hz of a 3:1 mix of 1280x1024x128 frames and 640x512x64 ROI crops,
     submitted to the serving loop as they arrive. pairs = (full, roi) lists of input pairs. With
     maps = (full_maps, roi_maps), the inputs are raw sensor frames at twice the stereo resolution,
-    read in place from pinned memory and rectified on the device with the resident maps. Returns
-    latency percentiles (arrival -> results in host memory) and the sustained Mpix*disp/s."""
+    read in place from pinned memory and rectified on the device with the resident maps. `warmup`
+    frames of both kinds run first, untimed (first launches load code objects). Returns latency
+    percentiles (scheduled arrival -> results in host memory), their split into waiting for
+    admission and service, and the sustained Mpix*disp/s."""
     rng = np.random.default_rng(seed)
     full, roi = pairs
     kinds = rng.random(n_frames) < 0.75
     gaps = rng.exponential(1.0 / rate_hz, n_frames)
-    sv = sgm.Server(sgm.SgmParams(**PARAMS), 1280, 1024, 128, slots=slots)
+    sv = sgm.Server(sgm.SgmParams(**PARAMS), 1280, 1024, 128, slots=slots, compute_streams=compute_streams)
     try:
         ids = (0, 0)
         if maps is not None:
             ids = tuple(sv.add_rectification(m.left_x, m.left_y, m.right_x, m.right_y, p[0].left.shape[1],
                                              p[0].left.shape[0]) for m, p in zip(maps, pairs))
+
+        def submit(i, k, tag):
+            p = full[i % len(full)] if k == 0 else roi[i % len(roi)]
+            sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=("disparity", "disparity_median"),
+                      tag=tag, rectification=ids[k], borrow=maps is not None)
+            h, w = p.left.shape
+            return (w * h // 4 if maps is not None else w * h) * p.num_disparities
+
+        left = warmup
+        for i in range(warmup):
+            submit(i, i % 2, -1)
+        t_end = time.time() + 60
+        while left and time.time() < t_end:
+            for c in sv.poll(64, 200000):
+                sv.release(c)
+                left -= 1
+        if left:
+            raise RuntimeError(f"served warm-up: {left} of {warmup} frames did not complete")
         t0 = sgm.Server.now_ns() + 2_000_000
         arrival = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
-        lat, done_t, pxd, got = [], [], 0, 0
-        for i in range(n_frames):
-            k = 0 if kinds[i] else 1
-            p = full[i % len(full)] if k == 0 else roi[i % len(roi)]
-            while sgm.Server.now_ns() < arrival[i]:
-                for c in sv.poll(64, 0):
-                    lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
-                    done_t.append(c["t_done_ns"])
-                    sv.release(c)
-                    got += 1
-                time.sleep(0.0001)
-            sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=("disparity", "disparity_median"),
-                      tag=i, rectification=ids[k], borrow=maps is not None)
-            h, w = p.left.shape
-            pxd += (w * h // 4 if maps is not None else w * h) * p.num_disparities
-        while got < n_frames:
-            for c in sv.poll(64, 200000):
-                lat.append((c["t_done_ns"] - arrival[c["tag"]]) / 1e6)
-                done_t.append(c["t_done_ns"])
+        lat = np.zeros(n_frames)
+        wait = np.zeros(n_frames)
+        gpu = np.zeros(n_frames)
+        done_t = np.zeros(n_frames, np.int64)
+        pxd, got = 0, 0
+
+        def take(timeout_us):
+            nonlocal got
+            for c in sv.poll(64, timeout_us):
+                i = c["tag"]
+                lat[i] = (c["t_done_ns"] - arrival[i]) / 1e6
+                wait[i] = (c["t_admit_ns"] - arrival[i]) / 1e6
+                gpu[i] = c["gpu_ms"]
+                done_t[i] = c["t_done_ns"]
                 sv.release(c)
                 got += 1
-        span_s = (max(done_t) - arrival[0]) / 1e9
-        lat = np.array(lat)
-        return {"rate_hz": rate_hz, "frames": n_frames, "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
+                if progress_every and got % progress_every == 0:
+                    log(f"served {rate_hz:.0f} Hz: {got}/{n_frames} frames done")
+
+        for i in range(n_frames):
+            while sgm.Server.now_ns() < arrival[i]:
+                take(0)
+                time.sleep(0.0001)
+            pxd += submit(i, 0 if kinds[i] else 1, i)
+        t_end = time.time() + 120
+        while got < n_frames and time.time() < t_end:
+            take(200000)
+        if got < n_frames:
+            raise RuntimeError(f"served stream: {n_frames - got} of {n_frames} frames did not complete")
+        span_s = (done_t.max() - arrival[0]) / 1e9
+        pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+        worst = int(np.argmax(lat))
+        return {"rate_hz": rate_hz, "frames": n_frames, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                "p999_ms": pct(lat, 99.9), "max_ms": round(float(lat.max()), 3), "max_frame": worst,
+                "max_frame_wait_ms": round(float(wait[worst]), 3),
+                "wait_p50_ms": pct(wait, 50), "wait_p99_ms": pct(wait, 99),
+                "service_p50_ms": pct(lat - wait, 50), "service_p99_ms": pct(lat - wait, 99),
+                "gpu_full_p50_ms": pct(gpu[kinds], 50) if kinds.any() else None,
+                "gpu_roi_p50_ms": pct(gpu[~kinds], 50) if (~kinds).any() else None,
                 "value": round(pxd / span_s / 1e6, 0), "offered": round(pxd / ((arrival[-1] - arrival[0]) / 1e9) / 1e6, 0),
-                "failed": sv.stats()["failed"]}
+                "failed": sv.stats()["failed"], "slots": slots, "compute_streams": compute_streams}
     finally:
         sv.stop()
 
@@ -279,11 +361,20 @@ def large_batched(sgm, speckle_pair, resident=16, total=64, distinct=8, verify=T
                         threads=min(len(os.sched_getaffinity(0)), 64))
             bit_exact = all(np.array_equal(np.asarray(ref[f]).view(np.uint8), got[f].view(np.uint8))
                             for f in ("disparity", "disparity_int", "status"))
+        plan = b.plan()
+        flow = total * dataflow_bytes(W5, H5, D5, 8, bool(plan["cost_volume"]), median=False)
+
+        def hbm(sec):  # the dataflow's bytes over the time, against HBM peak
+            gbs = flow / sec / 1e9
+            return {"gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
         return {"pairs": total, "resident": resident, "width": W5, "height": H5, "disparities": D5,
-                "device_gb": round(b.device_bytes() / 1e9, 1), "plan": b.plan(), "plan_s": round(plan_s, 2),
+                "device_gb": round(b.device_bytes() / 1e9, 1), "plan": plan, "plan_s": round(plan_s, 2),
                 "streamed_value": round(pxd / streamed_s / 1e6, 0), "graph_value": round(pxd / graph_s / 1e6, 0),
                 "plain_value": round(pxd / plain_s / 1e6, 0), "ratio": round(plain_s / graph_s, 3),
-                "ms_per_pair": round(graph_s * 1e3 / total, 3), "bit_exact_pair0": bit_exact}
+                "ms_per_pair": round(graph_s * 1e3 / total, 3), "bit_exact_pair0": bit_exact,
+                "dataflow_bytes_per_pair": flow // total, "dataflow_bytes_per_pxd": round(flow / pxd, 2),
+                "hbm_graph": hbm(graph_s), "hbm_streamed": hbm(streamed_s), "hbm_plain": hbm(plain_s)}
     finally:
         b.close()
 
@@ -310,9 +401,15 @@ def main():
     ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
     ap.add_argument("--no-large", action="store_true", help="skip the 64 x 2048x1536x256 streamed batch")
     ap.add_argument("--no-chain", action="store_true", help="skip the full compute_disparity chain")
-    ap.add_argument("--serve-frames", type=int, default=600, help="frames of the 60 Hz served stream (0: skip)")
-    ap.add_argument("--serve-sweep", type=str, default="240,480,720,960",
-                    help="arrival rates (Hz) of the saturation sweep, 300 frames each ('' to skip)")
+    ap.add_argument("--serve-frames", type=int, default=10000,
+                    help="frames of the 60 Hz served stream of raw frames (0: skip)")
+    ap.add_argument("--serve-sweep", type=str, default="480,720,960,1200,1440,1920",
+                    help="arrival rates (Hz) of the saturation sweep ('' to skip)")
+    ap.add_argument("--sweep-frames", type=int, default=2000, help="frames per sweep rate")
+    ap.add_argument("--sweep-modes", type=str, default="0,-1",
+                    help="server compute_streams settings of the sweep (0: one shared compute stream; -1: per slot)")
+    ap.add_argument("--rect-serve-frames", type=int, default=600,
+                    help="frames of the 60 Hz stream of already rectified inputs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -407,6 +504,7 @@ def main():
         if not args.no_cpu:
             med = float(np.median(runs[1:]))  # first run is the warm-up
             cpu = {"value": round(pxd / med / 1e6, 1), "unit": "Mpix·disp/s", "cores": threads, "kind": "port",
+                   "cpu_model": cpu_model(),
                    "sample": f"oracle sgm_disparity+median3 on the same 1280x1024x128 8-path pair, median of "
                              f"{len(runs) - 1} runs after 1 warm-up ({med * 1e3:.0f} ms each)"}
             log(f"cpu baseline: {cpu}")
@@ -449,16 +547,26 @@ def main():
     d_raw.free()
     rect_buf.free()
 
-    agg_bytes = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
-    achieved = agg_bytes / (paths_ms * 1e-3) / 1e9
+    # The path kernel's own algorithmic bytes (its dataflow), so frac is a fraction of HBM peak.
+    alg_bytes = path_kernel_bytes(W, H, D, 8, bool(best_cv))
+    achieved = alg_bytes / (paths_ms * 1e-3) / 1e9
     kernel = (f"paths_kernel<{8 if best_group == 16 else 4}, false, {'true' if best_cv else 'false'}, {best_group}, true>")
     traffic = traffic_from_profiles(kernel)
+    canon = AGG_BYTES_PER_PXD * pxd + AGG_BYTES_PER_PX * W * H
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kernel, "kernel_ms": round(paths_ms, 4), "algorithmic_bytes": agg_bytes}
+                "kernel": kernel, "kernel_ms": round(paths_ms, 4), "algorithmic_bytes": alg_bytes,
+                "algorithmic_bytes_per_unit": ("8 directions x (dp cost bytes read + dp path-cost bytes written "
+                                               "+ 4 penalty bytes) per pixel = 16 B/px*disp + 32 B/px at dp = D = 128"
+                                               if best_cv else "8 directions x (16 census + 4 penalty + dp) B/px"),
+                "traffic_source": "profiles/sgm_1280x1024x128_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+                                  "copy-calibrated); FETCH_SIZE counts reads served by the 256 MB Infinity Cache "
+                                  "as well as HBM reads",
+                "vs_canonical": {"bytes": canon, "equivalent_gbs": round(canon / (paths_ms * 1e-3) / 1e9, 1),
+                                 "note": "the reference's per-direction dataflow (u16 sum volume read-modified-"
+                                         "written per direction: 38 B/px*disp + 8 B/px) for the same work; this "
+                                         "kernel never moves those bytes, so the figure may exceed HBM peak"}}
     if traffic:
-        # The canonical figure counts bytes this dataflow never moves (no u16 sum volume); the bytes
-        # the kernel really moves, over the same duration, are its share of HBM bandwidth.
         roofline["moved_gbs"] = round(traffic / (paths_ms * 1e-3) / 1e9, 1)
         roofline["moved_frac"] = round(traffic / (paths_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
@@ -482,11 +590,16 @@ def main():
                 run()
             b.synchronize()
             res[mode] = (time.perf_counter() - t) * 1e3 / args.batch_reps
-        batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1), "plan": b.plan(),
+        plan = b.plan()
+        flow = sum(dataflow_bytes(w, h, d, 8, bool(plan["cost_volume"])) for w, h, d in sizes)
+        batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1), "plan": plan,
                    "batched_ms": round(res["graph"], 3), "plain_ms": round(res["plain"], 3),
                    "batched_value": round(total_pxd / (res["graph"] * 1e-3) / 1e6, 0),
                    "plain_value": round(total_pxd / (res["plain"] * 1e-3) / 1e6, 0),
-                   "ratio": round(res["plain"] / res["graph"], 3)}
+                   "ratio": round(res["plain"] / res["graph"], 3),
+                   "dataflow_bytes": flow, "dataflow_bytes_per_pxd": round(flow / total_pxd, 2),
+                   "batched_dataflow_gbs": round(flow / (res["graph"] * 1e-3) / 1e9, 1),
+                   "plain_dataflow_gbs": round(flow / (res["plain"] * 1e-3) / 1e9, 1)}
         log(f"batched: {batched}")
         b.close()
 
@@ -501,26 +614,33 @@ def main():
         log(f"large batched: {large}")
 
     served = None
-    if args.serve_frames > 0:
+    rates = [float(x) for x in args.serve_sweep.split(",") if x.strip()]
+    if args.serve_frames > 0 or rates or args.rect_serve_frames > 0:
         # Raw sensor frames rectified on the device with resident maps (the served form), then the
         # same stream of already rectified pairs.
         pool = sgm.PinnedPool()
         raw_pairs, maps = raw_serving_inputs(sgm, speckle_pair, pool)
-        served = serve_stream(sgm, 60.0, args.serve_frames, raw_pairs, maps=maps)
-        served["inputs"] = "raw 2560x2048 / 1280x1024 frames, resident rectification maps, read in place"
-        log(f"served 60 Hz (raw): {served}")
+        served = {}
+        if args.serve_frames > 0:
+            served = serve_stream(sgm, 60.0, args.serve_frames, raw_pairs, maps=maps, progress_every=1000)
+            served["inputs"] = "raw 2560x2048 / 1280x1024 frames, resident rectification maps, read in place"
+            log(f"served 60 Hz (raw): {served}")
         sweep = []
-        for r in [float(x) for x in args.serve_sweep.split(",") if x.strip()]:
-            sweep.append(serve_stream(sgm, r, 300, raw_pairs, seed=int(r), maps=maps))
-            log(f"served {r:.0f} Hz (raw): {sweep[-1]}")
-        ok = [x for x in sweep if x["p99_ms"] < 5.0]
+        for r in rates:
+            for cs in [int(x) for x in args.sweep_modes.split(",") if x.strip()]:
+                sweep.append(serve_stream(sgm, r, args.sweep_frames, raw_pairs, seed=int(r), maps=maps,
+                                          compute_streams=cs))
+                log(f"served {r:.0f} Hz (raw, compute_streams={cs}): {sweep[-1]}")
+        ok = [x for x in sweep if x["p99_ms"] < 5.0 and x["compute_streams"] == 0]
         served["sweep"] = sweep
         served["max_rate_p99_under_5ms_hz"] = max((x["rate_hz"] for x in ok), default=None)
-        full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
-        roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
-        rect = serve_stream(sgm, 60.0, args.serve_frames, (full, roi))
-        log(f"served 60 Hz (rectified inputs, copied at submission): {rect}")
-        served["rectified_inputs"] = rect
+        served["saturated_value"] = max((x["value"] for x in sweep), default=None)
+        if args.rect_serve_frames > 0:
+            full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
+            roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
+            rect = serve_stream(sgm, 60.0, args.rect_serve_frames, (full, roi))
+            log(f"served 60 Hz (rectified inputs, copied at submission): {rect}")
+            served["rectified_inputs"] = rect
         pool.free_all()
 
     line = {
@@ -537,7 +657,12 @@ def main():
         "dtype": "u16",
         "data": "synthetic",
         "config": {"workload": "SGM 1280x1024 D=128, 8 paths, census 9x7, P1=10 P2=120 adaptive, subpixel, median3",
-                   "pairs_per_step": 1, "width": W, "height": H, "disparities": D, "paths": 8},
+                   "pairs_per_step": 1, "width": W, "height": H, "disparities": D, "paths": 8,
+                   "min_disparity": pair.min_disparity, "num_disparities": D, "census_window": "9x7",
+                   "p1": PARAMS["p1"], "p2": PARAMS["p2"], "adaptive_p2": PARAMS["adaptive_p2"],
+                   "eight_paths": PARAMS["eight_paths"], "slant_steps": PARAMS["slant_steps"],
+                   "uniqueness": PARAMS["uniqueness"], "lr_max_diff": PARAMS["lr_max_diff"],
+                   "subpixel": PARAMS["subpixel"], "outputs": OUT_FIELDS},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bit_exact": bit_exact,
