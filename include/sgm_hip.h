@@ -451,6 +451,9 @@ typedef struct {
                                       admission order, while other requests' copies overlap them;
                                       k in 1..slots: k shared compute streams; -1: each slot computes on
                                       its own stream (requests' kernels share the GPU) */
+    int32_t path_group;            /* lanes per path scanline: 0 (default) = chosen per request shape by
+                                      timing 16 and 32 lanes on the first requests of that shape; 16 or 32
+                                      = fixed. Results never depend on it. */
 } sgm_server_config;
 
 typedef struct {

@@ -163,11 +163,12 @@ namespace sgm {
 // Entry points shared with the serving loop (sgm_server.hip).
 int validate_params_ext(const sgm_params* p, std::string& err) { return validate_params(p, err); }
 SgmConsts make_consts_ext(const sgm_params& p) { return make_consts(p); }
-hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5) {
+hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
+                            int path_group) {
     Stages st;
     if (ev5)
         for (int i = 0; i < 5; ++i) st.ev[i] = ev5[i];
-    return enqueue_pair(d, c, padded, s, ev5 ? &st : nullptr, 16, 32);  // 32-lane scanlines where allowed
+    return enqueue_pair(d, c, padded, s, ev5 ? &st : nullptr, 16, path_group);  // 32 lanes: where allowed
 }
 void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) { set_geometry(d, w, h, min_d, nd, ndirs); }
 size_t census_words_ext(int w, int h, int min_d, int nd) { return census_words(w, h, min_d, nd); }

@@ -19,7 +19,8 @@ namespace sgm {
 // Shared with sgm_api.hip.
 int validate_params_ext(const sgm_params* p, std::string& err);
 SgmConsts make_consts_ext(const sgm_params& p);
-hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5);
+hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
+                            int path_group);
 void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs);
 size_t census_words_ext(int w, int h, int min_d, int nd);
 void set_thread_error(const std::string& e);
@@ -139,7 +140,7 @@ int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, 
     }
     // Every scratch byte a kernel reads was written earlier in the same call (census pads are read
     // only where the costs they give are replaced by the invalid cost), so nothing is cleared.
-    e = enqueue_pair_ext(d, make_consts_ext(*params), d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr);
+    e = enqueue_pair_ext(d, make_consts_ext(*params), d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr, 32);
     if (e != hipSuccess) {
         set_thread_error(std::string("sgm_compute_device_noalloc: launch: ") + hipGetErrorString(e));
         return SGM_ERR_IO;

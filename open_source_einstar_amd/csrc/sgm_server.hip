@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -34,7 +35,8 @@ namespace sgm {
 // Shared with sgm_api.hip.
 int validate_params_ext(const sgm_params* p, std::string& err);
 SgmConsts make_consts_ext(const sgm_params& p);
-hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5);
+hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
+                            int path_group);
 void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs);
 size_t census_words_ext(int w, int h, int min_d, int nd);
 void set_thread_error(const std::string& e);
@@ -174,6 +176,33 @@ struct sgm_server {
     std::vector<std::unique_ptr<Compute>> compute;
     bool broken = false;  // a stream could not be drained: nothing new is admitted
 
+    // Lanes per path scanline by request shape (width, height, disparities): fixed by the config, or
+    // chosen by timing the path kernel of both widths on the first requests of the shape (two each,
+    // the faster minimum wins). Either width gives the same results.
+    int fixed_group = 0;
+    struct LaneChoice {
+        float best_ms[2] = {0, 0};  // 16, 32 lanes
+        int samples[2] = {0, 0};
+        int chosen = 0;
+    };
+    std::map<std::tuple<int, int, int>, LaneChoice> lanes;
+    int pick_group(int w, int h, int nd) {  // with mu held
+        if (fixed_group) return fixed_group;
+        LaneChoice& c = lanes[{w, h, nd}];
+        if (c.chosen) return c.chosen;
+        const bool can32 = dpl_for(nd) % 4 == 0;
+        if (!can32) return c.chosen = 16;
+        return c.samples[0] <= c.samples[1] ? 16 : 32;
+    }
+    void record_group(int w, int h, int nd, int group, float paths_ms) {  // with mu held
+        LaneChoice& c = lanes[{w, h, nd}];
+        if (c.chosen || paths_ms <= 0) return;
+        const int i = group == 32 ? 1 : 0;
+        c.best_ms[i] = c.samples[i] ? std::min(c.best_ms[i], paths_ms) : paths_ms;
+        ++c.samples[i];
+        if (c.samples[0] >= 2 && c.samples[1] >= 2) c.chosen = c.best_ms[1] < c.best_ms[0] ? 32 : 16;
+    }
+
     void dispatch(const std::vector<serve::Admission>& adm) {  // with mu held
         const int64_t t = now_ns();
         for (const auto& a : adm) {
@@ -248,6 +277,11 @@ void sgm_server::worker(int si) {
         };
         // Copies on the slot's stream; kernels on the compute stream (the slot's own stream when
         // there are no shared ones), handed over by events.
+        int group = 16;
+        {
+            std::lock_guard lk(mu);
+            group = pick_group(r.w, r.h, r.nd);
+        }
         Compute* cs = compute.empty() ? nullptr : compute[(size_t)si % compute.size()].get();
         hipStream_t ks = cs ? cs->stream : s.stream;
         hipError_t e;
@@ -282,7 +316,7 @@ void sgm_server::worker(int si) {
                 e = hipEventRecord(s.ev[5], ks);
                 if (e == hipSuccess) e = launch_rectify(rd, ks);
             }
-            if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, ks, s.ev);
+            if (e == hipSuccess) e = enqueue_pair_ext(d, consts, d.dp != d.nd, ks, s.ev, group);
             if (e == hipSuccess && cs) e = hipEventRecord(s.ev_done, ks);
         }
         if (e == hipSuccess && cs) e = hipStreamWaitEvent(s.stream, s.ev_done, 0);
@@ -297,8 +331,11 @@ void sgm_server::worker(int si) {
             if (e != hipSuccess && compute_enqueued) ok = hipStreamSynchronize(ks) == hipSuccess && ok;
             return ok;
         });
-        float gpu_ms = 0;
-        if (out.ok) (void)hipEventElapsedTime(&gpu_ms, r.rect.d_maps ? s.ev[5] : s.ev[0], s.ev[4]);
+        float gpu_ms = 0, paths_ms = 0;
+        if (out.ok) {
+            (void)hipEventElapsedTime(&gpu_ms, r.rect.d_maps ? s.ev[5] : s.ev[0], s.ev[4]);
+            (void)hipEventElapsedTime(&paths_ms, s.ev[1], s.ev[2]);
+        }
         sgm_completion cpl{};
         cpl.ticket = r.ticket;
         cpl.user_tag = r.tag;
@@ -315,6 +352,7 @@ void sgm_server::worker(int si) {
                 ++failed;
             }
             ++completed;
+            if (out.ok) record_group(r.w, r.h, r.nd, group, paths_ms);
             if (r.pinned) pinned_free[r.pinned_class].push_back(r.pinned);
             reqs.erase(r.ticket);
             s.has_job = false;
@@ -395,9 +433,10 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     sgm_params pmax = *params;
     pmax.num_disparities = cfg->max_disparities;
     if (validate_params_ext(&pmax, err) != SGM_OK || cfg->max_width < 1 || cfg->max_width > kMaxWidth || cfg->max_height < 1 ||
-        cfg->slots < 1 || cfg->slots > 64 || cfg->compute_streams < -1 || cfg->compute_streams > cfg->slots) {
-        set_thread_error("server config: max sizes, max_disparities (1..512), slots (1..64) or compute_streams "
-                         "(-1..slots) out of range" +
+        cfg->slots < 1 || cfg->slots > 64 || cfg->compute_streams < -1 || cfg->compute_streams > cfg->slots ||
+        (cfg->path_group != 0 && cfg->path_group != 16 && cfg->path_group != 32)) {
+        set_thread_error("server config: max sizes, max_disparities (1..512), slots (1..64), compute_streams "
+                         "(-1..slots) or path_group (0, 16, 32) out of range" +
                          (err.empty() ? std::string() : ": " + err));
         return SGM_ERR_INVALID_ARGUMENT;
     }
@@ -408,6 +447,7 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     sv->max_w = cfg->max_width;
     sv->max_h = cfg->max_height;
     sv->max_nd = cfg->max_disparities;
+    sv->fixed_group = cfg->path_group;
     size_t free_b = 0, total_b = 0;
     hipError_t e = hipMemGetInfo(&free_b, &total_b);
     size_t budget = cfg->hbm_budget_bytes > 0 ? (size_t)cfg->hbm_budget_bytes : std::min(free_b / 2, (size_t)64 << 30);

@@ -74,7 +74,8 @@ class CSgmStageTotals(C.Structure):
 
 class CSgmServerConfig(C.Structure):
     _fields_ = [("max_width", C.c_int32), ("max_height", C.c_int32), ("max_disparities", C.c_int32),
-                ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64), ("compute_streams", C.c_int32)]
+                ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64), ("compute_streams", C.c_int32),
+                ("path_group", C.c_int32)]
 
 
 class CSgmRequest(C.Structure):
@@ -749,10 +750,12 @@ class Server:
     until `release(completion)` returns them to the pool (or the server stops)."""
 
     def __init__(self, params: SgmParams, max_width: int, max_height: int, max_disparities: int, slots: int = 4,
-                 hbm_budget_bytes: int = 0, compute_streams: int = 0):
+                 hbm_budget_bytes: int = 0, compute_streams: int = 0, path_group: int = 0):
         """compute_streams: 0 = one shared compute stream (requests' kernels in admission order, copies
-        overlapping them), k = k shared streams, -1 = each slot computes on its own stream."""
-        cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes, compute_streams)
+        overlapping them), k = k shared streams, -1 = each slot computes on its own stream.
+        path_group: lanes per path scanline, 0 = timed per request shape, or 16 / 32."""
+        cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes, compute_streams,
+                               path_group)
         h = C.c_void_p()
         cp = params.to_c()
         rc = lib().sgm_server_start(C.byref(cp), C.byref(cfg), C.byref(h))

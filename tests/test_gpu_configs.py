@@ -82,8 +82,9 @@ def served_inputs():
     return out
 
 
-@pytest.mark.parametrize("compute_streams", [0, -1], ids=["shared-compute-stream", "per-slot-streams"])
-def test_served_stream_of_full_frames_and_rois(served_inputs, compute_streams):
+@pytest.mark.parametrize("compute_streams,path_group", [(0, 0), (-1, 32), (2, 16)],
+                         ids=["shared-compute-stream-timed-lanes", "per-slot-streams-32-lanes", "two-compute-streams-16-lanes"])
+def test_served_stream_of_full_frames_and_rois(served_inputs, compute_streams, path_group):
     import time
 
     rng = np.random.default_rng(7)
@@ -92,7 +93,8 @@ def test_served_stream_of_full_frames_and_rois(served_inputs, compute_streams):
     rng.shuffle(kinds)
     gaps = rng.exponential(1.0 / 60.0, n)
     pool = sgm.PinnedPool()
-    with sgm.Server(sgm.SgmParams(**PARAMS), 1280, 1024, 128, slots=4, compute_streams=compute_streams) as sv:
+    with sgm.Server(sgm.SgmParams(**PARAMS), 1280, 1024, 128, slots=4, compute_streams=compute_streams,
+                    path_group=path_group) as sv:
         ids = []
         for k, m, items in served_inputs:
             rw, rh = k["raw"]
