@@ -23,12 +23,22 @@ hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipSt
 hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
     if (c.ndirs != 4 && c.ndirs != 8) return hipErrorInvalidValue;
-    const size_t lds = (size_t)max_w * 15 + 16;
+    if (c.wide_costs) return launch_rows_w16(dpl, padded, t, c, max_w, stream);
+    // Rows up to kMaxStagedWidth stage their left results in LDS (15 bytes per column); wider rows keep
+    // only the right WTA keys there (4 bytes per column).
+    const bool wide = max_w > kMaxStagedWidth;
+    const size_t lds = (size_t)max_w * (wide ? 4 : 15) + 16;
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c); };
     switch (dpl) {
-#define SGM_ROWS(D, PD)                                                                                                  \
-    do {                                                                                                                 \
-        if (c.ndirs == 8) hipLaunchKernelGGL((rows_kernel<D, PD, 8>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);   \
-        else hipLaunchKernelGGL((rows_kernel<D, PD, 4>), dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c);                \
+#define SGM_ROWS(D, PD)                                                                          \
+    do {                                                                                         \
+        if (c.ndirs == 8) {                                                                      \
+            if (wide) go(rows_kernel<D, PD, 8, true>);                                           \
+            else go(rows_kernel<D, PD, 8, false>);                                               \
+        } else {                                                                                 \
+            if (wide) go(rows_kernel<D, PD, 4, true>);                                           \
+            else go(rows_kernel<D, PD, 4, false>);                                               \
+        }                                                                                        \
     } while (0)
 #define SGM_CASE(D)                  \
     case D:                          \

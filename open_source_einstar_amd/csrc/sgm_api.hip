@@ -52,15 +52,33 @@ int validate_params(const sgm_params* p, std::string& err) {
         err = "census window too large for 64 bits (at most 9x7)";
         return SGM_ERR_INVALID_ARGUMENT;
     }
-    if (p->p1 < 0 || p->p1 > 32767 || p->p2 < 0) {
-        err = "p1 must be in [0, 32767] and p2 >= 0";
+    if (p->p1 < 0 || p->p1 > (1 << 30) || p->p2 < 0) {
+        err = "p1 must be in [0, 2^30] and p2 >= 0";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    // MetalStereo::create's check and message (metal_stereo.cpp:118-120). Path costs are bytes where
+    // the largest step penalty allows it and u16 otherwise (wide_path_costs).
     const int cells = (2 * p->census_radius_x + 1) * (2 * p->census_radius_y + 1);
-    const int pen = p->adaptive_p2 ? std::max(p->p2, p->p1 + 1) : p->p2;
-    if (cells + pen > 255) {
-        err = "census window + P2 exceed the byte-sized path costs (cells + max(P2, P1+1 if adaptive) <= 255)";
+    if (cells + p->p2 > 255) {
+        err = "census window + P2 exceed the byte-sized SGM costs";
         return SGM_ERR_INVALID_ARGUMENT;
+    }
+    return SGM_OK;
+}
+
+// Rows wider than kMaxStagedWidth need 4 bytes of LDS per column in one workgroup (the row kernel's
+// right WTA keys, the occlusion filter's claims): refused where the device cannot give that much.
+int check_row_lds(int w, std::string& err) {
+    if (w <= kMaxStagedWidth) return SGM_OK;
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+        err = "cannot query the device's LDS size for a row of " + std::to_string(w) + " columns";
+        return SGM_ERR_IO;
+    }
+    if ((int64_t)w * 4 + 16 > lds) {
+        err = "rows of " + std::to_string(w) + " columns need " + std::to_string((int64_t)w * 4 + 16) +
+              " bytes of LDS per workgroup; this device has " + std::to_string(lds);
+        return SGM_ERR_UNSUPPORTED;
     }
     return SGM_OK;
 }
@@ -71,6 +89,7 @@ SgmConsts make_consts(const sgm_params& p) {
     c.p2 = p.p2;
     c.adaptive_p2 = p.adaptive_p2 ? 1 : 0;
     c.slant_steps = p.slant_steps ? 1 : 0;
+    c.wide_costs = wide_path_costs((2 * p.census_radius_x + 1) * (2 * p.census_radius_y + 1), p.p1, p.p2, p.adaptive_p2 != 0) ? 1 : 0;
     c.ndirs = p.eight_paths ? 8 : 4;
     c.census_rx = p.census_radius_x;
     c.census_ry = p.census_radius_y;
@@ -137,7 +156,8 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
     return hipSuccess;
 }
 
-void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) {
+// es: bytes per stored path cost (1, or 2 for u16 path costs).
+void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs, int es) {
     d.w = w;
     d.h = h;
     d.min_d = min_d;
@@ -145,7 +165,7 @@ void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) {
     d.dpl = dpl_for(nd);
     d.dp = d.dpl * kGroup;
     d.ndirs = ndirs;
-    d.lvol_px_stride = (int64_t)ndirs * d.dp;
+    d.lvol_px_stride = (int64_t)ndirs * d.dp * es;
     census_layout(w, min_d, d.dp, d.census_padl, d.census_pitch);
     fill_lines(d);
 }
@@ -162,6 +182,7 @@ size_t census_words(int w, int h, int min_d, int nd) {
 namespace sgm {
 // Entry points shared with the serving loop (sgm_server.hip).
 int validate_params_ext(const sgm_params* p, std::string& err) { return validate_params(p, err); }
+int check_row_lds_ext(int w, std::string& err) { return check_row_lds(w, err); }
 SgmConsts make_consts_ext(const sgm_params& p) { return make_consts(p); }
 hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
                             int path_group) {
@@ -170,7 +191,9 @@ hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, 
         for (int i = 0; i < 5; ++i) st.ev[i] = ev5[i];
     return enqueue_pair(d, c, padded, s, ev5 ? &st : nullptr, 16, path_group);  // 32 lanes: where allowed
 }
-void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs) { set_geometry(d, w, h, min_d, nd, ndirs); }
+void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs, int es) {
+    set_geometry(d, w, h, min_d, nd, ndirs, es);
+}
 size_t census_words_ext(int w, int h, int min_d, int nd) { return census_words(w, h, min_d, nd); }
 void set_thread_error(const std::string& e) { g_last_error = e; }
 }  // namespace sgm
@@ -367,8 +390,9 @@ static int create_context(const sgm_params* params, int sgm_w, int sgm_h, int st
     if (e == hipSuccess) e = dalloc(&ctx->d_img, 2 * std::max(npx, spx), b);
     ctx->census_half = census_words(sgm_w, sgm_h, params->min_disparity, params->num_disparities);
     // Census words of both images, then the left image's step-penalty words (4 bytes per pixel).
-    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half + (npx + 1) / 2, b);
-    if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp, b);
+    const bool wide = ctx->consts.wide_costs != 0;
+    if (e == hipSuccess) e = dalloc(&ctx->d_census, 2 * ctx->census_half + (npx * edge_bytes_per_px(wide) + 7) / 8, b);
+    if (e == hipSuccess) e = dalloc(&ctx->d_lvol, (size_t)ctx->ndirs * npx * dp * (wide ? 2 : 1), b);
     // Cost-volume mode is the default wherever the volume fits one buffer view.
     ctx->use_cvol = npx * dp <= kMaxViewBytes;
     if (e == hipSuccess && ctx->use_cvol) e = dalloc(&ctx->d_cvol, npx * dp, b);
@@ -425,6 +449,10 @@ int sgm_create(const sgm_params* params, int max_width, int max_height, sgm_cont
         g_last_error = "image too large: the census words of one image exceed 4 GiB - 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if (int rc = check_row_lds(max_width, err); rc != SGM_OK) {
+        g_last_error = err;
+        return rc;
+    }
     return create_context(params, max_width, max_height, max_width, max_height, sgm_only(*params), out);
 }
 
@@ -463,6 +491,10 @@ int sgm_create_stereo(const sgm_stereo_params* sp, int width, int height, sgm_co
         g_last_error = "image too large: the census words of one image exceed 4 GiB - 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if (int rc = check_row_lds(width, err); rc != SGM_OK) {  // the occlusion filter runs at the full width
+        g_last_error = err;
+        return rc;
+    }
     return create_context(&sp->sgm, sw, sh, width, height, *sp, out);
 }
 
@@ -489,7 +521,7 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
             if (e == hipSuccess && ctx->last_stream) e = hipStreamSynchronize(ctx->last_stream);
             if (e != hipSuccess) return ctx->hip_fail("sgm_set_option: synchronize", e);
             const size_t npx = (size_t)ctx->max_w * ctx->max_h;
-            const size_t census_bytes = (2 * ctx->census_half + (npx + 1) / 2) * 8;
+            const size_t census_bytes = 2 * ctx->census_half * 8 + npx * edge_bytes_per_px(ctx->consts.wide_costs != 0);
             const size_t cvol_bytes = npx * (size_t)dpl_for(ctx->p.num_disparities) * kGroup;
             void *cb = nullptr, *cp = nullptr, *vb = nullptr, *vp = nullptr;
             e = alloc_at_low_word(census_bytes, 0x80000100u, &cb, &cp);
@@ -661,7 +693,7 @@ static PairDesc context_desc(sgm_context* ctx, const uint8_t* dl, const uint8_t*
     d.edge = reinterpret_cast<uint32_t*>(ctx->d_census + 2 * ctx->census_half);
     d.lvol = ctx->d_lvol;
     d.cvol = ctx->use_cvol ? ctx->d_cvol : nullptr;
-    set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs);
+    set_geometry(d, w, h, ctx->p.min_disparity, ctx->p.num_disparities, ctx->ndirs, ctx->consts.wide_costs ? 2 : 1);
     d.disp = (o && o->disparity) ? o->disparity : ctx->d_disp;
     if (o) {
         d.median = o->disparity_median;
@@ -1827,6 +1859,10 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
             g_last_error = "pair " + std::to_string(i) + ": the census words of one image exceed 4 GiB - 1";
             return SGM_ERR_INVALID_ARGUMENT;
         }
+        if (int rc = check_row_lds(q.width, err); rc != SGM_OK) {
+            g_last_error = "pair " + std::to_string(i) + ": " + err;
+            return rc;
+        }
     }
     // The cost-volume mode needs every pair's volume within one buffer view.
     bool cv_fits = true;
@@ -1849,11 +1885,11 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         const size_t dp = (size_t)dpl_for(pairs[i].num_disparities) * kGroup;
         off_census[i] = total;
         total += (2 * census_words(pairs[i].width, pairs[i].height, pairs[i].min_disparity, pairs[i].num_disparities) * 8 +
-                  npx * 4 + 255) & ~(size_t)255;  // census words of both images, then the step-penalty words
+                  npx * edge_bytes_per_px(b->consts.wide_costs != 0) + 255) & ~(size_t)255;  // census words, then penalties
         off_cvol[i] = total;
         if (cv_fits) total += (npx * dp + 255) & ~(size_t)255;
         off_lvol[i] = total;
-        total += ((size_t)b->ndirs * npx * dp + 255) & ~(size_t)255;
+        total += ((size_t)b->ndirs * npx * dp * (b->consts.wide_costs ? 2 : 1) + 255) & ~(size_t)255;
     }
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = dalloc(&b->d_scratch, total, b->bytes);
@@ -1873,7 +1909,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         d.edge = reinterpret_cast<uint32_t*>(d.census_r + census_words(q.width, q.height, q.min_disparity, q.num_disparities));
         d.cvol = cv_fits ? b->d_scratch + off_cvol[i] : nullptr;
         d.lvol = b->d_scratch + off_lvol[i];
-        set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs);
+        set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs, b->consts.wide_costs ? 2 : 1);
         d.disp = q.out.disparity;
         d.median = q.out.disparity_median;
         d.conf = q.out.confidence;

@@ -14,7 +14,10 @@ constexpr int kBlock = 256;                   // threads per block (4 wave64s)
 constexpr int kLinesPerBlock = kBlock / kGroup;
 constexpr int kMaxBatch = 64;                 // pairs per launch table
 constexpr int kMaxDirs = 8;
-constexpr int kMaxWidth = 4096;               // the row kernel keeps 15 bytes of LDS per column
+// Widest image row: the row and occlusion kernels keep 4 bytes of LDS per column (128 KiB here);
+// rows up to kMaxStagedWidth also stage their left results there (15 bytes per column).
+constexpr int kMaxWidth = 32768;
+constexpr int kMaxStagedWidth = 4096;
 // The path kernel reads the census rows, the step penalties and the cost volume through buffer
 // views addressed by 32-bit byte offsets, so each of them must stay within 4 GiB - 1. Census rows
 // beyond that are refused (the penalties are half their size); a cost volume beyond it is not used
@@ -56,6 +59,7 @@ struct PairDesc {
 // Parameters shared by every pair of a launch (depth::SgmParams minus the disparity range).
 struct SgmConsts {
     int32_t p1, p2, adaptive_p2, slant_steps;
+    int32_t wide_costs;   // 1: path costs and step penalties are u16 (see wide_path_costs), 0: bytes
     int32_t ndirs;  // 4 or 8 path directions
     int32_t cv_from_dir;  // cost-volume launches: directions below this recompute their costs
     int32_t census_rx, census_ry, invalid_cost;
@@ -170,6 +174,15 @@ int median_blocks(int w, int h);
 
 // Disparities per lane for a disparity count, or 0 when unsupported (> 512).
 int dpl_for(int nd);
+// Path costs fit a byte when cells + the largest step penalty (P2, or max(P2, P1 + 1) with adaptive P2)
+// <= 255 and P1 <= 32767 (the byte kernels' 16-bit lane arithmetic adds P1 to a 0x7FFF sentinel);
+// otherwise they are u16, added with saturation where a sum only competes in a minimum.
+inline bool wide_path_costs(int cells, int p1, int p2, bool adaptive) {
+    const int64_t pen = adaptive ? (p2 > (int64_t)p1 + 1 ? p2 : (int64_t)p1 + 1) : p2;
+    return cells + pen > 255 || p1 > 32767;
+}
+// Bytes of step penalties per pixel: four axes, one byte or one u16 each.
+inline size_t edge_bytes_per_px(bool wide) { return wide ? 8 : 4; }
 // Census row layout for width w: x - min_d - k must address a word of the row for every pixel x
 // and lane disparity k < dp, so the path kernel loads disparity windows without clamping.
 inline void census_layout(int w, int min_d, int dp, int32_t& padl, int32_t& pitch) {

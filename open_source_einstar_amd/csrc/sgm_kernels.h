@@ -51,6 +51,10 @@ __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
 }
+// Saturating at 0xFFFF (v_pk_add_u16 with clamp).
+__device__ __forceinline__ uint32_t pk_add_sat(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
 
 // DPP controls (gfx9 encoding).
 constexpr int kQuadSwap1 = 0xB1;      // quad_perm [1,0,3,2]
@@ -387,6 +391,45 @@ __device__ __forceinline__ void store_bytes(SGM_GLOBAL uint8_t* dst, const uint3
     store_packed<DPL, NT>(dst, pack_bytes<DPL>(v));
 }
 
+// A pixel's DPL u16 path costs (W16): the lane's packed pairs as they are.
+template <int NW>
+struct Words {
+    uint32_t w[NW];
+};
+template <int NW, bool NT = true>
+__device__ __forceinline__ void store_words(SGM_GLOBAL uint8_t* dst, const uint32_t (&w)[NW]) {
+    if constexpr (NW % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < NW / 4; ++i)
+            put<NT>(u32x4{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]}, reinterpret_cast<SGM_GLOBAL u32x4*>(dst) + i);
+    } else if constexpr (NW % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < NW / 2; ++i) put<NT>(u32x2{w[2 * i], w[2 * i + 1]}, reinterpret_cast<SGM_GLOBAL u32x2*>(dst) + i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) put<NT>(w[i], reinterpret_cast<SGM_GLOBAL uint32_t*>(dst) + i);
+    }
+}
+// The stored form of a pixel's path costs: bytes (Packed) or u16 (Words).
+template <int DPL, bool W16>
+using PathOut = std::conditional_t<W16, Words<DPL / 2>, Packed<DPL>>;
+template <int DPL, bool W16>
+__device__ __forceinline__ PathOut<DPL, W16> make_out(const uint32_t (&v)[DPL / 2]) {
+    if constexpr (W16) {
+        Words<DPL / 2> o;
+#pragma unroll
+        for (int i = 0; i < DPL / 2; ++i) o.w[i] = v[i];
+        return o;
+    } else {
+        return pack_bytes<DPL>(v);
+    }
+}
+template <int DPL, bool W16>
+__device__ __forceinline__ void store_out(SGM_GLOBAL uint8_t* dst, const PathOut<DPL, W16>& o) {
+    if constexpr (W16) store_words<DPL / 2>(dst, o.w);
+    else store_packed<DPL>(dst, o);
+}
+
 // Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value (or zero) into
 // the line's end lanes, which keep = 0 / sent = sentinel replace. (A select would let the compiler run the
 // shift with the end lanes disabled, and a shift never reads a disabled lane's value.)
@@ -401,15 +444,21 @@ struct LineEdge {
 // With slant steps the P1 neighbourhood of k is {k-2, k-1, k+1, k+2}, which is min(a(k-2), a(k+1))
 // for the pairwise minima a(j) = min(L'(j), L'(j+1)): one pairwise pass, one shifted pass. An
 // excluded neighbour is the sentinel, which never wins a minimum against a real value.
-template <int DPL, bool PAD, int G, bool SLANT>
+//
+// W16 (u16 path costs): the sentinel is 0xFFFF and the P1 candidates are added with saturation; every
+// candidate of the minimum is >= the previous step's minimum, and L'(k) <= 0xFFFF is always one of
+// them, so a saturated candidate wins exactly when the unsaturated one would (stereo.cpp:71-80 forms
+// them in int), and L = C + best - pm then wraps at 16 bits as the reference's u16 cast does.
+template <int DPL, bool PAD, int G, bool SLANT, bool W16 = false>
 __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint32_t (&cc)[DPL / 2], uint32_t pmpm,
                                           uint32_t jj, uint32_t p1p1, const uint32_t (&padm)[DPL / 2], uint32_t& mn,
                                           const LineEdge& edge) {
     constexpr int R = DPL / 2;
+    constexpr uint32_t kSent = W16 ? 0xFFFFFFFFu : kSentinel;
     uint32_t nb_lo, nb_hi;  // (k0-2, k0-1) and (k0+DPL, k0+DPL+1)
     if constexpr (G == 16) {
-        nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSentinel);
-        nb_hi = dpp_or<kRowShl1>(prev[0], kSentinel);
+        nb_lo = dpp_or<kRowShr1>(prev[R - 1], kSent);
+        nb_hi = dpp_or<kRowShl1>(prev[0], kSent);
     } else {
         nb_lo = (dpp_zero<kWaveShr1>(prev[R - 1]) & edge.keep_lo) | edge.sent_lo;
         nb_hi = (dpp_zero<kWaveShl1>(prev[0]) & edge.keep_hi) | edge.sent_hi;
@@ -436,7 +485,8 @@ __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint3
     mn = 0xFFFFFFFFu;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const uint32_t best = pk_min(pk_min(prev[r], pk_add(nbr[r], p1p1)), jj);
+        const uint32_t cand = W16 ? pk_add_sat(nbr[r], p1p1) : pk_add(nbr[r], p1p1);
+        const uint32_t best = pk_min(pk_min(prev[r], cand), jj);
         uint32_t v = pk_add(pk_sub(best, pmpm), cc[r]);
         if (PAD) v = pk_max(v, padm[r]);
         prev[r] = v;
@@ -458,10 +508,14 @@ __device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
     return v;
 }
 
-// Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation.
-template <int DPL, bool PAD, bool CV, int G, bool SLANT>
+// Sixteen scanlines (lb * 16 ...) of the pair's path enumeration; no block synchronisation. W16: u16
+// path costs and step penalties (SgmConsts::wide_costs).
+template <int DPL, bool PAD, bool CV, int G, bool SLANT, bool W16 = false>
 __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c, int lb) {
     constexpr int R = DPL / 2;
+    constexpr uint32_t kSent = W16 ? 0xFFFFFFFFu : kSentinel;
+    constexpr uint32_t kPad = W16 ? 0xFFFFu : 0x7FFFu;
+    constexpr int ES = W16 ? 2 : 1;  // bytes per stored path cost
     const int line = lb * (int)(blockDim.x / G) + (int)(threadIdx.x / G);  // 4, 8 or 16 lines per block
     if (line >= P.total_lines) return;
     int dir = 0, off = 0;
@@ -479,19 +533,19 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     const int g = threadIdx.x & (G - 1);
     LineEdge edge;
     edge.keep_lo = g == 0 ? 0u : 0xFFFFFFFFu;
-    edge.sent_lo = g == 0 ? kSentinel : 0u;
+    edge.sent_lo = g == 0 ? kSent : 0u;
     edge.keep_hi = g == G - 1 ? 0u : 0xFFFFFFFFu;
-    edge.sent_hi = g == G - 1 ? kSentinel : 0u;
+    edge.sent_hi = g == G - 1 ? kSent : 0u;
     const int k0 = g * DPL;
     const int min_d = P.min_d;
     const uint32_t inv = (uint32_t)c.invalid_cost;
-    const uint32_t p1p1 = (uint32_t)c.p1 * 0x10001u;
+    const uint32_t p1p1 = (uint32_t)(W16 ? min(c.p1, 0xFFFF) : c.p1) * 0x10001u;
     // The whole group's window [x - min_d - (dp-1), x - min_d] lies in [0, W) for these x.
     const int full_lo = min_d + P.dp - 1, full_hi = W - 1 + min_d;
     uint32_t padm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        padm[r] = PAD ? ((k0 + 2 * r >= P.nd ? 0x7FFFu : 0u) | (k0 + 2 * r + 1 >= P.nd ? 0x7FFF0000u : 0u)) : 0u;
+        padm[r] = PAD ? ((k0 + 2 * r >= P.nd ? kPad : 0u) | (k0 + 2 * r + 1 >= P.nd ? kPad << 16 : 0u)) : 0u;
 
     const int x = ln.x, y = ln.y, n = ln.n;
     // Step penalties: byte dir/2 of the word of the pixel stepped into (directions 0, 2, 4, 6 run
@@ -499,11 +553,12 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     // both halves.
     const bool along = (dir & 1) == 0;
     const uint32_t axis = (uint32_t)(dir >> 1);
-    const uint32_t pen_sel = 0x0C000C00u | (axis << 16) | axis;
+    // W16: half axis & 1 of the pixel's word axis >> 1, in both halves.
+    const uint32_t pen_sel = W16 ? ((axis & 1) ? 0x03020302u : 0x01000100u) : (0x0C000C00u | (axis << 16) | axis);
     // Path costs are stored [y][x][dir][dp]: the row kernel then reads one pixel's directions as one
     // contiguous run, and each store here is still a whole dp-byte run.
     const int64_t step_l = ((int64_t)dy * W + dx) * P.lvol_px_stride;
-    SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + ((int64_t)y * W + x) * P.lvol_px_stride + (int64_t)dir * P.dp + k0;
+    SGM_GLOBAL uint8_t* lv = gptr(P.lvol) + ((int64_t)y * W + x) * P.lvol_px_stride + ((int64_t)dir * P.dp + k0) * ES;
 
     // The walk, for one source of matching costs: the u8 cost volume (CW) or recomputation from the
     // census words. A cost-volume launch may still recompute the costs of its first cv_from_dir
@@ -525,9 +580,10 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             wk.o_cr = (uint32_t)(((int64_t)y * cp + P.census_padl + (x - min_d - k0 - (DPL - 1))) * 8);
             wk.s_c = (uint32_t)(((int64_t)dy * cp + dx) * 8);
         }
-        wk.ed = buffer_view(P.edge, (uint64_t)W * H * 4);
-        wk.o_e = (uint32_t)((((int64_t)y * W + x) - (along ? 0 : ((int64_t)dy * W + dx))) * 4);
-        wk.s_e = (uint32_t)(((int64_t)dy * W + dx) * 4);
+        constexpr int kEB = W16 ? 8 : 4;  // penalty bytes per pixel
+        wk.ed = buffer_view(P.edge, (uint64_t)W * H * kEB);
+        wk.o_e = (uint32_t)((((int64_t)y * W + x) - (along ? 0 : ((int64_t)dy * W + dx))) * kEB + (W16 && axis >= 2 ? 4 : 0));
+        wk.s_e = (uint32_t)(((int64_t)dy * W + dx) * kEB);
         // Loads and stores. Two banks of K pixels alternate: while one bank's pixels are computed, the
         // other's loads are in flight. A bank is refilled (2K pixels further on) right after its
         // pixels are computed and before their path costs are stored, so on this hardware, where
@@ -536,7 +592,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
         // never used.
         // Seven cost-volume pixels a bank: the deepest prefetch that keeps three waves per SIMD at
         // DPL = 8 (160 VGPRs); this walk is bound by the loads in flight, not by occupancy.
-        constexpr int K = CW ? 7 : 2;
+        constexpr int K = CW ? (W16 ? 5 : 7) : 2;
         StepIn<DPL, CW> first, bank_a[K], bank_b[K];
         load_step<DPL, CW>(first, wk);  // pixel 0
     #pragma unroll
@@ -554,7 +610,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
                 prev[r] = PAD ? pk_max(cc[r], padm[r]) : cc[r];
                 mn = pk_min(mn, prev[r]);
             }
-            store_bytes<DPL>(lv, prev);
+            store_out<DPL, W16>(lv, make_out<DPL, W16>(prev));
             pmpm = pair_min_line<G>(mn);
         }
         // Pixels base .. base + K - 1 from bank bk, then the bank's refill, then the stores. In the main
@@ -566,7 +622,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             // Keeps the scheduler from hoisting the other bank's unpacking (and with it the wait for
             // loads still in flight) into this half.
             __builtin_amdgcn_sched_barrier(0);
-            Packed<DPL> out[K];
+            PathOut<DPL, W16> out[K];
     #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (kFull || base + j < n) {
@@ -575,8 +631,10 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
                     uint32_t cc[R];
                     step_costs<DPL, CW>(bk[j], xc >= full_lo && xc <= full_hi, xc - min_d - k0, W, inv, cc);
                     uint32_t mn;
-                    path_step<DPL, PAD, G, SLANT>(prev, cc, pmpm, pmpm + p2p2, p1p1, padm, mn, edge);
-                    out[j] = pack_bytes<DPL>(prev);
+                    // Byte penalties: pm + p2 < 2^16 in each half; u16: saturated.
+                    const uint32_t jj = W16 ? pk_add_sat(pmpm, p2p2) : pmpm + p2p2;
+                    path_step<DPL, PAD, G, SLANT, W16>(prev, cc, pmpm, jj, p1p1, padm, mn, edge);
+                    out[j] = make_out<DPL, W16>(prev);
                     pmpm = pair_min_line<G>(mn);
                 }
             }
@@ -585,7 +643,7 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
             for (int j = 0; j < K; ++j) load_step<DPL, CW>(bk[j], wk);  // pixels base + 2K ..
     #pragma unroll
             for (int j = 0; j < K; ++j)
-                if (kFull || base + j < n) store_packed<DPL>(lvh + j * step_l, out[j]);
+                if (kFull || base + j < n) store_out<DPL, W16>(lvh + j * step_l, out[j]);
             lvh += K * step_l;
         };
         int i = 1;
@@ -604,12 +662,12 @@ __device__ __forceinline__ void paths_body(const PairDesc& P, const SgmConsts& c
     }
 }
 
-template <int DPL, bool PAD, bool CV, int G, bool SLANT>
+template <int DPL, bool PAD, bool CV, int G, bool SLANT, bool W16 = false>
 __global__ __launch_bounds__(kBlock) void paths_kernel(LaunchTable t, SgmConsts c) {
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    paths_body<DPL, PAD, CV, G, SLANT>(P, c, lb);
+    paths_body<DPL, PAD, CV, G, SLANT, W16>(P, c, lb);
 }
 
 // -------------------------------------------------------------------------------- cost volume
@@ -692,13 +750,33 @@ template <int DPL>
 struct RowIn {
     uint32_t w[kMaxDirs][(DPL + 3) / 4];
 };
+// u16 path costs (W16): the lane's DPL costs of a direction as DPL/2 packed pairs.
+template <int DPL>
+struct RowIn16 {
+    uint32_t w[kMaxDirs][DPL / 2];
+};
+template <int DPL, bool W16>
+using RowInT = std::conditional_t<W16, RowIn16<DPL>, RowIn<DPL>>;
 
 // The direction count is a template parameter so that every load is unconditional: the compiler can
 // then wait for each ring slot alone instead of for every load in flight.
-template <int DPL, int NDIRS>
-__device__ __forceinline__ void load_row_in(RowIn<DPL>& in, const SGM_GLOBAL uint8_t* src, int64_t dir_stride) {
+template <int DPL, int NDIRS, bool W16>
+__device__ __forceinline__ void load_row_in(RowInT<DPL, W16>& in, const SGM_GLOBAL uint8_t* src, int64_t dir_stride) {
 #pragma unroll
-    for (int d = 0; d < NDIRS; ++d) load_raw<DPL>(src + d * dir_stride, in.w[d]);
+    for (int d = 0; d < NDIRS; ++d) load_raw<W16 ? 2 * DPL : DPL>(src + d * dir_stride, in.w[d]);
+}
+
+// S from u16 path costs: packed 16-bit sums, which wrap as the reference's u16 sum volume does
+// (stereo.cpp:82-83).
+template <int DPL, int NDIRS>
+__device__ __forceinline__ void sum_dirs16(const uint32_t (&w)[kMaxDirs][DPL / 2], uint32_t (&s)[DPL / 2]) {
+#pragma unroll
+    for (int r = 0; r < DPL / 2; ++r) {
+        uint32_t acc = w[0][r];
+#pragma unroll
+        for (int d = 1; d < NDIRS; ++d) acc = pk_add(acc, w[d][r]);
+        s[r] = acc;
+    }
 }
 
 __device__ __forceinline__ uint32_t half16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
@@ -724,9 +802,12 @@ __device__ __forceinline__ uint32_t lane_elem(const uint32_t (&s)[DPL / 2], int 
 
 // Image row lb of the pair; every thread of the block calls it (it synchronises twice). smem holds
 // 15 bytes per column: the right WTA keys, and the left results that the last pass writes out
-// coalesced (keys, second-best sums, disparities, status).
-template <int DPL, bool PAD, int NDIRS>
+// coalesced (keys, second-best sums, disparities, status). WIDE rows (beyond kMaxStagedWidth columns)
+// keep only the right WTA keys in LDS (4 bytes per column): each pixel's left results go straight to
+// the outputs when computed, and the left/right check reads the disparities back from there.
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false>
 __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
+    constexpr int ES = W16 ? 2 : 1;  // bytes per stored path cost
     const int W = P.w, y = lb;
     SGM_LDS uint32_t* rkey = smem;
     SGM_LDS uint32_t* lkey = smem + W;
@@ -739,9 +820,9 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     const int grp = threadIdx.x >> 4, g = threadIdx.x & (kGroup - 1);
     const int k0 = g * DPL;
     const int nd = P.nd, min_d = P.min_d;
-    const int64_t dp = P.dp;
+    const int64_t dp = P.dp * ES;  // bytes between a pixel's directions
     const int64_t pxs = P.lvol_px_stride;  // [y][x][dir][dp]: a pixel's directions are contiguous
-    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + k0;
+    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + k0 * ES;
     const int64_t row0 = (int64_t)y * W;
 
     // One pixel's results between its computation and its commit (right-WTA atomics and stores).
@@ -753,8 +834,9 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     };
     // Sums, left WTA with uniqueness and subpixel (stereo.cpp:412-432); no memory side effects, so
     // the computations of several pixels can overlap.
-    auto compute = [&](const RowIn<DPL>& in, int x, PxOut& o) {
-        sum_dirs<DPL, NDIRS>(in.w, o.s);
+    auto compute = [&](const RowInT<DPL, W16>& in, int x, PxOut& o) {
+        if constexpr (W16) sum_dirs16<DPL, NDIRS>(in.w, o.s);
+        else sum_dirs<DPL, NDIRS>(in.w, o.s);
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -841,10 +923,19 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             }
         }
         if (g == 0) {
-            drow[x] = o.d;
-            srow[x] = o.st;
-            lkey[x] = o.key;
-            lsec[x] = (uint16_t)o.sec;
+            if constexpr (WIDE) {
+                const int bk = o.key == 0xFFFFFFFFu ? -1 : (int)(o.key & 0xFFFFu);
+                P.disp[row0 + x] = o.d;  // before the left/right check, which reads it back
+                if (P.status) P.status[row0 + x] = o.st;
+                if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
+                if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? o.key >> 16 : 0xFFFFu);
+                if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? o.sec : 0xFFFFu);
+            } else {
+                drow[x] = o.d;
+                srow[x] = o.st;
+                lkey[x] = o.key;
+                lsec[x] = (uint16_t)o.sec;
+            }
         }
         if (P.sum_volume) {
             uint16_t* sv = P.sum_volume + (row0 + x) * nd;
@@ -858,11 +949,11 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     // previous ring is computed; the ring's pixels are computed together (their work overlaps),
     // the slots refilled, then the pixels committed. Reads past the row's end are clamped to its
     // last pixel and never used.
-    constexpr int kRowAhead = row_ahead(DPL);
-    RowIn<DPL> ring[kRowAhead];
+    constexpr int kRowAhead = row_ahead(W16 ? 2 * DPL : DPL);
+    RowInT<DPL, W16> ring[kRowAhead];
 #pragma unroll
     for (int j = 0; j < kRowAhead; ++j)
-        load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * pxs, dp);
+        load_row_in<DPL, NDIRS, W16>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * pxs, dp);
     for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
         PxOut o[kRowAhead];
 #pragma unroll
@@ -872,7 +963,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         }
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j)
-            load_row_in<DPL, NDIRS>(ring[j], lrow + (int64_t)min(x + (j + kRowAhead) * kLinesPerBlock, W - 1) * pxs, dp);
+            load_row_in<DPL, NDIRS, W16>(ring[j], lrow + (int64_t)min(x + (j + kRowAhead) * kLinesPerBlock, W - 1) * pxs, dp);
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
             const int xj = x + j * kLinesPerBlock;
@@ -883,8 +974,14 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     // Left/right consistency, stereo.cpp:450-461 (lround: halves away from zero).
     for (int x = threadIdx.x; x < W; x += kBlock) {
         const uint32_t rk = rkey[x];
-        float d = drow[x];
-        uint8_t st = srow[x];
+        float d;
+        uint8_t st = 0;
+        if constexpr (WIDE) {
+            d = P.disp[row0 + x];  // written above by this block (the barrier orders it)
+        } else {
+            d = drow[x];
+            st = srow[x];
+        }
         if (d > -1.0e8f) {
             const long xr = lroundf((float)x - d);
             bool ok = xr >= 0 && xr < W;
@@ -896,24 +993,28 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             else d = kInvalid;
         }
         P.disp[row0 + x] = d;
-        if (P.status) P.status[row0 + x] = st;
-        const uint32_t lk = lkey[x];
-        const int bk = lk == 0xFFFFFFFFu ? -1 : (int)(lk & 0xFFFFu);
-        if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
-        if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? lk >> 16 : 0xFFFFu);
-        if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? lsec[x] : 0xFFFFu);
+        if constexpr (WIDE) {
+            if (P.status && (st & 4)) P.status[row0 + x] |= 4;
+        } else {
+            if (P.status) P.status[row0 + x] = st;
+            const uint32_t lk = lkey[x];
+            const int bk = lk == 0xFFFFFFFFu ? -1 : (int)(lk & 0xFFFFu);
+            if (P.dint) P.dint[row0 + x] = bk >= 0 ? min_d + bk : INT_MIN;
+            if (P.best) P.best[row0 + x] = (uint16_t)(bk >= 0 ? lk >> 16 : 0xFFFFu);
+            if (P.second) P.second[row0 + x] = (uint16_t)(bk >= 0 ? lsec[x] : 0xFFFFu);
+        }
         if (P.rdisp) P.rdisp[row0 + x] = rk == 0xFFFFFFFFu ? INT_MIN : min_d + (int)(rk & 0xFFFFu);
         if (P.conf) P.conf[row0 + x] = d > -1.0e8f ? 1.0f : 0.0f;
     }
 }
 
-template <int DPL, bool PAD, int NDIRS>
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false>
 __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
     extern __shared__ uint32_t smem[];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    rows_body<DPL, PAD, NDIRS>(P, c, lb, lds(smem));
+    rows_body<DPL, PAD, NDIRS, WIDE, W16>(P, c, lb, lds(smem));
 }
 
 
@@ -927,13 +1028,13 @@ struct PathLaunch {
 
 namespace {
 
-template <int D, int G>
+template <int D, int G, bool W16 = false>
 hipError_t launch_paths_variant(const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
     const dim3 grid(t.total_blocks), block(pl.threads);
     auto go = [&](auto pad, auto cv) {
         constexpr bool PD = decltype(pad)::value, CV = decltype(cv)::value;
-        if (pl.slant) hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, true>), grid, block, 0, stream, t, c);
-        else hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, false>), grid, block, 0, stream, t, c);
+        if (pl.slant) hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, true, W16>), grid, block, 0, stream, t, c);
+        else hipLaunchKernelGGL((paths_kernel<D, PD, CV, G, false, W16>), grid, block, 0, stream, t, c);
     };
     if (pl.cv) {
         if (pl.padded) go(std::true_type{}, std::true_type{});
@@ -953,5 +1054,9 @@ hipError_t launch_paths_variant(const PathLaunch& pl, const LaunchTable& t, cons
 hipError_t launch_paths_g16_lo(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_paths_g16_hi(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_paths_g32(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+// u16 path costs (SgmConsts::wide_costs), 16 lanes per scanline (paths_w16.hip), and their row kernels
+// (rows_w16.hip).
+hipError_t launch_paths_w16(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_rows_w16(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 
 }  // namespace sgm

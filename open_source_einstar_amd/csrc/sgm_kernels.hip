@@ -34,7 +34,8 @@ namespace {
 // path axes r = (1, 0), (0, 1), (1, 1), (1, -1) as four bytes: byte k is the penalty of the step
 // into this pixel from p - r_k, max(P1 + 1, P2 / (|dI|/8 + 1)) when adaptive, P2 otherwise (0
 // where p - r_k leaves the image; no path steps from there). A step along -r_k uses byte k of the
-// pixel it leaves. Every penalty fits a byte (cells + penalty <= 255 is enforced).
+// pixel it leaves. With byte path costs every penalty fits a byte (cells + penalty <= 255); with u16
+// path costs (SgmConsts::wide_costs) the four penalties are u16 halves of two words.
 
 constexpr int kTile = 16;  // median tiles
 constexpr int kMaxRx = 4, kMaxRy = 3;
@@ -60,7 +61,7 @@ __device__ __forceinline__ void census_gather(uint32_t t0, uint32_t t1, uint32_t
 // the block calls it (it synchronises once).
 template <int RX, int RY>  // the census radii when known at compile time, else -1
 __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint16_t* tile,
-                                            SGM_LDS int32_t* p2q) {
+                                            SGM_LDS uint32_t* p2q) {
     const int tx_n = (P.w + kCenW - 1) / kCenW, ty_n = (P.h + kCenH - 1) / kCenH;
     const int side = lb / (tx_n * ty_n);
     const int tl = lb - side * tx_n * ty_n;
@@ -83,7 +84,7 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
         const int i = (int)threadIdx.x + j * kBlock;
         if (i < kTileN) tile[i] = (uint16_t)px[j];
     }
-    if (threadIdx.x < 32) p2q[threadIdx.x] = c.adaptive_p2 ? c.p2_by_q[threadIdx.x] : c.p2;
+    if (threadIdx.x < 32) p2q[threadIdx.x] = (uint32_t)(c.adaptive_p2 ? c.p2_by_q[threadIdx.x] : c.p2);
     __syncthreads();
     const int lx = (int)(threadIdx.x % (kCenW / kCenPx)) * kCenPx, ly = (int)(threadIdx.x / (kCenW / kCenPx));
     const int x = x0 + lx, y = y0 + ly;
@@ -165,9 +166,17 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
             const SGM_LDS uint16_t* q = ctr + p;
             const int I = q[0];
             auto pen = [&](int v, bool inside) { return inside ? (uint32_t)p2q[abs(I - v) >> 3] : 0u; };
-            const uint32_t e = pen(q[-1], xp > 0) | (pen(q[-kTileW], y > 0) << 8) | (pen(q[-kTileW - 1], xp > 0 && y > 0) << 16) |
-                               (pen(q[kTileW - 1], xp > 0 && y + 1 < P.h) << 24);
-            P.edge[(int64_t)y * P.w + xp] = e;
+            const uint32_t e0 = pen(q[-1], xp > 0), e1 = pen(q[-kTileW], y > 0), e2 = pen(q[-kTileW - 1], xp > 0 && y > 0),
+                           e3 = pen(q[kTileW - 1], xp > 0 && y + 1 < P.h);
+            if (c.wide_costs) {
+                // u16 penalties (saturated: a step penalty only competes in a minimum against values
+                // below 65536), axes 0-1 then 2-3.
+                auto s16 = [](uint32_t v) { return v > 0xFFFFu ? 0xFFFFu : v; };
+                reinterpret_cast<u32x2*>(P.edge)[(int64_t)y * P.w + xp] =
+                    u32x2{s16(e0) | (s16(e1) << 16), s16(e2) | (s16(e3) << 16)};
+            } else {
+                P.edge[(int64_t)y * P.w + xp] = e0 | (e1 << 8) | (e2 << 16) | (e3 << 24);
+            }
         }
     }
 }
@@ -175,7 +184,7 @@ __device__ __forceinline__ void census_body(const PairDesc& P, const SgmConsts& 
 template <int RX, int RY>
 __global__ __launch_bounds__(kBlock) void census_kernel(LaunchTable t, SgmConsts c) {
     __shared__ __attribute__((aligned(16))) uint16_t tile[kTileH * kTileW];
-    __shared__ int32_t p2q[32];
+    __shared__ uint32_t p2q[32];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
@@ -364,9 +373,11 @@ bool paths_group_ok(int dpl, int group, int lines_per_block) {
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, int group,
                         hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;
+    if (c.wide_costs) group = 16;  // u16 path costs: 16 lanes per scanline only
     if (!paths_group_ok(dpl, group, lines_per_block)) return hipErrorInvalidValue;
     const PathLaunch pl{padded, (t.n == 1 ? t.one.cvol : nullptr) != nullptr || t.cost_volume != 0, c.slant_steps != 0,
                         (unsigned)(lines_per_block * group)};
+    if (c.wide_costs) return launch_paths_w16(dpl, pl, t, c, stream);
     if (group == 16) return dpl <= 8 ? launch_paths_g16_lo(dpl, pl, t, c, stream) : launch_paths_g16_hi(dpl, pl, t, c, stream);
     return launch_paths_g32(dpl, pl, t, c, stream);
 }

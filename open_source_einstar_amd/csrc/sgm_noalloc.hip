@@ -18,10 +18,11 @@ using namespace sgm;
 namespace sgm {
 // Shared with sgm_api.hip.
 int validate_params_ext(const sgm_params* p, std::string& err);
+int check_row_lds_ext(int w, std::string& err);
 SgmConsts make_consts_ext(const sgm_params& p);
 hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
                             int path_group);
-void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs);
+void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs, int es);
 size_t census_words_ext(int w, int h, int min_d, int nd);
 void set_thread_error(const std::string& e);
 }  // namespace sgm
@@ -44,15 +45,15 @@ struct Layout {
     bool use_cvol;
 };
 
-Layout layout_for(int w, int h, int min_d, int nd, int ndirs, bool need_disp) {
+Layout layout_for(int w, int h, int min_d, int nd, int ndirs, bool need_disp, bool wide) {
     const size_t npx = (size_t)w * h;
     const size_t dp = (size_t)dpl_for(nd) * kGroup;
     Layout L{};
     size_t o = 0;
     L.census = o;
-    o = up256(o + 2 * census_words_ext(w, h, min_d, nd) * 8 + ((npx + 1) / 2) * 8);  // both images, then penalties
+    o = up256(o + 2 * census_words_ext(w, h, min_d, nd) * 8 + npx * edge_bytes_per_px(wide));  // both images, then penalties
     L.lvol = o;
-    o = up256(o + (size_t)ndirs * npx * dp);
+    o = up256(o + (size_t)ndirs * npx * dp * (wide ? 2 : 1));
     L.disp = o;
     if (need_disp) o = up256(o + npx * 4);
     // The byte cost volume (read once per direction instead of recomputing the costs) where it still fits.
@@ -84,9 +85,15 @@ int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, 
         set_thread_error("sgm_compute_device_noalloc: census outputs are only available from sgm_compute");
         return SGM_ERR_UNSUPPORTED;
     }
+    if (int rc = check_row_lds_ext(width, err); rc != SGM_OK) {
+        set_thread_error("sgm_compute_device_noalloc: " + err);
+        return rc;
+    }
     const int ndirs = params->eight_paths ? 8 : 4;
     const bool need_disp = !(d_out && d_out->disparity);
-    const Layout L = layout_for(width, height, params->min_disparity, params->num_disparities, ndirs, need_disp);
+    const SgmConsts consts = make_consts_ext(*params);
+    const Layout L = layout_for(width, height, params->min_disparity, params->num_disparities, ndirs, need_disp,
+                                consts.wide_costs != 0);
     if (L.total > kScratchBytes) {
         set_thread_error("sgm_compute_device_noalloc: " + std::to_string(width) + "x" + std::to_string(height) + " with " +
                          std::to_string(params->num_disparities) + " disparities and " + std::to_string(ndirs) +
@@ -119,7 +126,7 @@ int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, 
     d.left = d_left;
     d.right = d_right;
     d.pitch = pitch;
-    set_geometry_ext(d, width, height, params->min_disparity, params->num_disparities, ndirs);
+    set_geometry_ext(d, width, height, params->min_disparity, params->num_disparities, ndirs, consts.wide_costs ? 2 : 1);
     const size_t cw = census_words_ext(width, height, params->min_disparity, params->num_disparities);
     d.census_l = reinterpret_cast<uint64_t*>(base + L.census);
     d.census_r = d.census_l + cw;
@@ -140,7 +147,7 @@ int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, 
     }
     // Every scratch byte a kernel reads was written earlier in the same call (census pads are read
     // only where the costs they give are replaced by the invalid cost), so nothing is cleared.
-    e = enqueue_pair_ext(d, make_consts_ext(*params), d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr, 32);
+    e = enqueue_pair_ext(d, consts, d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr, 32);
     if (e != hipSuccess) {
         set_thread_error(std::string("sgm_compute_device_noalloc: launch: ") + hipGetErrorString(e));
         return SGM_ERR_IO;

@@ -34,10 +34,11 @@ using namespace sgm;
 namespace sgm {
 // Shared with sgm_api.hip.
 int validate_params_ext(const sgm_params* p, std::string& err);
+int check_row_lds_ext(int w, std::string& err);
 SgmConsts make_consts_ext(const sgm_params& p);
 hipError_t enqueue_pair_ext(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, hipEvent_t* ev5,
                             int path_group);
-void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs);
+void set_geometry_ext(PairDesc& d, int w, int h, int min_d, int nd, int ndirs, int es);
 size_t census_words_ext(int w, int h, int min_d, int nd);
 void set_thread_error(const std::string& e);
 }  // namespace sgm
@@ -57,7 +58,8 @@ struct Footprint {
     size_t raw, img, census, cvol, lvol, disp, median, conf, dint, status, rdisp, best, second, total;
 };
 
-Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px) {
+// wide: u16 path costs and step penalties (SgmConsts::wide_costs).
+Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px, bool wide) {
     const size_t npx = (size_t)w * h;
     const size_t dp = (size_t)dpl_for(nd) * kGroup;
     Footprint f{};
@@ -67,11 +69,11 @@ Footprint footprint(int w, int h, int min_d, int nd, int ndirs, size_t raw_px) {
     f.img = o;
     o = align_up(o + 2 * npx);
     f.census = o;
-    o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8 + npx * 4);  // census words, then step-penalty words
+    o = align_up(o + 2 * census_words_ext(w, h, min_d, nd) * 8 + npx * edge_bytes_per_px(wide));  // census words, then penalties
     f.cvol = o;  // matching costs as bytes: the path kernel's cost-volume mode, where the volume fits a view
     if (npx * dp <= kMaxViewBytes) o = align_up(o + npx * dp);
     f.lvol = o;
-    o = align_up(o + (size_t)ndirs * npx * dp);
+    o = align_up(o + (size_t)ndirs * npx * dp * (wide ? 2 : 1));
     f.disp = o;
     o = align_up(o + npx * 4);
     f.median = o;
@@ -230,7 +232,7 @@ void sgm_server::worker(int si) {
             r = std::move(reqs.at(s.ticket));  // the worker owns the request until it completes
         }
         const size_t raw_px = r.rect.d_maps ? (size_t)r.rect.raw_w * r.rect.raw_h : 0;
-        const Footprint f = footprint(r.w, r.h, r.min_d, r.nd, ndirs, raw_px);
+        const Footprint f = footprint(r.w, r.h, r.min_d, r.nd, ndirs, raw_px, consts.wide_costs != 0);
         uint8_t* base = d_arena + r.offset;
         const size_t npx = (size_t)r.w * r.h;
         PairDesc d{};
@@ -242,7 +244,7 @@ void sgm_server::worker(int si) {
         d.edge = reinterpret_cast<uint32_t*>(d.census_r + census_words_ext(r.w, r.h, r.min_d, r.nd));
         d.lvol = base + f.lvol;
         d.cvol = npx * (size_t)dpl_for(r.nd) * kGroup <= kMaxViewBytes ? base + f.cvol : nullptr;
-        set_geometry_ext(d, r.w, r.h, r.min_d, r.nd, ndirs);
+        set_geometry_ext(d, r.w, r.h, r.min_d, r.nd, ndirs, consts.wide_costs ? 2 : 1);
         d.disp = reinterpret_cast<float*>(base + f.disp);
         d.median = r.out.disparity_median ? reinterpret_cast<float*>(base + f.median) : nullptr;
         d.conf = r.out.confidence ? reinterpret_cast<float*>(base + f.conf) : nullptr;
@@ -440,6 +442,10 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
                          (err.empty() ? std::string() : ": " + err));
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if (int rc = check_row_lds_ext(cfg->max_width, err); rc != SGM_OK) {
+        set_thread_error("server config: " + err);
+        return rc;
+    }
     auto* sv = new sgm_server;
     sv->p = *params;
     sv->consts = make_consts_ext(*params);
@@ -518,7 +524,7 @@ int sgm_server_submit(sgm_server* sv, const sgm_request* rq, uint64_t* ticket) {
     r.nd = rq->num_disparities;
     r.out = rq->out;
     const size_t in_px = (size_t)in_w * in_h;
-    r.bytes = footprint(r.w, r.h, r.min_d, r.nd, sv->ndirs, ms.d_maps ? in_px : 0).total;
+    r.bytes = footprint(r.w, r.h, r.min_d, r.nd, sv->ndirs, ms.d_maps ? in_px : 0, sv->consts.wide_costs != 0).total;
     r.t_submit = now_ns();
     if (rq->borrow_inputs) {
         r.borrowed[0] = rq->left;

@@ -45,7 +45,7 @@ def test_defaults_match_reference_sgm_params():
 @pytest.mark.parametrize("bad", [
     dict(num_disparities=0), dict(num_disparities=513), dict(census_radius_x=5), dict(census_radius_y=4),
     dict(p1=-1), dict(p2=-1), dict(census_radius_x=4, census_radius_y=3, p2=200),
-    dict(census_radius_x=4, census_radius_y=3, p1=200, p2=10),
+    dict(census_radius_x=4, census_radius_y=3, p1=(1 << 30) + 1, p2=10),
 ])
 def test_invalid_parameters_are_rejected_before_any_gpu_work(bad):
     with pytest.raises(sgm.SgmError) as e:
@@ -53,8 +53,16 @@ def test_invalid_parameters_are_rejected_before_any_gpu_work(bad):
     assert e.value.errc == "invalid_argument"
 
 
+def test_penalty_limit_is_metal_stereos():
+    """cells + P2 <= 255 is MetalStereo::create's limit and message (metal_stereo.cpp:118-120); P1 has
+    no such limit there (large P1 with adaptive P2 runs with u16 path costs here)."""
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.HipStereo.create(sgm.SgmParams(census_radius_x=4, census_radius_y=3, p2=193), 64, 48)
+    assert e.value.errc == "invalid_argument" and "census window + P2 exceed the byte-sized SGM costs" in str(e.value)
+
+
 def test_invalid_sizes_are_rejected():
-    for (w, h) in [(0, 10), (10, 0), (5000, 10)]:
+    for (w, h) in [(0, 10), (10, 0), (32769, 10)]:
         with pytest.raises(sgm.SgmError) as e:
             sgm.HipStereo.create(sgm.SgmParams(), w, h)
         assert e.value.errc == "invalid_argument"

@@ -188,3 +188,20 @@ def test_random_batches_bit_exact(params, pairs, tuning, seed0):
                 assert_same(refs[i], b.results(i), FIELDS, tag=f"{mode} pair {i} {pairs[i]}")
     finally:
         b.close()
+
+
+def test_batch_with_wide_and_narrow_pairs():
+    """A lane class holding a pair wider than 4096 columns runs the wide row kernel for all its pairs;
+    every pair equals the oracle, graph and plain."""
+    sizes = [(5000, 20, 64), (300, 80, 64), (4100, 12, 128), (200, 50, 128)]
+    specs, refs = make_specs(sizes, 6000)
+    b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=dict(cost_volume=1, path_lines=8, branches=1, path_group=16))
+    try:
+        for mode in ("graph", "plain"):
+            b.clear_outputs(0xEE)
+            b.run() if mode == "graph" else b.run_plain()
+            b.synchronize()
+            for i in range(len(sizes)):
+                assert_same(refs[i], b.results(i), FIELDS, tag=f"wide batch {mode} pair {i} {sizes[i]}")
+    finally:
+        b.close()

@@ -218,3 +218,10 @@ def test_random_compute_disparity_sweep(w, h, nd, seed, stereo, full, eight):
     with sgm.HipStereo.create_stereo(sgm.SgmParams(**kw), sgm.StereoParams(**stereo), w, h) as ctx:
         got = ctx.compute_disparity(pair.left, pair.right, fl, fr)
     assert_maps(ref, got, f"chain sweep {w}x{h}")
+
+
+def test_wide_frame_compute_disparity():
+    """compute_disparity on a frame wider than the row kernel's LDS-staged rows: 9000 columns with one
+    pyramid level (SGM at 4500 columns, wide rows), refinement, detail refinement, the occlusion filter
+    (one LDS claim per column of the 9000-wide row), spikes and speckles, against the oracle."""
+    run(9000, 40, 96, 31, dict(pyramid_levels=1), 64, full=True)

@@ -107,24 +107,50 @@ def test_wide_lanes_second_best_full_window():
              uniqueness=0.95)
 
 
-def test_maximum_width_4096_with_512_disparities():
-    """The widest row the library accepts (4096 columns: the row kernel keeps 15 bytes of LDS per
-    column) with the largest disparity count, 8 paths and the 9x7 census, with volumes, then in both
-    cost modes; one column more is refused at create time."""
-    w, h, nd = 4096, 24, 512
-    pair = speckle_pair(w, h, nd, 93)
+def test_staged_row_width_4096_and_first_wide_row_4097():
+    """The widest row whose left results the row kernel stages in LDS (4096 columns, 15 bytes each) and
+    one column more, where the kernel keeps only the right WTA keys in LDS and writes the left results
+    straight to the outputs; the largest disparity count, 8 paths and the 9x7 census, with volumes,
+    then in both cost modes."""
+    for w in (4096, 4097):
+        h, nd = 24, 512
+        pair = speckle_pair(w, h, nd, 93)
+        params = dict(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=True, census_radius_x=4,
+                      census_radius_y=3, p1=10, p2=120)
+        print(f"width {w}x{h} D={nd} {params}", flush=True)
+        ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+        with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+            got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+            assert_same(ref, got, FIELDS + VOLUMES, tag=f"{w}x{h}x{nd}")
+            for cv in (1, 0):
+                st.set_option(sgm.OPT_COST_VOLUME, cv)
+                got = st.compute(pair.left, pair.right, outputs=FIELDS)
+                assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} cost volume {cv}")
+        assert (ref["disparity"] > -1e8).mean() > 0.5
+
+
+def test_maximum_width_32768():
+    """The widest row the library accepts (32768 columns: 4 bytes of LDS per column in the row and
+    occlusion kernels), both cost modes and both path-lane widths, a narrower image on the same
+    context, and one column more refused at create time."""
+    w, h, nd = 32768, 8, 64
+    pair = speckle_pair(w, h, nd, 94)
     params = dict(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=True, census_radius_x=4,
                   census_radius_y=3, p1=10, p2=120)
-    print(f"maximum width {w}x{h} D={nd} {params}", flush=True)
-    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    print(f"maximum width {w}x{h} D={nd}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params))
+    small = speckle_pair(5000, 6, nd, 95)
+    ref_small = O.sgm(small.left, small.right, O.SgmParams(**dict(params, min_disparity=small.min_disparity)))
     with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
-        got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
-        assert_same(ref, got, FIELDS + VOLUMES, tag=f"{w}x{h}x{nd}")
         for cv in (1, 0):
-            st.set_option(sgm.OPT_COST_VOLUME, cv)
-            got = st.compute(pair.left, pair.right, outputs=FIELDS)
-            assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} cost volume {cv}")
-    assert (ref["disparity"] > -1e8).mean() > 0.5
+            for group in (16, 32):
+                st.set_option(sgm.OPT_COST_VOLUME, cv)
+                st.set_option(sgm.OPT_PATH_GROUP, group)
+                got = st.compute(pair.left, pair.right, outputs=FIELDS)
+                assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} cost volume {cv} lanes {group}")
+    with sgm.HipStereo.create(sgm.SgmParams(**dict(params, min_disparity=small.min_disparity)), w, h) as st:
+        got = st.compute(small.left, small.right, outputs=FIELDS)
+        assert_same(ref_small, got, FIELDS, tag="5000x6 on a 32768-wide context")
     with pytest.raises(sgm.SgmError) as e:
         sgm.HipStereo.create(sgm.SgmParams(**params), w + 1, h)
     assert "max_width" in str(e.value)
@@ -289,3 +315,92 @@ def test_view_bases_with_bit31_set(cv, group):
         for _ in range(2):
             got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
             assert_same(ref, got, FIELDS + VOLUMES, tag=f"high view bases cv={cv} lanes={group}")
+
+
+WIDE_COSTS = [
+    # (w, h, nd, seed, params): each needs u16 path costs (cells + max step penalty > 255, or P1 > 32767)
+    (300, 60, 64, 41, dict(eight_paths=True, census_radius_x=4, census_radius_y=3, p1=200, p2=100)),
+    (96, 64, 32, 42, dict(adaptive_p2=False, p1=40000, p2=30)),
+    (3000, 12, 64, 43, dict(eight_paths=True, census_radius_x=4, census_radius_y=3, p1=100000, p2=120)),
+    (200, 100, 100, 44, dict(eight_paths=True, p1=500, p2=60, min_disparity=-7)),
+    (150, 40, 48, 45, dict(census_radius_x=2, census_radius_y=2, p1=1 << 30, p2=230, slant_steps=False)),
+    (5000, 10, 64, 46, dict(eight_paths=True, census_radius_x=4, census_radius_y=3, p1=70000, p2=120)),
+]
+
+
+@pytest.mark.parametrize("w,h,nd,seed,kw", WIDE_COSTS, ids=[f"{c[0]}x{c[1]}x{c[2]}-p1_{c[4]['p1']}" for c in WIDE_COSTS])
+def test_u16_path_costs_bit_exact(w, h, nd, seed, kw):
+    """Penalties beyond what byte path costs hold (MetalStereo accepts any P1 with cells + P2 <= 255):
+    u16 path costs and step penalties, saturating candidate sums, 16-bit wrap of path costs and sums as
+    the reference's u16 casts, with volumes, in both cost modes; the long rows with P1 >= 70000 carry
+    path costs past 2^16."""
+    ref, _ = run_case(w, h, nd, seed, volumes=w * h * nd <= 2_000_000, **kw)
+    pair = speckle_pair(w, h, max(nd, 6), seed)
+    params = dict(min_disparity=kw.get("min_disparity", pair.min_disparity), num_disparities=nd, **{k: v for k, v in kw.items() if k != "min_disparity"})
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        for cv, group in ((0, 16), (1, 32)):
+            st.set_option(sgm.OPT_COST_VOLUME, cv)
+            st.set_option(sgm.OPT_PATH_GROUP, group)
+            got = st.compute(pair.left, pair.right, outputs=FIELDS)
+            assert_same(ref, got, FIELDS, tag=f"u16 path costs {w}x{h}x{nd} cv={cv}")
+
+
+def test_u16_path_costs_in_a_batch_and_served():
+    params = dict(eight_paths=True, census_radius_x=4, census_radius_y=3, p1=300, p2=120)
+    sizes = [(200, 80, 64), (150, 60, 96), (64, 48, 128)]
+    pairs = [speckle_pair(w, h, d, 7000 + i) for i, (w, h, d) in enumerate(sizes)]
+    refs = [O.sgm(p.left, p.right, O.SgmParams(min_disparity=p.min_disparity, num_disparities=d, **params))
+            for p, (_, _, d) in zip(pairs, sizes)]
+    specs = [sgm.PairSpec(p.left, p.right, p.min_disparity, d, outputs=tuple(FIELDS)) for p, (_, _, d) in zip(pairs, sizes)]
+    b = sgm.Batch(sgm.SgmParams(**params), specs, tuning=dict(cost_volume=1, path_lines=8, branches=1, path_group=32))
+    try:
+        for mode in ("graph", "plain"):
+            b.clear_outputs(0xEE)
+            b.run() if mode == "graph" else b.run_plain()
+            b.synchronize()
+            for i in range(len(sizes)):
+                assert_same(refs[i], b.results(i), FIELDS, tag=f"u16 batch {mode} pair {i}")
+    finally:
+        b.close()
+    with sgm.Server(sgm.SgmParams(**params), 200, 80, 128, slots=2) as sv:
+        for i, (p, (_, _, d)) in enumerate(zip(pairs, sizes)):
+            sv.submit(p.left, p.right, p.min_disparity, d, outputs=FIELDS, tag=i)
+        got = {}
+        import time
+        t0 = time.time()
+        while len(got) < len(sizes) and time.time() - t0 < 60:
+            for c in sv.poll(8, timeout_us=100000):
+                got[c["tag"]] = {f: a.copy() for f, a in c["outputs"].items()}
+                sv.release(c)
+    for i in range(len(sizes)):
+        assert_same(refs[i], got[i], FIELDS, tag=f"u16 served request {i}")
+
+
+def test_u16_path_costs_wrap_on_long_rows():
+    """Random-noise rows shifted by a constant disparity: every wrong disparity costs about 31 at each
+    pixel, and with P1 = 100000 (adaptive P2: step penalty 100001) its horizontal path cost keeps its
+    own previous value plus that cost while the line minimum stays near 0, so it passes 2^16 after
+    about 2,100 pixels of a 5000-pixel row and wraps; the sums wrap with it. Bit-exact in both cost
+    modes (the guard below confirms, from the oracle's own cost volume, that the unwrapped
+    left-to-right path cost exceeds 2^16)."""
+    w, h, nd, shift = 5000, 8, 64, 20
+    rng = np.random.default_rng(4711)
+    right = rng.integers(0, 256, (h, w + shift), dtype=np.uint8)
+    left = np.ascontiguousarray(right[:, :w])
+    right = np.ascontiguousarray(right[:, shift:shift + w])
+    kw = dict(min_disparity=0, num_disparities=nd, eight_paths=True, census_radius_x=4, census_radius_y=3,
+              p1=100000, p2=120)
+    ref = O.sgm(left, right, O.SgmParams(**kw), volumes=True)
+    C = ref["cost_volume"][h // 2].astype(np.int64)  # an interior row (census rows 3..h-4)
+    L, mx = C[0].copy(), 0
+    for x in range(1, w):  # rows of constant-free noise: p2 = max(P1 + 1, ...) = P1 + 1
+        pm = L.min()
+        best = np.minimum(L, pm + kw["p1"] + 1)
+        L = C[x] + best - pm
+        mx = max(mx, int(L.max()))
+    assert mx > 65535, mx
+    with sgm.HipStereo.create(sgm.SgmParams(**kw), w, h) as st:
+        for cv in (1, 0):
+            st.set_option(sgm.OPT_COST_VOLUME, cv)
+            got = st.compute(left, right, outputs=FIELDS + ["sum_volume"])
+            assert_same(ref, got, FIELDS + ["sum_volume"], tag=f"wrapping u16 path costs cv={cv}")

@@ -142,3 +142,25 @@ def test_bounds_and_thread_independence():
     np.testing.assert_array_equal(S1, S8)
     # Per direction 0 <= L - C <= p2max, so C*P <= S <= (C + p2max)*P.
     assert (S1 >= cost * 8).all() and (S1 <= (cost.astype(int) + 120) * 8).all()
+
+
+def test_path_costs_wrap_at_16_bits_with_large_penalties():
+    """stereo.cpp:79-83 casts each path cost and each sum to u16: with a P1 above every path cost and
+    adaptive P2 (step penalty P1 + 1 on a flat guide), a disparity whose cost is 63 at every pixel of a
+    row keeps its own previous cost (best = L'(k)) while the line minimum stays 0, so along the row
+    L_n(k) = 63 (n + 1) mod 2^16, which passes 2^16 after 1040 steps; the four-path sum wraps the same way."""
+    w, nd = 1200, 4
+    cost = np.zeros((1, w, nd), np.uint16)
+    cost[:, :, 1] = 63
+    guide = np.full((1, w), 100, np.uint8)
+    p = O.SgmParams(num_disparities=nd, census_radius_x=4, census_radius_y=3, p1=100000, p2=120, adaptive_p2=True,
+                    eight_paths=False, slant_steps=True)
+    n = np.arange(w, dtype=np.int64)
+    lr = O.aggregate_direction(cost, guide, p, 1, 0)
+    assert np.array_equal(lr[0, :, 1].astype(np.int64), (63 * (n + 1)) % 65536)
+    assert (lr[0, :, [0, 2, 3]] == 0).all()
+    rl = O.aggregate_direction(cost, guide, p, -1, 0)
+    assert np.array_equal(rl[0, :, 1].astype(np.int64), (63 * (w - n)) % 65536)
+    s = O.aggregate(cost, guide, p)
+    assert np.array_equal(s[0, :, 1].astype(np.int64), (63 * (n + 1) + 63 * (w - n) + 2 * 63) % 65536)
+    assert int(lr[0, 1040, 1]) == 47  # 63 * 1041 - 65536

@@ -1,9 +1,9 @@
 """The ZNCC kernels test only the first and last sample of a window row against the right image's
 bounds. That equals testing every sample because, within a row, the float sample position strictly
 increases with the column: each step adds 1 - gx >= 0.1 (|gx| <= 0.9) and the roundings of the
-position's terms are a few ulps of values below 2^15. This evaluates the kernels' float32 expressions,
+position's terms are a few ulps of values below 2^17. This evaluates the kernels' float32 expressions,
 in their operation order, over random pixels, disparities, slopes and rows in the supported ranges
-(images up to 2 x 4096 columns) and checks the order directly.
+(images up to 32768 columns, full-resolution pairs up to 2 x 32768) and checks the order directly.
 """
 import numpy as np
 
@@ -35,8 +35,8 @@ def test_refine_rows_increase_strictly():
     rng = np.random.default_rng(7)
     for r in (1, 2, 3, 4, 5):
         m = 200_000
-        x = rng.integers(0, 8192, m)
-        d = rng.uniform(-1100.0, 9000.0, m).astype(F)
+        x = rng.integers(0, 32768, m)
+        d = rng.uniform(-1100.0, 33000.0, m).astype(F)
         gx = rng.uniform(-0.9, 0.9, m).astype(F)
         gx[:1000] = F(0.9)  # the steepest slopes
         gy = rng.uniform(-0.9, 0.9, m).astype(F)
@@ -49,11 +49,11 @@ def test_detail_rows_increase_strictly():
     rng = np.random.default_rng(8)
     for n in (2, 4, 6, 8, 10, 12, 32):
         m = 200_000
-        x = rng.integers(0, 4096, m)
-        D = rng.uniform(-2200.0, 18000.0, m).astype(F)
+        x = rng.integers(0, 32768, m)
+        D = rng.uniform(-2200.0, 66000.0, m).astype(F)
         gx = rng.uniform(-0.9, 0.9, m).astype(F)
         gx[:1000] = F(0.9)
         gy = rng.uniform(-0.9, 0.9, m).astype(F)
-        Y = rng.integers(0, 8192, m)
+        Y = rng.integers(0, 65536, m)
         xb = detail_positions(x, D, gx, gy, Y, n)
         assert (np.diff(xb, axis=1) > 0).all(), n
