@@ -57,7 +57,9 @@ enum {
 /* depth::SgmParams (stereo.hpp:26-40) plus StereoParams::subpixel (stereo.hpp:87). Booleans are
  * 0/1. Defaults: sgm_params_default(). Limits (those of MetalStereo::create,
  * metal_stereo.cpp:115-120): 1 <= num_disparities <= 512, census window <= 9x7,
- * window cells + max penalty <= 255 (path costs are bytes), 0 <= p1 <= 32767, p2 >= 0. */
+ * window cells + p2 <= 255; and 0 <= p1 <= 2^30, p2 >= 0. Path costs are bytes where the largest step
+ * penalty allows (cells + max(p2, p1 + 1 if adaptive) <= 255 and p1 <= 32767), u16 otherwise; results
+ * are the same either way. Image widths up to 32768 columns. */
 typedef struct {
     int32_t min_disparity;
     int32_t num_disparities;
