@@ -5,7 +5,10 @@
 //
 // The ZNCC scores accumulate in double in the reference's order (window rows outer, columns inner)
 // and the float sample positions keep its operation order; with -ffp-contract=off and IEEE double
-// add/mul/div/sqrt every score, and so every winner and subpixel offset, equals the CPU's.
+// add/mul/div/sqrt every score, and so every winner and subpixel offset, equals the CPU's. The sums
+// of products use fused multiply-adds: a sample is a float (24-bit significand) and a left pixel a
+// byte, so ib * ib (<= 48 bits) and ia * ib (<= 32 bits) are exact in double, and rounding once after
+// the add gives what rounding the exact product and then the sum gives.
 #include <climits>
 
 #include "sgm_internal.h"
@@ -107,8 +110,8 @@ __device__ float zncc_slanted(const uint8_t* a, const uint8_t* b, int64_t pitch,
             const double ib = (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
             const double ia = ra[x + u];
             sb += ib;
-            sbb += ib * ib;
-            sab += ia * ib;
+            sbb = __builtin_fma(ib, ib, sbb);
+            sab = __builtin_fma(ia, ib, sab);
         }
     }
     const double va = saa - sa * sa / n;
@@ -129,13 +132,23 @@ __device__ float zncc_slanted(const uint8_t* a, const uint8_t* b, int64_t pitch,
 // the first is >= 0 and the last is < bw - 1, and only those two are tested; finite slopes and
 // disparities leave no NaN to test for.
 
+// The two taps rb[x0], rb[x0 + 1] of a bilinear sample as one 16-bit load (unaligned where x0 is odd).
+typedef uint16_t u16_unaligned __attribute__((aligned(1)));
+__device__ __forceinline__ void taps(const uint8_t* p, float& a, float& b) {
+    const uint32_t v = *reinterpret_cast<const u16_unaligned*>(p);
+    a = (float)(v & 0xFFu);
+    b = (float)(v >> 8);
+}
+
 // The bilinear sample at xb of row rb, its left pixel clamped into [0, bw - 2].
 __device__ __forceinline__ double sample(const uint8_t* rb, int bw, float xb) {
     const float fl = __builtin_floorf(xb);
     const int x0 = (int)__builtin_amdgcn_fmed3f(fl, 0.0f, (float)(bw - 2));
     __builtin_assume(x0 >= 0);
     const float f = xb - fl;
-    return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
+    float a, b;
+    taps(rb + x0, a, b);
+    return a * (1.0f - f) + b * f;
 }
 
 __device__ __forceinline__ float zncc_finish(double sa, double saa, double sb, double sbb, double sab, double n) {
@@ -170,8 +183,9 @@ __device__ __forceinline__ double sample_row(const RightRows& rr, int row, float
         const RK_LDS float* t = (const RK_LDS float*)(uintptr_t)(rr.t + (uint32_t)(row * kRightCols * 4) + 4u * (uint32_t)x0);
         return t[0] * (1.0f - f) + t[1] * f;
     } else {
-        const uint8_t* rb = rr.b + (int64_t)row * rr.pitch;
-        return (float)rb[x0] * (1.0f - f) + (float)rb[x0 + 1] * f;
+        float a, b;
+        taps(rr.b + (int64_t)row * rr.pitch + x0, a, b);
+        return a * (1.0f - f) + b * f;
     }
 }
 
@@ -196,8 +210,8 @@ __device__ float zncc_slanted_w(const RK_LDS double* ta, const RightRows& rr, in
             const double ib = sample_row<TILE>(rr, v + R, xb);
             const double ia = ra[u];
             sb += ib;
-            sbb += ib * ib;
-            sab += ia * ib;
+            sbb = __builtin_fma(ib, ib, sbb);
+            sab = __builtin_fma(ia, ib, sab);
         }
     }
     if (!ok) return -2.0f;
@@ -372,8 +386,8 @@ __device__ float zncc_window(const uint8_t* a, const uint8_t* b, int64_t pitch, 
             const double ib = (float)rb[xi] * (1.0f - f) + (float)rb[xi + 1] * f;
             const double ia = ra[X];
             sb += ib;
-            sbb += ib * ib;
-            sab += ia * ib;
+            sbb = __builtin_fma(ib, ib, sbb);
+            sab = __builtin_fma(ia, ib, sab);
         }
     }
     const double m = (double)n * n;
@@ -409,8 +423,8 @@ __device__ float zncc_window_w(const RK_LDS double* ta, const uint8_t* b, int64_
             const double ib = sample(rb, bw, xb);
             const double ia = ra[i];
             sb += ib;
-            sbb += ib * ib;
-            sab += ia * ib;
+            sbb = __builtin_fma(ib, ib, sbb);
+            sab = __builtin_fma(ia, ib, sab);
         }
     }
     if (!ok) return -2.0f;
