@@ -394,6 +394,11 @@ typedef struct {
     int32_t branches;    /* 0: one stream; 1: a graph branch per lane class; 2: two per class */
     int32_t path_group;  /* lanes per scanline: 16, or 32 (with at most 8 lines; lane classes that do not
                             allow it use 16) */
+    int32_t sweep_band;  /* 0: every direction walked line by line; > 0: band sweeps (8 paths, byte path
+                            costs, cost volume): the six non-horizontal directions aggregated row by row,
+                            this many rows per launch, into one u16 sum per sweep */
+    int32_t sweep_tile;  /* band sweeps: columns per block, 16, 32, 48 or 64 (lane classes whose LDS
+                            would not hold it use less) */
 } sgm_batch_tuning;
 
 /* The plan a batch runs and how it was chosen. */
@@ -403,6 +408,8 @@ typedef struct {
     double best_ms, worst_ms; /* replay time of the fastest and slowest candidate */
     int32_t from_cache;
     int32_t path_group;       /* lanes per scanline (16 or 32) */
+    int32_t sweep_band;       /* rows per band-sweep launch; 0: line schedule */
+    int32_t sweep_tile;       /* band sweeps: columns per block (unused by the line schedule) */
 } sgm_batch_plan;
 
 /* Plans a batch of n pairs: allocates the scratch of all pairs, times every candidate plan on the

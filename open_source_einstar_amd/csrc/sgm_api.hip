@@ -1578,10 +1578,22 @@ struct BatchPlan {
     int path_lines = 16;  // scanlines per path-kernel block
     int branches = 1;     // 0: one stream, 1: a branch per lane class, 2: two per class
     int path_group = 16;  // lanes per scanline: 16, or 32 (at most 8 lines) for classes that allow it
+    int sweep_band = 0;   // > 0: band sweeps of this many rows (sweep_kernels.hip); 0: lines only
+    int sweep_tile = 64;  // band sweeps: columns per block
 };
 
 // Lanes per scanline of a class under a plan: 32 only where the lane layout allows it.
 int class_group(const BatchPlan& pl, int dpl) { return paths_group_ok(dpl, pl.path_group, pl.path_lines) ? pl.path_group : 16; }
+
+// A lane class's sweep block shape under a plan: the plan's tile and band, narrowed (tile first) until
+// the kernel takes it with the block's LDS in half a CU's, so that two blocks share a CU.
+constexpr size_t kSweepLdsLimit = 80 * 1024;
+void sweep_shape(const BatchPlan& pl, int dpl, int& tile, int& band) {
+    tile = pl.sweep_tile;
+    band = pl.sweep_band;
+    while (!sweep_shape_ok(dpl, tile, band, kSweepLdsLimit) && tile > 16) tile -= 16;
+    while (!sweep_shape_ok(dpl, tile, band, kSweepLdsLimit) && band > 1) band = (band + 1) / 2;
+}
 
 std::mutex g_plan_mu;
 std::map<std::string, BatchPlan> g_plan_cache;
@@ -1599,6 +1611,9 @@ struct sgm_batch {
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> branch;  // extra streams for parallel branches
     std::vector<hipEvent_t> events;
+    bool sweep_ok = false;            // band sweeps possible: 8 paths, byte path costs, cost volumes
+    std::vector<hipStream_t> sweep_streams;  // one per lane class: its horizontal line launch
+    std::vector<hipEvent_t> sweep_events;    // two per lane class (fork, join)
     BatchPlan plan;
     std::vector<void*> plan_allocs;  // device tables of the current plan's graph
     hipGraph_t graph = nullptr;
@@ -1647,7 +1662,25 @@ struct Group {
     int max_w = 1;
     std::vector<PairDesc> ds;
     LaunchTable cost, paths, rows;
+    // Band sweeps: the tables of both sweeps, the block shape and the number of band launches.
+    LaunchTable sweep;
+    int tile = 0, band = 0, nbands = 0;
 };
+
+// Band-sweep plans: a pair's path-cost buffer holds one record per pixel, [S_down[dp] u16,
+// S_up[dp] u16, L0[dp] u8, L1[dp] u8] (6 dp bytes, within the 8 dp of the line schedule's layout); the
+// line kernel writes only directions 0 and 1, into the record's last 2 dp bytes.
+PairDesc sweep_desc(PairDesc d) {
+    d.lvol_px_stride = (int64_t)6 * d.dp;
+    return d;
+}
+PairDesc horizontal_desc(PairDesc d) {
+    d.lvol = d.lvol + (size_t)4 * d.dp;
+    d.lvol_px_stride = (int64_t)6 * d.dp;
+    d.ndirs = 2;
+    fill_lines(d);
+    return d;
+}
 
 void drop_plan(sgm_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -1692,17 +1725,30 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         }
     }
     hipError_t e = hipSuccess;
+    const bool sweeps = plan.sweep_band > 0;
+    if (sweeps && (!b->sweep_ok || !plan.cost_volume)) return hipErrorInvalidValue;
     for (Group& g : groups) {
-        std::vector<int> cb, pb, rb;
+        std::vector<int> cb, pb, rb, sb;
+        std::vector<PairDesc> hds, sds;
+        if (sweeps) sweep_shape(plan, g.dpl, g.tile, g.band);
         for (const PairDesc& d : g.ds) {
             cb.push_back(cost8_blocks(d.w, d.h));
-            pb.push_back(path_blocks(d, plan.path_lines));
             rb.push_back(row_blocks(d));
+            if (sweeps) {
+                hds.push_back(horizontal_desc(d));
+                sds.push_back(sweep_desc(d));
+                pb.push_back(path_blocks(hds.back(), plan.path_lines));
+                sb.push_back(sweep_blocks(d.w, g.tile));
+                g.nbands = std::max(g.nbands, (d.h + g.band - 1) / g.band);
+            } else {
+                pb.push_back(path_blocks(d, plan.path_lines));
+            }
         }
         if (e == hipSuccess && plan.cost_volume) e = make_table(b, g.ds, cb, g.cost);
-        if (e == hipSuccess) e = make_table(b, g.ds, pb, g.paths);
+        if (e == hipSuccess) e = make_table(b, sweeps ? hds : g.ds, pb, g.paths);
         g.paths.cost_volume = plan.cost_volume ? 1 : 0;  // the path kernel reads the costs the cost launch wrote
-        if (e == hipSuccess) e = make_table(b, g.ds, rb, g.rows);
+        if (e == hipSuccess) e = make_table(b, sweeps ? sds : g.ds, rb, g.rows);
+        if (e == hipSuccess && sweeps) e = make_table(b, sds, sb, g.sweep);
     }
     LaunchTable census{}, median{};
     std::vector<int> blocks;
@@ -1728,6 +1774,17 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e == hipSuccess) b->events.push_back(ev);
     }
+    // Band sweeps: the down and up sweeps of each class run beside its horizontal line launch.
+    while (e == hipSuccess && sweeps && b->sweep_streams.size() < groups.size()) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) b->sweep_streams.push_back(st);
+    }
+    while (e == hipSuccess && sweeps && b->sweep_events.size() < 2 * groups.size()) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->sweep_events.push_back(ev);
+    }
     if (e != hipSuccess) return e;
     hipStream_t s = b->stream;
     e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -1741,8 +1798,24 @@ hipError_t build_plan(sgm_batch* b, const BatchPlan& plan) {
         const Group& g = groups[gi];
         hipStream_t gs = (nbr == 1 || gi == 0) ? s : b->branch[gi - 1];
         if (plan.cost_volume) le = launch_cost8(g.dpl, g.cost, b->consts, gs);
-        if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, class_group(plan, g.dpl), gs);
-        if (le == hipSuccess) le = launch_rows(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
+        if (sweeps) {
+            // The class's band sweeps (down and up in each launch) stay on the class's stream; its
+            // horizontal line launch runs beside them on a stream of its own.
+            hipStream_t hs = b->sweep_streams[gi];
+            hipEvent_t* ev = &b->sweep_events[2 * gi];
+            if (le == hipSuccess) le = hipEventRecord(ev[0], gs);
+            if (le == hipSuccess) le = hipStreamWaitEvent(hs, ev[0], 0);
+            if (le == hipSuccess)
+                le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, class_group(plan, g.dpl), hs);
+            for (int band = 0; le == hipSuccess && band < g.nbands; ++band)
+                le = launch_sweep(g.dpl, g.padded, g.sweep, b->consts, g.tile, g.band, band, gs);
+            if (le == hipSuccess) le = hipEventRecord(ev[1], hs);
+            if (le == hipSuccess) le = hipStreamWaitEvent(gs, ev[1], 0);
+            if (le == hipSuccess) le = launch_rows_sweep(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
+        } else {
+            if (le == hipSuccess) le = launch_paths(g.dpl, g.padded, g.paths, b->consts, plan.path_lines, class_group(plan, g.dpl), gs);
+            if (le == hipSuccess) le = launch_rows(g.dpl, g.padded, g.rows, b->consts, g.max_w, gs);
+        }
     }
     for (size_t i = 1; le == hipSuccess && nbr > 1 && i < nbr; ++i) {
         le = hipEventRecord(b->events[i], b->branch[i - 1]);
@@ -1785,7 +1858,7 @@ std::string batch_signature(const sgm_params& p, int n, const sgm_pair* pairs) {
                         (pairs[i].out.disparity_median ? "m" : ""));
     }
     std::sort(items.begin(), items.end());
-    std::string k = "v2:" + std::to_string(p.eight_paths ? 8 : 4) + ":" + std::to_string(p.census_radius_x) + "x" +
+    std::string k = "v3:" + std::to_string(p.eight_paths ? 8 : 4) + ":" + std::to_string(p.census_radius_x) + "x" +
                     std::to_string(p.census_radius_y) + ":s" + std::to_string(p.slant_steps ? 1 : 0);
     for (const std::string& it : items) k += ";" + it;
     return k;
@@ -1803,6 +1876,8 @@ void sgm_batch_destroy(sgm_batch* b) {
     if (b->order_ev) (void)hipEventDestroy(b->order_ev);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
     for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : b->sweep_events) (void)hipEventDestroy(e);
+    for (hipStream_t st : b->sweep_streams) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1832,13 +1907,15 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         g_last_error = "batch size must be in [1, " + std::to_string(kMaxBatch) + "]";
         return SGM_ERR_INVALID_ARGUMENT;
     }
-    sgm_batch_tuning tn{-1, -1, -1, -1};
+    sgm_batch_tuning tn{-1, -1, -1, -1, -1, -1};
     if (tuning) tn = *tuning;
     if ((tn.cost_volume < -1 || tn.cost_volume > 1) || (tn.path_lines != -1 && tn.path_lines != 4 && tn.path_lines != 8 && tn.path_lines != 16) ||
         (tn.branches < -1 || tn.branches > 2) || (tn.path_group != -1 && tn.path_group != 16 && tn.path_group != 32) ||
-        (tn.path_group == 32 && tn.path_lines == 16)) {
+        (tn.path_group == 32 && tn.path_lines == 16) || tn.sweep_band < -1 || tn.sweep_band > 1024 ||
+        (tn.sweep_tile != -1 && (tn.sweep_tile < 16 || tn.sweep_tile > 64 || tn.sweep_tile % 16 != 0))) {
         g_last_error = "tuning: cost_volume in {-1,0,1}, path_lines in {-1,4,8,16}, branches in {-1,0,1,2}, "
-                       "path_group in {-1,16,32} (32 with at most 8 lines)";
+                       "path_group in {-1,16,32} (32 with at most 8 lines), sweep_band in [-1, 1024], "
+                       "sweep_tile in {-1,16,32,48,64}";
         return SGM_ERR_INVALID_ARGUMENT;
     }
     for (int i = 0; i < n; ++i) {
@@ -1872,13 +1949,21 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         g_last_error = "tuning: a pair's cost volume exceeds 4 GiB - 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    const SgmConsts consts = make_consts(*params);
+    const bool sweep_ok = params->eight_paths && !consts.wide_costs && cv_fits;
+    if (tn.sweep_band > 0 && (!sweep_ok || tn.cost_volume == 0)) {
+        g_last_error = "tuning: band sweeps need 8 paths, byte path costs and the cost volume";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     auto* b = new sgm_batch;
     b->p = *params;
-    b->consts = make_consts(*params);
+    b->consts = consts;
     b->n = n;
     b->ndirs = params->eight_paths ? 8 : 4;
-    // Scratch per pair: census words (2 images, padded rows), u8 cost volume, u8 path costs.
-    std::vector<size_t> off_census(n), off_cvol(n), off_lvol(n);
+    b->sweep_ok = sweep_ok;
+    // Scratch per pair: census words (2 images, padded rows), u8 cost volume, u8 path costs, and for
+    // band sweeps the seed rows (two sweeps x two parities x three directions of W x dp bytes).
+    std::vector<size_t> off_census(n), off_cvol(n), off_lvol(n), off_seed(n);
     size_t total = 0;
     for (int i = 0; i < n; ++i) {
         const size_t npx = (size_t)pairs[i].width * pairs[i].height;
@@ -1890,6 +1975,8 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         if (cv_fits) total += (npx * dp + 255) & ~(size_t)255;
         off_lvol[i] = total;
         total += ((size_t)b->ndirs * npx * dp * (b->consts.wide_costs ? 2 : 1) + 255) & ~(size_t)255;
+        off_seed[i] = total;
+        if (sweep_ok) total += ((size_t)12 * pairs[i].width * dp + 255) & ~(size_t)255;
     }
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = dalloc(&b->d_scratch, total, b->bytes);
@@ -1909,6 +1996,7 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
         d.edge = reinterpret_cast<uint32_t*>(d.census_r + census_words(q.width, q.height, q.min_disparity, q.num_disparities));
         d.cvol = cv_fits ? b->d_scratch + off_cvol[i] : nullptr;
         d.lvol = b->d_scratch + off_lvol[i];
+        d.seed = sweep_ok ? b->d_scratch + off_seed[i] : nullptr;
         set_geometry(d, q.width, q.height, q.min_disparity, q.num_disparities, b->ndirs, b->consts.wide_costs ? 2 : 1);
         d.disp = q.out.disparity;
         d.median = q.out.disparity_median;
@@ -1934,21 +2022,37 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
                     if (tn.branches < 0 && br == 0) continue;  // one stream only when asked for
                     if (pg == 32 && pl == 16) continue;           // 32-lane lines: at most 8 per block
                     if (cv == 1 && !cv_fits) continue;
-                    cands.push_back({cv, pl, br, pg});
+                    if (tn.sweep_band <= 0) cands.push_back({cv, pl, br, pg, 0, 64});
+                    // Band sweeps: a few band heights and tiles, beside the horizontal line launch of
+                    // each knob setting that the caller fixed (8 lines of 16 lanes otherwise).
+                    if (tn.sweep_band == 0 || !sweep_ok || cv != 1) continue;
+                    if ((tn.path_lines < 0 && pl != 8) || (tn.path_group < 0 && pg != 16) || (tn.branches < 0 && br != 1))
+                        continue;
+                    const std::vector<int> bands = tn.sweep_band > 0 ? std::vector<int>{tn.sweep_band} : std::vector<int>{16, 32, 64};
+                    const std::vector<int> tiles = tn.sweep_tile > 0 ? std::vector<int>{tn.sweep_tile} : std::vector<int>{32, 64};
+                    for (int band : bands)
+                        for (int tile : tiles) cands.push_back({cv, pl, br, pg, band, tile});
                 }
+    if (cands.empty()) {
+        g_last_error = "tuning: no plan satisfies the fixed knobs";
+        sgm_batch_destroy(b);
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     b->signature = batch_signature(*params, n, pairs);
     BatchPlan best = cands.front();
     float best_ms = 0, worst_ms = 0;
     bool cached = false;
     // Only a search over every knob is cached, and a cached plan is taken only where it honours the
     // knobs this call fixed.
-    const bool all_open = tn.cost_volume < 0 && tn.path_lines < 0 && tn.branches < 0 && tn.path_group < 0;
+    const bool all_open = tn.cost_volume < 0 && tn.path_lines < 0 && tn.branches < 0 && tn.path_group < 0 &&
+                          tn.sweep_band < 0 && tn.sweep_tile < 0;
     if (cands.size() > 1) {
         std::lock_guard lk(g_plan_mu);
         auto it = g_plan_cache.find(b->signature);
         if (it != g_plan_cache.end() && std::any_of(cands.begin(), cands.end(), [&](const BatchPlan& c) {
                 return c.cost_volume == it->second.cost_volume && c.path_lines == it->second.path_lines &&
-                       c.branches == it->second.branches && c.path_group == it->second.path_group;
+                       c.branches == it->second.branches && c.path_group == it->second.path_group &&
+                       c.sweep_band == it->second.sweep_band && (c.sweep_band == 0 || c.sweep_tile == it->second.sweep_tile);
             })) {
             best = it->second;
             cached = true;
@@ -1983,6 +2087,8 @@ int sgm_batch_create_tuned(const sgm_params* params, int n, const sgm_pair* pair
     b->info.path_lines = best.path_lines;
     b->info.path_group = best.path_group;
     b->info.branches = best.branches;
+    b->info.sweep_band = best.sweep_band;
+    b->info.sweep_tile = best.sweep_tile;
     b->info.candidates_timed = timed;
     b->info.best_ms = best_ms;
     b->info.worst_ms = worst_ms;
@@ -2001,7 +2107,7 @@ int sgm_tune_cache_load(const char* path, int* entries) {
         g_last_error = std::string("sgm_tune_cache_load: cannot open ") + path;
         return SGM_ERR_NOT_FOUND;
     }
-    // One entry a line: key, a tab, then cost_volume path_lines branches path_group.
+    // One entry a line: key, a tab, then cost_volume path_lines branches path_group sweep_band sweep_tile.
     std::map<std::string, BatchPlan> got;
     char line[8192];
     int rc = SGM_OK;
@@ -2012,9 +2118,12 @@ int sgm_tune_cache_load(const char* path, int* entries) {
         const size_t tab = ln.find('\t');
         BatchPlan pl;
         if (tab == std::string::npos || tab == 0 ||
-            std::sscanf(ln.c_str() + tab + 1, "%d %d %d %d", &pl.cost_volume, &pl.path_lines, &pl.branches, &pl.path_group) != 4 ||
+            std::sscanf(ln.c_str() + tab + 1, "%d %d %d %d %d %d", &pl.cost_volume, &pl.path_lines, &pl.branches, &pl.path_group,
+                        &pl.sweep_band, &pl.sweep_tile) != 6 ||
             (pl.cost_volume != 0 && pl.cost_volume != 1) || (pl.path_lines != 4 && pl.path_lines != 8 && pl.path_lines != 16) ||
-            pl.branches < 0 || pl.branches > 2 || (pl.path_group != 16 && pl.path_group != 32)) {
+            pl.branches < 0 || pl.branches > 2 || (pl.path_group != 16 && pl.path_group != 32) || pl.sweep_band < 0 ||
+            pl.sweep_band > 1024 || (pl.sweep_band > 0 && pl.cost_volume != 1) || pl.sweep_tile < 16 || pl.sweep_tile > 64 ||
+            pl.sweep_tile % 16 != 0) {
             g_last_error = "sgm_tune_cache_load: malformed line: " + ln.substr(0, 200);
             rc = SGM_ERR_PROTOCOL;
             break;
@@ -2041,7 +2150,8 @@ int sgm_tune_cache_save(const char* path, int* entries) {
         return SGM_ERR_IO;
     }
     for (const auto& [k, v] : g_plan_cache)
-        std::fprintf(f, "%s\t%d %d %d %d\n", k.c_str(), v.cost_volume, v.path_lines, v.branches, v.path_group);
+        std::fprintf(f, "%s\t%d %d %d %d %d %d\n", k.c_str(), v.cost_volume, v.path_lines, v.branches, v.path_group,
+                     v.sweep_band, v.sweep_tile);
     const bool ok = std::fclose(f) == 0;
     if (!ok) {
         g_last_error = std::string("sgm_tune_cache_save: write failed: ") + path;

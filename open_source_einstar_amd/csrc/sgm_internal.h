@@ -45,6 +45,7 @@ struct PairDesc {
     uint16_t* second;        // best non-adjacent S (0xFFFF: none)
     uint16_t* sum_volume;    // optional debug S [y][x][nd]
     uint16_t* cost_volume;   // optional debug C [y][x][nd]
+    uint8_t* seed;           // band sweeps: [sweep][parity][kind][x][dp] path-cost bytes of a band's last row
     int64_t pitch;           // input row pitch in bytes
     int32_t census_pitch;    // census words per row: pads, then the w words of the row, then pads
     int32_t census_padl;     // words before x = 0 (every disparity window stays inside the row)
@@ -144,6 +145,17 @@ hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmCons
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+// Band sweeps of 8-path byte-cost batches (sweep_kernels.hip): band `band` of rows [band * band_rows,
+// ...) of every pair of `t` (tile-column blocks per pair, sweep_blocks), both sweeps in one launch; and
+// the row kernels that read the sweep records (rows_sweep.hip).
+size_t sweep_lds_bytes(int dpl, int tile, int band_rows);
+// A sweep block shape the kernel takes: tile in {16, 32, 48, 64}, a staged row within the block's
+// loads, and at most lds_limit bytes of LDS.
+bool sweep_shape_ok(int dpl, int tile, int band_rows, size_t lds_limit);
+int sweep_blocks(int w, int tile);
+hipError_t launch_sweep(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int tile, int band_rows, int band,
+                        hipStream_t stream);
+hipError_t launch_rows_sweep(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
 hipError_t launch_rectify(const RectifyDesc& r, hipStream_t stream);
 
 // Host-side launchers of the stages around SGM (refine_kernels.hip).

@@ -757,6 +757,13 @@ struct RowIn16 {
 };
 template <int DPL, bool W16>
 using RowInT = std::conditional_t<W16, RowIn16<DPL>, RowIn<DPL>>;
+// Band-sweep records (sweep_kernels.hip): the two horizontal directions' path-cost bytes and the
+// down and up sweeps' u16 sums of the other six.
+template <int DPL>
+struct RowInSw {
+    uint32_t w[kMaxDirs][(DPL + 3) / 4];  // w[0], w[1]: directions 0 and 1
+    uint32_t s[2][DPL / 2];
+};
 
 // The direction count is a template parameter so that every load is unconditional: the compiler can
 // then wait for each ring slot alone instead of for every load in flight.
@@ -805,8 +812,12 @@ __device__ __forceinline__ uint32_t lane_elem(const uint32_t (&s)[DPL / 2], int 
 // coalesced (keys, second-best sums, disparities, status). WIDE rows (beyond kMaxStagedWidth columns)
 // keep only the right WTA keys in LDS (4 bytes per column): each pixel's left results go straight to
 // the outputs when computed, and the left/right check reads the disparities back from there.
-template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false>
+// SW: the pixels are band-sweep records [S_down[dp] u16, S_up[dp] u16, L0[dp] u8, L1[dp] u8]
+// (NDIRS = 2 byte directions).
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false>
 __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
+    static_assert(!SW || (NDIRS == 2 && !W16), "sweep records hold two byte directions");
+    using In = std::conditional_t<SW, RowInSw<DPL>, RowInT<DPL, W16>>;
     constexpr int ES = W16 ? 2 : 1;  // bytes per stored path cost
     const int W = P.w, y = lb;
     SGM_LDS uint32_t* rkey = smem;
@@ -822,8 +833,19 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     const int nd = P.nd, min_d = P.min_d;
     const int64_t dp = P.dp * ES;  // bytes between a pixel's directions
     const int64_t pxs = P.lvol_px_stride;  // [y][x][dir][dp]: a pixel's directions are contiguous
-    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + k0 * ES;
+    const SGM_GLOBAL uint8_t* lrow = gptr(P.lvol) + (int64_t)y * W * pxs + (SW ? 4 * P.dp : 0) + k0 * ES;
+    const SGM_GLOBAL uint8_t* sums = gptr(P.lvol) + (int64_t)y * W * pxs + 2 * k0;  // SW: the u16 sums
     const int64_t row0 = (int64_t)y * W;
+    auto load_in = [&](In& in, int x) {
+        if constexpr (SW) {
+            load_raw<DPL>(lrow + (int64_t)x * pxs, in.w[0]);
+            load_raw<DPL>(lrow + (int64_t)x * pxs + dp, in.w[1]);
+            load_raw<2 * DPL>(sums + (int64_t)x * pxs, in.s[0]);
+            load_raw<2 * DPL>(sums + (int64_t)x * pxs + 2 * P.dp, in.s[1]);
+        } else {
+            load_row_in<DPL, NDIRS, W16>(in, lrow + (int64_t)x * pxs, dp);
+        }
+    };
 
     // One pixel's results between its computation and its commit (right-WTA atomics and stores).
     struct PxOut {
@@ -834,9 +856,16 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     };
     // Sums, left WTA with uniqueness and subpixel (stereo.cpp:412-432); no memory side effects, so
     // the computations of several pixels can overlap.
-    auto compute = [&](const RowInT<DPL, W16>& in, int x, PxOut& o) {
-        if constexpr (W16) sum_dirs16<DPL, NDIRS>(in.w, o.s);
-        else sum_dirs<DPL, NDIRS>(in.w, o.s);
+    auto compute = [&](const In& in, int x, PxOut& o) {
+        if constexpr (SW) {
+            sum_dirs<DPL, 2>(in.w, o.s);
+#pragma unroll
+            for (int r = 0; r < DPL / 2; ++r) o.s[r] = pk_add(o.s[r], pk_add(in.s[0][r], in.s[1][r]));
+        } else if constexpr (W16) {
+            sum_dirs16<DPL, NDIRS>(in.w, o.s);
+        } else {
+            sum_dirs<DPL, NDIRS>(in.w, o.s);
+        }
 
         // stereo.cpp:419-421: candidates whose right pixel x - (min_d + k) lies in the image.
         const int min_k = max(0, x - min_d - (W - 1));
@@ -950,10 +979,9 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     // the slots refilled, then the pixels committed. Reads past the row's end are clamped to its
     // last pixel and never used.
     constexpr int kRowAhead = row_ahead(W16 ? 2 * DPL : DPL);
-    RowInT<DPL, W16> ring[kRowAhead];
+    In ring[kRowAhead];
 #pragma unroll
-    for (int j = 0; j < kRowAhead; ++j)
-        load_row_in<DPL, NDIRS, W16>(ring[j], lrow + (int64_t)min(grp + j * kLinesPerBlock, W - 1) * pxs, dp);
+    for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(grp + j * kLinesPerBlock, W - 1));
     for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
         PxOut o[kRowAhead];
 #pragma unroll
@@ -962,8 +990,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
             if (xj < W) compute(ring[j], xj, o[j]);
         }
 #pragma unroll
-        for (int j = 0; j < kRowAhead; ++j)
-            load_row_in<DPL, NDIRS, W16>(ring[j], lrow + (int64_t)min(x + (j + kRowAhead) * kLinesPerBlock, W - 1) * pxs, dp);
+        for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(x + (j + kRowAhead) * kLinesPerBlock, W - 1));
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
             const int xj = x + j * kLinesPerBlock;
@@ -1008,13 +1035,13 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     }
 }
 
-template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false>
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false>
 __global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
     extern __shared__ uint32_t smem[];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    rows_body<DPL, PAD, NDIRS, WIDE, W16>(P, c, lb, lds(smem));
+    rows_body<DPL, PAD, NDIRS, WIDE, W16, SW>(P, c, lb, lds(smem));
 }
 
 

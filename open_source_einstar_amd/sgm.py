@@ -98,13 +98,14 @@ class CSgmServerStats(C.Structure):
 
 class CSgmBatchTuning(C.Structure):
     _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32),
-                ("path_group", C.c_int32)]
+                ("path_group", C.c_int32), ("sweep_band", C.c_int32), ("sweep_tile", C.c_int32)]
 
 
 class CSgmBatchPlan(C.Structure):
     _fields_ = [("cost_volume", C.c_int32), ("path_lines", C.c_int32), ("branches", C.c_int32),
                 ("candidates_timed", C.c_int32), ("best_ms", C.c_double), ("worst_ms", C.c_double),
-                ("from_cache", C.c_int32), ("path_group", C.c_int32)]
+                ("from_cache", C.c_int32), ("path_group", C.c_int32), ("sweep_band", C.c_int32),
+                ("sweep_tile", C.c_int32)]
 
 
 class CSgmPair(C.Structure):
@@ -607,7 +608,8 @@ class Batch:
     """A planned batch (sgm_batch_create): device copies of the pairs, outputs, and the graph."""
 
     def __init__(self, params: SgmParams, pairs: list[PairSpec], tuning: dict | None = None):
-        """tuning: cost_volume / path_lines / branches fixed to a value (omitted = autotuned)."""
+        """tuning: cost_volume / path_lines / branches / path_group / sweep_band / sweep_tile fixed to a
+        value (omitted = autotuned)."""
         self.params = params
         self.specs = pairs
         self._bufs = []
@@ -640,7 +642,7 @@ class Batch:
         cp = params.to_c()
         t = tuning or {}
         tn = CSgmBatchTuning(t.get("cost_volume", -1), t.get("path_lines", -1), t.get("branches", -1),
-                             t.get("path_group", -1))
+                             t.get("path_group", -1), t.get("sweep_band", -1), t.get("sweep_tile", -1))
         rc = lib().sgm_batch_create_tuned(C.byref(cp), len(pairs), cps, C.byref(tn), C.byref(h))
         if rc != 0:
             raise SgmError(rc, (lib().sgm_batch_last_error(None) or b"").decode())

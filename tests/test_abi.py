@@ -123,3 +123,29 @@ def test_noalloc_entry_point_refuses_oversize_pairs_before_any_gpu_work():
     bad = sgm.SgmParams(num_disparities=600).to_c()
     rc = L.sgm_compute_device_noalloc(C.byref(bad), 16, 16, 64, 64, 48, C.byref(sgm.CSgmOutputs()), None)
     assert sgm.ERRC[rc] == "invalid_argument"
+
+
+@pytest.mark.parametrize("tuning,eight,p1", [
+    (dict(sweep_band=-2), True, 10), (dict(sweep_band=1025), True, 10), (dict(sweep_tile=24), True, 10),
+    (dict(sweep_tile=80), True, 10), (dict(sweep_band=16), False, 10),       # band sweeps need 8 paths
+    (dict(sweep_band=16, cost_volume=0), True, 10),                         # ... and the cost volume
+    (dict(sweep_band=16), True, 200),                                       # ... and byte path costs
+])
+def test_batch_tuning_refusals_before_any_gpu_work(tuning, eight, p1):
+    """sgm_batch_create_tuned refuses band-sweep knobs it cannot honour before it allocates anything
+    (the pair pointers here are never dereferenced). P1 = 200 with adaptive P2: cells (63) +
+    max(P2, P1 + 1) > 255, so the path costs are u16."""
+    L = sgm.lib()
+    pairs = (sgm.CSgmPair * 1)()
+    pairs[0].left, pairs[0].right, pairs[0].pitch = 16, 16, 64
+    pairs[0].width, pairs[0].height, pairs[0].min_disparity, pairs[0].num_disparities = 64, 48, 0, 32
+    pairs[0].out.disparity = 16
+    t = dict(cost_volume=-1, path_lines=-1, branches=-1, path_group=-1, sweep_band=-1, sweep_tile=-1)
+    t.update(tuning)
+    tn = sgm.CSgmBatchTuning(*[t[f] for f, _ in sgm.CSgmBatchTuning._fields_])
+    cp = sgm.SgmParams(num_disparities=32, eight_paths=eight, census_radius_x=4, census_radius_y=3, p1=p1,
+                       p2=120).to_c()
+    h = C.c_void_p()
+    rc = L.sgm_batch_create_tuned(C.byref(cp), 1, pairs, C.byref(tn), C.byref(h))
+    assert sgm.ERRC[rc] == "invalid_argument", (tuning, rc)
+    assert b"tuning" in L.sgm_last_error(None) and not h.value

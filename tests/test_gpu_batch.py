@@ -55,14 +55,22 @@ def test_single_pair_batch():
         b.close()
 
 
-@pytest.mark.parametrize("tuning", [dict(cost_volume=0, path_lines=4, branches=0, path_group=16),
-                                    dict(cost_volume=1, path_lines=8, branches=2, path_group=32),
-                                    dict(cost_volume=1, path_lines=16, branches=1, path_group=16),
-                                    dict(cost_volume=0, path_lines=8, branches=1, path_group=32)],
+@pytest.mark.parametrize("tuning", [dict(cost_volume=0, path_lines=4, branches=0, path_group=16, sweep_band=0),
+                                    dict(cost_volume=1, path_lines=8, branches=2, path_group=32, sweep_band=0),
+                                    dict(cost_volume=1, path_lines=16, branches=1, path_group=16, sweep_band=0),
+                                    dict(cost_volume=0, path_lines=8, branches=1, path_group=32, sweep_band=0),
+                                    dict(cost_volume=1, path_lines=8, branches=1, path_group=16, sweep_band=32, sweep_tile=64),
+                                    dict(cost_volume=1, path_lines=16, branches=0, path_group=16, sweep_band=5, sweep_tile=16),
+                                    dict(cost_volume=1, path_lines=4, branches=2, path_group=32, sweep_band=1, sweep_tile=48),
+                                    dict(cost_volume=1, path_lines=8, branches=1, path_group=16, sweep_band=200, sweep_tile=32)],
                          ids=["recompute-4-onestream", "volume-8-split-32lanes", "volume-16-perclass",
-                              "recompute-8-perclass-32lanes"])
+                              "recompute-8-perclass-32lanes", "sweeps-32rows-64cols", "sweeps-5rows-16cols-onestream",
+                              "sweeps-1row-48cols-split", "sweeps-one-band-32cols"])
 def test_every_plan_gives_the_same_bytes(tuning):
-    sizes = [(96, 64, 64), (130, 70, 96), (64, 48, 128), (77, 91, 64), (40, 30, 24)]
+    """Line and band-sweep plans: every pair equals the oracle, the graph and the plain launches. The
+    sizes hold tiles and bands that do not divide the image, padded lanes (D = 24, 96, 192) and a
+    pair taller than one band of 64."""
+    sizes = [(96, 64, 64), (130, 70, 96), (64, 48, 128), (77, 91, 64), (40, 30, 24), (33, 130, 192)]
     specs, refs = make_specs(sizes, 3000)
     b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=tuning)
     try:
@@ -78,14 +86,41 @@ def test_every_plan_gives_the_same_bytes(tuning):
         b.close()
 
 
+@pytest.mark.parametrize("slant,adaptive", [(True, True), (False, False)], ids=["slant-adaptive", "plain-steps"])
+def test_band_sweep_sum_volumes(slant, adaptive):
+    """The sum volume S under band sweeps (the swept u16 sums of six directions plus the two horizontal
+    path costs, added by the row kernel) equals the oracle's for every pixel and disparity, on pairs
+    whose tiles and bands do not divide them, with negative and positive disparity offsets."""
+    params = dict(BASE, slant_steps=slant, adaptive_p2=adaptive)
+    sizes = [(150, 75, 64, -5), (70, 140, 40, 3), (23, 9, 16, 0)]
+    specs, refs = [], []
+    for i, (w, h, nd, min_d) in enumerate(sizes):
+        pair = speckle_pair(w, h, nd, 7100 + i)
+        specs.append(sgm.PairSpec(pair.left, pair.right, min_d, nd, outputs=tuple(FIELDS) + ("sum_volume",)))
+        refs.append(O.sgm(pair.left, pair.right, O.SgmParams(min_disparity=min_d, num_disparities=nd, **params),
+                          volumes=True))
+    b = sgm.Batch(sgm.SgmParams(**params), specs,
+                  tuning=dict(cost_volume=1, path_lines=8, branches=1, path_group=16, sweep_band=12, sweep_tile=32))
+    try:
+        for rep in range(2):
+            b.clear_outputs(0xEE)
+            b.run()
+            b.synchronize()
+            for i in range(len(sizes)):
+                assert_same(refs[i], b.results(i), FIELDS + ["sum_volume"], tag=f"sweep rep {rep} pair {i} {sizes[i]}")
+    finally:
+        b.close()
+
+
 def test_autotuner_times_candidates_and_caches_the_choice(tmp_path):
     sizes = [(96, 64, 64), (64, 48, 128), (40, 30, 24)]
     specs, refs = make_specs(sizes, 4000)
     b1 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     p1 = b1.plan()
     b1.close()
-    assert p1["candidates_timed"] == 20 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
-    knobs = ("cost_volume", "path_lines", "branches", "path_group")
+    # 20 line plans and 6 band-sweep plans (3 band heights x 2 tiles)
+    assert p1["candidates_timed"] == 26 and p1["from_cache"] == 0 and p1["best_ms"] <= p1["worst_ms"]
+    knobs = ("cost_volume", "path_lines", "branches", "path_group", "sweep_band", "sweep_tile")
     b2 = sgm.Batch(sgm.SgmParams(**BASE), specs)
     try:
         p2 = b2.plan()
@@ -125,9 +160,11 @@ def test_autotuner_times_candidates_and_caches_the_choice(tmp_path):
     # buckets, disparities per lane (p: padded) and median flag, sorted.
     items = sorted(f"{-(-w // 64)}x{-(-h // 64)}d{-(-nd // 16) if nd > 32 else 2}{'p' if nd % 16 else ''}m"
                    for w, h, nd in sizes)
-    key = "v2:8:4x3:s1;" + ";".join(items)
+    key = "v3:8:4x3:s1;" + ";".join(items)
     assert key + "\t" + " ".join(str(p1[k]) for k in knobs) in path.read_text().splitlines(), (key, path.read_text())
-    alt = dict(cost_volume=1 - p1["cost_volume"], path_lines=4, branches=2, path_group=16)
+    alt = dict(cost_volume=1 - p1["cost_volume"], path_lines=4, branches=2, path_group=16, sweep_band=0, sweep_tile=64)
+    if alt["cost_volume"] == 1:  # a loaded band-sweep plan
+        alt.update(path_lines=8, branches=1, sweep_band=16, sweep_tile=32)
     path.write_text(key + "\t" + " ".join(str(alt[k]) for k in knobs) + "\n")
     assert sgm.tune_cache_load(str(path)) == 1
     b6 = sgm.Batch(sgm.SgmParams(**BASE), specs)
@@ -162,6 +199,10 @@ def batch_sweep(n, seed):
                       branches=int(rng.integers(0, 3)), path_group=int(rng.choice([16, 32])))
         if tuning["path_group"] == 32:
             tuning["path_lines"] = min(tuning["path_lines"], 8)  # 32-lane scanlines take at most 8 lines
+        # Band sweeps (8 paths with the cost volume) in about half of those batches.
+        sweep = params["eight_paths"] and tuning["cost_volume"] == 1 and rng.random() < 0.6
+        tuning["sweep_band"] = int(rng.integers(1, 70)) if sweep else 0
+        tuning["sweep_tile"] = int(rng.choice([16, 32, 48, 64]))
         cases.append((params, pairs, tuning, 5000 + 100 * i))
     return cases
 

@@ -51,6 +51,19 @@ def test_batched_32_uneven_pairs():
         print(f"plan {b.plan()}", flush=True)
     finally:
         b.close()
+    # The same batch under band sweeps of each band height the autotuner tries.
+    for band in (16, 32, 64):
+        b = sgm.Batch(sgm.SgmParams(**PARAMS), specs, tuning=dict(cost_volume=1, path_lines=8, branches=1,
+                                                                   path_group=16, sweep_band=band, sweep_tile=64))
+        try:
+            b.clear_outputs(0xEE)
+            b.run()
+            b.synchronize()
+            for i, sz in enumerate(sizes):
+                assert_same(refs[i], b.results(i), FIELDS, tag=f"sweeps band {band} pair {i} {sz}")
+            print(f"band sweeps of {band} rows bit-exact", flush=True)
+        finally:
+            b.close()
 
 
 SERVE_FIELDS = ["disparity", "disparity_median", "disparity_int", "status", "right_disparity", "best_cost",
