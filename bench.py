@@ -62,17 +62,25 @@ def path_kernel_bytes(w, h, nd, paths=8, cost_volume=True):
     return paths * w * h * per_px_dir
 
 
-def dataflow_bytes(w, h, nd, paths=8, cost_volume=True, median=True):
+def dataflow_bytes(w, h, nd, paths=8, cost_volume=True, median=True, sweeps=False):
     """Algorithmic bytes of the whole per-pair pipeline as this library moves them: census (both
     images read, both census words and the step penalties written), the cost volume (census words
     read, dp cost bytes written), the paths (above), the rows (P*dp path-cost bytes read; disparity,
-    integer disparity and status written) and the median (disparity read, median written)."""
+    integer disparity and status written) and the median (disparity read, median written).
+    sweeps: the band-sweep schedule of 8-path batches instead -- the two horizontal directions on the
+    path kernel (2 x (dp cost bytes read + dp path-cost bytes written + 4 penalty bytes) per pixel),
+    each of the two sweeps reading dp cost bytes and 8 penalty bytes and writing a 2 dp u16 sum per
+    pixel (halo recomputation and seed rows not counted), and the rows reading 6 dp bytes per pixel."""
     dp = lane_dp(nd)
     b = w * h * (2 + 16 + 4)
     if cost_volume:
         b += w * h * (16 + dp)
-    b += path_kernel_bytes(w, h, nd, paths, cost_volume)
-    b += w * h * (paths * dp + 4 + 4 + 1)
+    if sweeps:
+        b += w * h * (2 * (2 * dp + 4) + 2 * (dp + 8 + 2 * dp))
+        b += w * h * (6 * dp + 4 + 4 + 1)
+    else:
+        b += path_kernel_bytes(w, h, nd, paths, cost_volume)
+        b += w * h * (paths * dp + 4 + 4 + 1)
     if median:
         b += w * h * 8
     return b
@@ -396,6 +404,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--batch-tuning", type=str, default="",
+                    help='fixed plan knobs of the batched leg as JSON, e.g. {"sweep_band": 16, "sweep_tile": 32} '
+                         "(omitted knobs are autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-verify", action="store_true", help="skip the bit-exact check against the oracle")
@@ -578,7 +589,7 @@ def main():
             p = speckle_pair(w, h, d, seed=1000 + i)
             specs.append(sgm.PairSpec(p.left, p.right, p.min_disparity, d, outputs=("disparity", "disparity_median",
                                                                                       "disparity_int", "status")))
-        b = sgm.Batch(sgm.SgmParams(**PARAMS), specs)
+        b = sgm.Batch(sgm.SgmParams(**PARAMS), specs, tuning=json.loads(args.batch_tuning) if args.batch_tuning else None)
         total_pxd = sum(w * h * d for w, h, d in sizes)
         res = {}
         for mode in ("plain", "graph"):
@@ -591,7 +602,8 @@ def main():
             b.synchronize()
             res[mode] = (time.perf_counter() - t) * 1e3 / args.batch_reps
         plan = b.plan()
-        flow = sum(dataflow_bytes(w, h, d, 8, bool(plan["cost_volume"])) for w, h, d in sizes)
+        flow = sum(dataflow_bytes(w, h, d, 8, bool(plan["cost_volume"]), sweeps=plan["sweep_band"] > 0)
+                   for w, h, d in sizes)
         batched = {"pairs": len(sizes), "mpix_disp": round(total_pxd / 1e6, 1), "plan": plan,
                    "batched_ms": round(res["graph"], 3), "plain_ms": round(res["plain"], 3),
                    "batched_value": round(total_pxd / (res["graph"] * 1e-3) / 1e6, 0),
