@@ -231,12 +231,14 @@ def test_random_batches_bit_exact(params, pairs, tuning, seed0):
         b.close()
 
 
-def test_batch_with_wide_and_narrow_pairs():
-    """A lane class holding a pair wider than 4096 columns runs the wide row kernel for all its pairs;
-    every pair equals the oracle, graph and plain."""
+@pytest.mark.parametrize("sweep_band", [0, 8], ids=["lines", "band-sweeps"])
+def test_batch_with_wide_and_narrow_pairs(sweep_band):
+    """A lane class holding a pair wider than 4096 columns runs the wide row kernel for all its pairs
+    (over band-sweep records too); every pair equals the oracle, graph and plain."""
     sizes = [(5000, 20, 64), (300, 80, 64), (4100, 12, 128), (200, 50, 128)]
     specs, refs = make_specs(sizes, 6000)
-    b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=dict(cost_volume=1, path_lines=8, branches=1, path_group=16))
+    b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=dict(cost_volume=1, path_lines=8, branches=1, path_group=16,
+                                                            sweep_band=sweep_band, sweep_tile=64))
     try:
         for mode in ("graph", "plain"):
             b.clear_outputs(0xEE)
@@ -244,5 +246,22 @@ def test_batch_with_wide_and_narrow_pairs():
             b.synchronize()
             for i in range(len(sizes)):
                 assert_same(refs[i], b.results(i), FIELDS, tag=f"wide batch {mode} pair {i} {sizes[i]}")
+    finally:
+        b.close()
+
+
+def test_band_sweeps_on_degenerate_shapes():
+    """Band sweeps on a 1x1 pair, a single column, a single row, three rows and a pair narrower than
+    its disparity range, with bands of 2 rows and 16-column tiles: every pair equals the oracle."""
+    sizes = [(1, 1, 16), (1, 40, 32), (40, 1, 64), (17, 3, 128), (9, 33, 96)]
+    specs, refs = make_specs(sizes, 6500)
+    b = sgm.Batch(sgm.SgmParams(**BASE), specs, tuning=dict(cost_volume=1, path_lines=4, branches=1, path_group=16,
+                                                            sweep_band=2, sweep_tile=16))
+    try:
+        b.clear_outputs(0xEE)
+        b.run()
+        b.synchronize()
+        for i in range(len(sizes)):
+            assert_same(refs[i], b.results(i), FIELDS, tag=f"degenerate sweep pair {i} {sizes[i]}")
     finally:
         b.close()

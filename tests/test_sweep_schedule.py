@@ -206,6 +206,12 @@ CASES = [  # w, h, nd, dp, min_d, tile, band, slant, adaptive
     (33, 40, 32, 32, 5, 16, 7, True, False),
     (70, 9, 10, 32, 2, 64, 16, True, True),
     (21, 30, 24, 32, -6, 48, 1, False, False),
+    # degenerate shapes with 2-row bands and 16-column tiles
+    (1, 1, 16, 32, 0, 16, 2, True, True),
+    (1, 40, 32, 32, 0, 16, 2, True, True),
+    (40, 1, 16, 32, 0, 16, 2, True, True),
+    (17, 3, 16, 32, 1, 16, 2, True, True),
+    (9, 33, 16, 32, -2, 16, 2, False, True),
 ]
 
 
