@@ -419,6 +419,7 @@ def main():
     ap.add_argument("--sweep-frames", type=int, default=2000, help="frames per sweep rate")
     ap.add_argument("--sweep-modes", type=str, default="0,-1",
                     help="server compute_streams settings of the sweep (0: one shared compute stream; -1: per slot)")
+    ap.add_argument("--serve-slots", type=int, default=4, help="server slots (requests in flight) of the served legs")
     ap.add_argument("--rect-serve-frames", type=int, default=600,
                     help="frames of the 60 Hz stream of already rectified inputs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -634,14 +635,15 @@ def main():
         raw_pairs, maps = raw_serving_inputs(sgm, speckle_pair, pool)
         served = {}
         if args.serve_frames > 0:
-            served = serve_stream(sgm, 60.0, args.serve_frames, raw_pairs, maps=maps, progress_every=1000)
+            served = serve_stream(sgm, 60.0, args.serve_frames, raw_pairs, maps=maps, progress_every=1000,
+                                  slots=args.serve_slots)
             served["inputs"] = "raw 2560x2048 / 1280x1024 frames, resident rectification maps, read in place"
             log(f"served 60 Hz (raw): {served}")
         sweep = []
         for r in rates:
             for cs in [int(x) for x in args.sweep_modes.split(",") if x.strip()]:
                 sweep.append(serve_stream(sgm, r, args.sweep_frames, raw_pairs, seed=int(r), maps=maps,
-                                          compute_streams=cs))
+                                          compute_streams=cs, slots=args.serve_slots))
                 log(f"served {r:.0f} Hz (raw, compute_streams={cs}): {sweep[-1]}")
         ok = [x for x in sweep if x["p99_ms"] < 5.0 and x["compute_streams"] == 0]
         served["sweep"] = sweep
@@ -650,7 +652,7 @@ def main():
         if args.rect_serve_frames > 0:
             full = [speckle_pair(1280, 1024, 128, seed=7000 + i) for i in range(3)]
             roi = [speckle_pair(640, 512, 64, seed=7100 + i) for i in range(3)]
-            rect = serve_stream(sgm, 60.0, args.rect_serve_frames, (full, roi))
+            rect = serve_stream(sgm, 60.0, args.rect_serve_frames, (full, roi), slots=args.serve_slots)
             log(f"served 60 Hz (rectified inputs, copied at submission): {rect}")
             served["rectified_inputs"] = rect
         pool.free_all()
