@@ -140,12 +140,21 @@ __device__ __forceinline__ void taps(const uint8_t* p, float& a, float& b) {
     b = (float)(v >> 8);
 }
 
+// The left pixel and the fraction of a sample position, for the windowed kernels: floor(xb) clamped
+// into [lo, hi], and xb - floor(xb), wherever xb >= 0, which holds for every sample of a window that
+// is scored (the others' scores are discarded, and their left pixel only has to stay inside
+// [lo, hi]). For xb >= 0 and integers 0 <= lo <= hi, truncating xb clamped into [lo, hi] gives
+// floor(xb) clamped, and v_fract_f32 returns xb - floor(xb), exact for 0 <= xb < 2^24.
+__device__ __forceinline__ int left_pixel(float xb, float lo, float hi, float& f) {
+    f = __builtin_amdgcn_fractf(xb);
+    return (int)__builtin_amdgcn_fmed3f(xb, lo, hi);
+}
+
 // The bilinear sample at xb of row rb, its left pixel clamped into [0, bw - 2].
 __device__ __forceinline__ double sample(const uint8_t* rb, int bw, float xb) {
-    const float fl = __builtin_floorf(xb);
-    const int x0 = (int)__builtin_amdgcn_fmed3f(fl, 0.0f, (float)(bw - 2));
+    float f;
+    const int x0 = left_pixel(xb, 0.0f, (float)(bw - 2), f);
     __builtin_assume(x0 >= 0);
-    const float f = xb - fl;
     float a, b;
     taps(rb + x0, a, b);
     return a * (1.0f - f) + b * f;
@@ -175,9 +184,8 @@ constexpr int kRightCols = 256;
 // The bilinear sample at xb of row `row` (counted from the window's first).
 template <bool TILE>
 __device__ __forceinline__ double sample_row(const RightRows& rr, int row, float xb) {
-    const float fl = __builtin_floorf(xb);
-    const float f = xb - fl;
-    const int x0 = (int)__builtin_amdgcn_fmed3f(fl, rr.lo, rr.hi);
+    float f;
+    const int x0 = left_pixel(xb, rr.lo, rr.hi, f);
     __builtin_assume(x0 >= 0);
     if constexpr (TILE) {
         const RK_LDS float* t = (const RK_LDS float*)(uintptr_t)(rr.t + (uint32_t)(row * kRightCols * 4) + 4u * (uint32_t)x0);
