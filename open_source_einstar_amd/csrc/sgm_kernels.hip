@@ -287,13 +287,21 @@ constexpr int kRectW = 64, kRectH = kBlock / kRectW;
 __device__ __forceinline__ uint32_t remap_sample(const SGM_GLOBAL uint8_t* src, int64_t pitch, int sw, int sh, float sx,
                                                  float sy) {
     // floor(s) in [0, size - 2]  <=>  0 <= s < size - 1 for every finite s; NaN fails both tests.
-    if (!(sx >= 0.0f && sy >= 0.0f && sx < (float)(sw - 1) && sy < (float)(sh - 1))) return 0;
+    // Branch-free: a sample outside reads the 2x2 pixels at (0, 0) and its value is replaced by 0. Each
+    // row's two taps come in one 16-bit load (unaligned where the left pixel's column is odd). A frame
+    // narrower or lower than 2 pixels has no sample inside, and no 2x2 block to read (a uniform test).
+    if (sw < 2 || sh < 2) return 0;
+    const bool in = sx >= 0.0f && sy >= 0.0f && sx < (float)(sw - 1) && sy < (float)(sh - 1);
     const float flx = __builtin_floorf(sx), fly = __builtin_floorf(sy);
     const float fx = sx - flx, fy = sy - fly;
-    const SGM_GLOBAL uint8_t* p = src + (int64_t)(int)fly * pitch + (int)flx;
-    const float a = p[0], b = p[1], c = p[pitch], d = p[pitch + 1];
+    const int ix = in ? (int)flx : 0, iy = in ? (int)fly : 0;
+    const SGM_GLOBAL uint8_t* p = src + (int64_t)iy * pitch + ix;
+    typedef uint16_t u16u __attribute__((aligned(1)));
+    const uint32_t top = *reinterpret_cast<const SGM_GLOBAL u16u*>(p);
+    const uint32_t bot = *reinterpret_cast<const SGM_GLOBAL u16u*>(p + pitch);
+    const float a = (float)(top & 0xFFu), b = (float)(top >> 8), c = (float)(bot & 0xFFu), d = (float)(bot >> 8);
     const float v = (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy;
-    return (uint32_t)__builtin_fminf(v + 0.5f, 255.0f);  // v >= 0: the lower clamp never binds
+    return in ? (uint32_t)__builtin_fminf(v + 0.5f, 255.0f) : 0u;  // v >= 0: the lower clamp never binds
 }
 
 __global__ __launch_bounds__(kBlock) void rectify_kernel(RectifyDesc r) {

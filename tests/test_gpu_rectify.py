@@ -75,6 +75,25 @@ def test_boundary_coordinates():
     np.testing.assert_array_equal(got["rect_right"], O.rectify_half(raw_r, my.T.copy() * (rw / rh), mx.T.copy() * (rh / rw)))
 
 
+@pytest.mark.parametrize("rw,rh", [(1, 30), (40, 1), (1, 1), (2, 2)])
+def test_one_pixel_raw_frames(rw, rh):
+    """Raw frames one pixel wide or tall have no 2x2 neighbourhood: every sample is outside and the
+    rectified images are 0 (a 2x2 frame has one); the kernel reads nothing outside the frame."""
+    mw, mh = 16, 12
+    rng = np.random.default_rng(rw * 100 + rh)
+    mx = rng.uniform(-1, 3, (mh, mw)).astype(np.float32)
+    my = rng.uniform(-1, 3, (mh, mw)).astype(np.float32)
+    raw_l = rng.integers(0, 256, (rh, rw), dtype=np.uint8)
+    raw_r = rng.integers(0, 256, (rh, rw), dtype=np.uint8)
+    with sgm.HipStereo.create(sgm.SgmParams(num_disparities=8), mw // 2, mh // 2) as st:
+        st.set_rectification(mx, my, my, mx, rw, rh)
+        got = st.compute_raw(raw_l, raw_r, rectified=True)
+    np.testing.assert_array_equal(got["rect_left"], O.rectify_half(raw_l, mx, my))
+    np.testing.assert_array_equal(got["rect_right"], O.rectify_half(raw_r, my, mx))
+    if rw < 2 or rh < 2:
+        assert not got["rect_left"].any() and not got["rect_right"].any()
+
+
 def test_full_size_1280x1024x128_from_2560x2048_raw():
     mw, mh, nd = 2560, 2048, 128
     maps = rectification_maps(mw, mh, mw, mh, 11)
