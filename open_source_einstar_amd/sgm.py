@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -716,40 +717,47 @@ def tune_cache_save(path: str) -> int:
 
 
 class PinnedPool:
-    """Reusable pinned host arrays (sgm_host_alloc), so device copies into them are direct DMA."""
+    """Reusable pinned host arrays (sgm_host_alloc), so device copies into them are direct DMA.
+    Thread-safe."""
 
     def __init__(self):
         self._free = {}   # (nbytes) -> [ptr]
         self._all = []
+        self._lock = threading.Lock()
 
     def take(self, shape, dtype) -> np.ndarray:
         nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
-        lst = self._free.get(nbytes)
-        if lst:
-            ptr = lst.pop()
-        else:
+        with self._lock:
+            lst = self._free.get(nbytes)
+            ptr = lst.pop() if lst else None
+        if ptr is None:
             p = C.c_void_p()
             _check(lib().sgm_host_alloc(nbytes, C.byref(p)))
             ptr = p.value
-            self._all.append(ptr)
+            with self._lock:
+                self._all.append(ptr)
         buf = (C.c_uint8 * nbytes).from_address(ptr)
         a = np.frombuffer(buf, dtype=dtype).reshape(shape)
         return a
 
     def give(self, a: np.ndarray):
-        self._free.setdefault(a.nbytes, []).append(a.ctypes.data)
+        with self._lock:
+            self._free.setdefault(a.nbytes, []).append(a.ctypes.data)
 
     def free_all(self):
-        for p in self._all:
+        with self._lock:
+            ptrs, self._all, self._free = self._all, [], {}
+        for p in ptrs:
             lib().sgm_host_free(p)
-        self._all, self._free = [], {}
 
 
 class Server:
     """The serving loop (sgm_server_*): submit host pairs as they arrive, poll completions.
 
     Output arrays live in pinned host memory owned by this object; a completion's arrays stay valid
-    until `release(completion)` returns them to the pool (or the server stops)."""
+    until `release(completion)` returns them to the pool (or the server stops). submit, poll and
+    release may be called from any threads (the library's server is thread-safe; a lock here keeps a
+    ticket's output arrays registered before any poll can return its completion)."""
 
     def __init__(self, params: SgmParams, max_width: int, max_height: int, max_disparities: int, slots: int = 4,
                  hbm_budget_bytes: int = 0, compute_streams: int = 0, path_group: int = 0):
@@ -768,6 +776,7 @@ class Server:
         self._pool = PinnedPool()
         self._rect_sizes = {}  # map set id -> rectified (width, height)
         self._borrowed = {}    # ticket -> input arrays read in place, kept alive until polled
+        self._lock = threading.Lock()  # guards _pending and _borrowed across submit and poll
 
     @staticmethod
     def now_ns() -> int:
@@ -809,14 +818,18 @@ class Server:
         rq.rectification = rectification
         rq.borrow_inputs = int(borrow)
         t = C.c_uint64()
-        rc = lib().sgm_server_submit(self._h, C.byref(rq), C.byref(t))
+        # The completion can be polled as soon as the request is submitted: its outputs are
+        # registered under the lock that poll takes before it looks them up.
+        with self._lock:
+            rc = lib().sgm_server_submit(self._h, C.byref(rq), C.byref(t))
+            if rc == 0:
+                self._pending[t.value] = res
+                if borrow:
+                    self._borrowed[t.value] = (left, right)
         if rc != 0:
             for a in res.values():
                 self._pool.give(a)
             raise SgmError(rc, (lib().sgm_server_last_error(self._h) or b"").decode())
-        self._pending[t.value] = res
-        if borrow:
-            self._borrowed[t.value] = (left, right)
         return t.value
 
     def poll(self, max_n: int = 64, timeout_us: int = 0) -> list:
@@ -825,12 +838,13 @@ class Server:
         n = C.c_int()
         _check(lib().sgm_server_poll(self._h, arr, max_n, timeout_us, C.byref(n)))
         out = []
-        for i in range(n.value):
-            c = arr[i]
-            out.append({"ticket": c.ticket, "tag": c.user_tag, "status": c.status, "t_submit_ns": c.t_submit_ns,
-                        "t_admit_ns": c.t_admit_ns, "t_done_ns": c.t_done_ns, "gpu_ms": c.gpu_ms,
-                        "outputs": self._pending.pop(c.ticket, None)})
-            self._borrowed.pop(c.ticket, None)
+        with self._lock:
+            for i in range(n.value):
+                c = arr[i]
+                out.append({"ticket": c.ticket, "tag": c.user_tag, "status": c.status, "t_submit_ns": c.t_submit_ns,
+                            "t_admit_ns": c.t_admit_ns, "t_done_ns": c.t_done_ns, "gpu_ms": c.gpu_ms,
+                            "outputs": self._pending.pop(c.ticket, None)})
+                self._borrowed.pop(c.ticket, None)
         return out
 
     def release(self, completion: dict):
