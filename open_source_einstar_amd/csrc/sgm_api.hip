@@ -1870,8 +1870,9 @@ extern "C" {
 
 void sgm_batch_destroy(sgm_batch* b) {
     if (!b) return;
-    drop_plan(b);
+    // A run on a caller's stream may still use the graph, its tables and the scratch.
     if (b->last_stream && b->last_stream != b->stream) (void)hipStreamSynchronize(b->last_stream);
+    drop_plan(b);
     dfree(b->d_scratch);
     if (b->order_ev) (void)hipEventDestroy(b->order_ev);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
