@@ -158,8 +158,9 @@ def test_autotuner_times_candidates_and_caches_the_choice(tmp_path):
     assert sgm.tune_cache_save(str(path)) >= 1
     # The key: path count, census window, slant flag, then each pair's width and height in 64-pixel
     # buckets, disparities per lane (p: padded) and median flag, sorted.
-    items = sorted(f"{-(-w // 64)}x{-(-h // 64)}d{-(-nd // 16) if nd > 32 else 2}{'p' if nd % 16 else ''}m"
-                   for w, h, nd in sizes)
+    def dpl(nd):  # the library's lane layouts: 4, 8, 12, 16, 24 or 32 disparities per lane
+        return next(d for d in (4, 8, 12, 16, 24, 32) if 16 * d >= nd)
+    items = sorted(f"{-(-w // 64)}x{-(-h // 64)}d{dpl(nd)}{'p' if 16 * dpl(nd) != nd else ''}m" for w, h, nd in sizes)
     key = "v3:8:4x3:s1;" + ";".join(items)
     assert key + "\t" + " ".join(str(p1[k]) for k in knobs) in path.read_text().splitlines(), (key, path.read_text())
     alt = dict(cost_volume=1 - p1["cost_volume"], path_lines=4, branches=2, path_group=16, sweep_band=0, sweep_tile=64)

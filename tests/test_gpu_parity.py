@@ -93,7 +93,8 @@ LANES = [96, 128, 192, 256, 384, 512]
 
 @pytest.mark.parametrize("nd", LANES)
 def test_every_lane_layout(nd):
-    """One case per disparities-per-lane instantiation above 64 (6, 8, 12, 16, 24, 32)."""
+    """One case per disparities-per-lane layout above 64 (8 with 32 padded disparities, 8, 12, 16, 24,
+    32)."""
     run_case(nd + 40, 24, nd, 20 + nd, volumes=True, eight_paths=True, census_radius_x=4, census_radius_y=3,
              p1=10, p2=120)
 
