@@ -335,7 +335,7 @@ enum {
     SGM_OPT_COST_VOLUME = 1, /* 1 (default): matching costs written once as bytes and read by every
                                 path (W*H*dp bytes of HBM); 0: recomputed from census words in each path */
     SGM_OPT_PATH_LINES = 2,  /* scanlines per path-kernel block: 4, 8 or 16 (default) */
-    SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 16 or 32 (default; used when the
+    SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 16 (default) or 32 (used when the
                                 lane layout allows it -- disparities padded to a multiple of 64 -- with
                                 at most 8 lines per block, else 16) */
     SGM_OPT_RECOMPUTE_DIRS = 4, /* with the cost volume on: how many of the path directions (in the order

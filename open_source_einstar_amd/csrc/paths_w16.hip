@@ -6,6 +6,7 @@ namespace sgm {
 
 hipError_t launch_paths_w16(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
     switch (dpl) {
+        case 2: return launch_paths_variant<2, 16, true>(pl, t, c, stream);
         case 4: return launch_paths_variant<4, 16, true>(pl, t, c, stream);
         case 8: return launch_paths_variant<8, 16, true>(pl, t, c, stream);
         case 12: return launch_paths_variant<12, 16, true>(pl, t, c, stream);

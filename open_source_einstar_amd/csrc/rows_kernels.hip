@@ -3,7 +3,7 @@
 
 namespace sgm {
 
-#define SGM_FOR_DPL(X) X(4) X(8) X(12) X(16) X(24) X(32)
+#define SGM_FOR_DPL(X) X(2) X(4) X(8) X(12) X(16) X(24) X(32)
 
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream) {
     if (t.total_blocks <= 0) return hipSuccess;

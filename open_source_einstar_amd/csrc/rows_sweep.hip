@@ -21,6 +21,7 @@ hipError_t launch_rows_sweep(int dpl, bool padded, const LaunchTable& t, const S
         else by_pad(std::false_type{});
     };
     switch (dpl) {
+        case 2: pick(std::integral_constant<int, 2>{}); break;
         case 4: pick(std::integral_constant<int, 4>{}); break;
         case 8: pick(std::integral_constant<int, 8>{}); break;
         case 12: pick(std::integral_constant<int, 12>{}); break;

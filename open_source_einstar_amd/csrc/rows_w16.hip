@@ -24,6 +24,7 @@ hipError_t launch_rows_w16(int dpl, bool padded, const LaunchTable& t, const Sgm
         else by_dirs(std::false_type{});
     };
     switch (dpl) {
+        case 2: pick(std::integral_constant<int, 2>{}); break;
         case 4: pick(std::integral_constant<int, 4>{}); break;
         case 8: pick(std::integral_constant<int, 8>{}); break;
         case 12: pick(std::integral_constant<int, 12>{}); break;

@@ -162,7 +162,7 @@ void set_geometry(PairDesc& d, int w, int h, int min_d, int nd, int ndirs, int e
     d.h = h;
     d.min_d = min_d;
     d.nd = nd;
-    d.dpl = dpl_for(nd);
+    d.dpl = dpl_for_pair(nd, w, h);
     d.dp = d.dpl * kGroup;
     d.ndirs = ndirs;
     d.lvol_px_stride = (int64_t)ndirs * d.dp * es;
@@ -227,7 +227,7 @@ struct sgm_context {
     void* d_cvol_block = nullptr;
     bool use_cvol = false;
     int path_lines = 16;         // scanlines per path-kernel block
-    int path_group = 32;         // lanes per scanline in the path kernel (32 where the lane layout allows)
+    int path_group = 16;         // lanes per scanline in the path kernel (16 or 32)
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     // Resident rectification (sgm_set_rectification): both maps as interleaved (x, y) floats, and

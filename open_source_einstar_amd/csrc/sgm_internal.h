@@ -184,8 +184,11 @@ int row_blocks(const PairDesc& d);
 int cost8_blocks(int w, int h);
 int median_blocks(int w, int h);
 
-// Disparities per lane for a disparity count, or 0 when unsupported (> 512).
+// Disparities per lane for a disparity count, or 0 when unsupported (> 512): the layout with the
+// largest footprint, which allocations are sized by.
 int dpl_for(int nd);
+// The layout a w x h pair runs with (never wider than dpl_for).
+int dpl_for_pair(int nd, int w, int h);
 // Path costs fit a byte when cells + the largest step penalty (P2, or max(P2, P1 + 1) with adaptive P2)
 // <= 255 and P1 <= 32767 (the byte kernels' 16-bit lane arithmetic adds P1 to a 0x7FFF sentinel);
 // otherwise they are u16, added with saturation where a sum only competes in a minimum.

@@ -331,15 +331,26 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(u32x4* __restrict__ dst, c
 
 }  // namespace
 
-// 2 and 6 disparities per lane are never chosen: their lanes read and write path costs in 2-byte
-// pieces, which measured slower than the next wider layout with the pad masked (1280x1024, 8 paths:
-// path kernel 7.95 us per M px*d at 6 per lane, 2.95 at 8; 12.6 at 2 per lane, 3.8 at 4).
+// 6 disparities per lane is never chosen: its lanes read and write path costs in 2-byte pieces and ran
+// slower than 8 per lane with the pad masked at every size measured (8 paths, D = 96: path kernel
+// 1.00 against 0.49 ms at 1280x1024, 120 against 56 us at 320x256). 2 per lane has the same pieces;
+// it is kept for pairs of at most kSmallPairPx pixels, where the walks are short and their latency
+// rather than the bytes sets the time (D = 32: 34 against 39 us at 320x256, 129 against 103 us at
+// 640x512 for 2 and 4 per lane). dpl_for gives the layout with the largest footprint for a disparity
+// count (what allocations are sized by), dpl_for_pair the one a pair runs with.
+constexpr int64_t kSmallPairPx = 256 * 1024;
+
 int dpl_for(int nd) {
     if (nd <= 0) return 0;
     const int need = (nd + kGroup - 1) / kGroup;
     for (const int d : {4, 8, 12, 16, 24, 32})
         if (d >= need) return d;
     return 0;
+}
+
+int dpl_for_pair(int nd, int w, int h) {
+    if (nd > 0 && nd <= 2 * kGroup && (int64_t)w * h <= kSmallPairPx) return 2;
+    return dpl_for(nd);
 }
 
 void fill_lines(PairDesc& d) {

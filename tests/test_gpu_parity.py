@@ -99,6 +99,25 @@ def test_every_lane_layout(nd):
              p1=10, p2=120)
 
 
+PADDED_LARGE = [(640, 512, 24, False), (800, 600, 96, True), (520, 510, 32, True)]
+
+
+@pytest.mark.parametrize("w,h,nd,eight", PADDED_LARGE, ids=[f"{c[0]}x{c[1]}x{c[2]}" for c in PADDED_LARGE])
+def test_padded_layouts_on_large_pairs(w, h, nd, eight):
+    """Disparity counts whose narrow lane layouts (2 and 6 per lane) are not used on pairs above 256K
+    pixels: D <= 32 runs 4 per lane and D = 96 runs 8 per lane, with the padded disparities masked;
+    with volumes, in both cost modes."""
+    ref, _ = run_case(w, h, nd, 60 + nd, volumes=True, eight_paths=eight, census_radius_x=4, census_radius_y=3,
+                      p1=10, p2=120)
+    pair = speckle_pair(w, h, nd, 60 + nd)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd, eight_paths=eight, census_radius_x=4,
+                  census_radius_y=3, p1=10, p2=120)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, 0)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS)
+    assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} recomputed costs")
+
+
 def test_wide_lanes_second_best_full_window():
     """512 disparities (32 per lane) on an image whose right part sees every disparity, so the row
     kernel's full-window branch picks best disparities in every lane slot (1,051 pixels in slots 30
