@@ -47,17 +47,20 @@ AGG_BYTES_PER_PXD = 38   # canonical aggregation dataflow, 8 paths: W*H*D*(5P-2)
 AGG_BYTES_PER_PX = 8
 
 
-def lane_dp(nd):
-    """Disparities a scanline carries (16 lanes x the smallest of 2,4,6,8,12,16,24,32 per lane >= nd/16)."""
+def lane_dp(nd, w, h):
+    """Disparities a scanline carries, as the library lays them out: 16 lanes x the smallest of
+    4, 8, 12, 16, 24, 32 per lane >= nd/16, or 2 per lane for nd <= 32 on pairs of at most 256K pixels."""
+    if nd <= 32 and w * h <= 256 * 1024:
+        return 32
     need = -(-nd // 16)
-    return 16 * next(d for d in (2, 4, 6, 8, 12, 16, 24, 32) if d >= need)
+    return 16 * next(d for d in (4, 8, 12, 16, 24, 32) if d >= need)
 
 
 def path_kernel_bytes(w, h, nd, paths=8, cost_volume=True):
     """Algorithmic bytes of one path-kernel launch over one pair: per direction each pixel's dp
     matching costs read (cost volume; recomputing them reads the left census word and the right one
     the window slides onto, 16 B), its 4-byte step-penalty word read and its dp path costs written."""
-    dp = lane_dp(nd)
+    dp = lane_dp(nd, w, h)
     per_px_dir = (dp if cost_volume else 16) + 4 + dp
     return paths * w * h * per_px_dir
 
@@ -71,7 +74,7 @@ def dataflow_bytes(w, h, nd, paths=8, cost_volume=True, median=True, sweeps=Fals
     path kernel (2 x (dp cost bytes read + dp path-cost bytes written + 4 penalty bytes) per pixel),
     each of the two sweeps reading dp cost bytes and 8 penalty bytes and writing a 2 dp u16 sum per
     pixel (halo recomputation and seed rows not counted), and the rows reading 6 dp bytes per pixel."""
-    dp = lane_dp(nd)
+    dp = lane_dp(nd, w, h)
     b = w * h * (2 + 16 + 4)
     if cost_volume:
         b += w * h * (16 + dp)
