@@ -292,7 +292,7 @@ def sweep_cases(n, seed):
                       adaptive_p2=bool(rng.random() < 0.5), eight_paths=bool(rng.random() < 0.6),
                       slant_steps=bool(rng.random() < 0.5), uniqueness=float(np.float32(rng.uniform(0.5, 1.0))),
                       lr_max_diff=int(rng.integers(-1, 4)), subpixel=bool(rng.random() < 0.8))
-        knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([4, 8, 16])),
+        knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([0, 4, 8, 16])),
                  sgm.OPT_PATH_GROUP: int(rng.choice([16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9))}
         cases.append((int(rng.integers(1, 301)), int(rng.integers(1, 61)), 1000 + i, params, knobs))
     return cases
