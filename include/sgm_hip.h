@@ -334,8 +334,8 @@ int sgm_reset_stage_totals(sgm_context* ctx);
 enum {
     SGM_OPT_COST_VOLUME = 1, /* 1 (default): matching costs written once as bytes and read by every
                                 path (W*H*dp bytes of HBM); 0: recomputed from census words in each path */
-    SGM_OPT_PATH_LINES = 2,  /* scanlines per path-kernel block: 4, 8 or 16, or 0 (default): chosen per
-                                pair, 4 for 4-path configurations and pairs of at most 512K pixels, else 16 */
+    SGM_OPT_PATH_LINES = 2,  /* scanlines per path-kernel block: 4, 8 or 16, or 0 (default): 4 for
+                                4-path configurations, else 16 */
     SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 16 (default) or 32 (used when the
                                 lane layout allows it -- disparities padded to a multiple of 64 -- with
                                 at most 8 lines per block, else 16) */

@@ -117,12 +117,11 @@ void dfree(void* p) {
 
 // Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
 // Scanlines per path-kernel block when the caller leaves it open (0): one wave per block (4 lines)
-// for 4-path configurations and pairs of at most kFewLinesPx pixels, whose walks are few and long for
-// the card, so that a finished wave's slot takes new work at once; 16 elsewhere (1280x1024, 8 paths:
-// path kernel 0.47-0.49 ms either way; 4 paths 0.28 against 0.30 ms; 640x512, 8 paths 98 against
-// 100 us; 2048x1536x256 2.57 against 2.64 ms).
-constexpr int64_t kFewLinesPx = 512 * 1024;
-int auto_path_lines(const PairDesc& d) { return (d.ndirs == 4 || (int64_t)d.w * d.h <= kFewLinesPx) ? 4 : 16; }
+// for 4-path configurations, whose walks are few and long for the card, so that a finished wave's
+// slot takes new work at once (1280x1024x128: path kernel 0.28 against 0.30 ms with the cost volume,
+// 0.55 against 0.63 ms recomputing the costs); 16 with 8 paths (1280x1024: 0.47-0.49 ms either way;
+// 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the refinement chain: 33 against 41 us).
+int auto_path_lines(const PairDesc& d) { return d.ndirs == 4 ? 4 : 16; }
 
 hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 0,
                         int path_group = 16) {
