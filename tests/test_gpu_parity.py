@@ -118,6 +118,22 @@ def test_padded_layouts_on_large_pairs(w, h, nd, eight):
     assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} recomputed costs")
 
 
+def test_layout_changes_with_the_pair_size_on_one_context():
+    """D <= 32 runs 2 disparities per lane on pairs of at most 256K pixels and 4 per lane above, chosen
+    per call: one context sized for the larger layout computes pairs on both sides of the boundary
+    (512x512 and 513x512), a small one and its maximum size, each against the oracle."""
+    params = dict(min_disparity=4, num_disparities=24, eight_paths=True, census_radius_x=4, census_radius_y=3,
+                  p1=10, p2=120)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), 600, 600) as st:
+        for w, h, seed in ((512, 512, 71), (513, 512, 72), (200, 150, 73), (600, 600, 74), (512, 512, 75)):
+            pair = speckle_pair(w, h, 24, seed)
+            print(f"layout boundary {w}x{h}", flush=True)
+            ref = O.sgm(pair.left, pair.right, O.SgmParams(**params))
+            got = st.compute(pair.left, pair.right, outputs=FIELDS)
+            assert_same(ref, got, FIELDS, tag=f"{w}x{h}")
+            assert (ref["disparity"] > -1e8).mean() > 0.3
+
+
 def test_wide_lanes_second_best_full_window():
     """512 disparities (32 per lane) on an image whose right part sees every disparity, so the row
     kernel's full-window branch picks best disparities in every lane slot (1,051 pixels in slots 30
