@@ -218,6 +218,59 @@ def downsample2(img):
     return ((a[0::2, 0::2] + a[0::2, 1::2] + a[1::2, 0::2] + a[1::2, 1::2] + 2) // 4).astype(np.uint8)
 
 
+def reference_stage(sgm, speckle_pair, steps, warmup, verify=True):
+    """The SGM stage as the reference's stereo front end runs it (libs/pipeline/src/stereo_frontend.cpp:22-30):
+    quarter resolution 320x256, SgmParams defaults (census 5x5, P1 8, P2 96 adaptive, 4 paths, slant steps)
+    with about 247 signed disparities, the configuration of the reference's only published SGM timing
+    (docs/status.md:128-135: census + cost 1.2, paths 7.6, row WTA 1.0 ms on an AMD Radeon Pro 560X).
+    Device-resident inputs, K back-to-back steps; reported beside that figure, not as vs_baseline (other
+    hardware, other metric)."""
+    W, H, D = 320, 256, 247
+    pair = speckle_pair(W, H, D, seed=3)
+    params = sgm.SgmParams(min_disparity=pair.min_disparity, num_disparities=D)
+    st = sgm.HipStereo.create(params, W, H)
+    img = sgm.DeviceBuffer(2 * W * H)
+    img.upload(pair.left, 0)
+    img.upload(pair.right, W * H)
+    outs = {f: sgm.DeviceBuffer(W * H * np.dtype(sgm.OUTPUT_DTYPES[f]).itemsize) for f in OUT_FIELDS}
+    d_out = {f: b.ptr for f, b in outs.items()}
+    try:
+        def step():
+            st.compute_device(img.ptr, img.ptr + W * H, W, W, H, d_out)
+
+        for _ in range(warmup):
+            step()
+        st.synchronize()
+        st.reset_stage_totals()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        st.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        tot = st.stage_totals()
+        n = max(1, tot["count"])
+        out = {"workload": f"{W}x{H}, {D} disparities, 4 paths, census 5x5, P1 8, P2 96 (SgmParams defaults): "
+                           "the reference front end's quarter-resolution SGM",
+               "ms_per_step": round(ms, 4), "value": round(W * H * D / (ms * 1e-3) / 1e6, 0),
+               "unit": "Mpix·disp/s",
+               "stage_ms": {k: round(tot[k] / n, 4) for k in ("census_ms", "paths_ms", "rows_ms", "median_ms")},
+               "reference": {"ms_per_step": 9.8, "value": round(W * H * D / 9.8e-3 / 1e6, 0),
+                             "hardware": "AMD Radeon Pro 560X (Metal), docs/status.md:128-135"}}
+        if verify:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            ref = O.sgm(pair.left, pair.right, O.SgmParams(min_disparity=pair.min_disparity, num_disparities=D))
+            got = {f: b.download((H, W), sgm.OUTPUT_DTYPES[f]) for f, b in outs.items()}
+            out["bit_exact"] = all(np.array_equal(np.asarray(ref[f]).view(np.uint8), got[f].view(np.uint8))
+                                   for f in OUT_FIELDS)
+        return out
+    finally:
+        st.close()
+        img.free()
+        for b in outs.values():
+            b.free()
+
+
 def stereo_chain(sgm, speckle_pair, steps, warmup, verify=True, cpu=True):
     """depth::compute_disparity with the reference defaults on the served frame size: 1280x1024
     rectified pair plus its 2560x2048 full-resolution pair; SGM at 320x256 (D=32), two ZNCC
@@ -619,6 +672,9 @@ def main():
         log(f"batched: {batched}")
         b.close()
 
+    ref_stage = reference_stage(sgm, speckle_pair, args.steps, args.warmup, verify=not args.no_verify)
+    log(f"reference front-end SGM stage: {ref_stage}")
+
     chain = None
     if not args.no_chain:
         chain = stereo_chain(sgm, speckle_pair, args.steps, args.warmup, verify=not args.no_verify, cpu=not args.no_cpu)
@@ -692,6 +748,7 @@ def main():
         "batched": batched,
         "served": served,
         "large_batched": large,
+        "reference_sgm_stage": ref_stage,
         "compute_disparity": chain,
     }
     print(json.dumps(line, ensure_ascii=False), flush=True)
