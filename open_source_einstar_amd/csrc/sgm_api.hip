@@ -115,17 +115,41 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-// Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
-// Scanlines per path-kernel block when the caller leaves it open (0): one wave per block (4 lines)
-// for 4-path configurations, whose walks are few and long for the card, so that a finished wave's
-// slot takes new work at once (1280x1024x128: path kernel 0.28 against 0.30 ms with the cost volume,
-// 0.55 against 0.63 ms recomputing the costs); 16 with 8 paths (1280x1024: 0.47-0.49 ms either way;
-// 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the refinement chain: 33 against 41 us).
-int auto_path_lines(const PairDesc& d) { return d.ndirs == 4 ? 4 : 16; }
+// Lanes per path scanline when the caller leaves it open (0). 32 lanes halve each lane's share of a
+// step (the walk's serial chain) and double the lanes a scanline occupies, so they pay where the walks
+// are few for the card or long per step: 4-path pairs up to about 1M pixels (path kernel 0.66-0.83 of
+// the 16-lane time at 320x256 to 960x768 for D = 128 and 256, about 1.0 at 1280x1024 and above), pairs
+// that recompute the costs of half their directions or more (0.42-0.89 up to 2048x1536), and 8-path
+// pairs of 16 or more disparities per lane (16 lanes a line) up to 640x512 (0.78-0.93). A 32-lane line
+// holds half of those per lane, so it is used only where they are a multiple of 8: with 2 or 6 per lane
+// a lane moves its path costs in 2-byte pieces (D = 192, 12 per lane: 1.9-2.4 times the 16-lane time).
+int auto_path_group(const PairDesc& d, const SgmConsts& c) {
+    if (c.wide_costs || d.dpl % 8 != 0) return 16;
+    const int64_t px = (int64_t)d.w * d.h;
+    const int recomputed = d.cvol ? std::min(std::max(c.cv_from_dir, 0), d.ndirs) : d.ndirs;
+    if (2 * recomputed >= d.ndirs) return 32;
+    if (d.ndirs == 4) return px <= 1024 * 1024 ? 32 : 16;
+    return d.dpl >= 16 && px <= 640 * 512 ? 32 : 16;
+}
 
+// Scanlines per path-kernel block when the caller leaves it open (0). 16 lanes per line: one wave per
+// block (4 lines) for 4-path configurations, whose walks are few and long for the card, so that a
+// finished wave's slot takes new work at once (1280x1024x128: path kernel 0.28 against 0.30 ms with
+// the cost volume, 0.55 against 0.63 ms recomputing the costs); 16 with 8 paths (1280x1024: 0.47-0.49
+// ms either way; 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the refinement chain: 33
+// against 41 us). 32 lanes per line: 8 lines (two waves), except 4 for 4-path pairs of 24 or more
+// disparities per lane of the 16-lane layout (640x512, D = 384 and 512: 198 and 248 against 214 and
+// 284 us).
+int auto_path_lines(const PairDesc& d, int path_group) {
+    if (path_group == 32) return d.ndirs == 4 && d.dpl >= 24 ? 4 : 8;
+    return d.ndirs == 4 ? 4 : 16;
+}
+
+// Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
 hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 0,
-                        int path_group = 16) {
-    if (path_lines <= 0) path_lines = auto_path_lines(d);
+                        int path_group = 0) {
+    if (path_group <= 0) path_group = auto_path_group(d, c);
+    if (path_lines <= 0) path_lines = auto_path_lines(d, path_group);
     // A 32-lane line needs dpl % 4 == 0 and at most 8 lines per 256-thread block.
     if (path_group == 32 && !paths_group_ok(d.dpl, 32, path_lines)) {
         if (d.dpl % 4 == 0 && path_lines == 16) path_lines = 8;
@@ -235,7 +259,7 @@ struct sgm_context {
     void* d_cvol_block = nullptr;
     bool use_cvol = false;
     int path_lines = 0;          // scanlines per path-kernel block (0: auto_path_lines)
-    int path_group = 16;         // lanes per scanline in the path kernel (16 or 32)
+    int path_group = 0;          // lanes per scanline in the path kernel (0: auto_path_group, 16 or 32)
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     // Resident rectification (sgm_set_rectification): both maps as interleaved (x, y) floats, and
@@ -565,8 +589,8 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
             ctx->path_lines = (int)value;
             return SGM_OK;
         case SGM_OPT_PATH_GROUP:
-            if (value != 16 && value != 32)
-                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 16 or 32");
+            if (value != 0 && value != 16 && value != 32)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 0 (chosen per pair), 16 or 32");
             ctx->path_group = (int)value;
             return SGM_OK;
         case SGM_OPT_RECOMPUTE_DIRS:

@@ -147,7 +147,7 @@ int sgm_compute_device_noalloc(const sgm_params* params, const uint8_t* d_left, 
     }
     // Every scratch byte a kernel reads was written earlier in the same call (census pads are read
     // only where the costs they give are replaced by the invalid cost), so nothing is cleared.
-    e = enqueue_pair_ext(d, consts, d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr, 16);
+    e = enqueue_pair_ext(d, consts, d.dp != d.nd, static_cast<hipStream_t>(stream), nullptr, 0);
     if (e != hipSuccess) {
         set_thread_error(std::string("sgm_compute_device_noalloc: launch: ") + hipGetErrorString(e));
         return SGM_ERR_IO;

@@ -180,7 +180,9 @@ struct sgm_server {
 
     // Lanes per path scanline by request shape (width, height, disparities): fixed by the config, or
     // chosen by timing the path kernel of both widths on the first requests of the shape (two each,
-    // the faster minimum wins). Either width gives the same results.
+    // the faster minimum wins). Either width gives the same results. Lane layouts whose disparities
+    // per lane (16 lanes a line) are not a multiple of 8 run 16 lanes untimed: on 32 lanes each lane
+    // would move its path costs in 2-byte pieces.
     int fixed_group = 0;
     struct LaneChoice {
         float best_ms[2] = {0, 0};  // 16, 32 lanes
@@ -192,7 +194,7 @@ struct sgm_server {
         if (fixed_group) return fixed_group;
         LaneChoice& c = lanes[{w, h, nd}];
         if (c.chosen) return c.chosen;
-        const bool can32 = dpl_for(nd) % 4 == 0;
+        const bool can32 = dpl_for_pair(nd, w, h) % 8 == 0;
         if (!can32) return c.chosen = 16;
         return c.samples[0] <= c.samples[1] ? 16 : 32;
     }

@@ -118,6 +118,45 @@ def test_padded_layouts_on_large_pairs(w, h, nd, eight):
     assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} recomputed costs")
 
 
+PER_PAIR_GROUP = [
+    # (w, h, nd, seed, cost_volume, recompute_dirs, params): lanes per scanline left to the library
+    (320, 256, 247, 81, 1, 0, dict()),                      # 4 paths, 256 padded: 32 lanes, 8 lines
+    (300, 200, 256, 82, 1, 0, dict(eight_paths=True)),     # 8 paths, 256 disparities, small: 32 lanes
+    (256, 200, 384, 83, 1, 0, dict()),                      # 4 paths, 384: 32 lanes, 4 lines
+    (400, 300, 128, 84, 0, 0, dict(eight_paths=True)),     # recomputed costs: 32 lanes
+    (400, 300, 128, 85, 1, 4, dict(eight_paths=True)),     # half the directions recompute: 32 lanes
+    (400, 300, 192, 86, 0, 0, dict(eight_paths=True)),     # 12 per lane: 16 lanes
+]
+
+
+@pytest.mark.parametrize("w,h,nd,seed,cv,rd,kw", PER_PAIR_GROUP,
+                         ids=[f"{c[0]}x{c[1]}x{c[2]}-cv{c[4]}-rd{c[5]}" for c in PER_PAIR_GROUP])
+def test_lanes_per_scanline_chosen_per_pair(w, h, nd, seed, cv, rd, kw):
+    """With the scanline width and lines per block left open, each branch of the per-pair choice
+    (32 lanes for small 4-path pairs, for recomputed costs and for wide ranges on small 8-path pairs;
+    16 where the layout holds 12 disparities per lane) gives the oracle's bytes, and so does the
+    16-lane width forced on the same context."""
+    pair = speckle_pair(w, h, min(nd, 256), seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd, census_radius_x=4, census_radius_y=3,
+                  p1=10, p2=120)
+    params.update(kw)
+    volumes = w * h * nd <= 20_000_000
+    print(f"per-pair lanes {w}x{h} D={nd} cv={cv} rd={rd} {kw}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=volumes)
+    fields = FIELDS + (VOLUMES if volumes else [])
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, cv)
+        st.set_option(sgm.OPT_RECOMPUTE_DIRS, rd)
+        got = st.compute(pair.left, pair.right, outputs=fields)
+        assert_same(ref, got, fields, tag=f"{w}x{h}x{nd} lanes chosen per pair")
+        st.set_option(sgm.OPT_PATH_GROUP, 16)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS)
+        assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} 16 lanes")
+        st.set_option(sgm.OPT_PATH_GROUP, 0)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS)
+        assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} lanes chosen per pair again")
+
+
 def test_layout_changes_with_the_pair_size_on_one_context():
     """D <= 32 runs 2 disparities per lane on pairs of at most 256K pixels and 4 per lane above, chosen
     per call: one context sized for the larger layout computes pairs on both sides of the boundary
@@ -309,7 +348,7 @@ def sweep_cases(n, seed):
                       slant_steps=bool(rng.random() < 0.5), uniqueness=float(np.float32(rng.uniform(0.5, 1.0))),
                       lr_max_diff=int(rng.integers(-1, 4)), subpixel=bool(rng.random() < 0.8))
         knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([0, 4, 8, 16])),
-                 sgm.OPT_PATH_GROUP: int(rng.choice([16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9))}
+                 sgm.OPT_PATH_GROUP: int(rng.choice([0, 16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9))}
         cases.append((int(rng.integers(1, 301)), int(rng.integers(1, 61)), 1000 + i, params, knobs))
     return cases
 
