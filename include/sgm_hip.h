@@ -347,11 +347,15 @@ enum {
     SGM_OPT_RECOMPUTE_DIRS = 4, /* with the cost volume on: how many of the path directions (in the order
                                   left-right, right-left, down, up, then the diagonals) recompute their
                                   costs from census words instead of reading them, 0..8 */
-    SGM_OPT_VIEW_BASE_HIGH = 5 /* test aid, value 1: re-places the census words, step penalties and
+    SGM_OPT_VIEW_BASE_HIGH = 5, /* test aid, value 1: re-places the census words, step penalties and
                                   cost volume (the buffers the path kernel reads through 32-bit buffer
                                   views) at addresses whose low 32-bit word has bit 31 set, each inside
                                   an allocation 4 GiB larger than it, so that a sign extension of that
                                   word would show up as wrong results */
+    SGM_OPT_ROW_THREADS = 6  /* threads per row-kernel block (one image row a block): 0 (default) = 1024
+                                for pairs of at most 256 rows, else 256; or 256, or 1024 (512 with 8
+                                paths and 256 or more padded disparities, and with 512; rows of more
+                                than 4096 columns and u16 path costs always run 256) */
 };
 int sgm_set_option(sgm_context* ctx, int option, int64_t value);
 

@@ -20,7 +20,8 @@ hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream) {
+hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream,
+                       int threads) {
     if (t.total_blocks <= 0) return hipSuccess;
     if (c.ndirs != 4 && c.ndirs != 8) return hipErrorInvalidValue;
     if (c.wide_costs) return launch_rows_w16(dpl, padded, t, c, max_w, stream);
@@ -28,17 +29,20 @@ hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmCons
     // only the right WTA keys there (4 bytes per column).
     const bool wide = max_w > kMaxStagedWidth;
     const size_t lds = (size_t)max_w * (wide ? 4 : 15) + 16;
-    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(t.total_blocks), dim3(kBlock), lds, stream, t, c); };
+    const bool big = threads == 1024 && !wide;
+    auto go = [&](auto kern, int nt) { hipLaunchKernelGGL(kern, dim3(t.total_blocks), dim3(nt), lds, stream, t, c); };
     switch (dpl) {
-#define SGM_ROWS(D, PD)                                                                          \
-    do {                                                                                         \
-        if (c.ndirs == 8) {                                                                      \
-            if (wide) go(rows_kernel<D, PD, 8, true>);                                           \
-            else go(rows_kernel<D, PD, 8, false>);                                               \
-        } else {                                                                                 \
-            if (wide) go(rows_kernel<D, PD, 4, true>);                                           \
-            else go(rows_kernel<D, PD, 4, false>);                                               \
-        }                                                                                        \
+#define SGM_ROWS_N(D, PD, N)                                                                      \
+    do {                                                                                          \
+        if (wide) go(rows_kernel<D, PD, N, true>, kBlock);                                        \
+        else if (big) go(rows_kernel<D, PD, N, false, false, false, big_row_threads(D, N)>,      \
+                         big_row_threads(D, N));                                                  \
+        else go(rows_kernel<D, PD, N, false>, kBlock);                                            \
+    } while (0)
+#define SGM_ROWS(D, PD)                                   \
+    do {                                                  \
+        if (c.ndirs == 8) SGM_ROWS_N(D, PD, 8);           \
+        else SGM_ROWS_N(D, PD, 4);                        \
     } while (0)
 #define SGM_CASE(D)                  \
     case D:                          \
@@ -48,6 +52,7 @@ hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmCons
         SGM_FOR_DPL(SGM_CASE)
 #undef SGM_CASE
 #undef SGM_ROWS
+#undef SGM_ROWS_N
         default:
             return hipErrorInvalidValue;
     }

@@ -145,9 +145,18 @@ int auto_path_lines(const PairDesc& d, int path_group) {
     return d.ndirs == 4 ? 4 : 16;
 }
 
+// Threads per row-kernel block when the caller leaves it open (0): the large block (1024 threads, or
+// 512) for pairs of at most kBigRowBlockRows rows -- one row per CU of the MI355X, so the large blocks
+// still run in one round -- where 256-thread blocks would leave most wave slots empty and give each
+// 16-lane group a long serial run of pixels (row kernel 0.53-0.86 of the 256-thread time from 160x120
+// to 4096x200); 256 otherwise (384 rows and more: 1.01-1.08 times the 256-thread time).
+constexpr int kBigRowBlockRows = 256;
+int auto_row_threads(const PairDesc& d) { return d.h <= kBigRowBlockRows ? 1024 : kBlock; }
+
 // Enqueues the stages of one pair (or one launch table). Records stage events when ev != null.
 hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipStream_t s, Stages* st, int path_lines = 0,
-                        int path_group = 0) {
+                        int path_group = 0, int row_threads = 0) {
+    if (row_threads <= 0) row_threads = auto_row_threads(d);
     if (path_group <= 0) path_group = auto_path_group(d, c);
     if (path_lines <= 0) path_lines = auto_path_lines(d, path_group);
     // A 32-lane line needs dpl % 4 == 0 and at most 8 lines per 256-thread block.
@@ -175,7 +184,7 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
     if ((e = launch_paths(d.dpl, padded, t, c, path_lines, path_group, s)) != hipSuccess) return e;
     if (st && (e = hipEventRecord(st->ev[2], s)) != hipSuccess) return e;
     t.total_blocks = row_blocks(d);
-    if ((e = launch_rows(d.dpl, padded, t, c, d.w, s)) != hipSuccess) return e;
+    if ((e = launch_rows(d.dpl, padded, t, c, d.w, s, row_threads)) != hipSuccess) return e;
     if (st && (e = hipEventRecord(st->ev[3], s)) != hipSuccess) return e;
     if (d.median) {
         t.total_blocks = median_blocks(d.w, d.h);
@@ -260,6 +269,7 @@ struct sgm_context {
     bool use_cvol = false;
     int path_lines = 0;          // scanlines per path-kernel block (0: auto_path_lines)
     int path_group = 0;          // lanes per scanline in the path kernel (0: auto_path_group, 16 or 32)
+    int row_threads = 0;         // threads per row-kernel block (0: auto_row_threads, 256 or 1024)
     uint16_t* d_sum = nullptr;   // debug volumes, allocated on first request
     uint16_t* d_cost = nullptr;
     // Resident rectification (sgm_set_rectification): both maps as interleaved (x, y) floats, and
@@ -593,6 +603,11 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
                 return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 0 (chosen per pair), 16 or 32");
             ctx->path_group = (int)value;
             return SGM_OK;
+        case SGM_OPT_ROW_THREADS:
+            if (value != 0 && value != 256 && value != 1024)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "threads per row block must be 0 (chosen per pair), 256 or 1024");
+            ctx->row_threads = (int)value;
+            return SGM_OK;
         case SGM_OPT_RECOMPUTE_DIRS:
             if (value < 0 || value > 8)
                 return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "directions recomputing their costs must be 0..8");
@@ -817,7 +832,7 @@ int sgm_compute_device(sgm_context* ctx, const uint8_t* d_left, const uint8_t* d
     if (d_out && (d_out->census_left || d_out->census_right))
         return ctx->fail(SGM_ERR_UNSUPPORTED, "census outputs are only available from sgm_compute");
     const bool padded = d.dp != d.nd;
-    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx), ctx->path_lines, ctx->path_group);
+    const hipError_t e = enqueue_pair(d, ctx->consts, padded, s, next_stages(ctx), ctx->path_lines, ctx->path_group, ctx->row_threads);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
@@ -940,7 +955,7 @@ static int compute_host(sgm_context* ctx, const HostFrame& f, int width, int hei
     o.sum_volume = out->sum_volume ? ctx->d_sum : nullptr;
     o.cost_volume = out->cost_volume ? ctx->d_cost : nullptr;
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, &o);
-    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group);
+    e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group, ctx->row_threads);
     if (e != hipSuccess) return ctx->hip_fail((w + ": launch").c_str(), e);
     struct Copy {
         void* host;
@@ -1090,7 +1105,7 @@ int sgm_compute_raw_device(sgm_context* ctx, const uint8_t* d_raw_left, const ui
     hipError_t e = enqueue_rectify(ctx, d_raw_left, d_raw_right, raw_pitch, dl, dr, s, st);
     if (e != hipSuccess) return ctx->hip_fail("sgm_compute_raw_device: rectify", e);
     PairDesc d = context_desc(ctx, dl, dr, width, width, height, d_out);
-    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group)) != hipSuccess)
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group, ctx->row_threads)) != hipSuccess)
         return ctx->hip_fail("sgm_compute_raw_device: launch", e);
     ctx->last_stream = s;
     ctx->pending_wall_ms = 0;
@@ -1143,7 +1158,7 @@ static hipError_t enqueue_disparity(sgm_context* ctx, const uint8_t* dl, const u
     o.disparity = ctx->d_disp;
     o.disparity_median = L > 0 ? ctx->d_wm : nullptr;  // refine_level's median3 of the coarse disparity
     PairDesc d = context_desc(ctx, il[L], ir[L], ip[L], lw[L], lh[L], &o);
-    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group)) != hipSuccess) return e;
+    if ((e = enqueue_pair(d, ctx->consts, d.dp != d.nd, s, st, ctx->path_lines, ctx->path_group, ctx->row_threads)) != hipSuccess) return e;
     if (st) {
         if ((e = hipEventRecord(st->ev[6], s)) != hipSuccess) return e;
         st->post = true;

@@ -141,7 +141,11 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
 bool paths_group_ok(int dpl, int group, int lines_per_block);
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, int group,
                         hipStream_t stream);
-hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream);
+// threads: per row block, kBlock or 1024: the large block (1024 threads, 512 for the layouts where
+// those would spill) for rows staged in LDS with byte path costs; other forms run kBlock whatever is
+// asked.
+hipError_t launch_rows(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int max_w, hipStream_t stream,
+                       int threads = kBlock);
 hipError_t launch_median(const LaunchTable& t, hipStream_t stream);
 hipError_t launch_cost8(int dpl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);

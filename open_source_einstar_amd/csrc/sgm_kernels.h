@@ -745,6 +745,9 @@ __device__ __forceinline__ float subpixel_offset(float cm, float c0, float cp) {
 
 // Pixels per 16-lane group whose path costs are loaded ahead: three where the registers allow.
 constexpr int row_ahead(int dpl) { return dpl <= 8 ? 3 : (dpl <= 16 ? 2 : 1); }
+// Threads of the row kernel's large block for a layout: 1024 (at most 128 VGPRs), or 512 where the
+// 1024-thread form spills (8 paths with 16 or more disparities per lane, and 32 per lane).
+constexpr int big_row_threads(int dpl, int ndirs) { return dpl <= 12 || (ndirs == 4 && dpl <= 24) ? 1024 : 512; }
 
 template <int DPL>
 struct RowIn {
@@ -814,9 +817,10 @@ __device__ __forceinline__ uint32_t lane_elem(const uint32_t (&s)[DPL / 2], int 
 // the outputs when computed, and the left/right check reads the disparities back from there.
 // SW: the pixels are band-sweep records [S_down[dp] u16, S_up[dp] u16, L0[dp] u8, L1[dp] u8]
 // (NDIRS = 2 byte directions).
-template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false>
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false, int RT = kBlock>
 __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c, int lb, SGM_LDS uint32_t* smem) {
     static_assert(!SW || (NDIRS == 2 && !W16), "sweep records hold two byte directions");
+    constexpr int kRowGroups = RT / kGroup;  // 16-lane groups of the block
     using In = std::conditional_t<SW, RowInSw<DPL>, RowInT<DPL, W16>>;
     constexpr int ES = W16 ? 2 : 1;  // bytes per stored path cost
     const int W = P.w, y = lb;
@@ -825,7 +829,7 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     SGM_LDS float* drow = (SGM_LDS float*)(smem + 2 * W);
     SGM_LDS uint16_t* lsec = (SGM_LDS uint16_t*)(smem + 3 * W);
     SGM_LDS uint8_t* srow = (SGM_LDS uint8_t*)(smem + 3 * W) + 2 * W;
-    for (int x = threadIdx.x; x < W; x += kBlock) rkey[x] = 0xFFFFFFFFu;
+    for (int x = threadIdx.x; x < W; x += RT) rkey[x] = 0xFFFFFFFFu;
     __syncthreads();
 
     const int grp = threadIdx.x >> 4, g = threadIdx.x & (kGroup - 1);
@@ -974,32 +978,32 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
         }
     };
 
-    // Pixels grp, grp + 16, ...: a ring of kRowAhead pixels' path costs is in flight while the
+    // Pixels grp, grp + kRowGroups, ...: a ring of kRowAhead pixels' path costs is in flight while the
     // previous ring is computed; the ring's pixels are computed together (their work overlaps),
     // the slots refilled, then the pixels committed. Reads past the row's end are clamped to its
     // last pixel and never used.
     constexpr int kRowAhead = row_ahead(W16 ? 2 * DPL : DPL);
     In ring[kRowAhead];
 #pragma unroll
-    for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(grp + j * kLinesPerBlock, W - 1));
-    for (int x = grp; x < W; x += kRowAhead * kLinesPerBlock) {
+    for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(grp + j * kRowGroups, W - 1));
+    for (int x = grp; x < W; x += kRowAhead * kRowGroups) {
         PxOut o[kRowAhead];
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
-            const int xj = x + j * kLinesPerBlock;
+            const int xj = x + j * kRowGroups;
             if (xj < W) compute(ring[j], xj, o[j]);
         }
 #pragma unroll
-        for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(x + (j + kRowAhead) * kLinesPerBlock, W - 1));
+        for (int j = 0; j < kRowAhead; ++j) load_in(ring[j], min(x + (j + kRowAhead) * kRowGroups, W - 1));
 #pragma unroll
         for (int j = 0; j < kRowAhead; ++j) {
-            const int xj = x + j * kLinesPerBlock;
+            const int xj = x + j * kRowGroups;
             if (xj < W) commit(o[j], xj);
         }
     }
     __syncthreads();
     // Left/right consistency, stereo.cpp:450-461 (lround: halves away from zero).
-    for (int x = threadIdx.x; x < W; x += kBlock) {
+    for (int x = threadIdx.x; x < W; x += RT) {
         const uint32_t rk = rkey[x];
         float d;
         uint8_t st = 0;
@@ -1035,13 +1039,15 @@ __device__ __forceinline__ void rows_body(const PairDesc& P, const SgmConsts& c,
     }
 }
 
-template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false>
-__global__ __launch_bounds__(kBlock) void rows_kernel(LaunchTable t, SgmConsts c) {
+// RT threads per block, one image row a block: 256, or 1024 for pairs of few rows (four times the
+// waves on a CU and a quarter of each group's pixels).
+template <int DPL, bool PAD, int NDIRS, bool WIDE, bool W16 = false, bool SW = false, int RT = kBlock>
+__global__ __launch_bounds__(RT) void rows_kernel(LaunchTable t, SgmConsts c) {
     extern __shared__ uint32_t smem[];
     int pi, lb;
     locate(t, blockIdx.x, pi, lb);
     const PairDesc P = load_desc(t, pi);
-    rows_body<DPL, PAD, NDIRS, WIDE, W16, SW>(P, c, lb, lds(smem));
+    rows_body<DPL, PAD, NDIRS, WIDE, W16, SW, RT>(P, c, lb, lds(smem));
 }
 
 

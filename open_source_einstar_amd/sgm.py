@@ -29,6 +29,7 @@ OPT_PATH_LINES = 2
 OPT_PATH_GROUP = 3
 OPT_RECOMPUTE_DIRS = 4
 OPT_VIEW_BASE_HIGH = 5
+OPT_ROW_THREADS = 6
 
 ERRC = {0: "ok", 1: "not_found", 2: "io", 3: "timeout", 4: "protocol", 5: "invalid_argument", 6: "blocked",
         7: "unsupported", 8: "busy", 9: "disconnected"}

@@ -157,6 +157,37 @@ def test_lanes_per_scanline_chosen_per_pair(w, h, nd, seed, cv, rd, kw):
         assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} lanes chosen per pair again")
 
 
+ROW_BLOCKS = [
+    # (w, h, nd, seed, params): large row blocks of 1024 threads, or 512 where 1024 would spill
+    (320, 256, 247, 91, dict()),                                       # 16 per lane, 4 paths: 1024
+    (300, 120, 256, 92, dict(eight_paths=True)),                       # 16 per lane, 8 paths: 512
+    (200, 64, 512, 93, dict()),                                        # 32 per lane: 512
+    (257, 200, 96, 94, dict(eight_paths=True, min_disparity=-9)),      # 8 per lane, padded
+    (40, 300, 32, 95, dict(eight_paths=True)),                         # fewer columns than groups, 300 rows
+    (4096, 20, 64, 96, dict(eight_paths=True)),                        # the widest staged rows
+    (4200, 8, 64, 97, dict(eight_paths=True)),                         # wider rows run 256 threads whatever is asked
+]
+
+
+@pytest.mark.parametrize("w,h,nd,seed,kw", ROW_BLOCKS, ids=[f"{c[0]}x{c[1]}x{c[2]}" for c in ROW_BLOCKS])
+def test_row_block_sizes(w, h, nd, seed, kw):
+    """Every threads-per-row-block setting (chosen per pair, 256, the large block) gives the oracle's
+    bytes, sums included, on one context."""
+    pair = speckle_pair(w, h, min(nd, 256), seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd, census_radius_x=4, census_radius_y=3,
+                  p1=10, p2=120)
+    params.update(kw)
+    volumes = w * h * nd <= 20_000_000
+    fields = FIELDS + (VOLUMES if volumes else [])
+    print(f"row blocks {w}x{h} D={nd} {kw}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=volumes)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        for rt in (0, 256, 1024, 0):
+            st.set_option(sgm.OPT_ROW_THREADS, rt)
+            got = st.compute(pair.left, pair.right, outputs=fields)
+            assert_same(ref, got, fields, tag=f"{w}x{h}x{nd} row threads {rt}")
+
+
 def test_layout_changes_with_the_pair_size_on_one_context():
     """D <= 32 runs 2 disparities per lane on pairs of at most 256K pixels and 4 per lane above, chosen
     per call: one context sized for the larger layout computes pairs on both sides of the boundary
@@ -348,7 +379,8 @@ def sweep_cases(n, seed):
                       slant_steps=bool(rng.random() < 0.5), uniqueness=float(np.float32(rng.uniform(0.5, 1.0))),
                       lr_max_diff=int(rng.integers(-1, 4)), subpixel=bool(rng.random() < 0.8))
         knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([0, 4, 8, 16])),
-                 sgm.OPT_PATH_GROUP: int(rng.choice([0, 16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9))}
+                 sgm.OPT_PATH_GROUP: int(rng.choice([0, 16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9)),
+                 sgm.OPT_ROW_THREADS: int(rng.choice([0, 256, 1024]))}
         cases.append((int(rng.integers(1, 301)), int(rng.integers(1, 61)), 1000 + i, params, knobs))
     return cases
 
