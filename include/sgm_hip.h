@@ -336,14 +336,17 @@ enum {
                                 path (W*H*dp bytes of HBM); 0: recomputed from census words in each path */
     SGM_OPT_PATH_LINES = 2,  /* scanlines per path-kernel block: 4, 8 or 16, or 0 (default): with 16
                                 lanes per scanline 4 for 4-path configurations, else 16; with 32 lanes 8,
-                                or 4 for 4-path pairs of 384 or more padded disparities */
+                                or 4 for 4-path pairs of 384 or more padded disparities; with 64, 4 */
     SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 0 (default) = chosen per pair
-                                (32 where the padded disparity count is a multiple of 128 and the pair has
-                                4 paths and at most 1M pixels, recomputes the costs of half its directions
-                                or more, or has 8 paths, 256 or more padded disparities and at most
-                                640x512 pixels; else 16), or 16, or 32 (used when the lane layout allows
-                                it -- padded disparities a multiple of 64 -- with at most 8 lines per
-                                block, else 16) */
+                                (64 where the padded disparity count is a multiple of 256 and above 384,
+                                or a multiple of 256 with half the directions or more recomputing their
+                                costs; else 32 where it is a multiple of 128 and the pair recomputes half
+                                its directions or more, has 4 paths and at most 1M pixels, or has 8
+                                paths, 256 or more padded disparities and at most 640x512 pixels; else
+                                16), or 16, or 32 (used when the lane layout allows it -- padded
+                                disparities a multiple of 64 -- with at most 8 lines per block, else 16),
+                                or 64 (used with 4 lines per block where the padded disparities are a
+                                multiple of 256, else as 32) */
     SGM_OPT_RECOMPUTE_DIRS = 4, /* with the cost volume on: how many of the path directions (in the order
                                   left-right, right-left, down, up, then the diagonals) recompute their
                                   costs from census words instead of reading them, 0..8 */

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wextra", "-Wno-unused-parameter",
 ]
 
-HIP_SOURCES = ["sgm_kernels.hip", "paths_g16_lo.hip", "paths_g16_hi.hip", "paths_g32.hip", "paths_w16.hip",
+HIP_SOURCES = ["sgm_kernels.hip", "paths_g16_lo.hip", "paths_g16_hi.hip", "paths_g32.hip", "paths_g64.hip", "paths_w16.hip",
                "rows_kernels.hip", "rows_w16.hip", "sweep_kernels.hip", "rows_sweep.hip",
                "refine_kernels.hip", "frame_kernels.hip", "sgm_api.hip", "sgm_server.hip", "sgm_noalloc.hip"]
 OBJ_DIR = os.path.join(PKG, "_build")

@@ -115,19 +115,24 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-// Lanes per path scanline when the caller leaves it open (0). 32 lanes halve each lane's share of a
-// step (the walk's serial chain) and double the lanes a scanline occupies, so they pay where the walks
-// are few for the card or long per step: 4-path pairs up to about 1M pixels (path kernel 0.66-0.83 of
-// the 16-lane time at 320x256 to 960x768 for D = 128 and 256, about 1.0 at 1280x1024 and above), pairs
-// that recompute the costs of half their directions or more (0.42-0.89 up to 2048x1536), and 8-path
-// pairs of 16 or more disparities per lane (16 lanes a line) up to 640x512 (0.78-0.93). A 32-lane line
-// holds half of those per lane, so it is used only where they are a multiple of 8: with 2 or 6 per lane
-// a lane moves its path costs in 2-byte pieces (D = 192, 12 per lane: 1.9-2.4 times the 16-lane time).
+// Lanes per path scanline when the caller leaves it open (0). Wider lines cut each lane's share of a
+// step (the walk's serial chain) and multiply the lanes a scanline occupies, so they pay where the
+// walks are few for the card or long per step. 64 lanes (4 or 8 disparities per lane): layouts of 32
+// per 16-lane lane (D > 384; path kernel 0.54-0.87 of the 16-lane time from 320x256 to 2048x1536),
+// and layouts of 16 when half the directions or more recompute their costs (0.45-0.76; 0.77-1.04 of
+// the 32-lane time). 32 lanes: other layouts of a multiple of 8 per lane that recompute (0.42-0.89),
+// 4-path pairs up to about 1M pixels (0.66-0.83 at 320x256 to 960x768 for D = 128 and 256, about 1.0
+// at 1280x1024 and above), and 8-path pairs of 16 or more per lane up to 640x512 (0.78-0.93). With 2
+// or 6 per lane a lane would move its path costs in 2-byte pieces (D = 192, 12 per lane, on 32 lanes:
+// 1.9-2.4 times the 16-lane time), so wider lines need a multiple of 8 per 16-lane lane (32 lanes) or
+// of 16 (64 lanes).
 int auto_path_group(const PairDesc& d, const SgmConsts& c) {
     if (c.wide_costs || d.dpl % 8 != 0) return 16;
     const int64_t px = (int64_t)d.w * d.h;
     const int recomputed = d.cvol ? std::min(std::max(c.cv_from_dir, 0), d.ndirs) : d.ndirs;
-    if (2 * recomputed >= d.ndirs) return 32;
+    const bool mostly_recomputed = 2 * recomputed >= d.ndirs;
+    if (d.dpl % 16 == 0 && (d.dpl >= 32 || mostly_recomputed)) return 64;
+    if (mostly_recomputed) return 32;
     if (d.ndirs == 4) return px <= 1024 * 1024 ? 32 : 16;
     return d.dpl >= 16 && px <= 640 * 512 ? 32 : 16;
 }
@@ -139,8 +144,9 @@ int auto_path_group(const PairDesc& d, const SgmConsts& c) {
 // ms either way; 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the refinement chain: 33
 // against 41 us). 32 lanes per line: 8 lines (two waves), except 4 for 4-path pairs of 24 or more
 // disparities per lane of the 16-lane layout (640x512, D = 384 and 512: 198 and 248 against 214 and
-// 284 us).
+// 284 us). 64 lanes: 4 lines (one a wave).
 int auto_path_lines(const PairDesc& d, int path_group) {
+    if (path_group == 64) return 4;
     if (path_group == 32) return d.ndirs == 4 && d.dpl >= 24 ? 4 : 8;
     return d.ndirs == 4 ? 4 : 16;
 }
@@ -159,7 +165,12 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
     if (row_threads <= 0) row_threads = auto_row_threads(d);
     if (path_group <= 0) path_group = auto_path_group(d, c);
     if (path_lines <= 0) path_lines = auto_path_lines(d, path_group);
-    // A 32-lane line needs dpl % 4 == 0 and at most 8 lines per 256-thread block.
+    // A 64-lane line needs dpl % 16 == 0 and 4 lines per block, else it runs 32 lanes; a 32-lane line
+    // needs dpl % 4 == 0 and at most 8 lines per block, else it runs 16.
+    if (path_group == 64 && !paths_group_ok(d.dpl, 64, path_lines)) {
+        if (d.dpl % 16 == 0) path_lines = 4;
+        else path_group = 32;
+    }
     if (path_group == 32 && !paths_group_ok(d.dpl, 32, path_lines)) {
         if (d.dpl % 4 == 0 && path_lines == 16) path_lines = 8;
         else path_group = 16;
@@ -599,8 +610,8 @@ int sgm_set_option(sgm_context* ctx, int option, int64_t value) {
             ctx->path_lines = (int)value;
             return SGM_OK;
         case SGM_OPT_PATH_GROUP:
-            if (value != 0 && value != 16 && value != 32)
-                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 0 (chosen per pair), 16 or 32");
+            if (value != 0 && value != 16 && value != 32 && value != 64)
+                return ctx->fail(SGM_ERR_INVALID_ARGUMENT, "lanes per path scanline must be 0 (chosen per pair), 16, 32 or 64");
             ctx->path_group = (int)value;
             return SGM_OK;
         case SGM_OPT_ROW_THREADS:

@@ -137,7 +137,8 @@ struct PointsDesc {
 // Host-side launchers (sgm_kernels.hip). All enqueue on `stream` and return the launch status.
 hipError_t launch_census(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
-// group: lanes per scanline, 16 (any dpl, 4/8/16 lines per block) or 32 (dpl % 4 == 0, 4/8 lines).
+// group: lanes per scanline, 16 (any dpl, 4/8/16 lines per block), 32 (dpl % 4 == 0, 4/8 lines) or 64
+// (dpl % 16 == 0, 4 lines).
 bool paths_group_ok(int dpl, int group, int lines_per_block);
 hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmConsts& c, int lines_per_block, int group,
                         hipStream_t stream);

@@ -430,7 +430,7 @@ __device__ __forceinline__ void store_out(SGM_GLOBAL uint8_t* dst, const PathOut
     else store_packed<DPL>(dst, o);
 }
 
-// Edge handling of a 32-lane line: a wave shift brings the neighbouring line's value (or zero) into
+// Edge handling of a 32- or 64-lane line: a wave shift brings the neighbouring line's value (or zero) into
 // the line's end lanes, which keep = 0 / sent = sentinel replace. (A select would let the compiler run the
 // shift with the end lanes disabled, and a shift never reads a disabled lane's value.)
 struct LineEdge {
@@ -438,7 +438,8 @@ struct LineEdge {
 };
 
 // One step of the recurrence (stereo.cpp:62-81) on the lane's packed pairs. A line spans G = 16
-// lanes (one DPP row) or 32 lanes (two rows, exchanging across the row boundary by wave shifts).
+// lanes (one DPP row), 32 lanes (two rows, exchanging across the row boundary by wave shifts) or 64
+// (the whole wave, likewise).
 // pmpm holds the previous step's minimum in both halves, jj = pmpm + p2 in both halves.
 //
 // With slant steps the P1 neighbourhood of k is {k-2, k-1, k+1, k+2}, which is min(a(k-2), a(k+1))
@@ -495,14 +496,19 @@ __device__ __forceinline__ void path_step(uint32_t (&prev)[DPL / 2], const uint3
 }
 
 // Minimum over the line's lanes of both halves of mn, in both halves of the result; every lane of
-// the line receives it. (With equal halves, u32 order is the order of the halves.) Two rows (G = 32)
-// pair up as 0-1 and 2-3 through permlane16_swap.
+// the line receives it. (With equal halves, u32 order is the order of the halves.) Rows pair up as 0-1
+// and 2-3 through permlane16_swap (G = 32, 64), then the wave's halves through permlane32_swap (G = 64).
 template <int G>
 __device__ __forceinline__ uint32_t pair_min_line(uint32_t mn) {
+    static_assert(G == 16 || G == 32 || G == 64, "a line spans 16, 32 or 64 lanes");
     const u16x2 h = __builtin_bit_cast(u16x2, mn);
     uint32_t v = row_min(__builtin_bit_cast(uint32_t, __builtin_elementwise_min(h, h.yx)));
-    if constexpr (G == 32) {
+    if constexpr (G >= 32) {
         const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min((uint32_t)r[0], (uint32_t)r[1]);
+    }
+    if constexpr (G == 64) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
         v = min((uint32_t)r[0], (uint32_t)r[1]);
     }
     return v;
@@ -1082,11 +1088,12 @@ hipError_t launch_paths_variant(const PathLaunch& pl, const LaunchTable& t, cons
 }  // namespace
 
 // Path-kernel launchers by lane layout, one translation unit each: 16 lanes per scanline with
-// dpl <= 8 (paths_g16_lo.hip) or dpl >= 12 (paths_g16_hi.hip), and 32 lanes (paths_g32.hip; dpl is
-// the 16-lane layout's, each lane holding half of it).
+// dpl <= 8 (paths_g16_lo.hip) or dpl >= 12 (paths_g16_hi.hip), 32 lanes (paths_g32.hip; dpl is
+// the 16-lane layout's, each lane holding half of it) and 64 lanes (paths_g64.hip; a quarter).
 hipError_t launch_paths_g16_lo(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_paths_g16_hi(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 hipError_t launch_paths_g32(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
+hipError_t launch_paths_g64(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);
 // u16 path costs (SgmConsts::wide_costs), 16 lanes per scanline (paths_w16.hip), and their row kernels
 // (rows_w16.hip).
 hipError_t launch_paths_w16(int dpl, const PathLaunch& pl, const LaunchTable& t, const SgmConsts& c, hipStream_t stream);

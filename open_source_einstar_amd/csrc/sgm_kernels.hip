@@ -389,6 +389,7 @@ hipError_t launch_cost_volume(const LaunchTable& t, const SgmConsts& c, hipStrea
 
 bool paths_group_ok(int dpl, int group, int lines_per_block) {
     if (group == 16) return lines_per_block == 4 || lines_per_block == 8 || lines_per_block == 16;
+    if (group == 64) return dpl % 16 == 0 && lines_per_block == 4;
     return group == 32 && dpl % 4 == 0 && (lines_per_block == 4 || lines_per_block == 8);
 }
 
@@ -401,7 +402,8 @@ hipError_t launch_paths(int dpl, bool padded, const LaunchTable& t, const SgmCon
                         (unsigned)(lines_per_block * group)};
     if (c.wide_costs) return launch_paths_w16(dpl, pl, t, c, stream);
     if (group == 16) return dpl <= 8 ? launch_paths_g16_lo(dpl, pl, t, c, stream) : launch_paths_g16_hi(dpl, pl, t, c, stream);
-    return launch_paths_g32(dpl, pl, t, c, stream);
+    if (group == 32) return launch_paths_g32(dpl, pl, t, c, stream);
+    return launch_paths_g64(dpl, pl, t, c, stream);
 }
 
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {

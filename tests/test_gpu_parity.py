@@ -126,6 +126,8 @@ PER_PAIR_GROUP = [
     (400, 300, 128, 84, 0, 0, dict(eight_paths=True)),     # recomputed costs: 32 lanes
     (400, 300, 128, 85, 1, 4, dict(eight_paths=True)),     # half the directions recompute: 32 lanes
     (400, 300, 192, 86, 0, 0, dict(eight_paths=True)),     # 12 per lane: 16 lanes
+    (600, 40, 512, 87, 1, 0, dict()),                       # 32 per lane: 64 lanes
+    (300, 200, 256, 88, 0, 0, dict()),                      # 16 per lane, recomputed costs: 64 lanes
 ]
 
 
@@ -133,9 +135,10 @@ PER_PAIR_GROUP = [
                          ids=[f"{c[0]}x{c[1]}x{c[2]}-cv{c[4]}-rd{c[5]}" for c in PER_PAIR_GROUP])
 def test_lanes_per_scanline_chosen_per_pair(w, h, nd, seed, cv, rd, kw):
     """With the scanline width and lines per block left open, each branch of the per-pair choice
-    (32 lanes for small 4-path pairs, for recomputed costs and for wide ranges on small 8-path pairs;
-    16 where the layout holds 12 disparities per lane) gives the oracle's bytes, and so does the
-    16-lane width forced on the same context."""
+    (64 lanes for 32 disparities per lane and for recomputed costs at 16; 32 lanes for small 4-path
+    pairs, for recomputed costs and for wide ranges on small 8-path pairs; 16 where the layout holds 12
+    disparities per lane) gives the oracle's bytes, and so does the 16-lane width forced on the same
+    context."""
     pair = speckle_pair(w, h, min(nd, 256), seed)
     params = dict(min_disparity=pair.min_disparity, num_disparities=nd, census_radius_x=4, census_radius_y=3,
                   p1=10, p2=120)
@@ -363,6 +366,35 @@ def test_32_lane_scanlines_bit_exact(w, h, nd, seed, kw, cv):
             assert_same(ref, got, FIELDS + VOLUMES, tag=f"g32 {w}x{h}x{nd} lines {lines}")
 
 
+G64_CASES = [(300, 20, 256, 101, dict(eight_paths=True)), (280, 24, 247, 102, dict()),
+             (560, 12, 512, 103, dict(eight_paths=True)), (520, 10, 450, 104, dict(slant_steps=False)),
+             (200, 40, 200, 105, dict(eight_paths=True, min_disparity=-30, census_radius_x=4, census_radius_y=3,
+                                      p1=10, p2=120)),
+             (96, 64, 128, 106, dict(eight_paths=True)),  # 8 per 16-lane lane: runs 32 lanes
+             (120, 30, 192, 107, dict())]                 # 12 per 16-lane lane: runs 16 lanes
+
+
+@pytest.mark.parametrize("cv", [0, 1], ids=["recompute", "volume"])
+@pytest.mark.parametrize("w,h,nd,seed,kw", G64_CASES, ids=[f"g64-{c[0]}x{c[1]}x{c[2]}" for c in G64_CASES])
+def test_64_lane_scanlines_bit_exact(w, h, nd, seed, kw, cv):
+    """Scanlines spread over the whole wave (64 lanes, exchanging across the DPP rows and the wave's
+    halves) give the same bytes as the oracle for the layouts that allow them (16 and 32 disparities per
+    16-lane lane, padded or not), whatever lines per block are asked for; other layouts run 32 or 16
+    lanes instead."""
+    pair = speckle_pair(w, h, min(nd, 256), seed)
+    params = dict(min_disparity=pair.min_disparity, num_disparities=nd)
+    params.update(kw)
+    print(f"g64 {w}x{h} D={nd} cv={cv} {params}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), volumes=True)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, cv)
+        st.set_option(sgm.OPT_PATH_GROUP, 64)
+        for lines in (0, 4, 16):
+            st.set_option(sgm.OPT_PATH_LINES, lines)
+            got = st.compute(pair.left, pair.right, outputs=FIELDS + VOLUMES)
+            assert_same(ref, got, FIELDS + VOLUMES, tag=f"g64 {w}x{h}x{nd} lines {lines}")
+
+
 def sweep_cases(n, seed):
     """Seeded random cases over the whole accepted parameter space and every tuning knob: sizes
     1..300 x 1..60, 1..512 disparities, census windows 1x1..9x7, P1/P2 up to the byte bound (P1 > P2
@@ -379,7 +411,7 @@ def sweep_cases(n, seed):
                       slant_steps=bool(rng.random() < 0.5), uniqueness=float(np.float32(rng.uniform(0.5, 1.0))),
                       lr_max_diff=int(rng.integers(-1, 4)), subpixel=bool(rng.random() < 0.8))
         knobs = {sgm.OPT_COST_VOLUME: int(rng.integers(0, 2)), sgm.OPT_PATH_LINES: int(rng.choice([0, 4, 8, 16])),
-                 sgm.OPT_PATH_GROUP: int(rng.choice([0, 16, 32])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9)),
+                 sgm.OPT_PATH_GROUP: int(rng.choice([0, 16, 32, 64])), sgm.OPT_RECOMPUTE_DIRS: int(rng.integers(0, 9)),
                  sgm.OPT_ROW_THREADS: int(rng.choice([0, 256, 1024]))}
         cases.append((int(rng.integers(1, 301)), int(rng.integers(1, 61)), 1000 + i, params, knobs))
     return cases
