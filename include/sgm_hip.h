@@ -474,8 +474,9 @@ typedef struct {
                                       k in 1..slots: k shared compute streams; -1: each slot computes on
                                       its own stream (requests' kernels share the GPU) */
     int32_t path_group;            /* lanes per path scanline: 0 (default) = chosen per request shape by
-                                      timing 16 and 32 lanes on the first requests of that shape; 16 or 32
-                                      = fixed. Results never depend on it. */
+                                      timing 16, 32 and 64 lanes (those the shape's lane layout allows) on
+                                      the first requests of that shape; 16, 32 or 64 = fixed (as
+                                      SGM_OPT_PATH_GROUP). Results never depend on it. */
 } sgm_server_config;
 
 typedef struct {

@@ -179,14 +179,16 @@ struct sgm_server {
     bool broken = false;  // a stream could not be drained: nothing new is admitted
 
     // Lanes per path scanline by request shape (width, height, disparities): fixed by the config, or
-    // chosen by timing the path kernel of both widths on the first requests of the shape (two each,
-    // the faster minimum wins). Either width gives the same results. Lane layouts whose disparities
-    // per lane (16 lanes a line) are not a multiple of 8 run 16 lanes untimed: on 32 lanes each lane
-    // would move its path costs in 2-byte pieces.
+    // chosen by timing the path kernel at each width the lane layout allows on the first requests of
+    // the shape (two each, the fastest minimum wins). Every width gives the same results. 32 lanes
+    // need a multiple of 8 disparities per lane of the 16-lane layout, 64 lanes a multiple of 16:
+    // otherwise each lane would move its path costs in 2-byte pieces.
     int fixed_group = 0;
+    static constexpr int kWidths[3] = {16, 32, 64};
     struct LaneChoice {
-        float best_ms[2] = {0, 0};  // 16, 32 lanes
-        int samples[2] = {0, 0};
+        float best_ms[3] = {0, 0, 0};
+        int samples[3] = {0, 0, 0};
+        int allowed = 1;  // widths kWidths[0 .. allowed - 1] are timed
         int chosen = 0;
     };
     std::map<std::tuple<int, int, int>, LaneChoice> lanes;
@@ -194,17 +196,27 @@ struct sgm_server {
         if (fixed_group) return fixed_group;
         LaneChoice& c = lanes[{w, h, nd}];
         if (c.chosen) return c.chosen;
-        const bool can32 = dpl_for_pair(nd, w, h) % 8 == 0;
-        if (!can32) return c.chosen = 16;
-        return c.samples[0] <= c.samples[1] ? 16 : 32;
+        const int dpl = dpl_for_pair(nd, w, h);
+        c.allowed = dpl % 16 == 0 ? 3 : (dpl % 8 == 0 ? 2 : 1);
+        if (c.allowed == 1) return c.chosen = 16;
+        int i = 0;  // the least sampled width
+        for (int k = 1; k < c.allowed; ++k)
+            if (c.samples[k] < c.samples[i]) i = k;
+        return kWidths[i];
     }
     void record_group(int w, int h, int nd, int group, float paths_ms) {  // with mu held
         LaneChoice& c = lanes[{w, h, nd}];
         if (c.chosen || paths_ms <= 0) return;
-        const int i = group == 32 ? 1 : 0;
+        const int i = group == 64 ? 2 : (group == 32 ? 1 : 0);
+        if (i >= c.allowed) return;
         c.best_ms[i] = c.samples[i] ? std::min(c.best_ms[i], paths_ms) : paths_ms;
         ++c.samples[i];
-        if (c.samples[0] >= 2 && c.samples[1] >= 2) c.chosen = c.best_ms[1] < c.best_ms[0] ? 32 : 16;
+        for (int k = 0; k < c.allowed; ++k)
+            if (c.samples[k] < 2) return;
+        int best = 0;
+        for (int k = 1; k < c.allowed; ++k)
+            if (c.best_ms[k] < c.best_ms[best]) best = k;
+        c.chosen = kWidths[best];
     }
 
     void dispatch(const std::vector<serve::Admission>& adm) {  // with mu held
@@ -438,9 +450,9 @@ int sgm_server_start(const sgm_params* params, const sgm_server_config* cfg, sgm
     pmax.num_disparities = cfg->max_disparities;
     if (validate_params_ext(&pmax, err) != SGM_OK || cfg->max_width < 1 || cfg->max_width > kMaxWidth || cfg->max_height < 1 ||
         cfg->slots < 1 || cfg->slots > 64 || cfg->compute_streams < -1 || cfg->compute_streams > cfg->slots ||
-        (cfg->path_group != 0 && cfg->path_group != 16 && cfg->path_group != 32)) {
+        (cfg->path_group != 0 && cfg->path_group != 16 && cfg->path_group != 32 && cfg->path_group != 64)) {
         set_thread_error("server config: max sizes, max_disparities (1..512), slots (1..64), compute_streams "
-                         "(-1..slots) or path_group (0, 16, 32) out of range" +
+                         "(-1..slots) or path_group (0, 16, 32, 64) out of range" +
                          (err.empty() ? std::string() : ": " + err));
         return SGM_ERR_INVALID_ARGUMENT;
     }

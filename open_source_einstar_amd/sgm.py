@@ -764,7 +764,7 @@ class Server:
                  hbm_budget_bytes: int = 0, compute_streams: int = 0, path_group: int = 0):
         """compute_streams: 0 = one shared compute stream (requests' kernels in admission order, copies
         overlapping them), k = k shared streams, -1 = each slot computes on its own stream.
-        path_group: lanes per path scanline, 0 = timed per request shape, or 16 / 32."""
+        path_group: lanes per path scanline, 0 = timed per request shape, or 16 / 32 / 64."""
         cfg = CSgmServerConfig(max_width, max_height, max_disparities, slots, hbm_budget_bytes, compute_streams,
                                path_group)
         h = C.c_void_p()

@@ -180,3 +180,23 @@ def test_random_request_mix_bit_exact():
         l, r, md, nd = reqs[c["tag"]]
         kw = dict(BASE, min_disparity=md, num_disparities=nd)
         assert_same(O.sgm(l, r, O.SgmParams(**kw)), c["outputs"], SERVE_FIELDS, tag=f"request {c['tag']} {l.shape} D={nd}")
+
+
+@pytest.mark.parametrize("group", [0, 64])
+def test_lane_widths_timed_or_fixed(group):
+    """A shape whose lane layout allows 16, 32 and 64 lanes per path scanline (D = 256): with the width
+    left open the first requests time each width twice before one is kept; with 64 fixed every request
+    runs it. Every frame gives the bytes it gives alone and the oracle's."""
+    fs = [speckle_pair(200, 160, 256, 900 + i) for i in range(8)]
+    print(f"serving {len(fs)} frames of 200x160x256, path_group {group}", flush=True)
+    with sgm.Server(sgm.SgmParams(**BASE), 256, 200, 256, slots=2, path_group=group) as sv:
+        for i, p in enumerate(fs):
+            sv.submit(p.left, p.right, p.min_disparity, p.num_disparities, outputs=SERVE_FIELDS, tag=i)
+        got = drain(sv, len(fs))
+        st = sv.stats()
+    assert len(got) == len(fs) and all(c["status"] == 0 for c in got) and st["failed"] == 0
+    for c in got:
+        p = fs[c["tag"]]
+        kw = dict(BASE, min_disparity=p.min_disparity, num_disparities=p.num_disparities)
+        assert_same(alone(p), c["outputs"], SERVE_FIELDS, tag=f"frame {c['tag']} vs alone")
+        assert_same(O.sgm(p.left, p.right, O.SgmParams(**kw)), c["outputs"], SERVE_FIELDS, tag=f"frame {c['tag']} vs oracle")
