@@ -7,7 +7,8 @@
   Poisson gaps at 60 Hz while earlier requests run: each result equals compute_raw alone and the
   oracle on the oracle's rectified pair;
 * a resident batch of two 2048x1536 pairs with 256 disparities, streamed (inputs replaced, graph
-  replayed) and launched plain.
+  replayed) and launched plain;
+* whole frames through the path-kernel scanline widths and row-kernel block sizes chosen per pair.
 """
 import os
 
@@ -182,3 +183,30 @@ def test_large_pairs_streamed_through_a_resident_batch():
                 assert_same(refs[2 + i], b.results(i), list(fields), tag=f"streamed {mode} pair {2 + i}")
     finally:
         b.close()
+
+
+FULL_SIZE_FORMS = [
+    # (w, h, nd, params, cost_volume, recompute_dirs): the path-kernel and row-kernel forms chosen per pair
+    (1280, 1024, 512, dict(census_radius_x=4, census_radius_y=3, p1=10, p2=120), 1, 0),  # 64 lanes, 32 per lane
+    (1280, 1024, 256, PARAMS, 0, 0),                                                      # 64 lanes, recomputed
+    (1280, 1024, 128, PARAMS, 1, 4),                                                      # 32 lanes, half recomputed
+    (960, 768, 256, dict(), 1, 0),                                                        # 32 lanes, 4 paths
+    (320, 256, 247, dict(), 1, 0),  # the reference front end's stage: 32 lanes, 1024-thread row blocks
+    (2048, 200, 256, PARAMS, 1, 0),                                                       # 512-thread row blocks
+]
+
+
+@pytest.mark.parametrize("w,h,nd,kw,cv,rd", FULL_SIZE_FORMS,
+                         ids=[f"{c[0]}x{c[1]}x{c[2]}-cv{c[4]}-rd{c[5]}" for c in FULL_SIZE_FORMS])
+def test_full_size_pairs_with_the_forms_chosen_per_pair(w, h, nd, kw, cv, rd):
+    """Whole frames through the scanline widths and row-block sizes the library picks per pair, against
+    the oracle, every output field."""
+    pair = speckle_pair(w, h, min(nd, 256), seed=w + h + nd)
+    params = dict(kw, min_disparity=pair.min_disparity, num_disparities=nd)
+    print(f"full-size form {w}x{h}x{nd} cv={cv} rd={rd}", flush=True)
+    ref = O.sgm(pair.left, pair.right, O.SgmParams(**params), threads=THREADS)
+    with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
+        st.set_option(sgm.OPT_COST_VOLUME, cv)
+        st.set_option(sgm.OPT_RECOMPUTE_DIRS, rd)
+        got = st.compute(pair.left, pair.right, outputs=FIELDS)
+    assert_same(ref, got, FIELDS, tag=f"{w}x{h}x{nd} cv={cv} rd={rd}")
