@@ -296,6 +296,28 @@ int sgm_disparity_to_points_device(const float* d_disparity, const float* d_conf
 int sgm_pack_depth_device(const float* d_points_xyzw, const float* d_weights, int width, int height, uint8_t* d_out,
                           void* stream);
 
+/* Device workspace sgm_remove_speckles_device needs for a width x height map: 12 bytes a pixel (labels,
+ * tile-local counts, region sizes); 0 for an empty size. */
+size_t sgm_speckle_workspace_bytes(int width, int height);
+
+/* Replaces depth::remove_speckles (stereo.hpp:117-118, stereo.cpp:465-494) on device data: d_out receives
+ * d_disparity with every region (4-neighbourhood, valid neighbours |d_a - d_b| <= max_diff) of fewer
+ * than max_region_size pixels set to the invalid disparity; max_region_size <= 1 copies. d_out may be
+ * d_disparity. d_work: sgm_speckle_workspace_bytes(width, height) bytes, 4-byte aligned (unused and
+ * may be NULL when max_region_size <= 1). Enqueued on `stream` (NULL = the null stream), no wait;
+ * allocates nothing. At most 2^31 - 1 pixels. */
+int sgm_remove_speckles_device(const float* d_disparity, float* d_out, int width, int height, int max_region_size,
+                               float max_diff, void* d_work, void* stream);
+
+/* Replaces depth::filter_disparity (stereo.hpp:124-125, stereo.cpp:286-326) on device data, in place:
+ * the occlusion filter (FilterParams::occlusion, occlusion_tolerance), then the spike filter (spikes,
+ * spike_threshold), as the reference orders them. d_work: width*height floats for the spike filter's
+ * output, which is then copied back (may be NULL when spikes == 0). Widths up to 32768 (rows beyond 4096
+ * columns need 4 bytes of LDS a column for the occlusion filter). Enqueued on `stream`, no wait;
+ * allocates nothing. */
+int sgm_filter_disparity_device(float* d_disparity, int width, int height, int occlusion, float occlusion_tolerance,
+                                int spikes, float spike_threshold, float* d_work, void* stream);
+
 /* Replaces MetalStereo::set_point_params (metal_stereo.hpp:83-85): the geometry and limits the frame
  * entry points convert disparity with; allocates the context's point buffers. */
 int sgm_set_point_params(sgm_context* ctx, const sgm_rectified_geometry* geometry, const sgm_point_params* params);

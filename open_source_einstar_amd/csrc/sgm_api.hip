@@ -1522,6 +1522,63 @@ int sgm_pack_depth_device(const float* d_points_xyzw, const float* d_weights, in
     return SGM_OK;
 }
 
+size_t sgm_speckle_workspace_bytes(int width, int height) {
+    if (width < 1 || height < 1) return 0;
+    return (size_t)width * (size_t)height * 12;
+}
+
+int sgm_remove_speckles_device(const float* d_disparity, float* d_out, int width, int height, int max_region_size,
+                               float max_diff, void* d_work, void* stream) {
+    const bool speckle = max_region_size > 1;
+    if (!d_disparity || !d_out || width < 1 || height < 1 || (int64_t)width * height > INT32_MAX ||
+        (speckle && (!d_work || (reinterpret_cast<uintptr_t>(d_work) & 3u)))) {
+        g_last_error = "sgm_remove_speckles_device: null pointer, size < 1, more than 2^31 - 1 pixels, or a missing "
+                       "or misaligned workspace";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t npx = (size_t)width * height;
+    int32_t* labels = static_cast<int32_t*>(d_work);
+    uint32_t* local = reinterpret_cast<uint32_t*>(labels + npx);
+    uint32_t* sizes = local + npx;
+    hipError_t e = speckle ? launch_ccl(d_disparity, labels, local, sizes, width, height, max_diff, s) : hipSuccess;
+    if (e == hipSuccess)
+        e = launch_finalize(d_disparity, speckle ? labels : nullptr, sizes, max_region_size, nullptr, d_out, nullptr, width,
+                            height, s);
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_remove_speckles_device", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
+int sgm_filter_disparity_device(float* d_disparity, int width, int height, int occlusion, float occlusion_tolerance,
+                                int spikes, float spike_threshold, float* d_work, void* stream) {
+    std::string err;
+    if (!d_disparity || width < 1 || height < 1 || width > kMaxWidth || (spikes && !d_work) ||
+        (reinterpret_cast<uintptr_t>(d_disparity) & 3u) || (reinterpret_cast<uintptr_t>(d_work) & 3u)) {
+        g_last_error = "sgm_filter_disparity_device: null pointer, size < 1, width above " + std::to_string(kMaxWidth) +
+                       ", or a missing or misaligned workspace";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
+    if (int rc = occlusion ? check_row_lds(width, err) : SGM_OK; rc != SGM_OK) {
+        g_last_error = "sgm_filter_disparity_device: " + err;
+        return rc;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = occlusion ? launch_occlusion(d_disparity, width, height, occlusion_tolerance, s) : hipSuccess;
+    if (e == hipSuccess && spikes) {
+        e = launch_spikes(d_disparity, d_work, width, height, spike_threshold, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_disparity, d_work, (size_t)width * height * sizeof(float), hipMemcpyDeviceToDevice, s);
+    }
+    if (e != hipSuccess) {
+        g_last_error = hip_msg("sgm_filter_disparity_device", e);
+        return SGM_ERR_IO;
+    }
+    return SGM_OK;
+}
+
 int sgm_set_point_params(sgm_context* ctx, const sgm_rectified_geometry* geometry, const sgm_point_params* params) {
     if (!ctx) {
         g_last_error = "ctx is NULL";

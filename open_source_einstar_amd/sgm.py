@@ -255,6 +255,10 @@ def lib():
         L.sgm_disparity_to_points_device.argtypes = [vp, vp, i32, i32, P(CSgmRectifiedGeometry), P(CSgmPointParams),
                                                      P(CSgmFrame), vp]
         L.sgm_pack_depth_device.argtypes = [vp, vp, i32, i32, vp, vp]
+        L.sgm_speckle_workspace_bytes.argtypes = [i32, i32]
+        L.sgm_speckle_workspace_bytes.restype = C.c_size_t
+        L.sgm_remove_speckles_device.argtypes = [vp, vp, i32, i32, i32, C.c_float, vp, vp]
+        L.sgm_filter_disparity_device.argtypes = [vp, i32, i32, i32, C.c_float, i32, C.c_float, vp, vp]
         L.sgm_set_point_params.argtypes = [vp, P(CSgmRectifiedGeometry), P(CSgmPointParams)]
         L.sgm_compute_frame_raw.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp]
         L.sgm_compute_frame_raw_device.argtypes = [vp, vp, vp, C.c_ssize_t, i32, i32, P(CSgmFrame), vp, vp, vp]
@@ -901,6 +905,39 @@ def disparity_to_points(disparity, confidence, geometry: RectifiedGeometry, para
         for b in bufs:
             if b is not None:
                 b.free()
+
+
+def remove_speckles(disparity, max_region_size: int = 200, max_diff: float = 1.0) -> np.ndarray:
+    """depth::remove_speckles on the GPU (sgm_remove_speckles_device) for a host array: a copy with every
+    region (4-neighbourhood, |d_a - d_b| <= max_diff) of fewer than max_region_size pixels invalid."""
+    d = np.ascontiguousarray(disparity, np.float32)
+    h, w = d.shape
+    bd = DeviceBuffer(d.nbytes)
+    bw = DeviceBuffer(max(4, lib().sgm_speckle_workspace_bytes(w, h)))
+    try:
+        bd.upload(d, 0)
+        _check(lib().sgm_remove_speckles_device(bd.ptr, bd.ptr, w, h, int(max_region_size), float(max_diff), bw.ptr, None))
+        return bd.download((h, w), np.float32)
+    finally:
+        bd.free()
+        bw.free()
+
+
+def filter_disparity(disparity, occlusion: bool = True, occlusion_tolerance: float = 1.0, spikes: bool = True,
+                     spike_threshold: float = 1.0) -> np.ndarray:
+    """depth::filter_disparity on the GPU (sgm_filter_disparity_device) for a host array: a filtered copy
+    (occlusion filter, then spike filter, with the reference's FilterParams)."""
+    d = np.ascontiguousarray(disparity, np.float32)
+    h, w = d.shape
+    bd, bw = DeviceBuffer(d.nbytes), DeviceBuffer(d.nbytes)
+    try:
+        bd.upload(d, 0)
+        _check(lib().sgm_filter_disparity_device(bd.ptr, w, h, int(bool(occlusion)), float(occlusion_tolerance),
+                                                 int(bool(spikes)), float(spike_threshold), bw.ptr, None))
+        return bd.download((h, w), np.float32)
+    finally:
+        bd.free()
+        bw.free()
 
 
 def pack_depth_device(d_points: int, d_weights: int | None, width: int, height: int, d_out: int,
