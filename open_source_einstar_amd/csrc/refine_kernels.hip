@@ -632,67 +632,127 @@ __device__ __forceinline__ bool joined(float d, float e, float max_diff) {
     return valid_d(e) && !(fabsf(e - d) > max_diff);
 }
 
-constexpr int kCcl = 64, kCclThreads = 1024, kCclPx = kCcl * kCcl / kCclThreads;
+constexpr int kCcl = 64, kCclThreads = 1024, kCclRows = kCcl * kCcl / kCclThreads;
+static_assert(kCcl == 64, "a wave spans one tile row: the ballots below are one bit a column");
 
+// Tile-local components of a 64x64 tile. Wave v takes tile rows v, v + 16, v + 32, v + 48, a lane a
+// column. A ballot of the row's horizontal joins gives every pixel its run (the longest stretch of
+// joined neighbours), whose first pixel becomes its label at once: no atomics within a row. Vertical
+// joins are then united once per pair of runs: a pixel whose left neighbour joins it in both rows and
+// also joins down leaves the union to that neighbour (the same two runs). Labels point at smaller
+// indices throughout, so every find ends. Each run's first pixel adds the run's length to its root.
 __global__ __launch_bounds__(kCclThreads) void ccl_local_kernel(const float* disp, int32_t* L, uint32_t* local,
                                                                 uint32_t* sizes, int w, int h, float max_diff) {
     __shared__ int32_t ll[kCcl * kCcl];
     __shared__ uint32_t cnt[kCcl * kCcl];
+    __shared__ uint64_t rowj[kCcl];  // bit c: pixel c of the tile row joins pixel c + 1
     const int bx = (int)blockIdx.x * kCcl, by = (int)blockIdx.y * kCcl;
-    // Thread tid takes tile pixels tid + j * 1024: a wave covers 64 consecutive pixels of a row.
-    float d[kCclPx];
-    bool v[kCclPx];
+    const int lane = (int)threadIdx.x & (kCcl - 1), wv = (int)threadIdx.x / kCcl;
+    const int x = bx + lane;
+    bool v[kCclRows], jd[kCclRows];
+    uint64_t runj[kCclRows];
 #pragma unroll
-    for (int j = 0; j < kCclPx; ++j) {
-        const int t = (int)threadIdx.x + j * kCclThreads, x = bx + (t & (kCcl - 1)), y = by + t / kCcl;
-        d[j] = x < w && y < h ? disp[y * w + x] : kInvalid;
-        v[j] = valid_d(d[j]);
-        ll[t] = v[j] ? t : -1;
+    for (int j = 0; j < kCclRows; ++j) {
+        const int ly = wv + j * (kCclThreads / kCcl), y = by + ly, t = ly * kCcl + lane;
+        const bool in = x < w && y < h;
+        const int64_t i = (int64_t)y * w + x;
+        const float d = in ? disp[i] : kInvalid;
+        v[j] = valid_d(d);
+        const float dr = __shfl_down(d, 1, kCcl);  // the right neighbour when it is in the tile
+        const bool jr = v[j] && lane + 1 < kCcl && x + 1 < w && joined(d, dr, max_diff);
+        jd[j] = v[j] && ly + 1 < kCcl && y + 1 < h && joined(d, disp[i + w], max_diff);
+        const uint64_t M = __ballot(jr);
+        runj[j] = M;
+        // The run's first column: the last s <= lane with s == 0 or pixel s - 1 not joining s.
+        const uint64_t starts = ~(M << 1);
+        const uint64_t upto = lane == kCcl - 1 ? ~0ull : ((2ull << lane) - 1);
+        const int s0 = 63 - __builtin_clzll(starts & upto);
+        ll[t] = v[j] ? ly * kCcl + s0 : -1;
         cnt[t] = 0;
+        if (lane == 0) rowj[ly] = M;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kCclPx; ++j) {
-        const int t = (int)threadIdx.x + j * kCclThreads, lx = t & (kCcl - 1), ly = t / kCcl;
-        const int x = bx + lx, y = by + ly, i = y * w + x;
-        if (!v[j]) continue;
-        if (lx + 1 < kCcl && x + 1 < w && joined(d[j], disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + 1);
-        if (ly + 1 < kCcl && y + 1 < h && joined(d[j], disp[i + w], max_diff))
-            unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + kCcl);
+    for (int j = 0; j < kCclRows; ++j) {
+        const int ly = wv + j * (kCclThreads / kCcl), t = ly * kCcl + lane;
+        const uint64_t D = __ballot(jd[j]);
+        if (jd[j]) {
+            const uint64_t below = rowj[ly + 1];  // ly + 1 < kCcl wherever jd holds
+            const bool dup = lane > 0 && ((D & runj[j] & below) >> (lane - 1) & 1u);
+            if (!dup) unite<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t, t + kCcl);
+        }
     }
     __syncthreads();
-    int32_t r[kCclPx];
+    int32_t r[kCclRows];
 #pragma unroll
-    for (int j = 0; j < kCclPx; ++j) {
-        const int t = (int)threadIdx.x + j * kCclThreads;
+    for (int j = 0; j < kCclRows; ++j) {
+        const int ly = wv + j * (kCclThreads / kCcl), t = ly * kCcl + lane;
         r[j] = -1;
         if (v[j]) {
             r[j] = find_root<__HIP_MEMORY_SCOPE_WORKGROUP>(ll, t);
-            atomicAdd(&cnt[r[j]], 1u);
+            const uint64_t M = runj[j];
+            const bool head = lane == 0 || !((M >> (lane - 1)) & 1u);
+            if (head) {
+                // The run ends at the first column >= lane that does not join its right neighbour.
+                const int e = __builtin_ctzll(~M & (~0ull << lane));
+                atomicAdd(&cnt[r[j]], (uint32_t)(e - lane + 1));
+            }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kCclPx; ++j) {
-        const int t = (int)threadIdx.x + j * kCclThreads, x = bx + (t & (kCcl - 1)), y = by + t / kCcl;
+    for (int j = 0; j < kCclRows; ++j) {
+        const int ly = wv + j * (kCclThreads / kCcl), y = by + ly, t = ly * kCcl + lane;
         if (x >= w || y >= h) continue;
-        const int32_t i = y * w + x;
+        const int64_t i = (int64_t)y * w + x;
         L[i] = v[j] ? (by + r[j] / kCcl) * w + bx + (r[j] & (kCcl - 1)) : -1;
         local[i] = v[j] && r[j] == t ? cnt[t] : 0u;
         sizes[i] = 0;
     }
 }
 
-__global__ __launch_bounds__(kTw* kTh) void ccl_border_kernel(const float* disp, int32_t* L, int w, int h, float max_diff) {
-    const int x = (int)blockIdx.x * kTw + (int)threadIdx.x, y = (int)blockIdx.y * kTh + (int)threadIdx.y;
-    if (x >= w || y >= h) return;
-    const bool right = (x % kCcl == kCcl - 1) && x + 1 < w, down = (y % kCcl == kCcl - 1) && y + 1 < h;
-    if (!right && !down) return;
-    const int32_t i = y * w + x;
+// Joins across tile borders, one thread a border pair: the first (tiles_x - 1) * h threads take the
+// vertical borders (pixel (64k - 1, y) and its right neighbour), the rest the horizontal ones (pixel
+// (x, 64k - 1) and the one below). A pair whose predecessor along the same tile edge (the row above,
+// or the column to the left) is joined too, with both sides' links between the two pairs inside one
+// tile, leaves the union to that predecessor: the same two components. (Links across another border
+// are never relied on, so no two pairs can leave their union to each other.)
+__global__ __launch_bounds__(kBlock) void ccl_border_kernel(const float* disp, int32_t* L, int w, int h, float max_diff,
+                                                           int64_t n_vertical, int64_t n_total) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n_total) return;
+    int x, y, xn, yn, xp, yp, xpn, ypn;  // the pair, its predecessor
+    bool pred_local;
+    if (k < n_vertical) {
+        const int b = (int)(k / h);
+        y = (int)(k - (int64_t)b * h);
+        x = (b + 1) * kCcl - 1;
+        xn = x + 1;
+        yn = y;
+        xp = x, yp = y - 1, xpn = xn, ypn = y - 1;
+        pred_local = y % kCcl != 0;
+    } else {
+        const int64_t q = k - n_vertical;
+        const int b = (int)(q / w);
+        x = (int)(q - (int64_t)b * w);
+        y = (b + 1) * kCcl - 1;
+        xn = x;
+        yn = y + 1;
+        xp = x - 1, yp = y, xpn = x - 1, ypn = yn;
+        pred_local = x % kCcl != 0;
+    }
+    if (xn >= w || yn >= h) return;
+    const int32_t i = y * w + x, in_ = yn * w + xn;
     const float d = disp[i];
-    if (!valid_d(d)) return;
-    if (right && joined(d, disp[i + 1], max_diff)) unite<__HIP_MEMORY_SCOPE_AGENT>(L, i, i + 1);
-    if (down && joined(d, disp[i + w], max_diff)) unite<__HIP_MEMORY_SCOPE_AGENT>(L, i, i + w);
+    if (!valid_d(d) || !joined(d, disp[in_], max_diff)) return;
+    if (pred_local) {
+        const int32_t ip = yp * w + xp, ipn = ypn * w + xpn;
+        const float dp = disp[ip], dpn = disp[ipn];
+        if (valid_d(dp) && joined(dp, dpn, max_diff) && joined(dp, d, max_diff) && valid_d(dpn) &&
+            joined(dpn, disp[in_], max_diff))
+            return;
+    }
+    unite<__HIP_MEMORY_SCOPE_AGENT>(L, i, in_);
 }
 
 __global__ __launch_bounds__(kTw* kTh) void ccl_accum_kernel(int32_t* L, const uint32_t* local, uint32_t* sizes, int w, int h) {
@@ -806,12 +866,17 @@ hipError_t launch_spikes(const float* in, float* out, int w, int h, float thr, h
 hipError_t launch_ccl(const float* disp, int32_t* labels, uint32_t* local, uint32_t* sizes, int w, int h, float max_diff,
                       hipStream_t stream) {
     if (w <= 0 || h <= 0) return hipSuccess;
-    const dim3 tiles((unsigned)((w + kCcl - 1) / kCcl), (unsigned)((h + kCcl - 1) / kCcl));
-    hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(kCclThreads), 0, stream, disp, labels, local, sizes, w, h, max_diff);
+    const int tx = (w + kCcl - 1) / kCcl, ty = (h + kCcl - 1) / kCcl;
+    hipLaunchKernelGGL(ccl_local_kernel, dim3((unsigned)tx, (unsigned)ty), dim3(kCclThreads), 0, stream, disp, labels, local,
+                       sizes, w, h, max_diff);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ccl_border_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, disp, labels, w, h, max_diff);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int64_t n_vertical = (int64_t)(tx - 1) * h, n_total = n_vertical + (int64_t)(ty - 1) * w;
+    if (n_total > 0) {
+        hipLaunchKernelGGL(ccl_border_kernel, dim3((unsigned)((n_total + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, disp,
+                           labels, w, h, max_diff, n_vertical, n_total);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(ccl_accum_kernel, grid2d(w, h), dim3(kTw, kTh), 0, stream, labels, local, sizes, w, h);
     return hipGetLastError();
 }
