@@ -85,6 +85,13 @@ MAPS = {
     "noisy-20x3000": lambda: noisy_regions(20, 3000, 6),
     "single-pixel-1x1": lambda: np.full((1, 1), 3.0, np.float32),
     "row-1x777": lambda: noisy_regions(777, 1, 7, cells=(1, 4)),
+    "column-1x3000": lambda: noisy_regions(1, 3000, 8, cells=(1, 40)),
+    # a one-pixel staircase down the diagonal: one 4-connected region crossing tiles corner to corner
+    "staircase-200x200": lambda: np.where(np.equal(*np.indices((200, 200))) | np.equal(np.indices((200, 200))[0],
+                                                                                          np.indices((200, 200))[1] + 1),
+                                          np.float32(9), INVALID).astype(np.float32),
+    # the diagonal alone: its pixels touch only at corners, so each is a region of one
+    "diagonal-150x150": lambda: np.where(np.equal(*np.indices((150, 150))), np.float32(9), INVALID).astype(np.float32),
 }
 
 
