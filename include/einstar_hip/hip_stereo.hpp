@@ -479,4 +479,50 @@ inline Result<std::vector<std::uint8_t>> pack_depth(const FrameResult& f) {
     return bytes;
 }
 
+// depth::remove_speckles (libs/depth/include/einstar/depth/stereo.hpp:117-118) on the GPU, in place:
+// regions (4-neighbourhood, |d_a - d_b| <= max_diff) of fewer than max_region_size pixels become
+// kInvalidDisparity. Through temporary device buffers (sgm_remove_speckles_device).
+inline Result<void> remove_speckles(ImageF32& disparity, const SpeckleParams& p) {
+    const auto n = disparity.size();
+    if (n == 0) return {};
+    void* dd = nullptr;
+    void* dw = nullptr;
+    const std::size_t work = sgm_speckle_workspace_bytes(disparity.width(), disparity.height());
+    int rc = sgm_dev_malloc(4 * n, &dd);
+    if (rc == SGM_OK && p.max_region_size > 1) rc = sgm_dev_malloc(work, &dw);
+    if (rc == SGM_OK) rc = sgm_dev_upload(dd, disparity.data(), 4 * n);
+    if (rc == SGM_OK)
+        rc = sgm_remove_speckles_device(static_cast<const float*>(dd), static_cast<float*>(dd), disparity.width(),
+                                        disparity.height(), p.max_region_size, p.max_diff, dw, nullptr);
+    if (rc == SGM_OK) rc = sgm_dev_download(disparity.data(), dd, 4 * n);
+    const std::string err = rc == SGM_OK ? std::string() : sgm_last_error(nullptr);
+    for (void* q : {dd, dw})
+        if (q) sgm_dev_free(q);
+    if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), err);
+    return {};
+}
+
+// depth::filter_disparity (libs/depth/include/einstar/depth/stereo.hpp:124-125) on the GPU, in place:
+// the occlusion filter, then the spike filter. Through temporary device buffers
+// (sgm_filter_disparity_device).
+inline Result<void> filter_disparity(ImageF32& disparity, const FilterParams& p) {
+    const auto n = disparity.size();
+    if (n == 0) return {};
+    void* dd = nullptr;
+    void* dw = nullptr;
+    int rc = sgm_dev_malloc(4 * n, &dd);
+    if (rc == SGM_OK && p.spikes) rc = sgm_dev_malloc(4 * n, &dw);
+    if (rc == SGM_OK) rc = sgm_dev_upload(dd, disparity.data(), 4 * n);
+    if (rc == SGM_OK)
+        rc = sgm_filter_disparity_device(static_cast<float*>(dd), disparity.width(), disparity.height(), p.occlusion ? 1 : 0,
+                                         p.occlusion_tolerance, p.spikes ? 1 : 0, p.spike_threshold,
+                                         static_cast<float*>(dw), nullptr);
+    if (rc == SGM_OK) rc = sgm_dev_download(disparity.data(), dd, 4 * n);
+    const std::string err = rc == SGM_OK ? std::string() : sgm_last_error(nullptr);
+    for (void* q : {dd, dw})
+        if (q) sgm_dev_free(q);
+    if (rc != SGM_OK) return make_error(static_cast<Errc>(rc), err);
+    return {};
+}
+
 }  // namespace einstar::depth_hip

@@ -271,6 +271,35 @@ int main() {
         auto s3 = (*st2)->compute_sgm(l.view(), r.view());
         expect(!s3 && s3.error().code == Errc::unsupported, "compute_sgm with pyramid levels -> unsupported");
     }
+    // The post-filters as building blocks (stereo.hpp:117-118, 124-125) on a map of noisy 5x5 blocks with
+    // holes, against the oracle's remove_speckles and filter_disparity.
+    {
+        const int mw = 301, mh = 203;
+        ImageF32 map(mw, mh);
+        std::uint32_t rng = 12345u;
+        auto next = [&rng] { return (rng = rng * 1664525u + 1013904223u) >> 8; };
+        std::vector<float> block((mw / 5 + 1) * (mh / 5 + 1));
+        for (auto& b : block) b = 5.0f + static_cast<float>(next() % 5000) / 100.0f;
+        for (int y = 0; y < mh; ++y)
+            for (int x = 0; x < mw; ++x)
+                map(x, y) = next() % 16 == 0 ? kInvalidDisparity
+                                             : block[(y / 5) * (mw / 5 + 1) + x / 5] + static_cast<float>(next() % 60) / 100.0f;
+        orc_stereo_params os{};
+        os.filter_occlusion = 1;
+        os.filter_occlusion_tolerance = 1.0f;
+        os.filter_spikes = 1;
+        os.filter_spike_threshold = 1.0f;
+        os.speckle_max_region_size = 40;
+        os.speckle_max_diff = 1.0f;
+        std::vector<float> want(map.data(), map.data() + map.size());
+        orc_filter_disparity(want.data(), mw, mh, &os, 0);
+        ImageF32 got = map;
+        auto fres = filter_disparity(got, FilterParams{});
+        expect(fres.has_value() && same_bytes(got, want), "filter_disparity == the oracle's");
+        expect(orc_remove_speckles(want.data(), mw, mh, &os) == 0, "oracle remove_speckles");
+        auto sres = remove_speckles(got, SpeckleParams{40, 1.0f});
+        expect(sres.has_value() && same_bytes(got, want), "remove_speckles == the oracle's");
+    }
     StereoParams big = sp;
     big.sgm.num_disparities = 600;
     auto st3 = HipStereo::create(big, w, h);
