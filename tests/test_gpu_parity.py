@@ -371,7 +371,7 @@ G64_CASES = [(300, 20, 256, 101, dict(eight_paths=True)), (280, 24, 247, 102, di
              (200, 40, 200, 105, dict(eight_paths=True, min_disparity=-30, census_radius_x=4, census_radius_y=3,
                                       p1=10, p2=120)),
              (96, 64, 128, 106, dict(eight_paths=True)),  # 8 per 16-lane lane: runs 32 lanes
-             (120, 30, 192, 107, dict())]                 # 12 per 16-lane lane: runs 16 lanes
+             (120, 30, 192, 107, dict())]                 # 12 per 16-lane lane: runs 32 lanes (6 per lane)
 
 
 @pytest.mark.parametrize("cv", [0, 1], ids=["recompute", "volume"])
