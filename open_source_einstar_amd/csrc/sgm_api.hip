@@ -544,6 +544,10 @@ int sgm_create_stereo(const sgm_stereo_params* sp, int width, int height, sgm_co
         g_last_error = "image too large: the census words of one image exceed 4 GiB - 1";
         return SGM_ERR_INVALID_ARGUMENT;
     }
+    if ((int64_t)width * height > INT32_MAX) {  // speckle labels are 32-bit pixel indices
+        g_last_error = "image too large: more than 2^31 - 1 pixels at the full resolution";
+        return SGM_ERR_INVALID_ARGUMENT;
+    }
     if (int rc = check_row_lds(width, err); rc != SGM_OK) {  // the occlusion filter runs at the full width
         g_last_error = err;
         return rc;

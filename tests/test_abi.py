@@ -68,6 +68,15 @@ def test_invalid_sizes_are_rejected():
         assert e.value.errc == "invalid_argument"
 
 
+def test_stereo_frames_beyond_2_31_pixels_are_rejected_before_any_gpu_work():
+    """The full-resolution frame of a compute_disparity context is indexed with 32-bit pixel numbers
+    (speckle labels): 32768 x 65537 with two pyramid levels passes the SGM level's census limit and is
+    refused for its full-resolution size."""
+    with pytest.raises(sgm.SgmError) as e:
+        sgm.HipStereo.create_stereo(sgm.SgmParams(num_disparities=16), sgm.StereoParams(pyramid_levels=2), 32768, 65537)
+    assert e.value.errc == "invalid_argument" and "2^31 - 1 pixels" in str(e.value)
+
+
 @pytest.mark.parametrize("bad", [dict(pyramid_levels=5), dict(refine_search_radius=5), dict(refine_search_radius=0),
                                  dict(detail_search=5.0), dict(detail_step=0.0), dict(refine_zncc_radius=8)])
 def test_invalid_stereo_parameters_are_rejected_before_any_gpu_work(bad):
