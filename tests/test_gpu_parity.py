@@ -477,11 +477,11 @@ def test_u16_path_costs_bit_exact(w, h, nd, seed, kw):
     pair = speckle_pair(w, h, max(nd, 6), seed)
     params = dict(min_disparity=kw.get("min_disparity", pair.min_disparity), num_disparities=nd, **{k: v for k, v in kw.items() if k != "min_disparity"})
     with sgm.HipStereo.create(sgm.SgmParams(**params), w, h) as st:
-        for cv, group in ((0, 16), (1, 32)):
+        for cv, group in ((0, 16), (1, 32), (1, 64), (0, 0)):  # u16 path costs run 16 lanes whatever is asked
             st.set_option(sgm.OPT_COST_VOLUME, cv)
             st.set_option(sgm.OPT_PATH_GROUP, group)
             got = st.compute(pair.left, pair.right, outputs=FIELDS)
-            assert_same(ref, got, FIELDS, tag=f"u16 path costs {w}x{h}x{nd} cv={cv}")
+            assert_same(ref, got, FIELDS, tag=f"u16 path costs {w}x{h}x{nd} cv={cv} lanes={group}")
 
 
 def test_u16_path_costs_in_a_batch_and_served():
