@@ -30,6 +30,11 @@ HIP_FLAGS = [
     "-Wall", "-Wextra", "-Wno-unused-parameter",
 ]
 
+# The path kernel's walk loop runs faster with its header aligned to 64 bytes (1280x1024x128, 8 paths:
+# 459-460 against 468 us; 320x256x128: 47.8 against 53 us), where the code before it otherwise leaves it
+# at any offset; the ZNCC kernels run about 1% slower aligned, so only the path-kernel sources take it.
+PATH_SOURCE_FLAGS = ["-falign-loops=64"]
+
 HIP_SOURCES = ["sgm_kernels.hip", "paths_g16_lo.hip", "paths_g16_hi.hip", "paths_g32.hip", "paths_g64.hip", "paths_w16.hip",
                "rows_kernels.hip", "rows_w16.hip", "sweep_kernels.hip", "rows_sweep.hip",
                "refine_kernels.hip", "frame_kernels.hip", "sgm_api.hip", "sgm_server.hip", "sgm_noalloc.hip"]
@@ -46,7 +51,8 @@ def build_hip(verbose: bool = True) -> str:
     then links the shared library."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
-    jobs = [[HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, s), "-o", o]
+    jobs = [[HIPCC, *HIP_FLAGS, *(PATH_SOURCE_FLAGS if s.startswith("paths_") else []), f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+             os.path.join(CSRC, s), "-o", o]
             for s, o in zip(HIP_SOURCES, objs)]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1))
     with ThreadPoolExecutor(workers) as ex:
