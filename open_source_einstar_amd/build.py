@@ -32,7 +32,8 @@ HIP_FLAGS = [
 
 # The path kernel's walk loop runs faster with its header aligned to 64 bytes (1280x1024x128, 8 paths:
 # 459-460 against 468 us; 320x256x128: 47.8 against 53 us), where the code before it otherwise leaves it
-# at any offset; the ZNCC kernels run about 1% slower aligned, so only the path-kernel sources take it.
+# at any offset; the ZNCC kernels run about 1% slower aligned and the row kernel the same, so only the
+# path-kernel sources take it.
 PATH_SOURCE_FLAGS = ["-falign-loops=64"]
 
 HIP_SOURCES = ["sgm_kernels.hip", "paths_g16_lo.hip", "paths_g16_hi.hip", "paths_g32.hip", "paths_g64.hip", "paths_w16.hip",
