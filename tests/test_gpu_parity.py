@@ -265,11 +265,11 @@ def test_maximum_width_32768():
     assert "max_width" in str(e.value)
 
 
-def test_config1_640x400x64_four_paths():
+def test_whole_640x400x64_four_paths_census_9x7():
     run_case(640, 400, 64, 1, census_radius_x=4, census_radius_y=3, p1=10, p2=120)
 
 
-def test_config2_1280x1024x128_eight_paths():
+def test_whole_1280x1024x128_eight_paths_census_9x7():
     run_case(1280, 1024, 128, 2, census_radius_x=4, census_radius_y=3, p1=10, p2=120, eight_paths=True)
 
 
