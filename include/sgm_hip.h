@@ -357,8 +357,9 @@ enum {
     SGM_OPT_COST_VOLUME = 1, /* 1 (default): matching costs written once as bytes and read by every
                                 path (W*H*dp bytes of HBM); 0: recomputed from census words in each path */
     SGM_OPT_PATH_LINES = 2,  /* scanlines per path-kernel block: 4, 8 or 16, or 0 (default): with 16
-                                lanes per scanline 4 for 4-path configurations, else 16; with 32 lanes 8,
-                                or 4 for 4-path pairs of 384 or more padded disparities; with 64, 4 */
+                                lanes per scanline 16, or 4 for 4-path pairs of at most 640x400 pixels,
+                                more than 16 disparities a lane or costs mostly recomputed; with 32 lanes
+                                8, or 4 for 4-path pairs of 384 or more padded disparities; with 64, 4 */
     SGM_OPT_PATH_GROUP = 3,  /* lanes per scanline in the path kernel: 0 (default) = chosen per pair
                                 (64 where the padded disparity count is a multiple of 256 and above 384,
                                 or a multiple of 256 with half the directions or more recomputing their

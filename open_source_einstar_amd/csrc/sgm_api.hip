@@ -121,34 +121,46 @@ void dfree(void* p) {
 // per 16-lane lane (D > 384; path kernel 0.54-0.87 of the 16-lane time from 320x256 to 2048x1536),
 // and layouts of 16 when half the directions or more recompute their costs (0.45-0.76; 0.77-1.04 of
 // the 32-lane time). 32 lanes: other layouts of a multiple of 8 per lane that recompute (0.42-0.89),
-// 4-path pairs up to about 1M pixels (0.66-0.83 at 320x256 to 960x768 for D = 128 and 256, about 1.0
-// at 1280x1024 and above), and 8-path pairs of 16 or more per lane up to 640x512 (0.78-0.93). With 2
-// or 6 per lane a lane would move its path costs in 2-byte pieces (D = 192, 12 per lane, on 32 lanes:
-// 1.9-2.4 times the 16-lane time), so wider lines need a multiple of 8 per 16-lane lane (32 lanes) or
-// of 16 (64 lanes).
+// 4-path pairs from the cost volume at 16 per lane up to 2M pixels (D = 256: 0.91-0.97 of the best
+// 16-lane form from 320x256 to 1600x1200), at 8 per lane up to 128K pixels (320x256x128: 40 against
+// 46 us; larger pairs run 16 lanes of 16 lines faster) and at 24 or more per lane up to 1M pixels, and
+// 8-path pairs of 16 or more per lane up to 640x512 (0.78-0.93). With 2 or 6 per lane a lane would
+// move its path costs in 2-byte pieces (D = 192, 12 per lane, on 32 lanes: 1.9-2.4 times the 16-lane
+// time), so wider lines need a multiple of 8 per 16-lane lane (32 lanes) or of 16 (64 lanes).
+static bool mostly_recomputed(const PairDesc& d, const SgmConsts& c) {
+    const int recomputed = d.cvol ? std::min(std::max(c.cv_from_dir, 0), d.ndirs) : d.ndirs;
+    return 2 * recomputed >= d.ndirs;
+}
+
 int auto_path_group(const PairDesc& d, const SgmConsts& c) {
     if (c.wide_costs || d.dpl % 8 != 0) return 16;
     const int64_t px = (int64_t)d.w * d.h;
-    const int recomputed = d.cvol ? std::min(std::max(c.cv_from_dir, 0), d.ndirs) : d.ndirs;
-    const bool mostly_recomputed = 2 * recomputed >= d.ndirs;
-    if (d.dpl % 16 == 0 && (d.dpl >= 32 || mostly_recomputed)) return 64;
-    if (mostly_recomputed) return 32;
-    if (d.ndirs == 4) return px <= 1024 * 1024 ? 32 : 16;
+    const bool recomputing = mostly_recomputed(d, c);
+    if (d.dpl % 16 == 0 && (d.dpl >= 32 || recomputing)) return 64;
+    if (recomputing) return 32;
+    if (d.ndirs == 4) {
+        const int64_t limit = d.dpl == 8 ? 128 * 1024 : d.dpl == 16 ? 2048 * 1024 : 1024 * 1024;
+        return px <= limit ? 32 : 16;
+    }
     return d.dpl >= 16 && px <= 640 * 512 ? 32 : 16;
 }
 
-// Scanlines per path-kernel block when the caller leaves it open (0). 16 lanes per line: one wave per
-// block (4 lines) for 4-path configurations, whose walks are few and long for the card, so that a
-// finished wave's slot takes new work at once (1280x1024x128: path kernel 0.28 against 0.30 ms with
-// the cost volume, 0.55 against 0.63 ms recomputing the costs); 16 with 8 paths (1280x1024: 0.47-0.49
-// ms either way; 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the refinement chain: 33
-// against 41 us). 32 lanes per line: 8 lines (two waves), except 4 for 4-path pairs of 24 or more
+// Scanlines per path-kernel block when the caller leaves it open (0). 16 lanes per line, 4 paths: one
+// wave per block (4 lines) for pairs up to 640x400 and for pairs that recompute most directions' costs
+// or hold more than 16 disparities per lane (1280x1024x128 recomputing: 0.55 against 0.63 ms); 16
+// lines for larger cost-volume pairs of up to 16 per lane (0.68-0.97 of the 4-line time from 640x480
+// to 2048x1536 at D = 64 to 192, except 1.01-1.05 at 1280x1024 and 1600x1200x192). 8 paths: 16 lines
+// (1280x1024: 0.47-0.49 ms either way; 2048x1536x256: 2.57 against 2.64 ms; the 320x256x32 pair of the
+// refinement chain: 33 against 41 us). 32 lanes per line: 8 lines (two waves), except 4 for 4-path pairs of 24 or more
 // disparities per lane of the 16-lane layout (640x512, D = 384 and 512: 198 and 248 against 214 and
 // 284 us). 64 lanes: 4 lines (one a wave).
-int auto_path_lines(const PairDesc& d, int path_group) {
+int auto_path_lines(const PairDesc& d, const SgmConsts& c, int path_group) {
     if (path_group == 64) return 4;
     if (path_group == 32) return d.ndirs == 4 && d.dpl >= 24 ? 4 : 8;
-    return d.ndirs == 4 ? 4 : 16;
+    if (d.ndirs != 4) return 16;
+    const bool long_lines =
+        (int64_t)d.w * d.h > 640 * 400 && d.dpl <= 16 && !c.wide_costs && !mostly_recomputed(d, c);
+    return long_lines ? 16 : 4;
 }
 
 // Threads per row-kernel block when the caller leaves it open (0): the large block (1024 threads, or
@@ -164,7 +176,7 @@ hipError_t enqueue_pair(const PairDesc& d, const SgmConsts& c, bool padded, hipS
                         int path_group = 0, int row_threads = 0) {
     if (row_threads <= 0) row_threads = auto_row_threads(d);
     if (path_group <= 0) path_group = auto_path_group(d, c);
-    if (path_lines <= 0) path_lines = auto_path_lines(d, path_group);
+    if (path_lines <= 0) path_lines = auto_path_lines(d, c, path_group);
     // A 64-lane line needs dpl % 16 == 0 and 4 lines per block, else it runs 32 lanes; a 32-lane line
     // needs dpl % 4 == 0 and at most 8 lines per block, else it runs 16.
     if (path_group == 64 && !paths_group_ok(d.dpl, 64, path_lines)) {
